@@ -1,0 +1,145 @@
+/* lumen_hip.h -- the C ABI of the MI355X photon-transport engine (liblumen_hip.so).
+ *
+ * The engine replaces the reference's photon loop behind `lumen.simulate`: `run.trace(model, ops, rows, key, ...)`
+ * plus `run.rebuild` (src/lumen/run.py:693-733, :534-557), which run tallymap's generated kernel over
+ * `transport.step.step` (src/lumen/transport/step.py:516-571) and bin deposit records into per-sensor cells
+ * (src/lumen/readout/fill.py:88-213).  Here one persistent HIP kernel traces every photon and accumulates the
+ * per-sensor tallies directly in FP64, so no records leave the device.
+ *
+ * Conventions: every entry point returns LH_OK (0) or an error code, with a message from lh_last_error().  Pointers
+ * named *_dev are device memory owned by the caller; everything else is host memory read during the call.  Calls on
+ * one model are not thread-safe; a `stream` of NULL is the default stream.  Units: m, ns, nm, eV.
+ */
+#ifndef LUMEN_HIP_H
+#define LUMEN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LH_OK 0
+#define LH_ERR_ARG 1    /* an argument is invalid (the message names it) */
+#define LH_ERR_HIP 2    /* a HIP runtime call failed */
+#define LH_ERR_LAYOUT 3 /* the packed layout is not this engine's */
+
+/* source kinds: an isotropic point (src/lumen/sources/point.py:73-86), a beam, a Cherenkov segment
+ * (src/lumen/sources/cherenkov.py:33-92) */
+#define LH_SRC_POINT 0
+#define LH_SRC_BEAM 1
+#define LH_SRC_CHERENKOV 2
+
+/* health cells (FP64), the reference's Health fates (src/lumen/api.py:445-495) plus counters */
+#define LH_H_ABSORBED_BULK 0
+#define LH_H_ABSORBED_SURFACE 1
+#define LH_H_LOST 2
+#define LH_H_ESCAPED 3
+#define LH_H_DISTANCE_CUT 4
+#define LH_H_ABANDONED 5
+#define LH_H_TRUNCATED 6
+#define LH_H_ROULETTED 7
+#define LH_H_REWEIGHTED 8
+#define LH_H_PUSHED 9
+#define LH_H_RECORDS 10       /* deposits at a sensor (the reference's deposit records) */
+#define LH_H_MOMENT_SPILLS 11 /* per-photon moment slots that overflowed (0 expected) */
+#define LH_H_VERTICES 12      /* steps taken */
+#define LH_H_PHOTONS 13       /* photons started */
+#define LH_H_START_WEIGHT 14  /* their row weights summed */
+#define LH_HEALTH_N 16
+
+typedef struct lh_model_s *lh_model;
+
+/* The navigator arrays, by the reference's names (src/lumen/geometry/navigator_build.py:371-397 `host`,
+ * src/lumen/geometry/csg_build.py:140-165 `T`, `K`, `G`).  Navigator ids: volumes 0..n_vol-1, the outside n_vol.
+ * R, t: x_world = R x_local + t (R row-major 3x3).  A leaf is {(x-q)'A(x-q) + 2 b.(x-q) + c <= 0 (typ 1)} and
+ * {n_k.(x-o) <= h_k}; a kind's program is postfix (op 0 push leaf arg, 1 union, 2 inter, 3 sub). */
+typedef struct {
+  int32_t n_vol, n_nodes, n_items, n_groups, n_leaves, n_planes, n_prog, n_kinds;
+  const float *R;          /* n_vol*9 */
+  const float *t;          /* n_vol*3 */
+  const int32_t *kind;     /* n_vol */
+  const int32_t *parent;   /* n_vol+1 (the outside's own entry is ignored) */
+  const int32_t *cls;      /* n_vol+1 class per navigator id */
+  const int32_t *sensor;   /* n_vol+1 sensor per navigator id, -1 none */
+  const int32_t *vg0, *vg1; /* n_vol+1 each navigator id's frame groups [vg0, vg1) */
+  const float *gR;         /* n_groups*9 each group's frame rotation */
+  const int32_t *g0, *g1;  /* n_groups each group's BVH nodes [g0, g1) */
+  const float *lo, *hi;    /* n_nodes*3 node boxes (group frame) */
+  const int32_t *skip, *start, *count; /* n_nodes depth-first order; a leaf has count > 0 */
+  const int32_t *items;    /* n_items daughter ids */
+  const int32_t *leaf_typ, *leaf_p0, *leaf_np; /* n_leaves */
+  const float *leaf_o, *leaf_q;                /* n_leaves*3 */
+  const float *leaf_A;                         /* n_leaves*9 */
+  const float *leaf_b;                         /* n_leaves*3 */
+  const float *leaf_c;                         /* n_leaves */
+  const float *plane_n;                        /* n_planes*3 */
+  const float *plane_h;                        /* n_planes */
+  const int32_t *kind_l0, *kind_nl, *kind_p0, *kind_pl; /* n_kinds */
+  const int32_t *prog_op, *prog_arg;                    /* n_prog */
+  float reach; /* the diagonal of the box holding every volume (m) */
+} lh_geometry;
+
+/* One emission source.  Photon i of it (0 <= i < n) has the id first_id + i: its random numbers are a function of
+ * (seed, id, vertex) only, so any split of a run into launches gives the same photons. */
+typedef struct {
+  int32_t kind;   /* LH_SRC_* */
+  int32_t medium; /* Cherenkov: the medium whose RINDEX sets the cone and the spectrum's span */
+  int32_t event;  /* the tally block this source adds into (0 <= event < n_events) */
+  int32_t flags;
+  float pos[3];   /* point position, or the segment's start (m) */
+  float dir[3];   /* beam direction, or the segment's unit direction */
+  float lam;      /* wavelength (nm) (point) */
+  float lam_hi;   /* > lam: wavelengths uniform on [lam, lam_hi] (point) */
+  float time;     /* emission time (ns) */
+  float weight;   /* photons each Monte Carlo photon stands for (n_photons / samples), before Cherenkov's own */
+  float length, beta, charge; /* Cherenkov segment */
+  float pad;
+  int64_t n;        /* photons of this source */
+  int64_t first_id; /* the id of its first photon */
+} lh_source;
+
+/* Run controls (src/lumen/transport/step.py:52-71 Controls, src/lumen/run.py:70 MAX_VERTICES). */
+typedef struct {
+  float roulette;       /* epsilon > 0 */
+  float max_distance;   /* <= 0: the geometry's reach */
+  int32_t max_vertices; /* the guard (steps per photon) */
+  int32_t chunk;        /* photons a wavefront takes from the work queue at once (0: the engine's default) */
+} lh_controls;
+
+/* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.
+ * Replaces the reference's Model + Operands build (src/lumen/transport/step.py:75-143). */
+int lh_model_create(const lh_geometry *geometry, const int32_t *layout, int32_t layout_words, lh_model *out);
+int lh_model_destroy(lh_model model);
+
+/* Query the model: number of sensors, of QE knots (charge cells per sensor) and tally doubles per event. */
+int lh_model_info(lh_model model, int32_t *n_sensors, int32_t *n_qe, int64_t *tally_per_event);
+
+/* Trace every photon of `sources` (host array of n_sources) and add into `tally_dev` (n_events blocks of
+ * [arrival S][arrival_t S][arrival2 S][charge S x n_qe], FP64) and `health_dev` (LH_HEALTH_N FP64).  Adds: zero
+ * them first for a fresh run.  `values_dev`: the packed values (float32, device, n_values of them, each table
+ * within them); `ref_values_dev`: the reference values every random choice is made at (NULL: values_dev); every
+ * source's event is below n_events.  Replaces run.trace + run.rebuild (src/lumen/run.py:93-145). */
+int lh_trace(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+             const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
+             double *tally_dev, int32_t n_events, double *health_dev, void *stream);
+
+/* Reverse-mode derivative by replay: each photon is traced twice from its Philox counter (no stored path): once to
+ * total its cotangent-weighted deposits, once to add each step's score and pathwise terms times the deposits still
+ * ahead.  `cotangent_dev`: dLoss/d(tally) in the tally layout (FP64); `slots_dev`: the gradient slot of each packed
+ * value (int32, -1 none); `grad_dev`: n_grad FP64 accumulators (added into).  Replaces the reference's jvp/vjp
+ * through tallymap's differentiated kernel (src/lumen/api.py:748-787). */
+int lh_adjoint(lh_model model, const float *values_dev, const float *ref_values_dev, const int32_t *slots_dev,
+               const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
+               const double *cotangent_dev, double *grad_dev, int32_t n_grad, double *health_dev, void *stream);
+
+/* The time of the last lh_trace / lh_adjoint's photon-loop kernel on its stream, in ms (HIP events). */
+int lh_last_kernel_ms(lh_model model, float *ms);
+
+const char *lh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUMEN_HIP_H */

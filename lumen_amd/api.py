@@ -1,0 +1,240 @@
+"""The drop-in surface: ``simulate(det, source, params, samples=, seed=) -> Light`` and ``Controls``.
+
+Mirrors the reference's ``lumen.simulate`` (``api.py:876-1180``) for the path it runs on the HIP engine: the source's
+photons are traced by the persistent kernel (``lh_trace``) into FP64 per-sensor tallies, from which ``Light`` gives
+``arrivals``, ``arrivals_error`` (the per-photon second moment's error, ``readout/grid.py:48-60``),
+``mean_arrival_time`` and the charge cells the readout reads, plus the run's ``health``.  Device memory and streams
+are torch's; the results are torch tensors on the device (``Light.to_dlpack`` hands them on, JAX included).
+
+Photon i of a run has the id ``first_id + i`` and its random numbers depend only on (seed, id, vertex), so a run's
+photons are the same however it is split (the reference's ``first_id`` chunks, ``api.py:1045-1111``).
+"""
+
+from __future__ import annotations
+
+import weakref
+from dataclasses import dataclass, field
+from typing import Any, NamedTuple
+
+import numpy as np
+
+from . import _lib, pack as packing
+from .params import Params
+from .sources import Batch
+
+MAX_VERTICES = 1 << 22
+ROULETTE = 0.01
+
+
+class Controls(NamedTuple):
+    """The step's controls (``transport/step.py:52-71``): ``max_distance`` (m; None: the geometry's reach),
+    ``roulette`` (epsilon > 0), ``max_vertices`` (the guard), ``chunk`` (photons per work-queue claim; 0 the
+    engine's default)."""  # noqa: D205, D209
+
+    max_distance: Any = None
+    roulette: float = ROULETTE
+    max_vertices: int = MAX_VERTICES
+    chunk: int = 0
+
+    def struct(self) -> _lib.LhControls:
+        if not (float(self.roulette) > 0.0):
+            raise ValueError(f"Controls: the roulette is a number > 0, got {self.roulette!r}")
+        return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
+                               int(self.max_vertices), int(self.chunk))
+
+
+class Health(NamedTuple):
+    """A run's health (the reference's ``api.py:445-495``): every photon's start weight ends in one fate."""
+
+    absorbed_bulk: float
+    absorbed_surface: float
+    lost: float
+    escaped: float
+    distance_cut: float
+    abandoned: float
+    truncated: float
+    rouletted: float
+    reweighted: float
+    pushed: float
+    records: float
+    moment_spills: float
+    vertices: float
+    photons: float
+    start_weight: float
+
+    FATES = ("absorbed_bulk", "absorbed_surface", "lost", "escaped", "distance_cut", "abandoned", "truncated",
+             "rouletted", "reweighted")
+
+    def balance(self) -> float:
+        """Return the start weight minus every fate: 0 to float32 rounding per step."""
+        return float(self.start_weight) - sum(float(getattr(self, k)) for k in self.FATES)
+
+    @staticmethod
+    def of(h: np.ndarray) -> "Health":
+        return Health(*[float(x) for x in np.asarray(h, np.float64)[:15]])
+
+
+@dataclass
+class Light:
+    """What ``simulate`` returns (the reference's ``api.py:515-537``): per sensor (with ``events``: per event and
+    sensor) the expected arrivals, their Monte Carlo error, the mean arrival time; the charge cells (S, K) on the QE
+    knots; the raw tallies; the health."""  # noqa: D205, D209
+
+    arrivals: Any
+    arrivals_error: Any
+    mean_arrival_time: Any
+    charge: Any
+    arrival2: Any
+    health: Health
+    samples: int
+    provenance: dict = field(default_factory=dict)
+    kernel_ms: float = float("nan")
+
+    def to_dlpack(self, name="arrivals"):
+        """Return a DLPack capsule of one output (device memory: ``jax.dlpack.from_dlpack`` takes it)."""
+        import torch.utils.dlpack
+
+        return torch.utils.dlpack.to_dlpack(getattr(self, name).contiguous())
+
+
+def moment_error(q2, s, n: int):
+    """Return sqrt(n / (n - 1) (q2 - s^2 / n)) clipped at 0: the error of a sum of n i.i.d. photons from its per-photon
+    second moment (``readout/grid.py:48-60``)."""  # noqa: D205, D209
+    import torch
+
+    n = float(max(int(n), 2))
+    return torch.sqrt(torch.clamp(n / (n - 1.0) * (q2 - s * s / n), min=0.0))
+
+
+class Engine:
+    """A detector's model on the device (built once per detector and device) and its run state."""
+
+    def __init__(self, det, device):
+        self.det = det
+        self.device = device
+        layout = packing.pack(det).layout
+        geo, self._keep = _lib.geometry_struct(det.nav, det.sensor_of_vol, det.reach)
+        h = _lib.C.c_void_p()
+        _lib.check(_lib.lib().lh_model_create(_lib.C.byref(geo), layout.ctypes.data_as(_lib._i), len(layout),
+                                              _lib.C.byref(h)), "lh_model_create")
+        self.handle = h
+        self.layout = layout
+        S, K, stride = _lib.C.c_int32(), _lib.C.c_int32(), _lib.C.c_int64()
+        _lib.check(_lib.lib().lh_model_info(h, _lib.C.byref(S), _lib.C.byref(K), _lib.C.byref(stride)),
+                   "lh_model_info")
+        self.S, self.K, self.stride = S.value, K.value, stride.value
+        weakref.finalize(self, _lib.lib().lh_model_destroy, h)
+
+    def kernel_ms(self) -> float:
+        ms = _lib.C.c_float()
+        _lib.check(_lib.lib().lh_last_kernel_ms(self.handle, _lib.C.byref(ms)), "lh_last_kernel_ms")
+        return ms.value
+
+
+_ENGINES: dict = {}
+
+
+def engine(det, device="cuda"):
+    """Return the detector's Engine on ``device`` (cached)."""
+    import torch
+
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError("simulate: the photon loop runs on the GPU (HIP); there is no CPU path in the product")
+    if not torch.cuda.is_available():
+        raise RuntimeError("simulate: no GPU is visible (torch.cuda.is_available() is False)")
+    key = (id(det), dev.index if dev.index is not None else torch.cuda.current_device())
+    ent = _ENGINES.get(key)
+    if ent is None or ent[0]() is not det:
+        with torch.cuda.device(key[1]):
+            eng = Engine(det, dev)
+        _ENGINES[key] = (weakref.ref(det), eng)
+        return eng
+    return ent[1]
+
+
+def source_rows(det, source, samples, first_id=0):
+    """Return (the sources' rows as an LhSource array, the number of events)."""
+    if isinstance(source, Batch):
+        rows = (_lib.LhSource * len(source.sources))()
+        first = int(first_id)
+        n_ev = 0
+        for i, (s, n) in enumerate(zip(source.sources, source.samples)):
+            r = s.row(det, max(int(n), 1))
+            r.n = int(n)
+            r.first_id = first
+            r.event = i if source.events is None else int(source.events[i])
+            n_ev = max(n_ev, r.event + 1)
+            first += int(n)
+            rows[i] = r
+        return rows, n_ev
+    if samples is None or int(samples) < 1:
+        raise ValueError("simulate: samples is a number of Monte Carlo photons >= 1")
+    rows = (_lib.LhSource * 1)()
+    r = source.row(det, int(samples))
+    r.n = int(samples)
+    r.first_id = int(first_id)
+    rows[0] = r
+    return rows, 1
+
+
+def simulate(det, source, params: Params | None = None, *, samples: int | None = None, seed: int = 0,
+             reference: Params | None = None, controls: Controls | None = None, first_id: int = 0,
+             device="cuda", stream=None) -> Light:
+    """Trace ``samples`` Monte Carlo photons of ``source`` through ``det`` and return the light at each sensor.
+
+    ``params``: the values (default ``det.params()``); ``reference``: the values every random choice is made at
+    (default: ``params``) -- held fixed while ``params`` moves, the light at a seed is smooth in ``params`` (common
+    random numbers).  A ``Batch`` source runs every source in one launch, one event each: the outputs gain a leading
+    event axis.
+    """
+    import torch
+
+    eng = engine(det, device)
+    controls = Controls() if controls is None else controls
+    p = det.params() if params is None else params
+    pk = packing.pack(det, p)
+    if not np.array_equal(pk.layout, eng.layout):
+        raise ValueError("simulate: these values change the detector's structure (a table's knot count)")
+    values = torch.as_tensor(pk.values, device=eng.device)
+    ref_values = None
+    if reference is not None:
+        pr = packing.pack(det, reference)
+        if not np.array_equal(pr.layout, eng.layout):
+            raise ValueError("simulate: the reference's structure differs from the detector's")
+        ref_values = torch.as_tensor(pr.values, device=eng.device)
+    rows, n_ev = source_rows(det, source, samples, first_id)
+    tally = torch.zeros((n_ev, eng.stride), dtype=torch.float64, device=eng.device)
+    health = torch.zeros(_lib.HEALTH_N, dtype=torch.float64, device=eng.device)
+    st = torch.cuda.current_stream(eng.device) if stream is None else stream
+    ctl = controls.struct()
+    _lib.check(_lib.lib().lh_trace(eng.handle, _lib.C.c_void_p(values.data_ptr()),
+                                   None if ref_values is None else _lib.C.c_void_p(ref_values.data_ptr()),
+                                   int(values.numel()), rows, len(rows),
+                                   _lib.C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), _lib.C.byref(ctl),
+                                   _lib.C.c_void_p(tally.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
+                                   _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
+    light = light_of(eng, tally, health.cpu().numpy(), samples if not isinstance(source, Batch) else None, source)
+    light.kernel_ms = eng.kernel_ms()
+    light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, controls=controls._asdict(),
+                            engine="liblumen_hip (gfx950)", reference=reference is not None)
+    return light
+
+
+def light_of(eng, tally, health, samples, source=None) -> Light:
+    """Return the Light of raw tallies (n_events, stride)."""
+    import torch
+
+    S, K = eng.S, eng.K
+    arr, arr_t, arr2 = tally[:, :S], tally[:, S:2 * S], tally[:, 2 * S:3 * S]
+    charge = tally[:, 3 * S:].reshape(-1, S, K)
+    if isinstance(source, Batch):
+        n = torch.as_tensor([max(int(x), 2) for x in source.samples], dtype=torch.float64, device=tally.device)
+        n = n[:, None] if source.events is None else torch.full_like(arr[:, :1], float(max(source.samples)))
+        err = torch.sqrt(torch.clamp(n / (n - 1.0) * (arr2 - arr * arr / n), min=0.0))
+    else:
+        err = moment_error(arr2, arr, samples)
+    mean_t = torch.where(arr > 0, arr_t / torch.where(arr > 0, arr, torch.ones_like(arr)), torch.zeros_like(arr))
+    if tally.shape[0] == 1 and not isinstance(source, Batch):
+        arr, err, mean_t, charge, arr2 = arr[0], err[0], mean_t[0], charge[0], arr2[0]
+    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0)

@@ -1,0 +1,892 @@
+// The photon loop on gfx950: one persistent kernel traces every photon of every source and tallies each sensor's
+// arrivals in FP64, behind the C ABI of include/lumen_hip.h.
+//
+// Restates the reference's loop (src/lumen/run.py:293-368 `_loop`: one item per photon stepping until it ends) and
+// its step (src/lumen/transport/step.py:516-571 `step`, :318-359 `_step_context`, :362-394 `_surface_event`,
+// :397-436 `_bulk`, :464-492 `_after`, :495-513 `_advance`), with the deposit records' readout fused in
+// (src/lumen/run.py:443-557 `rebuild`/`_moment_adds`, src/lumen/readout/fill.py:88-213): every deposit adds q w0 to
+// `arrival`, q w0 t to `arrival_t`, q w0 hat_k(lam) to `charge` and, when its photon ends, each sensor's square of
+// the photon's summed deposits to `arrival2` (the per-photon second moment).
+//
+// Schedule: each wavefront keeps 64 photons in flight; a lane whose photon ended takes the next one from a chunk its
+// wavefront claimed from the device work queue (one atomic per chunk), so lanes refill every step (the reference's
+// "step" schedule, src/lumen/run.py:81) and a launch of many sources with uneven photon counts keeps every lane busy.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lumen_hip.h"
+#include "nav.h"
+#include "optics.h"
+#include "philox.h"
+
+namespace lh {
+
+constexpr float ZERO_STEP_BANDS = 0.125f;
+constexpr int ZERO_STEPS_BEFORE_PUSH = 10;
+constexpr int ZERO_STEPS_BEFORE_KILL = 25;
+constexpr float PUSH_BANDS = 8.0f;
+constexpr float FINE_STRUCTURE = 1.0f / 137.035999084f;
+constexpr int MOMENT_SLOTS = 4;
+constexpr int MAX_FACETS = 256;
+constexpr int MAX_FACET_ATTEMPTS = 4096;
+constexpr int BLOCK = 256;
+constexpr int DEFAULT_CHUNK = 256;
+
+struct Args {
+  Geo g;
+  const int *__restrict__ L;
+  const float *__restrict__ P;
+  const float *__restrict__ Pr;
+  int has_ref;
+  const lh_source *__restrict__ src;
+  const long long *__restrict__ src_end;  // inclusive prefix sums of the sources' photon counts
+  int nsrc;
+  long long total;
+  uint32_t k0, k1;
+  float roulette, max_dist, reach;
+  int max_vert, chunk;
+  double *__restrict__ tally;
+  long long tally_stride;  // doubles per event
+  int S, K;
+  double *__restrict__ health;
+  unsigned long long *__restrict__ queue;
+};
+
+struct Photon {
+  float3 x, d, er, ei;
+  float t, lam, w, wr, w0;
+  int vol, srf, nz, vertex, cls, event;
+  lh_rng rng;
+  // the medium cache: the class it was evaluated for, live and reference values
+  int mcls;
+  MediumAt ml, mr;
+  // the per-photon moment: up to MOMENT_SLOTS sensors and the photon's summed deposits at each
+  int ms[MOMENT_SLOTS];
+  float mv[MOMENT_SLOTS];
+};
+
+struct Acc {  // per-lane health accumulators, flushed once per launch
+  float bulk, surf, esc, rou, rew, lost, dcut, aband, trunc, pushed, start_w;
+  int records, spills, vertices, photons;
+};
+
+__device__ __forceinline__ int class_medium(const Args &a, int cls) {
+  return cls < 0 ? -1 : a.L[a.L[L_CLASS_MEDIUM_AT] + cls];
+}
+
+__device__ __forceinline__ void tally_add(const Args &a, int event, int slot, double v) {
+  unsafeAtomicAdd(a.tally + (long long)event * a.tally_stride + slot, v);
+}
+
+__device__ inline void moment_flush_slot(const Args &a, Photon &p, int k) {
+  if (p.ms[k] >= 0) {
+    const double v = (double)p.mv[k];
+    tally_add(a, p.event, 2 * a.S + p.ms[k], v * v);
+    p.ms[k] = -1;
+    p.mv[k] = 0.0f;
+  }
+}
+
+__device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, float v) {
+  int free_k = -1;
+#pragma unroll
+  for (int k = 0; k < MOMENT_SLOTS; ++k) {
+    if (p.ms[k] == s) {
+      p.mv[k] += v;
+      return;
+    }
+    if (free_k < 0 && p.ms[k] < 0) free_k = k;
+  }
+  if (free_k < 0) {  // more sensors than slots: square the oldest now (its later deposits start a new sum)
+    moment_flush_slot(a, p, 0);
+#pragma unroll
+    for (int k = 0; k + 1 < MOMENT_SLOTS; ++k) {
+      p.ms[k] = p.ms[k + 1];
+      p.mv[k] = p.mv[k + 1];
+    }
+    p.ms[MOMENT_SLOTS - 1] = -1;
+    p.mv[MOMENT_SLOTS - 1] = 0.0f;
+    free_k = MOMENT_SLOTS - 1;
+    acc.spills += 1;
+  }
+  p.ms[free_k] = s;
+  p.mv[free_k] = v;
+}
+
+// one deposit at sensor s: arrival, arrival_t, the charge cells at the QE knots, the photon's moment
+__device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float q, float time) {
+  acc.records += 1;
+  const float v = q * p.w0;
+  if (v == 0.0f) return;
+  const double dv = (double)v;
+  tally_add(a, p.event, s, dv);
+  tally_add(a, p.event, a.S + s, dv * (double)time);
+  const int qoff = a.L[L_QE_TABLE];
+  if (a.K <= 1) {
+    tally_add(a, p.event, 3 * a.S + s, dv);
+  } else {  // hat weights on the QE wavelength knots (linear in wavelength, held beyond the ends)
+    const TabPos tp = tab_pos(a.P, qoff, a.K, p.lam);
+    tally_add(a, p.event, 3 * a.S + s * a.K + tp.i, dv * (double)(1.0f - tp.f));
+    if (tp.f > 0.0f) tally_add(a, p.event, 3 * a.S + s * a.K + tp.i + 1, dv * (double)tp.f);
+  }
+  moment_add(a, p, acc, s, v);
+}
+
+__device__ inline void photon_end(const Args &a, Photon &p) {
+#pragma unroll
+  for (int k = 0; k < MOMENT_SLOTS; ++k) moment_flush_slot(a, p, k);
+}
+
+// the source and local index of launch photon gi
+__device__ __forceinline__ int source_of(const Args &a, long long gi) {
+  int lo = 0, hi = a.nsrc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a.src_end[mid] > gi) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// emission: a source row's photon (sources/point.py:73-86, sources/cherenkov.py:33-66), located (item.py:24-36)
+__device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
+  const int j = source_of(a, gi);
+  const lh_source &S = a.src[j];
+  const long long local = gi - (j > 0 ? a.src_end[j - 1] : 0);
+  const unsigned long long id = (unsigned long long)(S.first_id + local);
+  p.rng = lh_rng{a.k0, a.k1, (uint32_t)id, (uint32_t)(id >> 32)};
+  float u[4], v[4];
+  p.rng.draw4(LH_SOURCE_VERTEX, 0, u);
+  p.rng.draw4(LH_SOURCE_VERTEX, 1, v);
+  p.x = mk3(S.pos[0], S.pos[1], S.pos[2]);
+  p.t = S.time;
+  p.w0 = S.weight;
+  p.er = p.ei = mk3(0.f, 0.f, 0.f);
+  p.event = S.event;
+  if (S.kind == LH_SRC_CHERENKOV) {
+    const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * S.medium;
+    const float e_lo = a.P[M[0]], e_hi = a.P[M[0] + M[1] - 1];
+    const float lam_min = HC / e_hi, lam_max = HC / e_lo;
+    const float zq = 1.0f / lam_min - 1.0f / lam_max;
+    p.lam = 1.0f / (1.0f / lam_min - u[0] * zq);
+    const float n = tab(a.P, M[0], M[1], HC / p.lam);
+    const float bn = S.beta * n;
+    const bool emits = bn > 1.0f;
+    const float s2 = emits ? 1.0f - 1.0f / (bn * bn) : 0.0f;
+    p.w0 = TWO_PI * FINE_STRUCTURE * S.charge * S.charge * S.length * s2 * zq * 1e9f * S.weight;
+    const float cos_c = emits ? 1.0f / bn : 1.0f;
+    const float sin_c = sqrtf(fmaxf(1.0f - cos_c * cos_c, 0.0f));
+    float sp, cp;
+    __sincosf(TWO_PI * u[2], &sp, &cp);
+    const float3 axis = mk3(S.dir[0], S.dir[1], S.dir[2]);
+    p.d = unit3(to_world(mk3(sin_c * cp, sin_c * sp, cos_c), axis));
+    const float xs = u[1] * S.length;
+    p.x = fma3(xs, axis, p.x);
+    p.t = S.time + xs / (S.beta * C_VACUUM);
+    p.er = unit3(fma3(dot3(axis, p.d), p.d, mul3(-1.0f, axis)));
+  } else {
+    p.lam = S.lam_hi > S.lam ? S.lam + u[2] * (S.lam_hi - S.lam) : S.lam;
+    if (S.kind == LH_SRC_BEAM) {
+      p.d = unit3(mk3(S.dir[0], S.dir[1], S.dir[2]));
+    } else {
+      const float c = 1.0f - 2.0f * u[0];
+      const float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float sp, cp;
+      __sincosf(TWO_PI * u[1], &sp, &cp);
+      p.d = mk3(s * cp, s * sp, c);
+    }
+  }
+  p.w = 1.0f;
+  p.wr = 1.0f;
+  p.vol = locate(a.g, p.x);
+  p.cls = a.g.vinfo[p.vol].z;
+  p.srf = -1;
+  p.nz = 0;
+  p.vertex = 0;
+  p.mcls = -2;
+#pragma unroll
+  for (int k = 0; k < MOMENT_SLOTS; ++k) {
+    p.ms[k] = -1;
+    p.mv[k] = 0.0f;
+  }
+  acc.photons += 1;
+  acc.start_w += p.w0;
+  if (p.vol >= a.g.nvol || class_medium(a, p.cls) < 0) {  // emitted outside the detector or in an opaque medium
+    acc.lost += p.w0;
+    return false;
+  }
+  return true;
+}
+
+// Geant4's unified facet path (facets.py:318-397): facets drawn for the current direction and mirrored about
+// until the direction leaves the surface; the polarisation mirrored about each facet's half vector
+__device__ inline float3 facet_path(const Photon &p, float sigma, float3 wi, float3 n, float3 &er, float3 &ei) {
+  float3 V = to_local_frame(mul3(-1.0f, wi), n);
+  float3 w = wi;
+  const float fmax = fminf(1.0f, 4.0f * sigma);
+  for (int k = 0; k < MAX_FACETS; ++k) {
+    float a = 0.0f, u2 = 0.5f;
+    for (int t = 0; t < MAX_FACET_ATTEMPTS; ++t) {
+      float u[4];
+      p.rng.draw4(p.vertex, LH_FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
+      const float al = sigma * sqrtf(-2.0f * __logf(u[0]));
+      if (al < HALF_PI && u[1] * al * PI_F < fminf(__sinf(al), fmax) * front_arc(V, al)) {
+        a = al;
+        u2 = u[2];
+        break;
+      }
+    }
+    const float psi = (2.0f * u2 - 1.0f) * front_arc(V, a);
+    const float ph = atan2f(V.y, V.x) + psi;
+    float sa, ca, sp, cp;
+    __sincosf(a, &sa, &ca);
+    __sincosf(ph, &sp, &cp);
+    const float3 m = mk3(sa * cp, sa * sp, ca);
+    const float3 Lo = fma3(2.0f * dot3(V, m), m, mul3(-1.0f, V));
+    const float3 wo = to_world(Lo, n);
+    pol_mirror(er, ei, unit3(sub3(wo, w)));
+    w = wo;
+    V = mul3(-1.0f, Lo);
+    if (Lo.z >= 0.0f) break;
+  }
+  return w;
+}
+
+__device__ inline void medium_cache(const Args &a, Photon &p) {
+  if (p.mcls == p.cls) return;
+  const int m = class_medium(a, p.cls);
+  p.ml = medium_at(a.L, a.P, m, p.lam);
+  p.mr = a.has_ref ? medium_at(a.L, a.Pr, m, p.lam) : p.ml;
+  p.mcls = p.cls;
+}
+
+__device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.0f ? live / ref : 0.0f; }
+
+// one step of one photon (transport/step.py:516-571); returns whether it goes on
+__device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
+  const Geo &g = a.g;
+  float b0 = band_at(maxabs3(p.x));
+  if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {  // G4Navigator's stuck-track push (step.py:269-282 `_recover`)
+    if (p.nz == ZERO_STEPS_BEFORE_PUSH) acc.pushed += p.w * p.w0;
+    p.x = fma3(PUSH_BANDS * b0, p.d, p.x);
+    p.vol = locate(g, p.x);
+    p.cls = g.vinfo[p.vol].z;
+    p.srf = -1;
+    b0 = band_at(maxabs3(p.x));
+  }
+  float u[4], ul[4];
+  p.rng.draw4(p.vertex, 0, u);   // path, channel, outcome, roulette
+  p.rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
+  float3 er0 = p.er, ei0 = p.ei;
+  pol_fill(p.d, er0, ei0, ul[3]);
+  medium_cache(a, p);
+  const MediumAt ml = p.ml, mr = p.mr;
+  const float mu = ml.mu_ray, mu_r = mr.mu_ray;
+  const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
+  const float max_dist = a.max_dist;
+  const Cross cr = query(g, p.x, p.d, p.vol, p.srf, max_dist);
+  const bool hit = cr.vol_after >= 0;
+  const float dist = cr.t;
+  const bool zero = dist <= ZERO_STEP_BANDS * b0;
+  const int cb = p.cls;
+  const int ca = hit ? g.vinfo[cr.vol_after].z : -1;
+  int row = hit ? a.L[a.L[L_ROW_OF_AT] + cb * a.L[L_CLASSES] + ca] : -1;
+  const bool cont = hit && zero && row < 0;  // a zero-length crossing no surface joins: a straight pass
+  if (cont) row = 0;
+  const bool scatter = mu_r > 0.0f && s < dist;
+  const float reach_live = __expf(-mu * dist), reach_ref = a.has_ref ? __expf(-mu_r * dist) : reach_live;
+  const float att = __expf(-ml.mu_abs * dist), att_r = a.has_ref ? __expf(-mr.mu_abs * dist) : att;
+  const float v_g = C_VACUUM / ml.ng;
+  SurfEval sl, sr;
+  int o = -1;
+  float pprop[N_OUT], pdraw[N_OUT];
+  const bool at_if = hit && row >= 0;
+  if (at_if) {
+    const int m2 = class_medium(a, ca);
+    const float n2 = index_at(a.L, a.P, m2, p.lam);
+    sl = surface_eval(a.L, a.P, row, p.lam, p.d, cr.n, ml.n, n2, er0, ei0);
+    if (cont) {
+      for (int k = 0; k < N_OUT; ++k) sl.p[k] = 0.0f;
+      sl.p[STRAIGHT] = 1.0f;
+      sl.detect = 0.0f;
+    }
+    if (a.has_ref) {
+      const float n2r = index_at(a.L, a.Pr, m2, p.lam);
+      sr = surface_eval(a.L, a.Pr, row, p.lam, p.d, cr.n, mr.n, n2r, er0, ei0);
+      if (cont) {
+        for (int k = 0; k < N_OUT; ++k) sr.p[k] = 0.0f;
+        sr.p[STRAIGHT] = 1.0f;
+        sr.detect = 0.0f;
+      }
+    } else {
+      sr = sl;
+    }
+    // the draw's probabilities: the reference's, or at an edge row the one defensive mixture over the row's
+    // support (estimator.py:96-153); the proposal normalises them
+    float tot = 0.0f;
+    int ncov = 0;
+    const int support = (sr.flags >> 8) & 63;
+    const bool edge = (sr.flags & FLAG_EDGE) && !cont;
+    for (int k = 0; k < N_OUT; ++k) {
+      pdraw[k] = sr.p[k];
+      if (edge && ((support >> k) & 1) && !(sr.p[k] > 0.0f)) ++ncov;
+    }
+    if (ncov > 0) {
+      const float eps = DEFENSIVE / (1.0f - DEFENSIVE) / (float)ncov;
+      for (int k = 0; k < N_OUT; ++k)
+        if (((support >> k) & 1) && !(sr.p[k] > 0.0f)) pdraw[k] = eps;
+    }
+    for (int k = 0; k < N_OUT; ++k) tot += pdraw[k];
+    for (int k = 0; k < N_OUT; ++k) pprop[k] = tot > 0.0f ? pdraw[k] / tot : 0.0f;
+    o = choose6(pprop, u[2]);
+    // the deposit: the expected detection at the crossing (step.py:522-524)
+    const float dep_q = p.w * reach_live * att * sl.detect;
+    const int sensor = g.vinfo[cr.vol_after].w;
+    if (sensor >= 0 && !cont) deposit(a, p, acc, sensor, dep_q, p.t + dist / v_g);
+  }
+  bool ok;
+  float w_new, wr_new;
+  if (scatter) {  // a bulk event at the free path's end (step.py:397-436): Rayleigh, the one channel here
+    const float w_path = ratio(mu * __expf(-mu * s), mu_r * __expf(-mu_r * s));
+    const float att_s = __expf(-ml.mu_abs * s);
+    const float w_at = p.w * w_path;
+    w_new = w_at * att_s;
+    wr_new = p.wr * (a.has_ref ? __expf(-mr.mu_abs * s) : att_s);
+    acc.bulk += w_at * (1.0f - att_s) * p.w0;
+    const float3 dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
+    pol_project(er0, ei0, dn);
+    p.x = fma3(s, p.d, p.x);
+    p.t += s / v_g;
+    p.d = unit3(dn);
+    p.er = er0;
+    p.ei = ei0;
+    const bool held = s <= ZERO_STEP_BANDS * b0 && p.srf >= 0;
+    if (!held) p.srf = -1;
+    ok = true;
+  } else if (hit) {  // the surface's drawn outcome (step.py:362-394)
+    const float w_at = p.w * ratio(reach_live, reach_ref);
+    float s_cont = 0.0f;
+    if (at_if)
+      for (int k = 0; k < N_OUT; ++k) s_cont += sl.p[k];
+    acc.bulk += w_at * (1.0f - att) * p.w0;
+    if (at_if) acc.surf += w_at * att * (1.0f - s_cont) * p.w0;
+    ok = at_if && o >= 0;
+    if (!at_if) acc.lost += w_at * att * p.w0;
+    if (ok) {
+      const float w_if = w_at * att * ratio(sl.p[o], pprop[o]);
+      const float r_o = ratio(pdraw[o], pprop[o]);
+      wr_new = p.wr * (reach_ref != 0.0f ? 1.0f : 0.0f) * att_r * r_o;
+      float3 dn = p.d;
+      const float3 wi = p.d, n = cr.n;
+      bool far = false;
+      if (o == STRAIGHT) {
+        far = true;
+      } else if (o == REFRACT) {
+        const float m2n = index_at(a.L, a.P, class_medium(a, ca), p.lam);
+        dn = refract_dir(wi, n, ml.n / m2n);
+        pol_recombine(er0, ei0, wi, s_axis(wi, n), sl.amps.ts, sl.amps.tp, unit3(dn));
+        far = true;
+      } else if (o == SPECULAR) {
+        dn = fma3(-2.0f * dot3(wi, n), n, wi);
+        pol_recombine(er0, ei0, wi, s_axis(wi, n), sl.amps.rs, sl.amps.rp, unit3(dn));
+      } else if (o == BACKSCATTER) {
+        dn = mul3(-1.0f, wi);
+        er0 = mul3(-1.0f, er0);
+        ei0 = mul3(-1.0f, ei0);
+      } else if (o == GLOSSY) {
+        dn = facet_path(p, sr.sigma, wi, n, er0, ei0);
+      } else {  // DIFFUSE
+        dn = lambert_sample(n, ul[0], ul[1]);
+        pol_mirror(er0, ei0, unit3(sub3(dn, wi)));
+      }
+      w_new = w_if;
+      acc.rew += (w_at * att * s_cont - w_new) * p.w0;
+      p.x = fma3(dist, p.d, p.x);
+      p.t += dist / v_g;
+      p.d = unit3(dn);
+      p.er = er0;
+      p.ei = ei0;
+      p.srf = cr.srf;
+      if (far) {
+        p.vol = cr.vol_after;
+        p.cls = ca;
+      }
+    } else {
+      w_new = wr_new = 0.0f;
+    }
+  } else {  // nothing met within max_distance and no bulk event: the photon left (step.py:551-560)
+    const float d_end = fminf(dist, max_dist);
+    const float w_far = p.w * ratio(__expf(-mu * d_end), __expf(-mu_r * d_end));
+    const float att_far = __expf(-ml.mu_abs * d_end);
+    if (max_dist < a.reach) acc.dcut += w_far * att_far * p.w0; else acc.esc += w_far * att_far * p.w0;
+    acc.bulk += w_far * (1.0f - att_far) * p.w0;
+    ok = false;
+    w_new = wr_new = 0.0f;
+  }
+  // the roulette on the reference weight, and the stuck rule (step.py:464-492)
+  bool alive = ok;
+  if (ok) {
+    if (wr_new < a.roulette) {
+      const float ps = wr_new / a.roulette;
+      if (u[3] < ps) {
+        const float wk = w_new / ps;
+        acc.rou += (w_new - wk) * p.w0;
+        w_new = wk;
+        wr_new = wr_new / ps;
+      } else {
+        acc.rou += w_new * p.w0;
+        w_new = wr_new = 0.0f;
+        alive = false;
+      }
+    }
+  }
+  p.nz = (hit && !scatter && dist <= ZERO_STEP_BANDS * b0) ? p.nz + 1 : 0;
+  p.w = w_new;
+  p.wr = wr_new;
+  p.vertex += 1;
+  acc.vertices += 1;
+  if (alive && p.nz >= ZERO_STEPS_BEFORE_KILL) {
+    acc.aband += p.w * p.w0;
+    alive = false;
+  }
+  if (alive && p.vertex >= a.max_vert) {
+    acc.trunc += p.w * p.w0;
+    alive = false;
+  }
+  return alive;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ inline void flush_health(const Args &a, const Acc &acc) {
+  const float vals[LH_HEALTH_N] = {acc.bulk, acc.surf, acc.lost, acc.esc, acc.dcut, acc.aband, acc.trunc, acc.rou,
+                                   acc.rew, acc.pushed, (float)acc.records, (float)acc.spills,
+                                   (float)acc.vertices, (float)acc.photons, acc.start_w, 0.0f};
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < LH_HEALTH_N; ++k) {
+    const float s = wave_sum(vals[k]);
+    if (lane == 0 && s != 0.0f) unsafeAtomicAdd(a.health + k, (double)s);
+  }
+}
+
+__global__ void reset_queue(unsigned long long *q) { *q = 0ull; }
+
+__global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
+  Photon p;
+  Acc acc{};
+  bool alive = false;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  long long cursor = 0, chunk_end = 0;
+  bool drained = false;
+  for (;;) {
+    // refill the lanes whose photon ended from the wavefront's chunk of the work queue
+    bool need = !alive;
+    unsigned long long mask = __ballot(need);
+    while (mask != 0ull && !drained) {
+      const long long avail = chunk_end - cursor;
+      if (avail <= 0) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+        base = __shfl(base, 0);
+        if ((long long)base >= a.total) {
+          drained = true;
+          break;
+        }
+        cursor = (long long)base;
+        chunk_end = min((long long)base + a.chunk, a.total);
+        continue;
+      }
+      const int rank = __popcll(mask & lt_mask);
+      const bool take = need && rank < avail;
+      if (take) alive = emit(a, cursor + rank, p, acc);
+      cursor += min((long long)__popcll(mask), avail);
+      need = need && !take;
+      mask = __ballot(need);
+    }
+    if (!__any(alive) && drained) break;
+    if (alive) {
+      alive = step(a, p, acc);
+      if (!alive) photon_end(a, p);
+    }
+  }
+  flush_health(a, acc);
+}
+
+}  // namespace lh
+
+// ---------------------------------------------------------------------------------------------------- the C ABI
+#define LH_API extern "C" __attribute__((visibility("default")))
+using namespace lh;
+
+struct lh_model_s {
+  Geo g;
+  void *geo_buf = nullptr;
+  int *layout = nullptr;
+  int layout_words = 0;
+  int S = 0, K = 1, n_media = 0;
+  long long tally_stride = 0;
+  unsigned long long *queue = nullptr;
+  lh_source *src = nullptr;
+  long long *src_end = nullptr;
+  int src_cap = 0;
+  int grid = 0;
+  long long values_needed = 0;  // the packed values a call must pass (the largest table's end)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int timed = 0;
+};
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail(LH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+LH_API const char *lh_last_error(void) { return g_err.c_str(); }
+
+LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t layout_words, lh_model *out) {
+  if (!G || !layout || !out) return fail(LH_ERR_ARG, "lh_model_create: null argument");
+  if (layout_words < 11 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  const int n = G->n_vol;
+  if (n <= 0 || n >= (1 << 21)) return fail(LH_ERR_ARG, "lh_model_create: n_vol out of range");
+  for (int i = 0; i < G->n_nodes; ++i) {
+    if (G->count[i] > 7) return fail(LH_ERR_ARG, "lh_model_create: a BVH leaf holds more than 7 items");
+    if (G->skip[i] < 0 || G->skip[i] > G->n_nodes) return fail(LH_ERR_ARG, "lh_model_create: a skip link is out of range");
+    if (G->count[i] > 0 && (G->start[i] < 0 || G->start[i] + G->count[i] > G->n_items))
+      return fail(LH_ERR_ARG, "lh_model_create: a leaf's items are out of range");
+  }
+  for (int i = 0; i < G->n_items; ++i)
+    if (G->items[i] < 0 || G->items[i] >= n) return fail(LH_ERR_ARG, "lh_model_create: an item is not a volume");
+  for (int i = 0; i < n; ++i)
+    if (G->kind[i] < 0 || G->kind[i] >= G->n_kinds) return fail(LH_ERR_ARG, "lh_model_create: a kind is out of range");
+  for (int k = 0; k < G->n_kinds; ++k) {
+    if (G->kind_l0[k] < 0 || G->kind_l0[k] + G->kind_nl[k] > G->n_leaves || G->kind_nl[k] < 1 ||
+        G->kind_nl[k] > MAX_CSG_LEAVES || G->kind_p0[k] < 0 || G->kind_p0[k] + G->kind_pl[k] > G->n_prog)
+      return fail(LH_ERR_ARG, "lh_model_create: a kind's leaves or program are out of range (at most 8 leaves)");
+    int depth = 0;
+    for (int j = 0; j < G->kind_pl[k]; ++j) {
+      const int op = G->prog_op[G->kind_p0[k] + j], arg = G->prog_arg[G->kind_p0[k] + j];
+      if (op == 0) {
+        if (arg < 0 || arg >= G->kind_nl[k]) return fail(LH_ERR_ARG, "lh_model_create: a program pushes a foreign leaf");
+        if (++depth > 31) return fail(LH_ERR_ARG, "lh_model_create: a program deeper than 31");
+      } else {
+        if (depth < 2 || op > 3) return fail(LH_ERR_ARG, "lh_model_create: a malformed program");
+        --depth;
+      }
+    }
+    if (depth != 1) return fail(LH_ERR_ARG, "lh_model_create: a program that does not leave one value");
+  }
+  for (int l = 0; l < G->n_leaves; ++l)
+    if (G->leaf_p0[l] < 0 || G->leaf_p0[l] + G->leaf_np[l] > G->n_planes)
+      return fail(LH_ERR_ARG, "lh_model_create: a leaf's planes are out of range");
+  for (int v = 0; v <= n; ++v) {
+    if (G->vg0[v] < 0 || G->vg1[v] > G->n_groups || G->vg0[v] > G->vg1[v])
+      return fail(LH_ERR_ARG, "lh_model_create: a frame group range is out of range");
+    if (v < n && (G->parent[v] < 0 || G->parent[v] > n)) return fail(LH_ERR_ARG, "lh_model_create: a parent is out of range");
+  }
+  for (int gi = 0; gi < G->n_groups; ++gi)
+    if (G->g0[gi] < 0 || G->g1[gi] > G->n_nodes || G->g0[gi] > G->g1[gi])
+      return fail(LH_ERR_ARG, "lh_model_create: a group's nodes are out of range");
+  const int nc = layout[2];
+  for (int v = 0; v <= n; ++v) {
+    if (G->cls[v] < 0 || G->cls[v] >= nc) return fail(LH_ERR_ARG, "lh_model_create: a class is out of range");
+    if (G->sensor[v] < -1 || G->sensor[v] >= layout[4]) return fail(LH_ERR_ARG, "lh_model_create: a sensor is out of range");
+  }
+  {  // the layout's own ranges: media, classes, the row table
+    const int nm = layout[1], nr = layout[3];
+    const long long need_media = (long long)layout[6] + (long long)MEDIUM_WORDS * nm;
+    const long long need_rows = (long long)layout[9] + (long long)ROW_WORDS * nr;
+    if (nm < 0 || nc < 1 || nr < 0 || layout[5] < 1 || layout[4] < 0 || need_media > layout_words ||
+        (long long)layout[7] + nc > layout_words || (long long)layout[8] + (long long)nc * nc > layout_words ||
+        need_rows > layout_words)
+      return fail(LH_ERR_LAYOUT, "lh_model_create: the layout's sections are out of range");
+    for (int c = 0; c < nc; ++c)
+      if (layout[layout[7] + c] < -1 || layout[layout[7] + c] >= nm)
+        return fail(LH_ERR_LAYOUT, "lh_model_create: a class's medium is out of range");
+    for (int c = 0; c < nc * nc; ++c)
+      if (layout[layout[8] + c] < -1 || layout[layout[8] + c] >= nr)
+        return fail(LH_ERR_LAYOUT, "lh_model_create: a surface row index is out of range");
+    for (int r = 0; r < nr; ++r) {
+      const int32_t *R = layout + layout[9] + ROW_WORDS * r;
+      if (R[0] < 0 || R[0] > 1) return fail(LH_ERR_LAYOUT, "lh_model_create: an unknown surface model");
+      for (int c = 0; c < N_COL; ++c)
+        if (R[2 + 2 * c] < 0 || R[3 + 2 * c] < 1) return fail(LH_ERR_LAYOUT, "lh_model_create: a surface column is empty");
+    }
+    for (int mi = 0; mi < nm; ++mi) {
+      const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
+      if (!(M[8] & 1) && (M[1] < 1 || M[3] < 1)) return fail(LH_ERR_LAYOUT, "lh_model_create: a medium without RINDEX");
+      for (int j = 0; j < 8; j += 2)
+        if (M[j] < 0 || M[j + 1] < 0) return fail(LH_ERR_LAYOUT, "lh_model_create: a negative table offset");
+    }
+  }
+
+  // the device layout: float4 / int4 arrays in one allocation
+  std::vector<float4> xf(3 * (size_t)n), gxf(3 * (size_t)std::max(G->n_groups, 1)), node(2 * (size_t)std::max(G->n_nodes, 1)),
+      leaf(4 * (size_t)G->n_leaves), plane((size_t)std::max(G->n_planes, 1));
+  std::vector<int4> vinfo((size_t)n + 1), kinds((size_t)G->n_kinds);
+  std::vector<int2> vgrp((size_t)n + 1), grange((size_t)std::max(G->n_groups, 1));
+  std::vector<int> items((size_t)std::max(G->n_items, 1)), prog((size_t)std::max(G->n_prog, 1));
+  for (int v = 0; v < n; ++v) {
+    const float *R = G->R + 9 * v, *t = G->t + 3 * v;
+    for (int r = 0; r < 3; ++r) {  // row r of R^T is column r of R; offset -(R^T t)_r
+      const float m0 = R[0 * 3 + r], m1 = R[1 * 3 + r], m2 = R[2 * 3 + r];
+      xf[3 * v + r] = make_float4(m0, m1, m2, -(m0 * t[0] + m1 * t[1] + m2 * t[2]));
+    }
+  }
+  for (int v = 0; v <= n; ++v) {
+    vinfo[v] = make_int4(v < n ? G->kind[v] : 0, v < n ? G->parent[v] : n, G->cls[v], G->sensor[v]);
+    vgrp[v] = make_int2(G->vg0[v], G->vg1[v]);
+  }
+  for (int gi = 0; gi < G->n_groups; ++gi) {
+    const float *R = G->gR + 9 * gi;
+    bool ident = true;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) ident &= R[3 * r + c] == (r == c ? 1.0f : 0.0f);
+    for (int r = 0; r < 3; ++r) gxf[3 * gi + r] = make_float4(R[0 * 3 + r], R[1 * 3 + r], R[2 * 3 + r], ident ? 1.0f : 0.0f);
+    grange[gi] = make_int2(G->g0[gi], G->g1[gi]);
+  }
+  for (int i = 0; i < G->n_nodes; ++i) {
+    int skip = G->skip[i], sc = G->count[i] > 0 ? G->start[i] * 8 + G->count[i] : 0;
+    float4 lo, hi;
+    memcpy(&lo.w, &skip, 4);
+    memcpy(&hi.w, &sc, 4);
+    lo.x = G->lo[3 * i]; lo.y = G->lo[3 * i + 1]; lo.z = G->lo[3 * i + 2];
+    hi.x = G->hi[3 * i]; hi.y = G->hi[3 * i + 1]; hi.z = G->hi[3 * i + 2];
+    node[2 * i] = lo;
+    node[2 * i + 1] = hi;
+  }
+  for (int i = 0; i < G->n_items; ++i) items[i] = G->items[i];
+  for (int l = 0; l < G->n_leaves; ++l) {
+    const float *q = G->leaf_q + 3 * l, *o = G->leaf_o + 3 * l, *A = G->leaf_A + 9 * l, *b = G->leaf_b + 3 * l;
+    int typ = G->leaf_typ[l], p0 = G->leaf_p0[l], np = G->leaf_np[l];
+    float4 L3;
+    L3.x = b[2];
+    memcpy(&L3.y, &typ, 4);
+    memcpy(&L3.z, &p0, 4);
+    memcpy(&L3.w, &np, 4);
+    leaf[4 * l] = make_float4(q[0], q[1], q[2], G->leaf_c[l]);
+    leaf[4 * l + 1] = make_float4(A[0], A[1], A[2], A[4]);
+    leaf[4 * l + 2] = make_float4(A[5], A[8], b[0], b[1]);
+    leaf[4 * l + 3] = L3;
+    for (int k = 0; k < np; ++k) {  // planes relative to q: n.(x - q) <= h - n.(q - o)
+      const float *nn = G->plane_n + 3 * (p0 + k);
+      const float h = G->plane_h[p0 + k] - (nn[0] * (q[0] - o[0]) + nn[1] * (q[1] - o[1]) + nn[2] * (q[2] - o[2]));
+      plane[p0 + k] = make_float4(nn[0], nn[1], nn[2], h);
+    }
+  }
+  for (int k = 0; k < G->n_kinds; ++k) kinds[k] = make_int4(G->kind_l0[k], G->kind_nl[k], G->kind_p0[k], G->kind_pl[k]);
+  for (int j = 0; j < G->n_prog; ++j) prog[j] = (G->prog_op[j] << 24) | (G->prog_arg[j] & 0xFFFFFF);
+
+  size_t off = 0;
+  auto place = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_xf = place(xf.size() * 16), o_gxf = place(gxf.size() * 16), o_node = place(node.size() * 16),
+               o_leaf = place(leaf.size() * 16), o_plane = place(plane.size() * 16), o_vinfo = place(vinfo.size() * 16),
+               o_kinds = place(kinds.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
+               o_items = place(items.size() * 4), o_prog = place(prog.size() * 4);
+  lh_model m = new lh_model_s();
+  char *buf = nullptr;
+  hipError_t e = hipMalloc((void **)&buf, off);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(LH_ERR_HIP, std::string("lh_model_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  m->geo_buf = buf;
+  std::vector<char> host(off);
+  memcpy(host.data() + o_xf, xf.data(), xf.size() * 16);
+  memcpy(host.data() + o_gxf, gxf.data(), gxf.size() * 16);
+  memcpy(host.data() + o_node, node.data(), node.size() * 16);
+  memcpy(host.data() + o_leaf, leaf.data(), leaf.size() * 16);
+  memcpy(host.data() + o_plane, plane.data(), plane.size() * 16);
+  memcpy(host.data() + o_vinfo, vinfo.data(), vinfo.size() * 16);
+  memcpy(host.data() + o_kinds, kinds.data(), kinds.size() * 16);
+  memcpy(host.data() + o_vgrp, vgrp.data(), vgrp.size() * 8);
+  memcpy(host.data() + o_grange, grange.data(), grange.size() * 8);
+  memcpy(host.data() + o_items, items.data(), items.size() * 4);
+  memcpy(host.data() + o_prog, prog.data(), prog.size() * 4);
+  HIP_TRY(hipMemcpy(buf, host.data(), off, hipMemcpyHostToDevice));
+  Geo &g = m->g;
+  g.nvol = n;
+  g.xf = (const float4 *)(buf + o_xf);
+  g.gxf = (const float4 *)(buf + o_gxf);
+  g.node = (const float4 *)(buf + o_node);
+  g.leaf = (const float4 *)(buf + o_leaf);
+  g.plane = (const float4 *)(buf + o_plane);
+  g.vinfo = (const int4 *)(buf + o_vinfo);
+  g.kinds = (const int4 *)(buf + o_kinds);
+  g.vgrp = (const int2 *)(buf + o_vgrp);
+  g.grange = (const int2 *)(buf + o_grange);
+  g.items = (const int *)(buf + o_items);
+  g.prog = (const int *)(buf + o_prog);
+  g.reach = G->reach;
+  HIP_TRY(hipMalloc((void **)&m->layout, (size_t)layout_words * 4));
+  HIP_TRY(hipMemcpy(m->layout, layout, (size_t)layout_words * 4, hipMemcpyHostToDevice));
+  m->layout_words = layout_words;
+  {
+    long long need = (long long)layout[10] + 2LL * layout[5];
+    for (int mi = 0; mi < layout[1]; ++mi) {
+      const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
+      for (int j = 0; j < 8; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
+    }
+    for (int r = 0; r < layout[3]; ++r) {
+      const int32_t *R = layout + layout[9] + ROW_WORDS * r;
+      for (int c = 0; c < N_COL; ++c) need = std::max(need, (long long)R[2 + 2 * c] + 2LL * R[3 + 2 * c]);
+    }
+    m->values_needed = need;
+  }
+  m->n_media = layout[1];
+  m->S = layout[4];
+  m->K = layout[5];
+  m->tally_stride = 3LL * m->S + (long long)m->S * m->K;
+  HIP_TRY(hipMalloc((void **)&m->queue, 256));
+  int dev = 0, cus = 0, per = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_kernel, BLOCK, 0));
+  m->grid = cus * (per > 0 ? per : 1);
+  HIP_TRY(hipEventCreate(&m->ev0));
+  HIP_TRY(hipEventCreate(&m->ev1));
+  *out = m;
+  return LH_OK;
+}
+
+LH_API int lh_model_destroy(lh_model m) {
+  if (!m) return LH_OK;
+  hipFree(m->geo_buf);
+  hipFree(m->layout);
+  hipFree(m->queue);
+  hipFree(m->src);
+  hipFree(m->src_end);
+  if (m->ev0) hipEventDestroy(m->ev0);
+  if (m->ev1) hipEventDestroy(m->ev1);
+  delete m;
+  return LH_OK;
+}
+
+LH_API int lh_model_info(lh_model m, int32_t *n_sensors, int32_t *n_qe, int64_t *tally_per_event) {
+  if (!m) return fail(LH_ERR_ARG, "lh_model_info: null model");
+  if (n_sensors) *n_sensors = m->S;
+  if (n_qe) *n_qe = m->K;
+  if (tally_per_event) *tally_per_event = m->tally_stride;
+  return LH_OK;
+}
+
+static int stage_sources(lh_model m, const lh_source *sources, int n_sources, long long *total, hipStream_t st) {
+  if (!sources || n_sources <= 0) return fail(LH_ERR_ARG, "no sources");
+  std::vector<long long> ends(n_sources);
+  long long acc = 0;
+  for (int i = 0; i < n_sources; ++i) {
+    const lh_source &s = sources[i];
+    if (s.n < 0) return fail(LH_ERR_ARG, "a source with a negative photon count");
+    if (s.kind < 0 || s.kind > 2) return fail(LH_ERR_ARG, "an unknown source kind");
+    if (s.event < 0) return fail(LH_ERR_ARG, "a negative event index");
+    if (s.kind == LH_SRC_CHERENKOV && (s.medium < 0 || s.medium >= m->n_media))
+      return fail(LH_ERR_ARG, "a Cherenkov source's medium is out of range");
+    if (s.kind == LH_SRC_CHERENKOV && !(s.beta > 0.0f)) return fail(LH_ERR_ARG, "a Cherenkov source needs beta > 0");
+    acc += s.n;
+    ends[i] = acc;
+  }
+  if (n_sources > m->src_cap) {
+    hipFree(m->src);
+    hipFree(m->src_end);
+    m->src = nullptr;
+    m->src_end = nullptr;
+    HIP_TRY(hipMalloc((void **)&m->src, sizeof(lh_source) * n_sources));
+    HIP_TRY(hipMalloc((void **)&m->src_end, sizeof(long long) * n_sources));
+    m->src_cap = n_sources;
+  }
+  HIP_TRY(hipMemcpyAsync(m->src, sources, sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(m->src_end, ends.data(), sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host staging vector goes out of scope
+  *total = acc;
+  return LH_OK;
+}
+
+static int fill_args(lh_model m, Args &a, const float *values, const float *ref, const lh_controls *ctl, uint64_t seed,
+                     double *tally, double *health) {
+  if (!values) return fail(LH_ERR_ARG, "null values");
+  if (!tally || !health) return fail(LH_ERR_ARG, "null tally or health");
+  a.g = m->g;
+  a.L = m->layout;
+  a.P = values;
+  a.Pr = ref ? ref : values;
+  a.has_ref = (ref && ref != values) ? 1 : 0;
+  a.k0 = (uint32_t)seed;
+  a.k1 = (uint32_t)(seed >> 32);
+  lh_controls c{0.01f, 0.0f, 1 << 22, 0};
+  if (ctl) c = *ctl;
+  if (!(c.roulette > 0.0f)) return fail(LH_ERR_ARG, "the roulette is a number > 0");
+  a.roulette = c.roulette;
+  a.reach = m->g.reach;
+  a.max_dist = c.max_distance > 0.0f ? c.max_distance : m->g.reach;
+  a.max_vert = c.max_vertices > 0 ? c.max_vertices : (1 << 22);
+  a.chunk = c.chunk > 0 ? c.chunk : DEFAULT_CHUNK;
+  a.tally = tally;
+  a.tally_stride = m->tally_stride;
+  a.S = m->S;
+  a.K = m->K;
+  a.health = health;
+  a.queue = m->queue;
+  a.src = m->src;
+  a.src_end = m->src_end;
+  return LH_OK;
+}
+
+LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
+                    int32_t n_sources, uint64_t seed, const lh_controls *ctl, double *tally, int32_t n_events,
+                    double *health, void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_trace: null model");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_trace: fewer packed values than the layout reads");
+  for (int i = 0; i < n_sources && sources; ++i)
+    if (sources[i].event >= n_events) return fail(LH_ERR_ARG, "lh_trace: a source's event is beyond n_events");
+  hipStream_t st = (hipStream_t)stream;
+  Args a;
+  long long total = 0;
+  int rc = stage_sources(m, sources, n_sources, &total, st);
+  if (rc) return rc;
+  rc = fill_args(m, a, values, ref, ctl, seed, tally, health);
+  if (rc) return rc;
+  a.nsrc = n_sources;
+  a.total = total;
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m->ev0, st));
+  hipLaunchKernelGGL(trace_kernel, dim3(m->grid), dim3(BLOCK), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m->ev1, st));
+  m->timed = 1;
+  return LH_OK;
+}
+
+LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, const int32_t *slots,
+                          const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *ctl,
+                          const double *cot, double *grad, int32_t n_grad, double *health, void *stream) {
+  (void)m; (void)values; (void)ref; (void)slots; (void)sources; (void)n_sources; (void)seed; (void)ctl; (void)cot;
+  (void)grad; (void)n_grad; (void)health; (void)stream;
+  return fail(LH_ERR_ARG, "lh_adjoint: not built in this version");
+}
+
+LH_API int lh_last_kernel_ms(lh_model m, float *ms) {
+  if (!m || !ms) return fail(LH_ERR_ARG, "lh_last_kernel_ms: null argument");
+  if (!m->timed) return fail(LH_ERR_ARG, "lh_last_kernel_ms: nothing launched yet");
+  HIP_TRY(hipEventSynchronize(m->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, m->ev0, m->ev1));
+  return LH_OK;
+}

@@ -1,0 +1,453 @@
+// The navigator on the device: the next visible crossing along a ray, and the volume holding a point.
+//
+// Restates the reference navigator's query for volumes that lie inside their mother (no overlaps, no meshes, no
+// invisible crossings): the nearer of the photon's own volume's exit and the first daughter entry, a daughter
+// winning a tie (src/lumen/geometry/navigate.py:663-693 `query`, :430-463 `_node`, src/lumen/rules/navigation.py
+// `entry_wins`).  Daughters are found by a stackless walk of each frame group's BVH (depth-first nodes with skip
+// links, src/lumen/geometry/bvh.py:1-10); a solid is its CSG leaves' intervals along the ray combined by the postfix
+// program, scanned endpoint by endpoint for the first change of membership (src/lumen/geometry/csg.py:417-449
+// `first_change`).  Quadric roots are taken at the closest approach (the vertex where the roots are near it), so
+// float32 never cancels the large terms (src/lumen/geometry/leaf_kernels.py:81-109 re-origination).
+//
+// Surface identity: the face a photon sits on (its `srf` code: volume, leaf, face) is not met again within K_ON
+// bands of the start; the band is BAND_ULPS float32 ulps of the coordinates plus a floor
+// (src/lumen/rules/tolerances.py:76-83).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lh {
+
+constexpr float HUGEF = 1e30f;
+constexpr float BIGF = 1e29f;
+constexpr float EPS32 = 1.1920928955078125e-7f;
+constexpr float BAND_ULPS = 8.0f;
+constexpr float BAND_FLOOR = 1e-9f;
+constexpr float K_ON = 4.0f;
+constexpr int MAX_CSG_LEAVES = 8;
+constexpr int MAX_GROUPS_WALK = 8;
+
+struct Geo {
+  int nvol;
+  const float4 *__restrict__ xf;    // 3 per volume: rows of R^T with the offset -R^T t (world -> local)
+  const int4 *__restrict__ vinfo;   // per navigator id: kind, parent, cls, sensor
+  const int2 *__restrict__ vgrp;    // per navigator id: frame groups [x, y)
+  const float4 *__restrict__ gxf;   // 3 per group: rows of gR^T (w: 1 if identity)
+  const int2 *__restrict__ grange;  // per group: BVH nodes [x, y)
+  const float4 *__restrict__ node;  // 2 per node: (lo, skip), (hi, start * 8 + count)
+  const int *__restrict__ items;
+  const float4 *__restrict__ leaf;  // 4 per leaf: (q, c), (A00 A01 A02 A11), (A12 A22 b0 b1), (b2, typ, p0, np)
+  const float4 *__restrict__ plane; // (n, h) relative to the leaf's q
+  const int4 *__restrict__ kinds;   // l0, nl, p0, pl
+  const int *__restrict__ prog;     // op << 24 | arg
+  float reach;
+};
+
+__device__ __forceinline__ float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul3(float s, float3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float3 fma3(float s, float3 a, float3 b) {
+  return mk3(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z));
+}
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 unit3(float3 a) {
+  const float s = dot3(a, a);
+  return s > 0.0f ? mul3(rsqrtf(s), a) : a;
+}
+__device__ __forceinline__ float maxabs3(float3 a) { return fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))); }
+
+// the band at a coordinate scale (src/lumen/rules/navigation.py:129-134)
+__device__ __forceinline__ float band_at(float scale) { return BAND_ULPS * EPS32 * scale + BAND_FLOOR; }
+
+__device__ __forceinline__ int srf_code(int vol, int leaf_local, int face) {
+  return (vol << 10) | (leaf_local << 5) | face;
+}
+
+// world -> a volume's frame
+__device__ __forceinline__ void to_local(const Geo &g, int v, float3 x, float3 d, float3 &xl, float3 &dl) {
+  const float4 r0 = g.xf[3 * v], r1 = g.xf[3 * v + 1], r2 = g.xf[3 * v + 2];
+  xl = mk3(fmaf(r0.x, x.x, fmaf(r0.y, x.y, fmaf(r0.z, x.z, r0.w))),
+           fmaf(r1.x, x.x, fmaf(r1.y, x.y, fmaf(r1.z, x.z, r1.w))),
+           fmaf(r2.x, x.x, fmaf(r2.y, x.y, fmaf(r2.z, x.z, r2.w))));
+  dl = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+           r2.x * d.x + r2.y * d.y + r2.z * d.z);
+}
+
+__device__ __forceinline__ float3 normal_to_world(const Geo &g, int v, float3 n) {
+  const float4 r0 = g.xf[3 * v], r1 = g.xf[3 * v + 1], r2 = g.xf[3 * v + 2];
+  return mk3(r0.x * n.x + r1.x * n.y + r2.x * n.z, r0.y * n.x + r1.y * n.y + r2.y * n.z,
+             r0.z * n.x + r1.z * n.y + r2.z * n.z);
+}
+
+struct LeafIv {
+  float lo[2], hi[2];
+  int flo[2], fhi[2];
+};
+
+// a leaf's (at most two) intervals [lo, hi) along o + t d in its kind's frame; face 0 the quadric, k + 1 plane k
+__device__ inline void leaf_intervals(const Geo &g, int li, float3 o, float3 d, LeafIv &iv) {
+  const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
+  const int typ = __float_as_int(L3.y), p0 = __float_as_int(L3.z), np = __float_as_int(L3.w);
+  const float3 y = sub3(o, mk3(L0.x, L0.y, L0.z));
+  float tp0 = -HUGEF, tp1 = HUGEF;
+  int fp0 = -1, fp1 = -1;
+  bool empty = false;
+  for (int k = 0; k < np; ++k) {
+    const float4 P = g.plane[p0 + k];
+    const float nd = P.x * d.x + P.y * d.y + P.z * d.z;
+    const float rhs = P.w - (P.x * y.x + P.y * y.y + P.z * y.z);
+    if (fabsf(nd) < 1e-12f) {
+      empty |= rhs < 0.0f;
+    } else {
+      const float tp = rhs / nd;
+      if (nd < 0.0f) {
+        if (tp > tp0) { tp0 = tp; fp0 = k + 1; }
+      } else {
+        if (tp < tp1) { tp1 = tp; fp1 = k + 1; }
+      }
+    }
+  }
+  // the quadric's pieces [a0, b0] and [a1, b1] (a piece with a > b is empty); qa/qb: the end is a root
+  float a0 = -HUGEF, b0 = HUGEF, a1 = HUGEF, b1 = -HUGEF;
+  bool qa0 = false, qb0 = false, qa1 = false;
+  if (typ == 1) {
+    const float A00 = L1.x, A01 = L1.y, A02 = L1.z, A11 = L1.w, A12 = L2.x, A22 = L2.y;
+    const float3 b = mk3(L2.z, L2.w, L3.x);
+    const float c = L0.w;
+    const float sa = fmaxf(fmaxf(fmaxf(fabsf(A00), fabsf(A01)), fmaxf(fabsf(A02), fabsf(A11))),
+                           fmaxf(fabsf(A12), fabsf(A22)));
+    auto Av = [&](float3 v) {
+      return mk3(A00 * v.x + A01 * v.y + A02 * v.z, A01 * v.x + A11 * v.y + A12 * v.z,
+                 A02 * v.x + A12 * v.y + A22 * v.z);
+    };
+    const float3 Ad = Av(d);
+    const float a = dot3(d, Ad);
+    const float bd = dot3(b, d);
+    const bool lin = fabsf(a) <= 16.0f * EPS32 * sa;
+    float tc = -dot3(y, d);
+    float3 yc = fma3(tc, d, y);
+    float3 Ay = Av(yc);
+    float beta = dot3(Ay, d) + bd;
+    float gam = dot3(yc, Ay) + 2.0f * dot3(b, yc) + c;
+    if (!lin && a * gam > 0.5f * beta * beta) {  // the roots lie near the vertex: take them there
+      tc -= beta / a;
+      yc = fma3(tc, d, y);
+      Ay = Av(yc);
+      beta = dot3(Ay, d) + bd;
+      gam = dot3(yc, Ay) + 2.0f * dot3(b, yc) + c;
+    }
+    if (lin) {
+      if (beta > 0.0f) {
+        a0 = -HUGEF; b0 = tc - gam / (2.0f * beta); qb0 = true;
+      } else if (beta < 0.0f) {
+        a0 = tc - gam / (2.0f * beta); b0 = HUGEF; qa0 = true;
+      } else if (gam > 0.0f) {
+        a0 = HUGEF; b0 = -HUGEF;
+      }
+    } else {
+      const float D = beta * beta - a * gam;
+      if (D < 0.0f) {
+        if (a > 0.0f) { a0 = HUGEF; b0 = -HUGEF; }
+      } else {
+        const float sq = sqrtf(D);
+        const float qq = -(beta + copysignf(sq, beta));
+        float r1 = qq / a, r2 = qq != 0.0f ? gam / qq : r1;
+        const float rl = fminf(r1, r2), rh = fmaxf(r1, r2);
+        if (a > 0.0f) {
+          a0 = tc + rl; b0 = tc + rh; qa0 = qb0 = true;
+        } else {
+          a0 = -HUGEF; b0 = tc + rl; qb0 = true;
+          a1 = tc + rh; b1 = HUGEF; qa1 = true;
+        }
+      }
+    }
+  }
+  // clip each piece by the planes' slab
+  {
+    const float lo = fmaxf(a0, tp0), hi = fminf(b0, tp1);
+    const bool ok = lo < hi && !empty;
+    iv.lo[0] = ok ? lo : HUGEF;
+    iv.hi[0] = ok ? hi : HUGEF;
+    iv.flo[0] = (a0 > tp0 && qa0) ? 0 : fp0;
+    iv.fhi[0] = (b0 < tp1 && qb0) ? 0 : fp1;
+  }
+  {
+    const float lo = fmaxf(a1, tp0), hi = fminf(b1, tp1);
+    const bool ok = lo < hi && !empty;
+    iv.lo[1] = ok ? lo : HUGEF;
+    iv.hi[1] = ok ? hi : HUGEF;
+    iv.flo[1] = (a1 > tp0 && qa1) ? 0 : fp0;
+    iv.fhi[1] = fp1;
+  }
+}
+
+// a leaf's membership of a point (its kind's frame)
+__device__ inline bool leaf_inside(const Geo &g, int li, float3 x) {
+  const float4 L0 = g.leaf[4 * li], L3 = g.leaf[4 * li + 3];
+  const int typ = __float_as_int(L3.y), p0 = __float_as_int(L3.z), np = __float_as_int(L3.w);
+  const float3 y = sub3(x, mk3(L0.x, L0.y, L0.z));
+  for (int k = 0; k < np; ++k) {
+    const float4 P = g.plane[p0 + k];
+    if (P.x * y.x + P.y * y.y + P.z * y.z > P.w) return false;
+  }
+  if (typ == 1) {
+    const float4 L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2];
+    const float3 Ay = mk3(L1.x * y.x + L1.y * y.y + L1.z * y.z, L1.y * y.x + L1.w * y.y + L2.x * y.z,
+                          L1.z * y.x + L2.x * y.y + L2.y * y.z);
+    const float q = dot3(y, Ay) + 2.0f * (L2.z * y.x + L2.w * y.y + L3.x * y.z) + L0.w;
+    if (q > 0.0f) return false;
+  }
+  return true;
+}
+
+// postfix program over leaf bits (bit k: leaf l0 + k is in)
+__device__ __forceinline__ bool run_program(const Geo &g, int p0, int pl, uint32_t bits) {
+  uint32_t st = 0;
+  int sp = 0;
+  for (int k = 0; k < pl; ++k) {
+    const int w = g.prog[p0 + k];
+    const int op = w >> 24, arg = w & 0xFFFFFF;
+    if (op == 0) {
+      st = (st & ~(1u << sp)) | (((bits >> arg) & 1u) << sp);
+      ++sp;
+    } else {
+      const uint32_t b = (st >> (sp - 1)) & 1u, a = (st >> (sp - 2)) & 1u;
+      const uint32_t r = op == 1 ? (a | b) : op == 2 ? (a & b) : (a & ~b & 1u);
+      sp -= 1;
+      st = (st & ~(1u << (sp - 1))) | (r << (sp - 1));
+    }
+  }
+  return (st & 1u) != 0u;
+}
+
+__device__ inline bool solid_inside(const Geo &g, int kind, float3 x) {
+  const int4 K = g.kinds[kind];
+  if (K.y == 1) return leaf_inside(g, K.x, x);
+  uint32_t bits = 0;
+  for (int k = 0; k < K.y && k < 32; ++k) bits |= (uint32_t)leaf_inside(g, K.x + k, x) << k;
+  return run_program(g, K.z, K.w, bits);
+}
+
+struct Change {
+  float t;
+  int leaf, face;  // leaf within its kind; -1: none
+};
+
+// the first change of the solid's membership along the ray after t = 0, from the side `want_from` (true: the
+// point starts inside and the change is an exit; false: an entry).  Endpoints on the face `skip_code` within
+// `skip_t` of the start count as passed.  `vol` names the volume in the surface codes.
+__device__ inline Change first_change(const Geo &g, int kind, int vol, float3 o, float3 d, bool exit_query,
+                                      int skip_code, float skip_t) {
+  const int4 K = g.kinds[kind];
+  Change ch{HUGEF, -1, -1};
+  if (K.y == 1) {  // one leaf: its intervals are the solid's
+    LeafIv iv;
+    leaf_intervals(g, K.x, o, d, iv);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float lo = iv.lo[p], hi = iv.hi[p];
+      if (lo >= HUGEF) continue;
+      if (srf_code(vol, 0, iv.flo[p]) == skip_code && fabsf(lo) <= skip_t) lo = -BIGF;
+      if (srf_code(vol, 0, iv.fhi[p]) == skip_code && fabsf(hi) <= skip_t) hi = -BIGF;
+      if (exit_query) {
+        if (lo <= 0.0f && 0.0f < hi && hi < ch.t) { ch.t = hi; ch.leaf = 0; ch.face = iv.fhi[p]; }
+      } else {
+        if (lo > 0.0f && lo < hi && lo < ch.t) { ch.t = lo; ch.leaf = 0; ch.face = iv.flo[p]; }
+      }
+    }
+    if (exit_query && ch.leaf < 0) {  // the start is outside its own solid (rounding): leave it where it is
+      ch.t = 0.0f;
+      ch.leaf = 0;
+      ch.face = 0;
+    }
+    return ch;
+  }
+  // several leaves: scan the endpoints in order for the first change of the program's value
+  const int nl = K.y < MAX_CSG_LEAVES ? K.y : MAX_CSG_LEAVES;
+  LeafIv ivs[MAX_CSG_LEAVES];
+  for (int k = 0; k < nl; ++k) {
+    leaf_intervals(g, K.x + k, o, d, ivs[k]);
+    for (int p = 0; p < 2; ++p) {
+      if (ivs[k].lo[p] >= HUGEF) continue;
+      if (srf_code(vol, k, ivs[k].flo[p]) == skip_code && fabsf(ivs[k].lo[p]) <= skip_t) ivs[k].lo[p] = -BIGF;
+      if (srf_code(vol, k, ivs[k].fhi[p]) == skip_code && fabsf(ivs[k].hi[p]) <= skip_t) ivs[k].hi[p] = -BIGF;
+    }
+  }
+  auto member = [&](float t) {
+    uint32_t bits = 0;
+    for (int k = 0; k < nl; ++k) {
+      const bool in = (ivs[k].lo[0] <= t && t < ivs[k].hi[0]) || (ivs[k].lo[1] <= t && t < ivs[k].hi[1]);
+      bits |= (uint32_t)in << k;
+    }
+    return run_program(g, K.z, K.w, bits);
+  };
+  const bool start = member(0.0f);
+  if (exit_query && !start) return Change{0.0f, 0, 0};
+  if (!exit_query && start) return ch;  // inside a daughter already (rounding): no entry ahead
+  float cur = 0.0f;
+  for (int it = 0; it < 4 * nl + 5; ++it) {
+    float nxt = HUGEF;
+    int nl_ = -1, nf = -1;
+    for (int k = 0; k < nl; ++k)
+      for (int p = 0; p < 2; ++p) {
+        if (ivs[k].lo[p] > cur && ivs[k].lo[p] < nxt) { nxt = ivs[k].lo[p]; nl_ = k; nf = ivs[k].flo[p]; }
+        if (ivs[k].hi[p] > cur && ivs[k].hi[p] < nxt && ivs[k].hi[p] < HUGEF) {
+          nxt = ivs[k].hi[p]; nl_ = k; nf = ivs[k].fhi[p];
+        }
+      }
+    if (nl_ < 0 || nxt >= HUGEF) break;
+    if (member(nxt) != start) return Change{nxt, nl_, nf};
+    cur = nxt;
+  }
+  if (exit_query) return Change{0.0f, 0, 0};
+  return ch;
+}
+
+struct Cross {
+  float t;
+  int vol_after;  // navigator id entered; -1: no crossing within reach
+  int srf;        // the crossed face's code
+  int own;        // 1: the exit of the photon's own volume
+  float3 n;       // unit normal facing the photon
+};
+
+// the surface normal of a crossing (world frame, facing the photon)
+__device__ inline float3 crossing_normal(const Geo &g, int v, int leaf_local, int face, float3 x, float3 d, float t) {
+  float3 xl, dl;
+  to_local(g, v, x, d, xl, dl);
+  const int li = g.kinds[g.vinfo[v].x].x + leaf_local;
+  const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
+  float3 nl;
+  if (face == 0) {
+    const float3 y = sub3(fma3(t, dl, xl), mk3(L0.x, L0.y, L0.z));
+    nl = mk3(L1.x * y.x + L1.y * y.y + L1.z * y.z + L2.z, L1.y * y.x + L1.w * y.y + L2.x * y.z + L2.w,
+             L1.z * y.x + L2.x * y.y + L2.y * y.z + L3.x);
+  } else {
+    const float4 P = g.plane[__float_as_int(L3.z) + face - 1];
+    nl = mk3(P.x, P.y, P.z);
+  }
+  float3 n = unit3(normal_to_world(g, v, nl));
+  if (dot3(n, d) > 0.0f) n = mul3(-1.0f, n);
+  return n;
+}
+
+// the next visible crossing from x along d in navigator id `vol`, within tmax; `last` the face the photon sits on
+__device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
+  const float skip_t = K_ON * band_at(maxabs3(x));
+  Cross best{tmax, -1, -1, 0, mk3(0.f, 0.f, 1.f)};
+  int best_leaf = -1, best_face = -1, best_vol = -1;
+  if (vol < g.nvol) {
+    float3 xl, dl;
+    to_local(g, vol, x, d, xl, dl);
+    const Change ch = first_change(g, g.vinfo[vol].x, vol, xl, dl, true, last, skip_t);
+    if (ch.leaf >= 0 && ch.t <= best.t) {
+      best.t = ch.t;
+      best.vol_after = g.vinfo[vol].y;
+      best.own = 1;
+      best_leaf = ch.leaf;
+      best_face = ch.face;
+      best_vol = vol;
+    }
+  }
+  const int2 gr = g.vgrp[vol];
+  for (int gi = gr.x; gi < gr.y; ++gi) {
+    const float4 q0 = g.gxf[3 * gi], q1 = g.gxf[3 * gi + 1], q2 = g.gxf[3 * gi + 2];
+    float3 xg = x, dg = d;
+    if (q0.w == 0.0f) {
+      xg = mk3(dot3(mk3(q0.x, q0.y, q0.z), x), dot3(mk3(q1.x, q1.y, q1.z), x), dot3(mk3(q2.x, q2.y, q2.z), x));
+      dg = mk3(dot3(mk3(q0.x, q0.y, q0.z), d), dot3(mk3(q1.x, q1.y, q1.z), d), dot3(mk3(q2.x, q2.y, q2.z), d));
+    }
+    const float3 inv = mk3(1.0f / (fabsf(dg.x) > 1e-30f ? dg.x : copysignf(1e-30f, dg.x)),
+                           1.0f / (fabsf(dg.y) > 1e-30f ? dg.y : copysignf(1e-30f, dg.y)),
+                           1.0f / (fabsf(dg.z) > 1e-30f ? dg.z : copysignf(1e-30f, dg.z)));
+    const int2 nr = g.grange[gi];
+    int ni = nr.x;
+    while (ni < nr.y) {
+      const float4 lo = g.node[2 * ni], hi = g.node[2 * ni + 1];
+      const float tx0 = (lo.x - xg.x) * inv.x, tx1 = (hi.x - xg.x) * inv.x;
+      const float ty0 = (lo.y - xg.y) * inv.y, ty1 = (hi.y - xg.y) * inv.y;
+      const float tz0 = (lo.z - xg.z) * inv.z, tz1 = (hi.z - xg.z) * inv.z;
+      const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+      const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
+      const int skip = __float_as_int(lo.w);
+      if (te > tl) {
+        ni = skip;
+        continue;
+      }
+      const int sc = __float_as_int(hi.w);
+      const int cnt = sc & 7;
+      if (cnt > 0) {
+        const int st = sc >> 3;
+        for (int k = 0; k < cnt; ++k) {
+          const int u = g.items[st + k];
+          float3 xl, dl;
+          to_local(g, u, x, d, xl, dl);
+          const Change ch = first_change(g, g.vinfo[u].x, u, xl, dl, false, last, skip_t);
+          if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
+            best.t = ch.t;
+            best.vol_after = u;
+            best.own = 0;
+            best_leaf = ch.leaf;
+            best_face = ch.face;
+            best_vol = u;
+          }
+        }
+        ni = skip;
+      } else {
+        ni += 1;
+      }
+    }
+  }
+  if (best_vol >= 0) {
+    best.srf = srf_code(best_vol, best_leaf, best_face);
+    best.n = crossing_normal(g, best_vol, best_leaf, best_face, x, d, best.t);
+  }
+  return best;
+}
+
+// the deepest navigator id holding x (the outside when no volume does)
+__device__ inline int locate(const Geo &g, float3 x) {
+  int cur = g.nvol;
+  for (int depth = 0; depth < 64; ++depth) {
+    int found = -1;
+    const int2 gr = g.vgrp[cur];
+    for (int gi = gr.x; gi < gr.y && found < 0; ++gi) {
+      const float4 q0 = g.gxf[3 * gi], q1 = g.gxf[3 * gi + 1], q2 = g.gxf[3 * gi + 2];
+      float3 xg = x;
+      if (q0.w == 0.0f)
+        xg = mk3(dot3(mk3(q0.x, q0.y, q0.z), x), dot3(mk3(q1.x, q1.y, q1.z), x), dot3(mk3(q2.x, q2.y, q2.z), x));
+      const int2 nr = g.grange[gi];
+      int ni = nr.x;
+      while (ni < nr.y && found < 0) {
+        const float4 lo = g.node[2 * ni], hi = g.node[2 * ni + 1];
+        const int skip = __float_as_int(lo.w);
+        if (xg.x < lo.x || xg.x > hi.x || xg.y < lo.y || xg.y > hi.y || xg.z < lo.z || xg.z > hi.z) {
+          ni = skip;
+          continue;
+        }
+        const int sc = __float_as_int(hi.w);
+        const int cnt = sc & 7;
+        if (cnt > 0) {
+          for (int k = 0; k < cnt; ++k) {
+            const int u = g.items[(sc >> 3) + k];
+            float3 xl, dl;
+            to_local(g, u, x, mk3(0.f, 0.f, 1.f), xl, dl);
+            if (solid_inside(g, g.vinfo[u].x, xl)) { found = u; break; }
+          }
+          ni = skip;
+        } else {
+          ni += 1;
+        }
+      }
+    }
+    if (found < 0) return cur;
+    cur = found;
+  }
+  return cur;
+}
+
+}  // namespace lh

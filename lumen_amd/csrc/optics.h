@@ -1,0 +1,321 @@
+// Optics on the device: tables, the bulk medium at a wavelength, the surface kernel and the direction samplers.
+//
+// Restates the reference's per-step physics (src/lumen/transport/step.py:318-359 `_step_context`,
+// src/lumen/media/bulk.py:76-110 `rate`/`evaluate`, src/lumen/media/laws.py `_table`, `geant4_group_index`,
+// src/lumen/surfaces/types.py:211-296 `chained_cuts`/`_none`/`_real`/`_geant4_chain`, src/lumen/surfaces/fresnel.py
+// `amplitudes`, src/lumen/surfaces/polar.py, src/lumen/surfaces/outcomes.py, src/lumen/media/phase.py:45-75
+// Rayleigh, src/lumen/lobes.py:60-78 Lambert, src/lumen/surfaces/facets.py:318-397 the unified facet path).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nav.h"
+
+namespace lh {
+
+constexpr float HC = 1239.8419843320026f;  // eV nm
+constexpr float C_VACUUM = 0.299792458f;   // m / ns
+constexpr float TWO_PI = 6.283185307179586f;
+constexpr float PI_F = 3.141592653589793f;
+constexpr float HALF_PI = 1.5707963267948966f;
+constexpr float FLOOR32 = 1e-30f;
+constexpr float TINY_NORMAL = 1.17549435e-38f;
+constexpr float DEFENSIVE = 0.01f;
+constexpr int N_OUT = 6;
+enum { STRAIGHT = 0, REFRACT = 1, SPECULAR = 2, BACKSCATTER = 3, GLOSSY = 4, DIFFUSE = 5 };
+enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1 };
+enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4 };
+enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, N_COL };
+// the packed layout's header words (lumen_amd/pack.py)
+enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
+       L_ROWS_AT, L_QE_TABLE };
+constexpr int MEDIUM_WORDS = 9;
+constexpr int ROW_WORDS = 2 + 2 * N_COL;
+
+struct TabPos {
+  int i;    // the knot pair (i, i + 1); for one knot or a clamped end, i with f 0
+  float f;  // the weight of knot i + 1
+};
+
+// a table linear in its knot coordinate between knots, held beyond them (G4PhysicsVector::Value)
+__device__ __forceinline__ TabPos tab_pos(const float *__restrict__ P, int off, int n, float x) {
+  if (n <= 1 || x <= P[off]) return TabPos{0, 0.0f};
+  if (x >= P[off + n - 1]) return TabPos{n - 1, 0.0f};
+  int i = 0;
+  while (i < n - 2 && P[off + i + 1] <= x) ++i;
+  const float e0 = P[off + i], e1 = P[off + i + 1];
+  return TabPos{i, (x - e0) / (e1 - e0)};
+}
+
+__device__ __forceinline__ float tab_at(const float *__restrict__ P, int off, int n, TabPos p) {
+  const float v0 = P[off + n + p.i];
+  return p.f == 0.0f ? v0 : fmaf(p.f, P[off + n + p.i + 1] - v0, v0);
+}
+
+__device__ __forceinline__ float tab(const float *__restrict__ P, int off, int n, float x) {
+  return tab_at(P, off, n, tab_pos(P, off, n, x));
+}
+
+// a mean free path's rate; a length of 0 (or below the smallest normal float) is 1 / FLT_MIN (media/bulk.py:61-73)
+__device__ __forceinline__ float rate_of(float L) { return L >= TINY_NORMAL ? 1.0f / L : 1.0f / TINY_NORMAL; }
+
+struct MediumAt {
+  float n, ng, mu_abs, mu_ray;
+};
+
+__device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__restrict__ P, int m, float lam) {
+  MediumAt r{1.0f, 1.0f, 0.0f, 0.0f};
+  if (m < 0) return r;
+  const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
+  if (M[8] & 1) return r;
+  const float E = HC / lam;
+  r.n = tab(P, M[0], M[1], E);
+  const TabPos pg = tab_pos(P, M[2], M[3], E);  // Geant4 interpolates the group velocity (1 / n_g) in energy
+  const float g0 = 1.0f / P[M[2] + M[3] + pg.i];
+  const float inv = pg.f == 0.0f ? g0 : fmaf(pg.f, 1.0f / P[M[2] + M[3] + pg.i + 1] - g0, g0);
+  r.ng = 1.0f / inv;
+  if (M[5] > 0) r.mu_abs = rate_of(tab(P, M[4], M[5], E));
+  if (M[7] > 0) r.mu_ray = rate_of(tab(P, M[6], M[7], E));
+  return r;
+}
+
+__device__ __forceinline__ float index_at(const int *__restrict__ L, const float *__restrict__ P, int m, float lam) {
+  if (m < 0) return 1.0f;
+  const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
+  if (M[8] & 1) return 1.0f;
+  return tab(P, M[0], M[1], HC / lam);
+}
+
+__device__ __forceinline__ float clip01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// ------------------------------------------------------------------------------------------- polarisation
+__device__ __forceinline__ void basis(float3 n, float3 &t1, float3 &t2) {  // Duff et al., JCGT 2017
+  const float sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  t1 = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  t2 = mk3(b, sign + n.y * n.y * a, -n.y);
+}
+
+__device__ __forceinline__ float3 to_world(float3 l, float3 n) {
+  float3 t1, t2;
+  basis(n, t1, t2);
+  return mk3(l.x * t1.x + l.y * t2.x + l.z * n.x, l.x * t1.y + l.y * t2.y + l.z * n.y,
+             l.x * t1.z + l.y * t2.z + l.z * n.z);
+}
+
+__device__ __forceinline__ float3 to_local_frame(float3 v, float3 n) {
+  float3 t1, t2;
+  basis(n, t1, t2);
+  return mk3(dot3(v, t1), dot3(v, t2), dot3(v, n));
+}
+
+// an unpolarised state (|E|^2 < 1/2) becomes a random linear one perpendicular to d (polar.py `fill`)
+__device__ __forceinline__ void pol_fill(float3 d, float3 &er, float3 &ei, float u) {
+  if (dot3(er, er) + dot3(ei, ei) < 0.5f) {
+    float3 t1, t2;
+    basis(d, t1, t2);
+    float s, c;
+    __sincosf(TWO_PI * u, &s, &c);
+    er = add3(mul3(c, t1), mul3(s, t2));
+    ei = mk3(0.f, 0.f, 0.f);
+  }
+}
+
+__device__ __forceinline__ void pol_normalise(float3 &er, float3 &ei) {
+  const float s = dot3(er, er) + dot3(ei, ei);
+  if (s > FLOOR32) {
+    const float r = rsqrtf(s);
+    er = mul3(r, er);
+    ei = mul3(r, ei);
+  } else {
+    er = ei = mk3(0.f, 0.f, 0.f);
+  }
+}
+
+// the part of E perpendicular to k, normalised (polar.py `project`)
+__device__ __forceinline__ void pol_project(float3 &er, float3 &ei, float3 k) {
+  er = fma3(-dot3(k, er), k, er);
+  ei = fma3(-dot3(k, ei), k, ei);
+  pol_normalise(er, ei);
+}
+
+// -E + 2 (E.h) h (polar.py `mirror`)
+__device__ __forceinline__ void pol_mirror(float3 &er, float3 &ei, float3 h) {
+  er = fma3(2.0f * dot3(er, h), h, mul3(-1.0f, er));
+  ei = fma3(2.0f * dot3(ei, h), h, mul3(-1.0f, ei));
+}
+
+struct cf {
+  float re, im;
+};
+__device__ __forceinline__ cf cmul(cf a, cf b) { return cf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cf cdiv(cf a, cf b) {
+  const float d = b.re * b.re + b.im * b.im;
+  return cf{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+__device__ __forceinline__ float cabs2(cf a) { return a.re * a.re + a.im * a.im; }
+
+// sqrt with Im >= 0 (the decaying wave); 0 near 0 (fresnel.py `decaying_sqrt`)
+__device__ __forceinline__ cf decaying_sqrt(cf z) {
+  const float m = sqrtf(z.re * z.re + z.im * z.im);
+  if (m < 1e-12f) return cf{0.f, 0.f};
+  float re = sqrtf(fmaxf(0.5f * (m + z.re), 0.0f));
+  float im = sqrtf(fmaxf(0.5f * (m - z.re), 0.0f));
+  if (z.im < 0.0f) im = -im;
+  if (im < 0.0f) { re = -re; im = -im; }
+  return cf{re, im};
+}
+
+struct Amps {
+  cf rs, rp, ts, tp;
+};
+
+// Fresnel amplitudes from real n1 into real n2 at incidence cos_i (fresnel.py `amplitudes`)
+__device__ inline Amps fresnel(float cos_i, float n1, float n2) {
+  const float c = clip01(cos_i);
+  const cf q = decaying_sqrt(cf{n2 * n2 - n1 * n1 * (1.0f - c * c), 0.0f});
+  const cf ds{n1 * c + q.re, q.im}, dp{n2 * n2 * c + n1 * q.re, n1 * q.im};
+  Amps a;
+  const bool oks = sqrtf(cabs2(ds)) > 1e-12f, okp = sqrtf(cabs2(dp)) > 1e-12f;
+  a.rs = oks ? cdiv(cf{n1 * c - q.re, -q.im}, ds) : cf{-1.f, 0.f};
+  a.rp = okp ? cdiv(cf{n2 * n2 * c - n1 * q.re, -n1 * q.im}, dp) : cf{1.f, 0.f};
+  a.ts = oks ? cdiv(cf{2.0f * n1 * c, 0.f}, ds) : cf{0.f, 0.f};
+  a.tp = okp ? cdiv(cf{2.0f * n1 * n2 * c, 0.f}, dp) : cf{0.f, 0.f};
+  return a;
+}
+
+// the s axis unit(wi x n); at normal incidence a fixed perpendicular of wi (polar.py `s_axis`)
+__device__ __forceinline__ float3 s_axis(float3 wi, float3 n) {
+  const float3 c = cross3(wi, n);
+  if (dot3(c, c) > EPS32 * EPS32) return unit3(c);
+  float3 t1, t2;
+  basis(wi, t1, t2);
+  return t1;
+}
+
+// the state a_s E_s s + a_p E_p (k x s), normalised (polar.py `recombine`)
+__device__ inline void pol_recombine(float3 &er, float3 &ei, float3 wi, float3 s, cf as, cf ap, float3 k) {
+  const float3 p_in = cross3(wi, s);
+  const cf es{dot3(er, s), dot3(ei, s)}, ep{dot3(er, p_in), dot3(ei, p_in)};
+  const cf cs = cmul(as, es), cp = cmul(ap, ep);
+  const float3 p = cross3(k, s);
+  er = add3(mul3(cs.re, s), mul3(cp.re, p));
+  ei = add3(mul3(cs.im, s), mul3(cp.im, p));
+  pol_normalise(er, ei);
+}
+
+// ------------------------------------------------------------------------------------------- the surface kernel
+struct SurfEval {
+  float p[N_OUT];  // the continuing shares
+  float detect;    // the detected share
+  float sigma;     // the glossy lobe's width
+  Amps amps;       // (r_s, r_p, t_s, t_p) of the deterministic outcomes
+  int flags;
+};
+
+__device__ __forceinline__ float col_at(const int *__restrict__ R, const float *__restrict__ P, int c, float E) {
+  return tab(P, R[2 + 2 * c], R[3 + 2 * c], E);
+}
+
+// one surface row at a crossing: Geant4's chain (types.py:211-296): r, t, a = chained_cuts(Rc, T), r split by the
+// reflection constants (or the Fresnel interface), t straight on, a detected with eff
+__device__ inline SurfEval surface_eval(const int *__restrict__ L, const float *__restrict__ P, int row, float lam,
+                                        float3 wi, float3 n, float n1, float n2, float3 er, float3 ei) {
+  SurfEval s;
+  const int *R = L + L[L_ROWS_AT] + ROW_WORDS * row;
+  const int model = R[0], flags = R[1];
+  s.flags = flags;
+  const float E = HC / lam;
+  const float Rv = col_at(R, P, COL_R, E), Tv = col_at(R, P, COL_T, E), eff = col_at(R, P, COL_EFF, E);
+  const float c1 = clip01(Rv), c2 = clip01(Rv + Tv);
+  const float r = c1, t = c2 - c1, a = 1.0f - c2;
+  s.sigma = col_at(R, P, COL_SIGMA, E);
+  for (int k = 0; k < N_OUT; ++k) s.p[k] = 0.0f;
+  s.amps = Amps{cf{-1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}};  // the perfect mirror's
+  if (model == MODEL_REFLECT) {
+    const float ps = col_at(R, P, COL_SPIKE, E), pl = col_at(R, P, COL_LOBE, E), pb = col_at(R, P, COL_BACK, E);
+    const float k1 = clip01(ps), k2 = clip01(ps + pl), k3 = clip01(ps + pl + pb);
+    const float ss = k1, sl = k2 - k1, bs = k3 - k2, lm = 1.0f - k3;
+    const bool normal = flags & FLAG_NORMAL;
+    s.p[SPECULAR] = r * (normal ? ss + sl : ss);
+    s.p[GLOSSY] = normal ? 0.0f : r * sl;
+    s.p[BACKSCATTER] = r * bs;
+    s.p[DIFFUSE] = r * lm;
+  } else {
+    const float cos_i = clip01(-dot3(wi, n));
+    s.amps = fresnel(cos_i, n1, n2);
+    const float3 sx = s_axis(wi, n);
+    const float3 px = cross3(wi, sx);
+    const float es2 = dot3(er, sx) * dot3(er, sx) + dot3(ei, sx) * dot3(ei, sx);
+    const float ep2 = dot3(er, px) * dot3(er, px) + dot3(ei, px) * dot3(ei, px);
+    const float F = clip01(cabs2(s.amps.rs) * es2 + cabs2(s.amps.rp) * ep2);
+    s.p[SPECULAR] = r * F;
+    s.p[REFRACT] = r * (1.0f - F);
+  }
+  s.p[STRAIGHT] = t;
+  if (flags & FLAG_FAR_OPAQUE) s.p[STRAIGHT] = s.p[REFRACT] = 0.0f;
+  s.detect = a * eff;
+  return s;
+}
+
+// the outcome drawn from q with one uniform: the first k with cumsum_k > u * total; -1 when q is all zero
+// (estimator.py `choose`)
+__device__ __forceinline__ int choose6(const float q[N_OUT], float u) {
+  float tot = 0.0f;
+  for (int k = 0; k < N_OUT; ++k) tot += q[k];
+  if (!(tot > 0.0f)) return -1;
+  const float th = u * tot;
+  float c = 0.0f;
+  int pick = -1;
+  for (int k = 0; k < N_OUT; ++k) {
+    c += q[k];
+    if (pick < 0 && c > th && q[k] > 0.0f) pick = k;
+  }
+  if (pick < 0)
+    for (int k = N_OUT - 1; k >= 0; --k)
+      if (q[k] > 0.0f) { pick = k; break; }
+  return pick;
+}
+
+// ------------------------------------------------------------------------------------------- directions
+// Rayleigh, polarised: the dipole axis er with probability |er|^2, else ei; c from p(c) = 3/4 (1 - c^2)
+// (phase.py `rayleigh_sample`, `dipole_cos`)
+__device__ inline float3 rayleigh_sample(float3 er, float3 ei, float u0, float u1, float u2) {
+  const float a2 = dot3(er, er), norm2 = a2 + dot3(ei, ei);
+  const float3 axis = unit3(u2 * norm2 < a2 ? er : ei);
+  const float c = fminf(fmaxf(2.0f * cosf((acosf(fminf(fmaxf(1.0f - 2.0f * u0, -1.0f), 1.0f)) - TWO_PI) / 3.0f),
+                              -1.0f), 1.0f);
+  const float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+  float sp, cp;
+  __sincosf(TWO_PI * u1, &sp, &cp);
+  return to_world(mk3(s * cp, s * sp, c), axis);
+}
+
+// Lambertian about n (lobes.py `_cosine`)
+__device__ __forceinline__ float3 lambert_sample(float3 n, float u0, float u1) {
+  const float r = sqrtf(u0);
+  float sp, cp;
+  __sincosf(TWO_PI * u1, &sp, &cp);
+  return to_world(mk3(r * cp, r * sp, sqrtf(fmaxf(1.0f - u0, 0.0f))), n);
+}
+
+// Snell's refraction at facing normal n with eta = n1 / n2; the mirror under total internal reflection
+// (outcomes.py `refract`)
+__device__ __forceinline__ float3 refract_dir(float3 wi, float3 n, float eta) {
+  const float cos_i = -dot3(wi, n);
+  const float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+  if (sin2_t >= 1.0f) return fma3(2.0f * cos_i, n, wi);
+  const float cos_t = sqrtf(fmaxf(1.0f - sin2_t, 0.0f));
+  return unit3(fma3(eta * cos_i - cos_t, n, mul3(eta, wi)));
+}
+
+// the half-width of the azimuth arc of facets at tilt a that face V (facets.py `_front_arc`)
+__device__ __forceinline__ float front_arc(float3 V, float a) {
+  const float sv = sqrtf(V.x * V.x + V.y * V.y);
+  float sa, ca;
+  __sincosf(a, &sa, &ca);
+  const float ratio = V.z * ca / fmaxf(sv * sa, FLOOR32);
+  return acosf(-fminf(ratio, 1.0f));
+}
+
+}  // namespace lh

@@ -1,0 +1,162 @@
+"""A detector as the photon loop reads it: navigator arrays, media, classes, the surface table and the sensors.
+
+This restates the structure of the reference's built detector on the path (``build/lookup.py:360-390`` the surface
+table, ``transport/step.py:280-295`` rows by class pair, ``media/bulk.py:76-110`` the media, ``surfaces/types.py``
+the surface types) for the types the engine implements:
+
+==================  ==========  ==========================================================================
+type                 model      reads (columns)
+==================  ==========  ==========================================================================
+ABSORBER             reflect    nothing: every photon absorbed, nothing detected
+SPECULAR_METAL       reflect    R (mirror), eff (the absorbed share detected)
+UNIFIED_METAL        reflect    R split by p_spike / p_lobe (facet path, sigma_alpha) / p_back / Lambertian, eff
+PAINTED              reflect    R split by f_spec (p_spike column: mirror) and Lambertian, eff
+DIELECTRIC           fresnel    the Fresnel interface of the two media (R fixed at 1)
+DIELECTRIC_RT        fresnel    R (into the interface), T (straight on), eff
+==================  ==========  ==========================================================================
+
+``row_of[c_from, c_to]`` is the surface row of a crossing from class ``c_from`` into ``c_to`` (-1: no surface
+joins them; the photon is lost).  A volume's class decides its medium (``class_medium``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ..params import Params, Table
+from .navarrays import NavArrays
+
+# surface models (the kernel's branches) and the six continuing outcomes, as the reference's
+# ``surfaces/types.py:288-296`` and ``surfaces/outcomes.py:36-53``
+MODEL_REFLECT, MODEL_FRESNEL = 0, 1
+STRAIGHT, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE = range(6)
+COLUMNS = ("R", "T", "eff", "p_spike", "p_lobe", "p_back", "sigma_alpha", "polish")
+NEUTRAL = dict(R=0.0, T=0.0, eff=0.0, p_spike=0.0, p_lobe=0.0, p_back=0.0, sigma_alpha=0.1, polish=0.0)
+FLAG_NORMAL, FLAG_FAR_OPAQUE, FLAG_EDGE = 1, 2, 4
+
+# Geant4 property names -> columns
+G4_COLUMN = {"REFLECTIVITY": "R", "TRANSMITTANCE": "T", "EFFICIENCY": "eff", "SPECULARSPIKECONSTANT": "p_spike",
+             "SPECULARLOBECONSTANT": "p_lobe", "BACKSCATTERCONSTANT": "p_back", "sigma_alpha": "sigma_alpha"}
+
+# the outcomes each type can give with its constants fixed (its support: what the defensive mixture covers,
+# ``surfaces/types.py:404-440`` support rules), as bits by outcome index
+SUPPORT = {
+    "ABSORBER": (),
+    "SPECULAR_METAL": (SPECULAR,),
+    "UNIFIED_METAL": (SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE),
+    "PAINTED": (SPECULAR, DIFFUSE),
+    "DIELECTRIC": (REFRACT, SPECULAR),
+    "DIELECTRIC_RT": (REFRACT, SPECULAR, STRAIGHT),
+}
+
+TYPES = {
+    "ABSORBER": (MODEL_REFLECT, (), {}, 0),
+    "SPECULAR_METAL": (MODEL_REFLECT, ("R", "eff"), {"p_spike": 1.0}, FLAG_NORMAL),
+    "UNIFIED_METAL": (MODEL_REFLECT, ("R", "p_spike", "p_lobe", "p_back", "sigma_alpha", "eff"), {}, 0),
+    "PAINTED": (MODEL_REFLECT, ("R", "p_spike", "eff"), {}, 0),
+    "DIELECTRIC": (MODEL_FRESNEL, (), {"R": 1.0}, 0),
+    "DIELECTRIC_RT": (MODEL_FRESNEL, ("R", "T", "eff"), {}, 0),
+}
+
+
+@dataclass
+class Medium:
+    """A bulk medium: its keys (None: absent).  ``rayleigh`` may be ``"water_rule"``: Geant4's table from RINDEX."""
+
+    name: str
+    rindex: str | None
+    absorption: str | None = None
+    rayleigh: str | None = None
+
+
+@dataclass
+class SurfaceRow:
+    """One row of the surface table: its type, and per column a parameter key or a constant."""
+
+    name: str
+    type: str
+    columns: dict = field(default_factory=dict)  # column -> key (str) or number
+    far_opaque: bool = False
+
+    @property
+    def model(self) -> int:
+        return TYPES[self.type][0]
+
+    def flags(self) -> int:
+        """Return the row's flag word: type flags, far-opaque, and its support's outcome bits at bit 8."""
+        sup = set(SUPPORT[self.type])
+        if self.type == "PAINTED" and not isinstance(self.columns.get("p_spike"), str):
+            fs = float(self.columns.get("p_spike", 0.0))  # a fixed f_spec of 0 (1) leaves no mirror (Lambertian)
+            sup -= {SPECULAR} if fs == 0.0 else {DIFFUSE} if fs == 1.0 else set()
+        if self.far_opaque:
+            sup -= {STRAIGHT, REFRACT}
+        bits = sum(1 << k for k in sup)
+        return TYPES[self.type][3] | (FLAG_FAR_OPAQUE if self.far_opaque else 0) | (bits << 8)
+
+    def column(self, c):
+        """Return the column's key, or its constant (the type's fixed value, else the neutral value)."""
+        if c in self.columns and c in TYPES[self.type][1]:
+            return self.columns[c]
+        return TYPES[self.type][2].get(c, NEUTRAL[c])
+
+
+@dataclass
+class SensorGroup:
+    """A sensor group: its sensors (indices into the detector's sensors), QE and efficiency keys."""
+
+    name: str
+    sensors: np.ndarray
+    qe: str | None = None
+    efficiency: str | None = None
+
+
+@dataclass
+class Detector:
+    """A loaded detector (module docstring).  ``params()`` gives its parameter values (the drop-in ``det.params()``)."""
+
+    name: str
+    nav: NavArrays
+    media: list
+    class_medium: np.ndarray  # (C,) medium index per class (-1: opaque, no medium)
+    class_names: list
+    rows: list
+    row_of: np.ndarray  # (C, C) int32
+    sensor_of_vol: np.ndarray  # (n + 1,) int32, -1 none
+    sensor_names: list
+    groups: list
+    values: Params
+    source_box: tuple = None
+
+    def params(self) -> Params:
+        """Return the detector's parameter values."""
+        return self.values
+
+    @property
+    def n_sensors(self) -> int:
+        return len(self.sensor_names)
+
+    @property
+    def reach(self) -> float:
+        """The farthest a straight path goes inside the geometry: the diagonal of the box holding every volume (m)."""
+        lo, hi = self.nav.lo_w.min(0), self.nav.hi_w.max(0)
+        return float(np.linalg.norm(hi - lo))
+
+    def sensor_table(self):
+        """Return (names, world positions (S, 3)) of the sensors in ``Light.arrivals`` order."""
+        pos = np.zeros((self.n_sensors, 3))
+        for v in range(self.nav.n):
+            s = self.sensor_of_vol[v]
+            if s >= 0:
+                pos[s] = self.nav.host["t"][v]
+        return list(self.sensor_names), pos
+
+    def __repr__(self):
+        return (f"{self.name}: {self.nav.n} volumes, {len(self.media)} media, {len(self.rows)} surface rows, "
+                f"{self.n_sensors} sensors")
+
+
+def table_at(tab: Table, x):
+    """Return a table at x (its knot coordinate), linear between knots, held beyond (G4PhysicsVector::Value)."""
+    return np.interp(x, np.asarray(tab.energy, np.float64), np.asarray(tab.value, np.float64))

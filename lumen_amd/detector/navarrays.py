@@ -1,0 +1,204 @@
+"""The navigator-array format: volumes, daughter BVHs in frame groups and CSG tables, as named host arrays.
+
+This is the reference's persistent navigator format (``geometry/navigator_build.py:51-90``, ``:371-397``,
+``geometry/csg_build.py:31-46``, ``:165-210``, on disk ``geometry/cache.py:187-264``): a dict ``host`` of per-node,
+per-group, per-navigator-id and per-volume arrays, and ``csg`` = (T, K, G) leaf, kind and program tables with the
+reference's column names.  ``save``/``load`` write and read ``arrays.npz`` plus ``meta.json`` in a directory.
+
+The builder here covers the solids this engine navigates (quadric-and-plane leaves; no meshes, no tori), volumes that
+lie inside their mother (no overlaps, so no ownership links) and world-frame daughter groups (``gR`` the identity).
+Navigator ids: volumes 0..n-1, the outside n.  ``R``, ``t`` map a volume's frame to the world: x_w = R x + t.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import bvh
+from .solids import Solid, program_depth
+
+HUGE = 1e30
+BOX_PAD_REL = 1e-6
+BOX_PAD_ABS = 1e-9
+STACK = 32
+MAX_DEPTH = 31
+
+
+@dataclass
+class Volume:
+    """A placed volume: its solid (kind index), world pose, mother (-1: the outside), class and name."""
+
+    name: str
+    kind: int
+    R: np.ndarray
+    t: np.ndarray
+    parent: int
+    cls: int
+
+
+@dataclass
+class NavArrays:
+    """The navigator's arrays (module docstring).  ``host``: named arrays; ``csg``: {"T", "K", "G"} tables."""
+
+    host: dict
+    csg: dict
+    n: int
+    names: tuple
+    lo_w: np.ndarray
+    hi_w: np.ndarray
+    n_mesh: int = 0
+    precision: str = "float32"
+    stats: dict = field(default_factory=dict)
+
+    # ------------------------------------------------------------------ on disk: arrays.npz + meta.json
+    def save(self, path: str) -> None:
+        """Write ``arrays.npz`` (every array, host then csg, by name) and ``meta.json`` into directory ``path``."""
+        os.makedirs(path, exist_ok=True)
+        arrs = {f"host/{k}": v for k, v in self.host.items()}
+        for tab in ("T", "K", "G"):
+            arrs |= {f"csg/{tab}/{k}": v for k, v in self.csg[tab].items()}
+        arrs["lo_w"], arrs["hi_w"] = self.lo_w, self.hi_w
+        np.savez(os.path.join(path, "arrays.npz"), **arrs)
+        meta = dict(format="lumen-navarrays/1", n=int(self.n), n_mesh=int(self.n_mesh), names=list(self.names),
+                    precision=self.precision, stats=self.stats, empty_leaf=int(self.csg["empty_leaf"]),
+                    arrays=sorted(arrs))
+        with open(os.path.join(path, "meta.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+
+    @staticmethod
+    def load(path: str) -> "NavArrays":
+        """Read a directory written by ``save`` (or by the reference's cache with the same names)."""
+        with open(os.path.join(path, "meta.json")) as f:
+            meta = json.load(f)
+        with np.load(os.path.join(path, "arrays.npz"), allow_pickle=False) as z:
+            host = {k[5:]: z[k] for k in z.files if k.startswith("host/")}
+            csg = {tab: {k[len(tab) + 5:]: z[k] for k in z.files if k.startswith(f"csg/{tab}/")}
+                   for tab in ("T", "K", "G")}
+            lo_w, hi_w = z["lo_w"], z["hi_w"]
+        csg["empty_leaf"] = meta.get("empty_leaf", len(csg["T"]["typ"]) - 1)
+        return NavArrays(host, csg, int(meta["n"]), tuple(meta["names"]), lo_w, hi_w, int(meta.get("n_mesh", 0)),
+                         meta.get("precision", "float32"), meta.get("stats", {}))
+
+
+# ---------------------------------------------------------------------------------------------- the CSG tables
+def csg_tables(kinds: list[Solid], dtype=np.float32) -> dict:
+    """Return the CSG tables of ``kinds``: T (leaves, one empty leaf past the end), K (leaf and program ranges), G.
+
+    Columns as the reference's (``geometry/csg_build.py:140-165``): ``typ``, ``p0``, ``np_``, ``sa``, ``c``,
+    ``R``, ``r``, ``o*``, ``b*``, ``q*``, ``bl*``, ``bh*``, ``A**``, ``m**`` per leaf; ``n*``, ``h`` per plane.
+    """
+    rows, planes, prog, krows = [], [], [], []
+    for kd in kinds:
+        if program_depth(kd.program) > MAX_DEPTH:
+            raise ValueError(f"csg tables: solid {kd.name!r} is deeper than {MAX_DEPTH} operands")
+        l0, p0 = len(rows), len(prog)
+        for lf in kd.leaves:
+            rows.append(dict(typ=lf.typ, o=lf.origin, q=lf.origin + lf.qo, A=lf.A, b=lf.b, c=lf.c,
+                             sa=max(float(np.max(np.abs(lf.A))), 1e-30), p0=len(planes), np_=len(lf.N)))
+            planes += [(lf.N[k], lf.H[k]) for k in range(len(lf.N))]
+        prog += list(kd.program)
+        krows.append((l0, len(kd.leaves), p0, len(kd.program)))
+    empty_leaf = len(rows)
+    rows.append(dict(typ=0, o=np.zeros(3), q=np.zeros(3), A=np.zeros((3, 3)), b=np.zeros(3), c=0.0, sa=1.0,
+                     p0=len(planes), np_=0))
+    f = lambda v: np.asarray(v, np.float64).astype(dtype)  # noqa: E731
+    T = dict(typ=np.asarray([r["typ"] for r in rows], np.int32), p0=np.asarray([r["p0"] for r in rows], np.int32),
+             np_=np.asarray([r["np_"] for r in rows], np.int32), sa=f([r["sa"] for r in rows]),
+             c=f([r["c"] for r in rows]), R=f(np.zeros(len(rows))), r=f(np.zeros(len(rows))))
+    for a in range(3):
+        T[f"o{a}"] = f([r["o"][a] for r in rows])
+        T[f"b{a}"] = f([r["b"][a] for r in rows])
+        T[f"q{a}"] = f([r["q"][a] for r in rows])
+        T[f"bl{a}"] = f(np.full(len(rows), -HUGE))
+        T[f"bh{a}"] = f(np.full(len(rows), HUGE))
+        for b in range(3):
+            T[f"A{a}{b}"] = f([r["A"][a, b] for r in rows])
+            T[f"m{a}{b}"] = f([1.0 if a == b else 0.0 for _ in rows])
+    planes = planes or [(np.zeros(3), 0.0)]
+    for a in range(3):
+        T[f"n{a}"] = f([p[0][a] for p in planes])
+    T["h"] = f([p[1] for p in planes])
+    K = {k: np.asarray([x[j] for x in krows], np.int32) for j, k in enumerate(("l0", "nl", "p0", "pl"))}
+    prog = prog or [(0, 0)]
+    G = dict(op=np.asarray([g[0] for g in prog], np.int32), arg=np.asarray([g[1] for g in prog], np.int32))
+    return dict(T=T, K=K, G=G, empty_leaf=empty_leaf,
+                kind_box=[(0.5 * (k.lo + k.hi), 0.5 * (k.hi - k.lo)) for k in kinds])
+
+
+# ---------------------------------------------------------------------------------------------- the builder
+def _world_box(kd: Solid, R, t):
+    c, h = 0.5 * (kd.lo + kd.hi), 0.5 * (kd.hi - kd.lo)
+    cw, hw = R @ c + t, np.abs(R) @ h
+    pad = BOX_PAD_REL * (np.abs(cw) + hw) + BOX_PAD_ABS
+    return cw - hw - pad, cw + hw + pad
+
+
+def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision="float32",
+          leaf_size: int = bvh.LEAF_SIZE) -> NavArrays:
+    """Return the NavArrays of placed ``volumes`` (in depth-first order: a mother before its daughters)."""
+    fdt = np.float32 if precision == "float32" else np.float64
+    n = len(volumes)
+    if n == 0:
+        raise ValueError("navarrays.build: no volumes")
+    parent = np.array([n if v.parent < 0 else v.parent for v in volumes], np.int64)
+    for i, p in enumerate(parent):
+        if p < n and p >= i:
+            raise ValueError("navarrays.build: volumes must come after their mother (depth-first order)")
+    csg = csg_tables(kinds, fdt)
+    lo_w, hi_w = np.zeros((n, 3)), np.zeros((n, 3))
+    for i, v in enumerate(volumes):
+        lo_w[i], hi_w[i] = _world_box(kinds[v.kind], v.R, v.t)
+    kids = [[] for _ in range(n + 1)]
+    for i in range(n):
+        kids[parent[i]].append(i)
+    nodes = dict(lo=[], hi=[], skip=[], start=[], count=[])
+    items, g0, g1, vg0, vg1 = [], [], [], np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
+    depth = 0
+    for u in range(n + 1):
+        vg0[u] = len(g0)
+        if kids[u]:
+            ks = np.asarray(kids[u], np.int64)
+            tr = bvh.build(lo_w[ks], hi_w[ks], leaf_size)
+            base_n, base_i = len(nodes["lo"]), len(items)
+            g0.append(base_n)
+            nodes["lo"] += list(tr["lo"])
+            nodes["hi"] += list(tr["hi"])
+            nodes["skip"] += list(tr["skip"] + base_n)
+            nodes["start"] += list(np.where(tr["count"] > 0, tr["start"] + base_i, 0))
+            nodes["count"] += list(tr["count"])
+            items += list(ks[tr["items"]])
+            g1.append(len(nodes["lo"]))
+            depth = max(depth, tr["depth"])
+        vg1[u] = len(g0)
+    if depth > STACK:
+        raise ValueError(f"navarrays.build: a BVH of depth {depth} exceeds the traversal stack ({STACK})")
+    G = len(g0)
+    size = np.ones(n, np.int64)
+    for i in range(n - 1, -1, -1):
+        if parent[i] < n:
+            size[parent[i]] += size[i]
+    nl = np.maximum(csg["K"]["nl"][[v.kind for v in volumes]], 1)
+    host = dict(
+        lo=np.asarray(nodes["lo"] or np.zeros((1, 3))).astype(fdt),
+        hi=np.asarray(nodes["hi"] or np.zeros((1, 3))).astype(fdt),
+        skip=np.asarray(nodes["skip"] or [0], np.int32), start=np.asarray(nodes["start"] or [0], np.int32),
+        count=np.asarray(nodes["count"] or [0], np.int32), items=np.asarray(items or [0], np.int32),
+        gR=np.tile(np.eye(3), (max(G, 1), 1, 1)).astype(fdt),
+        g0=np.asarray(g0 or [0], np.int32), g1=np.asarray(g1 or [0], np.int32),
+        vg0=vg0, vg1=vg1, vg1u=vg1.copy(),
+        parent=np.concatenate([parent, [n]]).astype(np.int32),
+        cls=np.asarray([v.cls for v in volumes] + [outside_cls], np.int32),
+        kind=np.asarray([v.kind for v in volumes], np.int32),
+        R=np.stack([v.R for v in volumes]).astype(fdt), t=np.stack([v.t for v in volumes]).astype(fdt),
+        prio=np.arange(n + 1, dtype=np.int32),
+        size=np.concatenate([size, [0]]).astype(np.int32),
+        piece=np.full(n + 1, -1, np.int32),
+        ub=np.concatenate([[0], np.cumsum(nl)]).astype(np.int32),
+    )
+    stats = dict(volumes=n, kinds=len(kinds), groups=G, bvh_depth=depth,
+                 max_daughters=max(len(k) for k in kids))
+    return NavArrays(host, csg, n, tuple(v.name for v in volumes), lo_w, hi_w, 0, precision, stats)

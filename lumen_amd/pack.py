@@ -1,0 +1,157 @@
+"""Packing: a detector's parameter values into the flat operands the photon loop reads (the reference's
+``build/pack.py:84-97`` ``pack.pack``, laid out for the HIP kernel and the C ABI).
+
+Two arrays per run: ``values`` (float32) holds every table as its knots then its values, ``[E_0..E_{k-1},
+V_0..V_{k-1}]``; ``layout`` (int32, fixed by the detector's structure) says where each table is.  A constant is a
+table of one knot.  ``slots`` (int32, one per ``values`` entry) names the gradient slot of each table value the run
+differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` declares the same):
+
+    [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
+    [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)
+    media:  per medium 9 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
+    rows:   per row 18 words: model, flags, then (offset, knots) of R, T, eff, p_spike, p_lobe, p_back, sigma_alpha,
+            polish
+
+The group index is Geant4's GROUPVEL rule on the RINDEX knots (``media/laws.py:150-181``); a Rayleigh table
+``"water_rule"`` is Geant4's Einstein-Smoluchowski table for Water on the RINDEX knots (``rules/optics.py:189-254``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import units
+from .detector.model import COLUMNS, FLAG_EDGE, Detector
+from .params import Params, Table
+
+MAGIC = 0x4C484F31
+HEADER = 11
+MEDIUM_WORDS = 9
+ROW_WORDS = 2 + 2 * len(COLUMNS)
+
+
+def group_index_knots(e, n):
+    """Return Geant4's GROUPVEL knots as group indices (energies, n_g) from RINDEX knots (e, n).
+
+    G4MaterialPropertiesTable::CalculateGROUPVEL as the reference restates it (``media/laws.py:150-166``): knot 0 at
+    e_0 with the first segment's slope, knot j (1..k-1) at the midpoint of segment (j-1, j) with that segment's
+    slope and mean index, knot k at e_k with the last segment's slope; n_g = max(n + dn/dlog E, n).
+    """
+    e, n = np.asarray(e, np.float64), np.asarray(n, np.float64)
+    k = len(e) - 1
+    if k < 1:
+        return e.copy(), n.copy()
+    eg = np.concatenate([e[:1], 0.5 * (e[:-2] + e[1:-1]), e[-1:]])
+    ng = np.zeros(k + 1)
+    for j in range(k + 1):
+        s = min(max(j - 1, 0), k - 1)
+        slope = (n[s + 1] - n[s]) / np.log(e[s + 1] / e[s])
+        base = n[j] if j in (0, k) else 0.5 * (n[s] + n[s + 1])
+        ng[j] = max(base + slope, base)
+    return eg, ng
+
+
+def water_rayleigh(e, n):
+    """Return Geant4's Rayleigh lengths (m) of Water at RINDEX knots (e eV, n): 1/L = c1 k^4 ((n^2-1)(n^2+2)/3)^2."""
+    e, n = np.asarray(e, np.float64), np.asarray(n, np.float64)
+    c1 = units.WATER_COMPRESSIBILITY * units.WATER_TEMPERATURE * units.K_BOLTZMANN / (6.0 * np.pi)
+    k = 2.0 * np.pi / units.HC_EV_M * e
+    n2 = n * n
+    return 1.0 / (c1 * k**4 * ((n2 - 1.0) * (n2 + 2.0) / 3.0) ** 2)
+
+
+@dataclass
+class Packed:
+    """A run's packed operands: ``values`` float32, ``layout`` int32, ``slots`` int32, and the gradient slot map
+    ``grad_keys`` = [(key, first slot, knots)]."""
+
+    values: np.ndarray
+    layout: np.ndarray
+    slots: np.ndarray
+    n_grad: int
+    grad_keys: list
+
+
+class _Builder:
+    def __init__(self, grad_keys):
+        self.vals, self.slots = [], []
+        self.grad = {k: None for k in grad_keys}
+        self.n_grad = 0
+        self.grad_map = []
+
+    def table(self, e, v, key=None):
+        e, v = np.atleast_1d(np.asarray(e, np.float64)), np.atleast_1d(np.asarray(v, np.float64))
+        if e.shape != v.shape or len(e) == 0:
+            raise ValueError(f"pack: table {key!r} has {len(e)} knots and {len(v)} values")
+        if len(e) > 1 and np.any(np.diff(e) <= 0):
+            raise ValueError(f"pack: table {key!r}: knots must ascend")
+        off = len(self.vals)
+        self.vals += list(e) + list(v)
+        slot = -1
+        if key is not None and key in self.grad:
+            if self.grad[key] is None:
+                self.grad[key] = self.n_grad
+                self.grad_map.append((key, self.n_grad, len(e)))
+                self.n_grad += len(e)
+            slot = self.grad[key]
+        self.slots += [-1] * len(e) + ([slot + i for i in range(len(e))] if slot >= 0 else [-1] * len(e))
+        return off, len(e)
+
+    def value(self, p: Params, key):
+        v = p[key]
+        if isinstance(v, Table):
+            return self.table(v.energy, v.value, key)
+        return self.table([0.0] * np.size(v) if np.size(v) == 1 else np.arange(np.size(v), dtype=float),
+                          np.atleast_1d(v), key)
+
+
+def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows=()) -> Packed:
+    """Return the Packed operands of ``det`` at ``params`` (default ``det.params()``).
+
+    ``grad_keys``: the keys whose table values get gradient slots (the adjoint's parameters).  ``edge_rows``: surface
+    rows drawn with the defensive mixture (the reference's edge rows, ``transport/estimator.py:131-153``).
+    """
+    p = det.params() if params is None else params
+    b = _Builder(grad_keys)
+    nm, nc, nr = len(det.media), len(det.class_medium), len(det.rows)
+    med = np.zeros((nm, MEDIUM_WORDS), np.int32)
+    for i, m in enumerate(det.media):
+        if m.rindex is None:
+            med[i, 8] = 1  # opaque
+            continue
+        rt = p[m.rindex]
+        med[i, 0:2] = b.table(rt.energy, rt.value, m.rindex)
+        eg, ng = group_index_knots(rt.energy, rt.value)
+        med[i, 2:4] = b.table(eg, ng)
+        if m.absorption is not None:
+            med[i, 4:6] = b.value(p, m.absorption)
+        if m.rayleigh == "water_rule":
+            med[i, 6:8] = b.table(rt.energy, water_rayleigh(rt.energy, rt.value))
+        elif m.rayleigh is not None:
+            med[i, 6:8] = b.value(p, m.rayleigh)
+    rows = np.zeros((nr, ROW_WORDS), np.int32)
+    for r, row in enumerate(det.rows):
+        rows[r, 0] = row.model
+        rows[r, 1] = row.flags() | (FLAG_EDGE if r in edge_rows else 0)
+        for j, c in enumerate(COLUMNS):
+            v = row.column(c)
+            rows[r, 2 + 2 * j: 4 + 2 * j] = b.value(p, v) if isinstance(v, str) else b.table([0.0], [float(v)])
+    qe_off, n_qe = 0, 1
+    q = p[det.groups[0].qe] if det.groups and det.groups[0].qe is not None else None
+    if isinstance(q, Table):  # the charge histogram's wavelength knots (nm); QE itself is read after tracking
+        qe_off, n_qe = b.table(q.energy, q.value)
+    else:  # no QE table, or one QE per sensor: a single charge cell per sensor
+        qe_off, n_qe = b.table([0.0], [1.0])
+    L = [MAGIC, nm, nc, nr, det.n_sensors, n_qe, 0, 0, 0, 0, qe_off]
+    L[6] = len(L)
+    L += list(med.ravel())
+    L[7] = len(L)
+    L += list(np.asarray(det.class_medium, np.int32))
+    L[8] = len(L)
+    L += list(np.asarray(det.row_of, np.int32).ravel())
+    L[9] = len(L)
+    L += list(rows.ravel())
+    return Packed(np.asarray(b.vals, np.float32), np.asarray(L, np.int32), np.asarray(b.slots, np.int32),
+                  b.n_grad, b.grad_map)

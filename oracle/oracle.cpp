@@ -1,0 +1,1104 @@
+// THE ORACLE -- test infrastructure, not the product.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it, and only as the checker (or as the CPU baseline that is timed); the product path
+// (lumen_amd) never calls it and has no CPU fallback.
+//
+// A C++17/OpenMP restatement of the reference's photon loop, in FP64, written from the reference's algorithm:
+//   loop and fates        src/lumen/run.py:293-368 (_loop), src/lumen/transport/item.py:20-36
+//   one step              src/lumen/transport/step.py:318-359 (_step_context), :362-394 (_surface_event),
+//                         :397-436 (_bulk), :464-492 (_after), :495-513 (_advance), :516-571 (step)
+//   estimator             src/lumen/transport/estimator.py:50-153 (proposal, choose, defensive, edge_floor)
+//   media                 src/lumen/media/bulk.py:61-110, src/lumen/media/laws.py:97-181 (tables, GROUPVEL)
+//   surfaces              src/lumen/surfaces/types.py:211-296, :352-389 (chained_cuts, _none, _real, kernel),
+//                         src/lumen/surfaces/fresnel.py:25-45, src/lumen/surfaces/outcomes.py:36-75,
+//                         src/lumen/surfaces/polar.py:39-118
+//   lobes                 src/lumen/media/phase.py:45-75 (Rayleigh), src/lumen/lobes.py:60-78 (Lambert),
+//                         src/lumen/surfaces/facets.py:318-397 (unified facet path)
+//   sources               src/lumen/sources/point.py:73-86, src/lumen/sources/cherenkov.py:33-66
+//   readout               src/lumen/run.py:483-528 (_moment_adds: a photon's deposits per sensor summed, then
+//                         squared), src/lumen/readout/fill.py:88-213
+//   navigator referee     src/lumen/compare/oracle.py:1-24, :208 (FP64 labels and the first change along a ray),
+//                         here by brute force over every daughter (no BVH), each solid's CSG leaves evaluated in FP64.
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11) with the same (seed, photon id, vertex, block) contract as the
+// engine, restated here independently.
+//
+// Parity status per function (the checks that pin it live in tests/test_oracle.py):
+//   philox            pinned: Random123 known-answer vectors
+//   tables, GROUPVEL  pinned: closed forms (Water Rayleigh lengths at the RINDEX knots, n and L_abs at 400 nm)
+//   navigator         pinned: closed-form ray-box / ray-cylinder / ray-sphere-cap distances
+//   facet path        pinned: Geant4 11.3.0 histograms (tests/golden/g4_unified_metal_*.npz), two-sample chi2
+//   Rayleigh, Lambert pinned: their closed-form angular densities (chi2 on the cosine)
+//   Fresnel           pinned: textbook normal-incidence and Brewster-angle values, energy conservation
+//   whole loop        pinned: the tutorial's published arrivals (example_tank), the specular-sphere closed form
+//   Cherenkov weight  pinned: Frank-Tamm photon yield per metre in closed form
+//   moment error      pinned: brute force on a small run
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/lumen_hip.h"
+
+namespace orc {
+
+// ------------------------------------------------------------------------------------------ the scalar types
+// Counted: a double that counts the floating-point operations made with it (add, sub, mul, div, compare and each
+// math function as one; fma as two), for the algorithmic work per photon.
+thread_local long long g_ops = 0;
+
+struct Counted {
+  double v;
+  Counted() : v(0) {}
+  Counted(double x) : v(x) {}
+  explicit operator double() const { return v; }
+};
+inline Counted operator+(Counted a, Counted b) { ++g_ops; return Counted(a.v + b.v); }
+inline Counted operator-(Counted a, Counted b) { ++g_ops; return Counted(a.v - b.v); }
+inline Counted operator*(Counted a, Counted b) { ++g_ops; return Counted(a.v * b.v); }
+inline Counted operator/(Counted a, Counted b) { ++g_ops; return Counted(a.v / b.v); }
+inline Counted operator-(Counted a) { return Counted(-a.v); }
+inline Counted &operator+=(Counted &a, Counted b) { ++g_ops; a.v += b.v; return a; }
+inline Counted &operator-=(Counted &a, Counted b) { ++g_ops; a.v -= b.v; return a; }
+inline Counted &operator*=(Counted &a, Counted b) { ++g_ops; a.v *= b.v; return a; }
+inline Counted &operator/=(Counted &a, Counted b) { ++g_ops; a.v /= b.v; return a; }
+inline bool operator<(Counted a, Counted b) { ++g_ops; return a.v < b.v; }
+inline bool operator>(Counted a, Counted b) { ++g_ops; return a.v > b.v; }
+inline bool operator<=(Counted a, Counted b) { ++g_ops; return a.v <= b.v; }
+inline bool operator>=(Counted a, Counted b) { ++g_ops; return a.v >= b.v; }
+inline bool operator==(Counted a, Counted b) { ++g_ops; return a.v == b.v; }
+inline bool operator!=(Counted a, Counted b) { ++g_ops; return a.v != b.v; }
+
+inline double val(double x) { return x; }
+inline double val(Counted x) { return x.v; }
+#define ORC_F1(name, f)                                  \
+  inline double name(double x) { return f(x); }           \
+  inline Counted name(Counted x) { ++g_ops; return Counted(f(x.v)); }
+ORC_F1(SQRT, std::sqrt)
+ORC_F1(EXP, std::exp)
+ORC_F1(LOG, std::log)
+ORC_F1(SIN, std::sin)
+ORC_F1(COS, std::cos)
+ORC_F1(ACOS, std::acos)
+ORC_F1(FABS, std::fabs)
+inline double ATAN2(double y, double x) { return std::atan2(y, x); }
+inline Counted ATAN2(Counted y, Counted x) { ++g_ops; return Counted(std::atan2(y.v, x.v)); }
+template <class R> inline R MIN(R a, R b) { return b < a ? b : a; }
+template <class R> inline R MAX(R a, R b) { return a < b ? b : a; }
+template <class R> inline R CLIP01(R x) { return MIN<R>(MAX<R>(x, R(0.0)), R(1.0)); }
+
+template <class R> struct V3 {
+  R x, y, z;
+};
+template <class R> inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline V3<R> unit(V3<R> a) {
+  R s = dot(a, a);
+  if (s > R(0.0)) return (R(1.0) / SQRT(s)) * a;
+  return a;
+}
+
+// ------------------------------------------------------------------------------------------ Philox4x32-10
+inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+// integer operations of one Philox call, counted for the work model: per round 2 32x32->64 multiplies (as 2 ops
+// each: low and high), 4 xors, 2 key adds
+constexpr long long PHILOX_INT_OPS = 10 * (4 + 4 + 2);
+thread_local long long g_int_ops = 0;
+
+struct Rng {
+  uint32_t k0, k1, id_lo, id_hi;
+  void draw4(uint32_t vertex, uint32_t block, double u[4]) const {
+    uint32_t c[4] = {id_lo, id_hi, vertex, block};
+    philox(c, k0, k1);
+    g_int_ops += PHILOX_INT_OPS;
+    for (int i = 0; i < 4; ++i) u[i] = ((double)(c[i] >> 9) + 0.5) * (1.0 / 8388608.0);
+  }
+};
+constexpr uint32_t FACET_BLOCK = 0x100u, SOURCE_VERTEX = 0xFFFFFFFFu;
+
+// ------------------------------------------------------------------------------------------ constants
+constexpr double HC = 1239.8419843320026, C_VACUUM = 0.299792458, TWO_PI = 6.283185307179586;
+constexpr double PI = 3.141592653589793, HALF_PI = 1.5707963267948966;
+constexpr double FLOOR32 = 1e-30, TINY_NORMAL = 1.1754943508222875e-38, DEFENSIVE = 0.01;
+constexpr double FINE_STRUCTURE = 1.0 / 137.035999084;
+constexpr double ZERO_STEP = 1e-9;  // m: a crossing this close to the start is a zero-length step (FP64)
+constexpr double ON_FACE = 1e-9;    // m: the face a photon sits on is not met again this close
+constexpr int ZERO_STEPS_BEFORE_PUSH = 10, ZERO_STEPS_BEFORE_KILL = 25;
+constexpr double PUSH = 1e-7;       // m
+enum { STRAIGHT = 0, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE, NOUT };
+enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, NCOL };
+
+// ------------------------------------------------------------------------------------------ the scene (FP64)
+struct Leaf {
+  int typ;
+  double q[3], A[9], b[3], c;
+  std::vector<double> n;  // 3 per plane, relative to q
+  std::vector<double> h;
+};
+struct Kind {
+  std::vector<Leaf> leaves;
+  std::vector<int> op, arg;
+};
+struct Scene {
+  int n;
+  std::vector<double> R, t;  // local -> world
+  std::vector<int> kind, parent, cls, sensor;
+  std::vector<std::vector<int>> kids;  // per navigator id, its daughters (brute force: no BVH)
+  std::vector<Kind> kinds;
+  double reach;
+};
+
+Scene scene_of(const lh_geometry *G) {
+  Scene s;
+  s.n = G->n_vol;
+  s.R.assign(G->R, G->R + 9 * s.n);
+  s.t.assign(G->t, G->t + 3 * s.n);
+  s.kind.assign(G->kind, G->kind + s.n);
+  s.parent.assign(G->parent, G->parent + s.n + 1);
+  s.cls.assign(G->cls, G->cls + s.n + 1);
+  s.sensor.assign(G->sensor, G->sensor + s.n + 1);
+  s.kids.assign(s.n + 1, {});
+  for (int v = 0; v < s.n; ++v) s.kids[s.parent[v]].push_back(v);
+  s.kinds.resize(G->n_kinds);
+  for (int k = 0; k < G->n_kinds; ++k) {
+    Kind &K = s.kinds[k];
+    for (int j = 0; j < G->kind_nl[k]; ++j) {
+      const int l = G->kind_l0[k] + j;
+      Leaf L;
+      L.typ = G->leaf_typ[l];
+      for (int a = 0; a < 3; ++a) L.q[a] = G->leaf_q[3 * l + a], L.b[a] = G->leaf_b[3 * l + a];
+      for (int a = 0; a < 9; ++a) L.A[a] = G->leaf_A[9 * l + a];
+      L.c = G->leaf_c[l];
+      for (int p = 0; p < G->leaf_np[l]; ++p) {
+        const int pi = G->leaf_p0[l] + p;
+        const float *nn = G->plane_n + 3 * pi;
+        const double o[3] = {G->leaf_o[3 * l], G->leaf_o[3 * l + 1], G->leaf_o[3 * l + 2]};
+        // n.(x - o) <= h  ->  n.(x - q) <= h - n.(q - o)
+        L.n.insert(L.n.end(), {nn[0], nn[1], nn[2]});
+        L.h.push_back(G->plane_h[pi] - (nn[0] * (L.q[0] - o[0]) + nn[1] * (L.q[1] - o[1]) + nn[2] * (L.q[2] - o[2])));
+      }
+      K.leaves.push_back(L);
+    }
+    for (int j = 0; j < G->kind_pl[k]; ++j) {
+      K.op.push_back(G->prog_op[G->kind_p0[k] + j]);
+      K.arg.push_back(G->prog_arg[G->kind_p0[k] + j]);
+    }
+  }
+  s.reach = G->reach;
+  return s;
+}
+
+template <class R> inline void to_local(const Scene &s, int v, V3<R> x, V3<R> d, V3<R> &xl, V3<R> &dl) {
+  const double *M = &s.R[9 * v], *t = &s.t[3 * v];
+  const V3<R> y = x - v3<R>(R(t[0]), R(t[1]), R(t[2]));
+  xl = v3<R>(R(M[0]) * y.x + R(M[3]) * y.y + R(M[6]) * y.z, R(M[1]) * y.x + R(M[4]) * y.y + R(M[7]) * y.z,
+             R(M[2]) * y.x + R(M[5]) * y.y + R(M[8]) * y.z);
+  dl = v3<R>(R(M[0]) * d.x + R(M[3]) * d.y + R(M[6]) * d.z, R(M[1]) * d.x + R(M[4]) * d.y + R(M[7]) * d.z,
+             R(M[2]) * d.x + R(M[5]) * d.y + R(M[8]) * d.z);
+}
+template <class R> inline V3<R> to_world_dir(const Scene &s, int v, V3<R> n) {
+  const double *M = &s.R[9 * v];
+  return v3<R>(R(M[0]) * n.x + R(M[1]) * n.y + R(M[2]) * n.z, R(M[3]) * n.x + R(M[4]) * n.y + R(M[5]) * n.z,
+               R(M[6]) * n.x + R(M[7]) * n.y + R(M[8]) * n.z);
+}
+
+// a leaf's intervals along y + t d, y relative to its q: each (lo, hi, face_lo, face_hi); face 0 the quadric
+template <class R> struct Iv {
+  R lo, hi;
+  int flo, fhi;
+};
+template <class R>
+int leaf_iv(const Leaf &L, V3<R> o, V3<R> d, Iv<R> out[2]) {
+  const R INF = R(1e300);
+  const V3<R> y = o - v3<R>(R(L.q[0]), R(L.q[1]), R(L.q[2]));
+  R t0 = -INF, t1 = INF;
+  int f0 = -1, f1 = -1;
+  for (size_t p = 0; p < L.h.size(); ++p) {
+    const V3<R> n = v3<R>(R(L.n[3 * p]), R(L.n[3 * p + 1]), R(L.n[3 * p + 2]));
+    const R nd = dot(n, d), rhs = R(L.h[p]) - dot(n, y);
+    if (FABS(nd) < R(1e-15)) {
+      if (rhs < R(0.0)) return 0;
+      continue;
+    }
+    const R tp = rhs / nd;
+    if (nd < R(0.0)) {
+      if (tp > t0) { t0 = tp; f0 = (int)p + 1; }
+    } else {
+      if (tp < t1) { t1 = tp; f1 = (int)p + 1; }
+    }
+  }
+  R qa[2] = {-INF, INF}, qb[2] = {INF, -INF};
+  bool ra[2] = {false, false}, rb[2] = {false, false};
+  if (L.typ == 1) {
+    auto Av = [&](V3<R> v) {
+      return v3<R>(R(L.A[0]) * v.x + R(L.A[1]) * v.y + R(L.A[2]) * v.z, R(L.A[3]) * v.x + R(L.A[4]) * v.y + R(L.A[5]) * v.z,
+                   R(L.A[6]) * v.x + R(L.A[7]) * v.y + R(L.A[8]) * v.z);
+    };
+    const V3<R> bb = v3<R>(R(L.b[0]), R(L.b[1]), R(L.b[2]));
+    const R a = dot(d, Av(d));
+    const R beta = dot(Av(y), d) + dot(bb, d);
+    const R gam = dot(y, Av(y)) + R(2.0) * dot(bb, y) + R(L.c);
+    double amax = 0;
+    for (int i = 0; i < 9; ++i) amax = std::max(amax, std::fabs(L.A[i]));
+    if (FABS(a) <= R(1e-13 * amax)) {
+      if (beta > R(0.0)) { qa[0] = -INF; qb[0] = -gam / (R(2.0) * beta); rb[0] = true; }
+      else if (beta < R(0.0)) { qa[0] = -gam / (R(2.0) * beta); qb[0] = INF; ra[0] = true; }
+      else if (gam > R(0.0)) { qa[0] = INF; qb[0] = -INF; }
+    } else {
+      const R D = beta * beta - a * gam;
+      if (D < R(0.0)) {
+        if (a > R(0.0)) { qa[0] = INF; qb[0] = -INF; }
+      } else {
+        const R sq = SQRT(D);
+        const R qq = beta > R(0.0) ? -(beta + sq) : -(beta - sq);
+        R r1 = qq / a, r2 = qq != R(0.0) ? gam / qq : r1;
+        if (r2 < r1) std::swap(r1, r2);
+        if (a > R(0.0)) { qa[0] = r1; qb[0] = r2; ra[0] = rb[0] = true; }
+        else { qa[0] = -INF; qb[0] = r1; rb[0] = true; qa[1] = r2; qb[1] = INF; ra[1] = true; }
+      }
+    }
+  }
+  int k = 0;
+  for (int p = 0; p < 2; ++p) {
+    const R lo = MAX<R>(qa[p], t0), hi = MIN<R>(qb[p], t1);
+    if (lo < hi) {
+      out[k].lo = lo;
+      out[k].hi = hi;
+      out[k].flo = (qa[p] > t0 && ra[p]) ? 0 : f0;
+      out[k].fhi = (qb[p] < t1 && rb[p]) ? 0 : f1;
+      ++k;
+    }
+  }
+  return k;
+}
+
+inline bool run_prog(const Kind &K, const std::vector<bool> &in) {
+  std::vector<bool> st;
+  for (size_t j = 0; j < K.op.size(); ++j) {
+    if (K.op[j] == 0) {
+      st.push_back(in[K.arg[j]]);
+    } else {
+      const bool b = st.back();
+      st.pop_back();
+      const bool a = st.back();
+      st.pop_back();
+      st.push_back(K.op[j] == 1 ? (a || b) : K.op[j] == 2 ? (a && b) : (a && !b));
+    }
+  }
+  return st.back();
+}
+
+template <class R> struct Change {
+  R t;
+  int leaf, face;
+};
+
+// the first change of a solid's membership along the ray after t = 0 (compare/oracle.py: FP64 leaf roots, the CSG
+// value scanned between consecutive endpoints); `exit_query`: from inside, else from outside.  Endpoints of the face
+// the photon sits on (volume `vol`, code `skip`) within ON_FACE of the start count as passed.
+template <class R>
+Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_query, long long skip) {
+  const Kind &K = s.kinds[s.kind[vol]];
+  const int nl = (int)K.leaves.size();
+  std::vector<std::array<Iv<R>, 2>> ivs(nl);
+  std::vector<int> cnt(nl);
+  for (int l = 0; l < nl; ++l) {
+    cnt[l] = leaf_iv(K.leaves[l], o, d, ivs[l].data());
+    for (int p = 0; p < cnt[l]; ++p) {
+      Iv<R> &iv = ivs[l][p];
+      const long long clo = ((long long)vol << 10) | (l << 5) | iv.flo, chi = ((long long)vol << 10) | (l << 5) | iv.fhi;
+      if (clo == skip && FABS(iv.lo) <= R(ON_FACE)) iv.lo = R(-1e299);
+      if (chi == skip && FABS(iv.hi) <= R(ON_FACE)) iv.hi = R(-1e299);
+    }
+  }
+  auto member = [&](R t) {
+    std::vector<bool> in(nl);
+    for (int l = 0; l < nl; ++l) {
+      bool b = false;
+      for (int p = 0; p < cnt[l]; ++p) b = b || (ivs[l][p].lo <= t && t < ivs[l][p].hi);
+      in[l] = b;
+    }
+    return run_prog(K, in);
+  };
+  const bool start = member(R(0.0));
+  if (exit_query && !start) return {R(0.0), 0, 0};
+  if (!exit_query && start) return {R(1e300), -1, -1};
+  R cur = R(0.0);
+  for (int it = 0; it < 4 * nl + 5; ++it) {
+    R nxt = R(1e300);
+    int bl = -1, bf = -1;
+    for (int l = 0; l < nl; ++l)
+      for (int p = 0; p < cnt[l]; ++p) {
+        const Iv<R> &iv = ivs[l][p];
+        if (iv.lo > cur && iv.lo < nxt) { nxt = iv.lo; bl = l; bf = iv.flo; }
+        if (iv.hi > cur && iv.hi < nxt && iv.hi < R(1e299)) { nxt = iv.hi; bl = l; bf = iv.fhi; }
+      }
+    if (bl < 0) break;
+    if (member(nxt) != start) return {nxt, bl, bf};
+    cur = nxt;
+  }
+  if (exit_query) return {R(0.0), 0, 0};
+  return {R(1e300), -1, -1};
+}
+
+template <class R> struct Cross {
+  R t;
+  int vol_after;
+  long long srf;
+  V3<R> n;
+};
+
+template <class R> V3<R> normal_at(const Scene &s, int v, int leaf, int face, V3<R> x, V3<R> d, R t) {
+  V3<R> xl, dl;
+  to_local(s, v, x, d, xl, dl);
+  const Leaf &L = s.kinds[s.kind[v]].leaves[leaf];
+  V3<R> nl;
+  if (face == 0) {
+    const V3<R> y = (xl + t * dl) - v3<R>(R(L.q[0]), R(L.q[1]), R(L.q[2]));
+    nl = v3<R>(R(L.A[0]) * y.x + R(L.A[1]) * y.y + R(L.A[2]) * y.z + R(L.b[0]),
+               R(L.A[3]) * y.x + R(L.A[4]) * y.y + R(L.A[5]) * y.z + R(L.b[1]),
+               R(L.A[6]) * y.x + R(L.A[7]) * y.y + R(L.A[8]) * y.z + R(L.b[2]));
+  } else {
+    nl = v3<R>(R(L.n[3 * (face - 1)]), R(L.n[3 * (face - 1) + 1]), R(L.n[3 * (face - 1) + 2]));
+  }
+  V3<R> n = unit(to_world_dir(s, v, nl));
+  if (dot(n, d) > R(0.0)) n = R(-1.0) * n;
+  return n;
+}
+
+// the next visible crossing: the own exit, then every daughter (a daughter wins a tie)
+template <class R> Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, long long last, R tmax) {
+  Cross<R> best{tmax, -1, -1, v3<R>(R(0.0), R(0.0), R(1.0))};
+  int bv = -1, bl = -1, bf = -1;
+  bool own = false;
+  if (vol < s.n) {
+    V3<R> xl, dl;
+    to_local(s, vol, x, d, xl, dl);
+    const Change<R> ch = first_change(s, vol, xl, dl, true, last);
+    if (ch.leaf >= 0 && ch.t <= best.t) {
+      best.t = ch.t; best.vol_after = s.parent[vol]; bv = vol; bl = ch.leaf; bf = ch.face; own = true;
+    }
+  }
+  for (int u : s.kids[vol]) {
+    V3<R> xl, dl;
+    to_local(s, u, x, d, xl, dl);
+    const Change<R> ch = first_change(s, u, xl, dl, false, last);
+    if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && own))) {
+      best.t = ch.t; best.vol_after = u; bv = u; bl = ch.leaf; bf = ch.face; own = false;
+    }
+  }
+  if (bv >= 0) {
+    best.srf = ((long long)bv << 10) | (bl << 5) | bf;
+    best.n = normal_at(s, bv, bl, bf, x, d, best.t);
+  }
+  return best;
+}
+
+inline bool inside(const Scene &s, int v, V3<double> x) {
+  V3<double> xl, dl;
+  to_local<double>(s, v, x, v3<double>(0, 0, 1), xl, dl);
+  const Kind &K = s.kinds[s.kind[v]];
+  std::vector<bool> in(K.leaves.size());
+  for (size_t l = 0; l < K.leaves.size(); ++l) {
+    const Leaf &L = K.leaves[l];
+    const double y[3] = {xl.x - L.q[0], xl.y - L.q[1], xl.z - L.q[2]};
+    bool b = true;
+    for (size_t p = 0; p < L.h.size(); ++p)
+      b = b && (L.n[3 * p] * y[0] + L.n[3 * p + 1] * y[1] + L.n[3 * p + 2] * y[2] <= L.h[p]);
+    if (L.typ == 1) {
+      double q = L.c;
+      for (int i = 0; i < 3; ++i) {
+        q += 2.0 * L.b[i] * y[i];
+        for (int j = 0; j < 3; ++j) q += y[i] * L.A[3 * i + j] * y[j];
+      }
+      b = b && q <= 0.0;
+    }
+    in[l] = b;
+  }
+  return run_prog(K, in);
+}
+
+inline int locate(const Scene &s, V3<double> x) {
+  int cur = s.n;
+  for (;;) {
+    int found = -1;
+    for (int u : s.kids[cur])
+      if (inside(s, u, x)) { found = u; break; }
+    if (found < 0) return cur;
+    cur = found;
+  }
+}
+
+// ------------------------------------------------------------------------------------------ optics (FP64)
+struct Optics {
+  const int32_t *L;
+  const float *P;
+  double tab(int off, int n, double x, int *i_out = nullptr, double *f_out = nullptr) const {
+    int i = 0;
+    double f = 0.0;
+    if (n > 1 && x > P[off]) {
+      if (x >= P[off + n - 1]) {
+        i = n - 1;
+      } else {
+        while (i < n - 2 && P[off + i + 1] <= x) ++i;
+        f = (x - P[off + i]) / ((double)P[off + i + 1] - P[off + i]);
+      }
+    }
+    if (i_out) *i_out = i, *f_out = f;
+    const double v0 = P[off + n + i];
+    return f == 0.0 ? v0 : v0 + f * ((double)P[off + n + i + 1] - v0);
+  }
+  int cls_medium(int c) const { return c < 0 ? -1 : L[L[7] + c]; }
+  int row_of(int a, int b) const { return L[L[8] + a * L[2] + b]; }
+  const int32_t *medium(int m) const { return L + L[6] + 9 * m; }
+  const int32_t *row(int r) const { return L + L[9] + 18 * r; }
+};
+
+inline double rate(double len) { return len >= TINY_NORMAL ? 1.0 / len : 1.0 / TINY_NORMAL; }
+
+template <class R> struct Med {
+  R n, ng, mu_abs, mu_ray;
+};
+
+template <class R> Med<R> medium_at(const Optics &o, int m, double lam) {
+  Med<R> r{R(1.0), R(1.0), R(0.0), R(0.0)};
+  if (m < 0) return r;
+  const int32_t *M = o.medium(m);
+  if (M[8] & 1) return r;
+  const double E = HC / lam;
+  r.n = R(o.tab(M[0], M[1], E));
+  int i;
+  double f;
+  o.tab(M[2], M[3], E, &i, &f);  // the group velocity (1 / n_g) linear in energy between GROUPVEL knots
+  const R g0 = R(1.0) / R(o.P[M[2] + M[3] + i]);
+  const R inv = f == 0.0 ? g0 : g0 + R(f) * (R(1.0) / R(o.P[M[2] + M[3] + i + 1]) - g0);
+  r.ng = R(1.0) / inv;
+  if (M[5] > 0) r.mu_abs = R(rate(o.tab(M[4], M[5], E)));
+  if (M[7] > 0) r.mu_ray = R(rate(o.tab(M[6], M[7], E)));
+  return r;
+}
+
+inline double index_at(const Optics &o, int m, double lam) {
+  if (m < 0) return 1.0;
+  const int32_t *M = o.medium(m);
+  if (M[8] & 1) return 1.0;
+  return o.tab(M[0], M[1], HC / lam);
+}
+
+template <class R> void basis(V3<R> n, V3<R> &t1, V3<R> &t2) {
+  const R sign = n.z >= R(0.0) ? R(1.0) : R(-1.0);
+  const R a = R(-1.0) / (sign + n.z);
+  const R b = n.x * n.y * a;
+  t1 = v3<R>(R(1.0) + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  t2 = v3<R>(b, sign + n.y * n.y * a, -n.y);
+}
+template <class R> V3<R> to_world(V3<R> l, V3<R> n) {
+  V3<R> t1, t2;
+  basis(n, t1, t2);
+  return l.x * t1 + l.y * t2 + l.z * n;
+}
+
+struct C2 {  // complex in double (Fresnel, Jones)
+  double re, im;
+};
+inline C2 cm(C2 a, C2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline C2 cd(C2 a, C2 b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+inline double ca2(C2 a) { return a.re * a.re + a.im * a.im; }
+
+struct Fres {
+  C2 rs, rp, ts, tp;
+};
+inline Fres fresnel_amps(double c, double n1, double n2) {
+  c = std::min(std::max(c, 0.0), 1.0);
+  const double z = n2 * n2 - n1 * n1 * (1 - c * c);
+  C2 q = z >= 0 ? C2{std::sqrt(z), 0.0} : C2{0.0, std::sqrt(-z)};  // Im q >= 0: the decaying wave
+  if (std::fabs(z) < 1e-24) q = {0, 0};
+  const C2 ds{n1 * c + q.re, q.im}, dp{n2 * n2 * c + n1 * q.re, n1 * q.im};
+  Fres f;
+  const bool oks = std::sqrt(ca2(ds)) > 1e-12, okp = std::sqrt(ca2(dp)) > 1e-12;
+  f.rs = oks ? cd({n1 * c - q.re, -q.im}, ds) : C2{-1, 0};
+  f.rp = okp ? cd({n2 * n2 * c - n1 * q.re, -n1 * q.im}, dp) : C2{1, 0};
+  f.ts = oks ? cd({2 * n1 * c, 0}, ds) : C2{0, 0};
+  f.tp = okp ? cd({2 * n1 * n2 * c, 0}, dp) : C2{0, 0};
+  return f;
+}
+
+template <class R> struct Surf {
+  R p[NOUT];
+  R detect, sigma;
+  Fres amps;
+  int flags;
+};
+
+template <class R> V3<R> s_axis(V3<R> wi, V3<R> n) {
+  const V3<R> c = cross(wi, n);
+  if (dot(c, c) > R(1e-28)) return unit(c);
+  V3<R> t1, t2;
+  basis(wi, t1, t2);
+  return t1;
+}
+
+template <class R>
+Surf<R> surface_eval(const Optics &o, int row, double lam, V3<R> wi, V3<R> n, double n1, double n2, V3<R> er, V3<R> ei) {
+  Surf<R> s;
+  const int32_t *Rw = o.row(row);
+  s.flags = Rw[1];
+  const double E = HC / lam;
+  auto col = [&](int c) { return R(o.tab(Rw[2 + 2 * c], Rw[3 + 2 * c], E)); };
+  const R c1 = CLIP01<R>(col(COL_R)), c2 = CLIP01<R>(col(COL_R) + col(COL_T));
+  const R r = c1, t = c2 - c1, a = R(1.0) - c2;
+  s.sigma = col(COL_SIGMA);
+  for (int k = 0; k < NOUT; ++k) s.p[k] = R(0.0);
+  s.amps = Fres{{-1, 0}, {1, 0}, {1, 0}, {1, 0}};
+  if (Rw[0] == 0) {
+    const R ps = col(COL_SPIKE), pl = col(COL_LOBE), pb = col(COL_BACK);
+    const R k1 = CLIP01<R>(ps), k2 = CLIP01<R>(ps + pl), k3 = CLIP01<R>(ps + pl + pb);
+    const bool normal = Rw[1] & 1;
+    s.p[SPECULAR] = r * (normal ? k2 : k1);
+    s.p[GLOSSY] = normal ? R(0.0) : r * (k2 - k1);
+    s.p[BACKSCATTER] = r * (k3 - k2);
+    s.p[DIFFUSE] = r * (R(1.0) - k3);
+  } else {
+    s.amps = fresnel_amps(val(-dot(wi, n)), n1, n2);
+    const V3<R> sx = s_axis(wi, n), px = cross(wi, sx);
+    const R es2 = dot(er, sx) * dot(er, sx) + dot(ei, sx) * dot(ei, sx);
+    const R ep2 = dot(er, px) * dot(er, px) + dot(ei, px) * dot(ei, px);
+    const R F = CLIP01<R>(R(ca2(s.amps.rs)) * es2 + R(ca2(s.amps.rp)) * ep2);
+    s.p[SPECULAR] = r * F;
+    s.p[REFRACT] = r * (R(1.0) - F);
+  }
+  s.p[STRAIGHT] = t;
+  if (Rw[1] & 2) s.p[STRAIGHT] = s.p[REFRACT] = R(0.0);
+  s.detect = a * col(COL_EFF);
+  return s;
+}
+
+template <class R> int choose(const R *q, int n, double u) {
+  R tot = R(0.0);
+  for (int k = 0; k < n; ++k) tot += q[k];
+  if (!(tot > R(0.0))) return -1;
+  R c = R(0.0);
+  for (int k = 0; k < n; ++k) {
+    c += q[k];
+    if (c > R(u) * tot && q[k] > R(0.0)) return k;
+  }
+  for (int k = n - 1; k >= 0; --k)
+    if (q[k] > R(0.0)) return k;
+  return -1;
+}
+
+template <class R> void pol_normalise(V3<R> &er, V3<R> &ei) {
+  const R s = dot(er, er) + dot(ei, ei);
+  if (s > R(FLOOR32)) {
+    const R r = R(1.0) / SQRT(s);
+    er = r * er;
+    ei = r * ei;
+  } else {
+    er = ei = v3<R>(R(0.0), R(0.0), R(0.0));
+  }
+}
+template <class R> void pol_mirror(V3<R> &er, V3<R> &ei, V3<R> h) {
+  er = R(2.0) * dot(er, h) * h - er;
+  ei = R(2.0) * dot(ei, h) * h - ei;
+}
+template <class R> void pol_recombine(V3<R> &er, V3<R> &ei, V3<R> wi, V3<R> s, C2 as, C2 ap, V3<R> k) {
+  const V3<R> pin = cross(wi, s);
+  const C2 es{val(dot(er, s)), val(dot(ei, s))}, ep{val(dot(er, pin)), val(dot(ei, pin))};
+  const C2 cs = cm(as, es), cp = cm(ap, ep);
+  const V3<R> p = cross(k, s);
+  er = R(cs.re) * s + R(cp.re) * p;
+  ei = R(cs.im) * s + R(cp.im) * p;
+  pol_normalise(er, ei);
+}
+
+// ------------------------------------------------------------------------------------------ the run
+struct Run {
+  const Scene *s;
+  Optics live, ref;
+  bool has_ref;
+  const lh_source *src;
+  int nsrc;
+  std::vector<long long> ends;
+  uint32_t k0, k1;
+  double roulette, max_dist;
+  int max_vert;
+  int S, K;
+  long long stride;
+};
+
+struct Tally {
+  std::vector<double> t;   // n_events * stride
+  double h[LH_HEALTH_N] = {0};
+};
+
+template <class R> struct Photon {
+  V3<R> x, d, er, ei;
+  R t, w, wr;
+  double lam, w0;
+  int vol, cls, nz, vertex, event;
+  long long srf;
+  Rng rng;
+};
+
+template <class R> V3<R> facet_path(const Photon<R> &p, double sigma, V3<R> wi, V3<R> n, V3<R> &er, V3<R> &ei) {
+  V3<R> t1, t2;
+  basis(n, t1, t2);
+  const V3<R> mwi = R(-1.0) * wi;
+  V3<R> V = v3<R>(dot(mwi, t1), dot(mwi, t2), dot(mwi, n));
+  V3<R> w = wi;
+  const double fmax = std::min(1.0, 4.0 * sigma);
+  auto arc = [&](R a) {
+    const R sv = SQRT(V.x * V.x + V.y * V.y);
+    const R ratio = V.z * COS(a) / MAX<R>(sv * SIN(a), R(FLOOR32));
+    return ACOS(-MIN<R>(ratio, R(1.0)));
+  };
+  for (int k = 0; k < 256; ++k) {
+    R a = R(0.0);
+    double u2 = 0.5;
+    for (int t = 0; t < 4096; ++t) {
+      double u[4];
+      p.rng.draw4(p.vertex, FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
+      const R al = R(sigma) * SQRT(R(-2.0) * LOG(R(u[0])));
+      if (al < R(HALF_PI) && R(u[1]) * al * R(PI) < MIN<R>(SIN(al), R(fmax)) * arc(al)) {
+        a = al;
+        u2 = u[2];
+        break;
+      }
+    }
+    const R psi = (R(2.0 * u2) - R(1.0)) * arc(a);
+    const R ph = ATAN2(V.y, V.x) + psi;
+    const V3<R> m = v3<R>(SIN(a) * COS(ph), SIN(a) * SIN(ph), COS(a));
+    const V3<R> Lo = R(2.0) * dot(V, m) * m - V;
+    const V3<R> wo = to_world(Lo, n);
+    pol_mirror(er, ei, unit(wo - w));
+    w = wo;
+    V = R(-1.0) * Lo;
+    if (Lo.z >= R(0.0)) break;
+  }
+  return w;
+}
+
+template <class R> struct Moments {  // a photon's deposits summed per sensor (exact, unbounded)
+  std::vector<std::pair<int, double>> m;
+  void add(int s, double v) {
+    for (auto &e : m)
+      if (e.first == s) { e.second += v; return; }
+    m.push_back({s, v});
+  }
+};
+
+template <class R> bool emit(const Run &r, long long gi, Photon<R> &p, Tally &T) {
+  const int j = (int)(std::upper_bound(r.ends.begin(), r.ends.end(), gi) - r.ends.begin());
+  const lh_source &S = r.src[j];
+  const long long local = gi - (j > 0 ? r.ends[j - 1] : 0);
+  const unsigned long long id = (unsigned long long)(S.first_id + local);
+  p.rng = Rng{r.k0, r.k1, (uint32_t)id, (uint32_t)(id >> 32)};
+  double u[4];
+  p.rng.draw4(SOURCE_VERTEX, 0, u);
+  p.x = v3<R>(R(S.pos[0]), R(S.pos[1]), R(S.pos[2]));
+  p.t = R(S.time);
+  p.w0 = S.weight;
+  p.er = p.ei = v3<R>(R(0.0), R(0.0), R(0.0));
+  p.event = S.event;
+  if (S.kind == LH_SRC_CHERENKOV) {
+    const int32_t *M = r.live.medium(S.medium);
+    const double e_lo = r.live.P[M[0]], e_hi = r.live.P[M[0] + M[1] - 1];
+    const double lam_min = HC / e_hi, lam_max = HC / e_lo, zq = 1.0 / lam_min - 1.0 / lam_max;
+    p.lam = 1.0 / (1.0 / lam_min - u[0] * zq);
+    const double n = r.live.tab(M[0], M[1], HC / p.lam), bn = S.beta * n;
+    const bool em = bn > 1.0;
+    const double s2 = em ? 1.0 - 1.0 / (bn * bn) : 0.0;
+    p.w0 = TWO_PI * FINE_STRUCTURE * S.charge * S.charge * S.length * s2 * zq * 1e9 * S.weight;
+    const double cc = em ? 1.0 / bn : 1.0, sc = std::sqrt(std::max(1.0 - cc * cc, 0.0));
+    const V3<R> axis = v3<R>(R(S.dir[0]), R(S.dir[1]), R(S.dir[2]));
+    p.d = unit(to_world(v3<R>(R(sc * std::cos(TWO_PI * u[2])), R(sc * std::sin(TWO_PI * u[2])), R(cc)), axis));
+    const double xs = u[1] * S.length;
+    p.x = p.x + R(xs) * axis;
+    p.t = R(S.time + xs / (S.beta * C_VACUUM));
+    p.er = unit(dot(axis, p.d) * p.d - axis);
+  } else {
+    p.lam = S.lam_hi > S.lam ? S.lam + u[2] * (S.lam_hi - S.lam) : S.lam;
+    if (S.kind == LH_SRC_BEAM) {
+      p.d = unit(v3<R>(R(S.dir[0]), R(S.dir[1]), R(S.dir[2])));
+    } else {
+      const double c = 1.0 - 2.0 * u[0], sn = std::sqrt(std::max(1.0 - c * c, 0.0));
+      p.d = v3<R>(R(sn * std::cos(TWO_PI * u[1])), R(sn * std::sin(TWO_PI * u[1])), R(c));
+    }
+  }
+  p.w = p.wr = R(1.0);
+  p.vol = locate(*r.s, v3<double>(val(p.x.x), val(p.x.y), val(p.x.z)));
+  p.cls = r.s->cls[p.vol];
+  p.srf = -1;
+  p.nz = 0;
+  p.vertex = 0;
+  T.h[LH_H_PHOTONS] += 1;
+  T.h[LH_H_START_WEIGHT] += p.w0;
+  if (p.vol >= r.s->n || r.live.cls_medium(p.cls) < 0) {
+    T.h[LH_H_LOST] += p.w0;
+    return false;
+  }
+  return true;
+}
+
+inline double ratio(double a, double b) { return b != 0.0 ? a / b : 0.0; }
+template <class R> R ratio(R a, R b) { return b != R(0.0) ? a / b : R(0.0); }
+
+template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &mom) {
+  const Scene &s = *r.s;
+  if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {
+    if (p.nz == ZERO_STEPS_BEFORE_PUSH) T.h[LH_H_PUSHED] += val(p.w) * p.w0;
+    p.x = p.x + R(PUSH) * p.d;
+    p.vol = locate(s, v3<double>(val(p.x.x), val(p.x.y), val(p.x.z)));
+    p.cls = s.cls[p.vol];
+    p.srf = -1;
+  }
+  double u[4], ul[4];
+  p.rng.draw4(p.vertex, 0, u);
+  p.rng.draw4(p.vertex, 1, ul);
+  V3<R> er0 = p.er, ei0 = p.ei;
+  if (val(dot(er0, er0) + dot(ei0, ei0)) < 0.5) {  // unpolarised: a random linear state (polar.py `fill`)
+    V3<R> t1, t2;
+    basis(p.d, t1, t2);
+    er0 = R(std::cos(TWO_PI * ul[3])) * t1 + R(std::sin(TWO_PI * ul[3])) * t2;
+    ei0 = v3<R>(R(0.0), R(0.0), R(0.0));
+  }
+  const int m = r.live.cls_medium(p.cls);
+  const Med<R> ml = medium_at<R>(r.live, m, p.lam);
+  const Med<R> mr = r.has_ref ? medium_at<R>(r.ref, m, p.lam) : ml;
+  const R mu = ml.mu_ray, mu_r = mr.mu_ray;
+  const R s_free = -LOG(R(u[0])) / MAX<R>(mu_r, R(FLOOR32));
+  const Cross<R> cr = query<R>(s, p.x, p.d, p.vol, p.srf, R(r.max_dist));
+  const bool hit = cr.vol_after >= 0;
+  const R dist = cr.t;
+  const bool zero = dist <= R(ZERO_STEP);
+  const int cb = p.cls, ca = hit ? s.cls[cr.vol_after] : -1;
+  int row = hit ? r.live.row_of(cb, ca) : -1;
+  const bool cont = hit && zero && row < 0;
+  if (cont) row = 0;
+  const bool scatter = mu_r > R(0.0) && s_free < dist;
+  const R reach_live = EXP(-mu * dist), reach_ref = r.has_ref ? EXP(-mu_r * dist) : reach_live;
+  const R att = EXP(-ml.mu_abs * dist), att_r = r.has_ref ? EXP(-mr.mu_abs * dist) : att;
+  const R v_g = R(C_VACUUM) / ml.ng;
+  Surf<R> sl{}, sr{};
+  R pdraw[NOUT], pprop[NOUT];
+  int o = -1;
+  const bool at_if = hit && row >= 0;
+  double *tal = &T.t[(size_t)p.event * r.stride];
+  if (at_if) {
+    const int m2 = r.live.cls_medium(ca);
+    sl = surface_eval<R>(r.live, row, p.lam, p.d, cr.n, val(ml.n), index_at(r.live, m2, p.lam), er0, ei0);
+    sr = r.has_ref ? surface_eval<R>(r.ref, row, p.lam, p.d, cr.n, val(mr.n), index_at(r.ref, m2, p.lam), er0, ei0)
+                   : sl;
+    if (cont) {
+      for (int k = 0; k < NOUT; ++k) sl.p[k] = sr.p[k] = R(0.0);
+      sl.p[STRAIGHT] = sr.p[STRAIGHT] = R(1.0);
+      sl.detect = sr.detect = R(0.0);
+    }
+    const int support = (sr.flags >> 8) & 63;
+    const bool edge = (sr.flags & 4) && !cont;
+    int ncov = 0;
+    for (int k = 0; k < NOUT; ++k) {
+      pdraw[k] = sr.p[k];
+      if (edge && ((support >> k) & 1) && !(sr.p[k] > R(0.0))) ++ncov;
+    }
+    if (ncov)
+      for (int k = 0; k < NOUT; ++k)
+        if (((support >> k) & 1) && !(sr.p[k] > R(0.0))) pdraw[k] = R(DEFENSIVE / (1 - DEFENSIVE) / ncov);
+    R tot = R(0.0);
+    for (int k = 0; k < NOUT; ++k) tot += pdraw[k];
+    for (int k = 0; k < NOUT; ++k) pprop[k] = tot > R(0.0) ? pdraw[k] / tot : R(0.0);
+    o = choose<R>(pprop, NOUT, u[2]);
+    const int sensor = s.sensor[cr.vol_after];
+    if (sensor >= 0 && !cont) {
+      const double q = val(p.w * reach_live * att * sl.detect), v = q * p.w0;
+      const double time = val(p.t + dist / v_g);
+      T.h[LH_H_RECORDS] += 1;
+      if (v != 0.0) {
+        tal[sensor] += v;
+        tal[r.S + sensor] += v * time;
+        if (r.K <= 1) {
+          tal[3 * r.S + sensor] += v;
+        } else {
+          int i;
+          double f;
+          r.live.tab(r.live.L[10], r.K, p.lam, &i, &f);
+          tal[3 * r.S + sensor * r.K + i] += v * (1 - f);
+          if (f > 0) tal[3 * r.S + sensor * r.K + i + 1] += v * f;
+        }
+        mom.add(sensor, v);
+      }
+    }
+  }
+  bool ok;
+  R w_new = R(0.0), wr_new = R(0.0);
+  if (scatter) {
+    const R w_path = ratio<R>(mu * EXP(-mu * s_free), mu_r * EXP(-mu_r * s_free));
+    const R att_s = EXP(-ml.mu_abs * s_free);
+    const R w_at = p.w * w_path;
+    w_new = w_at * att_s;
+    wr_new = p.wr * (r.has_ref ? EXP(-mr.mu_abs * s_free) : att_s);
+    T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s)) * p.w0;
+    // Rayleigh (phase.py `rayleigh_sample`): the dipole axis er w.p. |er|^2, c from 3/4 (1 - c^2)
+    const R a2 = dot(er0, er0), n2 = a2 + dot(ei0, ei0);
+    const V3<R> axis = unit(R(ul[2]) * n2 < a2 ? er0 : ei0);
+    const R c = MIN<R>(MAX<R>(R(2.0) * COS((ACOS(MIN<R>(MAX<R>(R(1.0 - 2.0 * ul[0]), R(-1.0)), R(1.0))) - R(TWO_PI)) / R(3.0)),
+                              R(-1.0)), R(1.0));
+    const R sn = SQRT(MAX<R>(R(1.0) - c * c, R(0.0)));
+    const V3<R> dn = unit(to_world(v3<R>(sn * COS(R(TWO_PI * ul[1])), sn * SIN(R(TWO_PI * ul[1])), c), axis));
+    er0 = er0 - dot(dn, er0) * dn;
+    ei0 = ei0 - dot(dn, ei0) * dn;
+    pol_normalise(er0, ei0);
+    p.x = p.x + s_free * p.d;
+    p.t = p.t + s_free / v_g;
+    p.d = dn;
+    p.er = er0;
+    p.ei = ei0;
+    if (!(s_free <= R(ZERO_STEP) && p.srf >= 0)) p.srf = -1;
+    ok = true;
+  } else if (hit) {
+    const R w_at = p.w * ratio<R>(reach_live, reach_ref);
+    R s_cont = R(0.0);
+    if (at_if)
+      for (int k = 0; k < NOUT; ++k) s_cont += sl.p[k];
+    T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att)) * p.w0;
+    if (at_if) T.h[LH_H_ABSORBED_SURFACE] += val(w_at * att * (R(1.0) - s_cont)) * p.w0;
+    else T.h[LH_H_LOST] += val(w_at * att) * p.w0;
+    ok = at_if && o >= 0;
+    if (ok) {
+      const R w_if = w_at * att * ratio<R>(sl.p[o], pprop[o]);
+      wr_new = p.wr * (reach_ref != R(0.0) ? R(1.0) : R(0.0)) * att_r * ratio<R>(pdraw[o], pprop[o]);
+      const V3<R> wi = p.d, n = cr.n;
+      V3<R> dn = wi;
+      bool far = false;
+      if (o == STRAIGHT) {
+        far = true;
+      } else if (o == REFRACT) {
+        const double eta = val(ml.n) / index_at(r.live, r.live.cls_medium(ca), p.lam);
+        const R ci = -dot(wi, n), s2t = R(eta * eta) * (R(1.0) - ci * ci);
+        if (s2t >= R(1.0)) dn = wi + R(2.0) * ci * n;
+        else dn = unit(R(eta) * wi + (R(eta) * ci - SQRT(MAX<R>(R(1.0) - s2t, R(0.0)))) * n);
+        pol_recombine(er0, ei0, wi, s_axis(wi, n), sl.amps.ts, sl.amps.tp, unit(dn));
+        far = true;
+      } else if (o == SPECULAR) {
+        dn = wi - R(2.0) * dot(wi, n) * n;
+        pol_recombine(er0, ei0, wi, s_axis(wi, n), sl.amps.rs, sl.amps.rp, unit(dn));
+      } else if (o == BACKSCATTER) {
+        dn = R(-1.0) * wi;
+        er0 = R(-1.0) * er0;
+        ei0 = R(-1.0) * ei0;
+      } else if (o == GLOSSY) {
+        dn = facet_path(p, val(sr.sigma), wi, n, er0, ei0);
+      } else {
+        const R rr = SQRT(R(ul[0]));
+        dn = to_world(v3<R>(rr * COS(R(TWO_PI * ul[1])), rr * SIN(R(TWO_PI * ul[1])), SQRT(MAX<R>(R(1.0 - ul[0]), R(0.0)))), n);
+        pol_mirror(er0, ei0, unit(dn - wi));
+      }
+      w_new = w_if;
+      T.h[LH_H_REWEIGHTED] += val(w_at * att * s_cont - w_new) * p.w0;
+      p.x = p.x + dist * p.d;
+      p.t = p.t + dist / v_g;
+      p.d = unit(dn);
+      p.er = er0;
+      p.ei = ei0;
+      p.srf = cr.srf;
+      if (far) {
+        p.vol = cr.vol_after;
+        p.cls = ca;
+      }
+    }
+  } else {
+    const R d_end = MIN<R>(dist, R(r.max_dist));
+    const R w_far = p.w * ratio<R>(EXP(-mu * d_end), EXP(-mu_r * d_end));
+    const R att_far = EXP(-ml.mu_abs * d_end);
+    T.h[r.max_dist < r.s->reach ? LH_H_DISTANCE_CUT : LH_H_ESCAPED] += val(w_far * att_far) * p.w0;
+    T.h[LH_H_ABSORBED_BULK] += val(w_far * (R(1.0) - att_far)) * p.w0;
+    ok = false;
+  }
+  bool alive = ok;
+  if (ok && wr_new < R(r.roulette)) {
+    const R ps = wr_new / R(r.roulette);
+    if (R(u[3]) < ps) {
+      const R wk = w_new / ps;
+      T.h[LH_H_ROULETTED] += val(w_new - wk) * p.w0;
+      w_new = wk;
+      wr_new = wr_new / ps;
+    } else {
+      T.h[LH_H_ROULETTED] += val(w_new) * p.w0;
+      w_new = wr_new = R(0.0);
+      alive = false;
+    }
+  }
+  p.nz = (hit && !scatter && zero) ? p.nz + 1 : 0;
+  p.w = w_new;
+  p.wr = wr_new;
+  p.vertex += 1;
+  T.h[LH_H_VERTICES] += 1;
+  if (alive && p.nz >= ZERO_STEPS_BEFORE_KILL) {
+    T.h[LH_H_ABANDONED] += val(p.w) * p.w0;
+    alive = false;
+  }
+  if (alive && p.vertex >= r.max_vert) {
+    T.h[LH_H_TRUNCATED] += val(p.w) * p.w0;
+    alive = false;
+  }
+  return alive;
+}
+
+template <class R> void trace_range(const Run &r, long long a, long long b, Tally &T) {
+  for (long long gi = a; gi < b; ++gi) {
+    Photon<R> p;
+    Moments<R> mom;
+    bool alive = emit<R>(r, gi, p, T);
+    while (alive) alive = step<R>(r, p, T, mom);
+    double *tal = &T.t[(size_t)p.event * r.stride];
+    for (auto &e : mom.m) tal[2 * r.S + e.first] += e.second * e.second;
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *values, const float *ref,
+                    const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl) {
+  r.s = &s;
+  r.live = Optics{layout, values};
+  r.ref = Optics{layout, ref ? ref : values};
+  r.has_ref = ref != nullptr && ref != values;
+  r.src = src;
+  r.nsrc = nsrc;
+  long long acc = 0;
+  r.ends.resize(nsrc);
+  for (int i = 0; i < nsrc; ++i) r.ends[i] = (acc += src[i].n);
+  r.k0 = (uint32_t)seed;
+  r.k1 = (uint32_t)(seed >> 32);
+  r.roulette = ctl ? ctl->roulette : 0.01;
+  r.max_dist = (ctl && ctl->max_distance > 0) ? ctl->max_distance : s.reach;
+  r.max_vert = (ctl && ctl->max_vertices > 0) ? ctl->max_vertices : (1 << 22);
+  r.S = layout[4];
+  r.K = layout[5];
+  r.stride = 3LL * r.S + (long long)r.S * r.K;
+  return 0;
+}
+
+// Trace every photon of the sources in FP64 on `threads` OpenMP threads; add into tally (n_events * stride) and
+// health (LH_HEALTH_N), both host arrays.
+ORC_API int oracle_trace(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
+                         const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally,
+                         int n_events, double *health, int threads) {
+  const Scene s = scene_of(G);
+  Run r;
+  make_run(r, s, layout, values, ref, src, nsrc, seed, ctl);
+  const long long total = r.ends.empty() ? 0 : r.ends.back();
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<Tally> parts;
+#pragma omp parallel
+  {
+#pragma omp single
+    parts.resize(omp_get_num_threads());
+    Tally &T = parts[omp_get_thread_num()];
+    T.t.assign((size_t)n_events * r.stride, 0.0);
+#pragma omp for schedule(dynamic, 1024)
+    for (long long gi = 0; gi < total; ++gi) trace_range<double>(r, gi, gi + 1, T);
+  }
+  for (auto &T : parts) {
+    for (size_t i = 0; i < T.t.size(); ++i) tally[i] += T.t[i];
+    for (int k = 0; k < LH_HEALTH_N; ++k) health[k] += T.h[k];
+  }
+  return 0;
+}
+
+// The algorithmic work of the first n photons: floating-point operations (Counted) and Philox integer operations,
+// and the vertices they took (single thread).
+ORC_API int oracle_count_ops(const lh_geometry *G, const int32_t *layout, const float *values, const lh_source *src,
+                             int nsrc, uint64_t seed, const lh_controls *ctl, long long n, double *out3) {
+  const Scene s = scene_of(G);
+  Run r;
+  make_run(r, s, layout, values, nullptr, src, nsrc, seed, ctl);
+  Tally T;
+  T.t.assign((size_t)64 * r.stride, 0.0);
+  const long long total = std::min(n, r.ends.empty() ? 0LL : r.ends.back());
+  g_ops = 0;
+  g_int_ops = 0;
+  for (long long gi = 0; gi < total; ++gi) {
+    Photon<Counted> p;
+    Moments<Counted> mom;
+    if (src[0].event >= 64) return 1;
+    bool alive = emit<Counted>(r, gi, p, T);
+    while (alive) alive = step<Counted>(r, p, T, mom);
+  }
+  out3[0] = (double)g_ops;
+  out3[1] = (double)g_int_ops;
+  out3[2] = T.h[LH_H_VERTICES];
+  return 0;
+}
+
+// The navigator referee: the next visible crossing of n rays (x, d: n*3; vol, last per ray), FP64.  Outputs t,
+// the volume entered, the normal (n*3).
+ORC_API int oracle_query(const lh_geometry *G, int n, const double *x, const double *d, const int32_t *vol,
+                         const int64_t *last, double tmax, double *t, int32_t *vol_after, double *normal,
+                         int64_t *srf) {
+  const Scene s = scene_of(G);
+  for (int i = 0; i < n; ++i) {
+    const Cross<double> c = query<double>(s, v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]),
+                                          unit(v3(d[3 * i], d[3 * i + 1], d[3 * i + 2])), vol[i], last[i], tmax);
+    t[i] = c.t;
+    vol_after[i] = c.vol_after;
+    normal[3 * i] = c.n.x, normal[3 * i + 1] = c.n.y, normal[3 * i + 2] = c.n.z;
+    srf[i] = c.srf;
+  }
+  return 0;
+}
+
+ORC_API int oracle_locate(const lh_geometry *G, int n, const double *x, int32_t *vol) {
+  const Scene s = scene_of(G);
+  for (int i = 0; i < n; ++i) vol[i] = locate(s, v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]));
+  return 0;
+}
+
+ORC_API void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// n facet paths off a flat surface (normal +z) at incidence wi with width sigma; outgoing directions (n*3)
+ORC_API void oracle_facet_paths(double sigma, const double *wi, long long n, uint64_t seed, double *out) {
+  for (long long i = 0; i < n; ++i) {
+    Photon<double> p;
+    p.rng = Rng{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)i, (uint32_t)(i >> 32)};
+    p.vertex = 0;
+    V3<double> er = v3(0.0, 1.0, 0.0), ei = v3(0.0, 0.0, 0.0);
+    const V3<double> w = facet_path(p, sigma, unit(v3(wi[0], wi[1], wi[2])), v3(0.0, 0.0, 1.0), er, ei);
+    out[3 * i] = w.x, out[3 * i + 1] = w.y, out[3 * i + 2] = w.z;
+  }
+}
+
+// Fresnel amplitudes (re, im of r_s, r_p, t_s, t_p) at incidence cosine c from n1 into n2
+ORC_API void oracle_fresnel(double c, double n1, double n2, double out8[8]) {
+  const Fres f = fresnel_amps(c, n1, n2);
+  const C2 a[4] = {f.rs, f.rp, f.ts, f.tp};
+  for (int i = 0; i < 4; ++i) out8[2 * i] = a[i].re, out8[2 * i + 1] = a[i].im;
+}
