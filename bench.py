@@ -1,0 +1,251 @@
+"""Benchmark: photons traced per second on one MI355X, the 1,000-PMT water tank with a 1 m Cherenkov track.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons 2**24] [--no-cpu] [--work-model]
+
+A step is one forward pass of the photon loop over one batch (2^24 photons, FP32 tracing, FP64 tallies) through
+the C ABI (``lh_trace``), the inputs resident on the device.  Prints one JSON line: the metric, whole-job photons/s,
+the roofline of the photon-loop kernel (the algorithmic FP32 work per photon from the oracle's op count, over the
+kernel's HIP-event time, against the MI355X FP32 vector peak) and the CPU baseline: the oracle (the FP64 C++/OpenMP
+restatement of the reference's loop) timed on this host's cores on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photons traced/s (fwd; fwd+adjoint) on 1 MI355X at fixed MC std-error"
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector/matrix peak (TFLOP/s)
+PEAK_HBM_GBS = 8000.0
+WORK_MODEL = os.path.join(ROOT, "profiles", "work_model.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "trace_kernel_water_tank_2p24_pmc.txt")
+
+
+def work_model(det, src, n=1 << 14):
+    """Return the algorithmic work per photon of this workload from the oracle's op-counting run (frozen in
+    profiles/work_model.json; recomputed with --work-model)."""
+    if os.path.exists(WORK_MODEL):
+        with open(WORK_MODEL) as f:
+            wm = json.load(f)
+        if "water_tank_cherenkov" in wm:
+            return wm["water_tank_cherenkov"]
+    from oracle import oracle as O
+
+    fp, ints, vert = O.OracleRun(det).count_ops(src, n, n, seed=0)
+    return dict(photons=n, fp_ops_per_photon=fp / n, int_ops_per_photon=ints / n, vertices_per_photon=vert / n,
+                fp_ops_per_vertex=fp / max(vert, 1))
+
+
+def pmc_traffic():
+    """Return HBM bytes per launch of the photon-loop kernel from the committed rocprofv3 counter summary, or None."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    vals = {}
+    with open(PMC_SUMMARY) as f:
+        for line in f:
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except ValueError:
+                    pass
+    if "hbm_bytes_per_launch" in vals:
+        return vals["hbm_bytes_per_launch"]
+    return None
+
+
+def cpu_baseline(det, src, seconds=12.0):
+    """Time the oracle (FP64, OpenMP on every host core) on a bounded sample of the workload."""
+    from oracle import oracle as O
+
+    threads = os.cpu_count() or 1
+    run = O.OracleRun(det)
+    n = 1 << 13
+    t0 = time.perf_counter()
+    run.trace(src, n, seed=99, threads=threads, bvh=True)
+    dt = time.perf_counter() - t0
+    n = int(min(1 << 24, max(n, n * seconds / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    run.trace(src, n, seed=100, threads=threads, bvh=True)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="photons/s", cores=threads, kind="port",
+                sample=f"{n} photons of the same workload (1,000-PMT tank, 1 m Cherenkov track), FP64 oracle with the "
+                       "reference's BVH walk, "
+                       f"{dt:.1f} s on {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--photons", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--steps-adjoint", type=int, default=3)
+    ap.add_argument("--batch-repeats", type=int, default=3)
+    ap.add_argument("--work-model", action="store_true", help="recompute and store the op-count work model")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run (the photon loop shards by photon "
+              "id ranges with one all-reduce of the FP64 tallies; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import lumen_amd as L
+    from lumen_amd.api import engine
+    from lumen_amd.detector import muon_track, water_tank
+
+    det = water_tank()
+    x0, d = muon_track()
+    src = L.sources.CherenkovSegment("Water", start=tuple(x0), direction=tuple(d), length=1.0)
+    if args.work_model:
+        if os.path.exists(WORK_MODEL):
+            os.remove(WORK_MODEL)
+        wm = work_model(det, src)
+        os.makedirs(os.path.dirname(WORK_MODEL), exist_ok=True)
+        with open(WORK_MODEL, "w") as f:
+            json.dump({"water_tank_cherenkov": wm, "source": "oracle op count (bench.py --work-model)"}, f, indent=1)
+    wm = work_model(det, src)
+    n = int(args.photons)
+    eng = engine(det)
+    dev = eng.device
+    for w in range(args.warmup):
+        L.simulate(det, src, samples=n, seed=w)
+    torch.cuda.synchronize(dev)
+    kernel_ms, rel_err = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        light = L.simulate(det, src, samples=n, seed=1000 + k)
+        kernel_ms.append(light.kernel_ms)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    a = light.arrivals.double()
+    rel_err = float(torch.sqrt(light.arrival2.sum()) / a.sum())
+    ms_step = 1e3 * wall / args.steps
+    value = n * args.steps / wall
+    k_ms = float(np.mean(kernel_ms))
+    fp_per_photon = wm["fp_ops_per_photon"]
+    achieved = n * fp_per_photon / (k_ms * 1e-3) / 1e12
+    traffic = pmc_traffic()
+    out = {
+        "metric": METRIC, "value": value, "unit": "photons/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "water_tank_1000pmt_cherenkov_fwd", "photons_per_step": n, "sensors": 1000,
+                   "source": "1.0 m beta=1 track, seed 2", "tracing": "f32", "tallies": "f64",
+                   "rel_std_error_total_arrivals": rel_err},
+        "kernel_ms": k_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "fp_ops_per_photon": fp_per_photon, "kernel": "lh::trace_kernel"},
+    }
+    if not args.no_adjoint:
+        out["fwd_adjoint"] = fwd_adjoint_leg(det, src, n, args.steps_adjoint)
+        out["value_fwd_adjoint"] = out["fwd_adjoint"]["value"]
+    if not args.no_batch:
+        out["batched"] = batched_leg(det, args.batch_repeats)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(det, src)
+    print(json.dumps(out), flush=True)
+
+
+C3_KEYS = ("materials/Water/ABSLENGTH", "materials/Water/RAYLEIGH", "surfaces/Liner/REFLECTIVITY",
+           "surfaces/Liner/sigma_alpha")
+
+
+def fwd_adjoint_leg(det, src, n, steps):
+    """Forward plus adjoint on the same tank: the Poisson NLL of pe against a target simulated at parameters
+    perturbed by N(0, 5 %) (seed 3); gradient over the 16 absorption knots, the 16 Rayleigh knots (the Rayleigh
+    scale's derivative is their dot product with the lengths), the liner's reflectivity and roughness, and the 1,000
+    PMT QEs (after tracking)."""  # noqa: D205, D209
+    import numpy as np
+    import torch
+
+    import lumen_amd as L
+    from lumen_amd.autograd import simulate_diff, table_tensor
+    from lumen_amd.params import Table
+
+    p0 = det.params()
+    rng = np.random.default_rng(3)
+    pt = p0
+    for k in C3_KEYS + ("sensors/pmt/qe",):
+        v = p0[k]
+        arr = v.value if isinstance(v, Table) else np.atleast_1d(v)
+        new = arr * (1.0 + 0.05 * rng.normal(size=arr.shape))
+        pt = pt.set(k, new if arr.size > 1 else float(new[0]))
+    qt = torch.as_tensor(np.asarray(pt["sensors/pmt/qe"], np.float64), device="cuda")
+    target = (L.simulate(det, src, pt, samples=n, seed=77).charge[:, 0] * qt).detach()
+    qe = torch.as_tensor(np.asarray(p0["sensors/pmt/qe"], np.float64), device="cuda", dtype=torch.float64)
+
+    def one(seed):
+        ts = {k: table_tensor(p0, k) for k in C3_KEYS}
+        q = qe.clone().requires_grad_(True)
+        light = simulate_diff(det, src, ts, samples=n, seed=seed)
+        pe = light.charge[:, 0] * q
+        loss = (pe - target * torch.log(torch.clamp(pe, min=1e-30))).sum()
+        loss.backward()
+        run = light.provenance["run"]
+        return run, ts, q
+
+    run, ts, q = one(500)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd_ms, adj_ms = [], []
+    for k in range(steps):
+        run, ts, q = one(600 + k)
+        fwd_ms.append(run.kernel_ms)
+        adj_ms.append(run.adjoint_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n_params = sum(int(t.numel()) for t in ts.values()) + int(q.numel())
+    t_step = wall / steps
+    return {"value": n / t_step, "unit": "photons/s", "ms_per_step": 1e3 * t_step, "photons_per_step": n,
+            "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
+            "parameters": n_params,
+            "speedup_vs_central_fd": (2 * n_params + 1) * float(np.mean(fwd_ms)) /
+                                     (float(np.mean(fwd_ms)) + float(np.mean(adj_ms))),
+            "loss": "Poisson NLL of pe vs a target at N(0, 5%)-perturbed parameters"}
+
+
+def batched_leg(det, repeats):
+    """256 point sources with uneven photon counts (2^12..2^20): one graph-captured persistent launch with the
+    autotuned work queue against one launch per source."""  # noqa: D205, D209
+    import numpy as np
+    import torch
+
+    from lumen_amd.batch import BatchRun, c4_batch
+
+    batch = c4_batch(det)
+    run = BatchRun(det, batch, seed=11)
+    run.forward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        run.forward()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / repeats
+    run.sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        run.sequential()
+    torch.cuda.synchronize()
+    t_seq = (time.perf_counter() - t0) / repeats
+    return {"sources": len(batch.sources), "photons": int(run.total), "batched_ms": 1e3 * t_batch,
+            "sequential_ms": 1e3 * t_seq, "speedup": t_seq / t_batch,
+            "photons_per_s_batched": run.total / t_batch,
+            "autotuned": {"chunk": run.controls.chunk, "blocks_per_cu": run.controls.blocks_per_cu}}
+
+
+if __name__ == "__main__":
+    main()

@@ -102,10 +102,12 @@ typedef struct {
 
 /* Run controls (src/lumen/transport/step.py:52-71 Controls, src/lumen/run.py:70 MAX_VERTICES). */
 typedef struct {
-  float roulette;       /* epsilon > 0 */
-  float max_distance;   /* <= 0: the geometry's reach */
-  int32_t max_vertices; /* the guard (steps per photon) */
-  int32_t chunk;        /* photons a wavefront takes from the work queue at once (0: the engine's default) */
+  float roulette;        /* epsilon > 0 */
+  float max_distance;    /* <= 0: the geometry's reach */
+  int32_t max_vertices;  /* the guard (steps per photon) */
+  int32_t chunk;         /* photons a wavefront takes from the work queue at once (0: the engine's default) */
+  int32_t blocks_per_cu; /* resident blocks of 256 per CU of the persistent grid (0: as many as fit) */
+  int32_t reserved;
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.
@@ -127,12 +129,38 @@ int lh_trace(lh_model model, const float *values_dev, const float *ref_values_de
 
 /* Reverse-mode derivative by replay: each photon is traced twice from its Philox counter (no stored path): once to
  * total its cotangent-weighted deposits, once to add each step's score and pathwise terms times the deposits still
- * ahead.  `cotangent_dev`: dLoss/d(tally) in the tally layout (FP64); `slots_dev`: the gradient slot of each packed
- * value (int32, -1 none); `grad_dev`: n_grad FP64 accumulators (added into).  Replaces the reference's jvp/vjp
+ * ahead.  `cotangent_dev`: dLoss/d(tally) in the tally layout (FP64); `slots_dev`: the gradient slot of each of the n_values packed
+ * value (int32, -1 none, each below n_grad <= 8192); `grad_dev`: n_grad FP64 accumulators (added into); the
+ * derivative is taken at `values_dev` with the draws at `ref_values_dev` (NULL: values_dev), the photons those of
+ * lh_trace with the same seed and sources.  The arrival_t and arrival2 cells carry no cotangent (no derivative).  Replaces the reference's jvp/vjp
  * through tallymap's differentiated kernel (src/lumen/api.py:748-787). */
-int lh_adjoint(lh_model model, const float *values_dev, const float *ref_values_dev, const int32_t *slots_dev,
-               const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
-               const double *cotangent_dev, double *grad_dev, int32_t n_grad, double *health_dev, void *stream);
+int lh_adjoint(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+               const int32_t *slots_dev, const lh_source *sources, int32_t n_sources, uint64_t seed,
+               const lh_controls *controls, const double *cotangent_dev, int32_t n_events, double *grad_dev,
+               int32_t n_grad, double *health_dev, void *stream);
+
+/* A launch split for graph capture: lh_stage uploads and checks the sources (it may synchronise); lh_launch and
+ * lh_launch_adjoint then only enqueue kernels on `stream` (no allocation, no host synchronisation), so a caller
+ * can capture them in a HIP graph and replay it with new values, seeds and output buffers.  lh_stage_slots checks
+ * the adjoint's slots once.  The staged sources have buffers of their own: lh_trace / lh_adjoint leave them as
+ * they are. */
+int lh_stage(lh_model model, const lh_source *sources, int32_t n_sources, int64_t *total_photons);
+int lh_stage_slots(lh_model model, const int32_t *slots_dev, int64_t n_values, int32_t n_grad);
+int lh_launch(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values, uint64_t seed,
+              const lh_controls *controls, double *tally_dev, int32_t n_events, double *health_dev, void *stream);
+int lh_launch_adjoint(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+                      const int32_t *slots_dev, uint64_t seed, const lh_controls *controls,
+                      const double *cotangent_dev, int32_t n_events, double *grad_dev, int32_t n_grad,
+                      double *health_dev, void *stream);
+
+/* The navigator on n rays (device arrays): from x (n*3) along d (n*3) in navigator id vol (-1: located here), the
+ * photon sitting on face `last` (-1: none), the next visible crossing within tmax (<= 0: the reach): its distance
+ * t, the navigator id entered (-1: none), the unit normal facing the ray (n*3), its surface code; `located` the
+ * navigator id holding x.  Replaces Navigator.query / locate (src/lumen/geometry/navigate.py:663-693, :226-241)
+ * as the reference's navigator tests call them (tests/unit/test_navigator.py:43-73). */
+int lh_query(lh_model model, int32_t n, const float *x_dev, const float *d_dev, const int32_t *vol_dev,
+             const int32_t *last_dev, float tmax, float *t_dev, int32_t *vol_after_dev, float *normal_dev,
+             int32_t *srf_dev, int32_t *located_dev, void *stream);
 
 /* The time of the last lh_trace / lh_adjoint's photon-loop kernel on its stream, in ms (HIP events). */
 int lh_last_kernel_ms(lh_model model, float *ms);

@@ -29,18 +29,19 @@ ROULETTE = 0.01
 class Controls(NamedTuple):
     """The step's controls (``transport/step.py:52-71``): ``max_distance`` (m; None: the geometry's reach),
     ``roulette`` (epsilon > 0), ``max_vertices`` (the guard), ``chunk`` (photons per work-queue claim; 0 the
-    engine's default)."""  # noqa: D205, D209
+    engine's default), ``blocks_per_cu`` (the persistent grid's resident blocks per CU; 0 as many as fit)."""  # noqa: D205, D209
 
     max_distance: Any = None
     roulette: float = ROULETTE
     max_vertices: int = MAX_VERTICES
     chunk: int = 0
+    blocks_per_cu: int = 0
 
     def struct(self) -> _lib.LhControls:
         if not (float(self.roulette) > 0.0):
             raise ValueError(f"Controls: the roulette is a number > 0, got {self.roulette!r}")
         return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
-                               int(self.max_vertices), int(self.chunk))
+                               int(self.max_vertices), int(self.chunk), int(self.blocks_per_cu), 0)
 
 
 class Health(NamedTuple):
@@ -125,6 +126,31 @@ class Engine:
         self.S, self.K, self.stride = S.value, K.value, stride.value
         weakref.finalize(self, _lib.lib().lh_model_destroy, h)
 
+    def query(self, x, d, vol=None, last=None, tmax=0.0):
+        """Return (t, vol_after, normal, srf, located) of rays x, d (n, 3) on the device navigator (float32)."""
+        import torch
+
+        dev = self.device
+        x = torch.as_tensor(np.asarray(x, np.float32).reshape(-1, 3), device=dev).contiguous()
+        d = torch.as_tensor(np.asarray(d, np.float32).reshape(-1, 3), device=dev).contiguous()
+        n = x.shape[0]
+        vol = torch.full((n,), -1, dtype=torch.int32, device=dev) if vol is None else \
+            torch.as_tensor(np.broadcast_to(np.asarray(vol, np.int32), (n,)).copy(), device=dev)
+        last = torch.full((n,), -1, dtype=torch.int32, device=dev) if last is None else \
+            torch.as_tensor(np.asarray(last, np.int32), device=dev)
+        t = torch.zeros(n, dtype=torch.float32, device=dev)
+        va = torch.zeros(n, dtype=torch.int32, device=dev)
+        nrm = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+        srf = torch.zeros(n, dtype=torch.int32, device=dev)
+        loc = torch.zeros(n, dtype=torch.int32, device=dev)
+        st = torch.cuda.current_stream(dev)
+        ptr = lambda a: _lib.C.c_void_p(a.data_ptr())  # noqa: E731
+        _lib.check(_lib.lib().lh_query(self.handle, n, ptr(x), ptr(d), ptr(vol), ptr(last), float(tmax), ptr(t),
+                                       ptr(va), ptr(nrm), ptr(srf), ptr(loc), _lib.C.c_void_p(st.cuda_stream)),
+                   "lh_query")
+        torch.cuda.synchronize(dev)
+        return t.cpu().numpy(), va.cpu().numpy(), nrm.cpu().numpy(), srf.cpu().numpy(), loc.cpu().numpy()
+
     def kernel_ms(self) -> float:
         ms = _lib.C.c_float()
         _lib.check(_lib.lib().lh_last_kernel_ms(self.handle, _lib.C.byref(ms)), "lh_last_kernel_ms")
@@ -180,26 +206,27 @@ def source_rows(det, source, samples, first_id=0):
 
 def simulate(det, source, params: Params | None = None, *, samples: int | None = None, seed: int = 0,
              reference: Params | None = None, controls: Controls | None = None, first_id: int = 0,
-             device="cuda", stream=None) -> Light:
+             edge_rows="structure", device="cuda", stream=None) -> Light:
     """Trace ``samples`` Monte Carlo photons of ``source`` through ``det`` and return the light at each sensor.
 
     ``params``: the values (default ``det.params()``); ``reference``: the values every random choice is made at
     (default: ``params``) -- held fixed while ``params`` moves, the light at a seed is smooth in ``params`` (common
     random numbers).  A ``Batch`` source runs every source in one launch, one event each: the outputs gain a leading
-    event axis.
+    event axis.  ``edge_rows``: the surface rows drawn with the defensive mixture ("structure": the reference's rule;
+    ``()``: none -- the same light in expectation, without the zero-weight photons an edge row sends on).
     """
     import torch
 
     eng = engine(det, device)
     controls = Controls() if controls is None else controls
     p = det.params() if params is None else params
-    pk = packing.pack(det, p)
+    pk = packing.pack(det, p, edge_rows=edge_rows)
     if not np.array_equal(pk.layout, eng.layout):
         raise ValueError("simulate: these values change the detector's structure (a table's knot count)")
     values = torch.as_tensor(pk.values, device=eng.device)
     ref_values = None
     if reference is not None:
-        pr = packing.pack(det, reference)
+        pr = packing.pack(det, reference, edge_rows=edge_rows)
         if not np.array_equal(pr.layout, eng.layout):
             raise ValueError("simulate: the reference's structure differs from the detector's")
         ref_values = torch.as_tensor(pr.values, device=eng.device)

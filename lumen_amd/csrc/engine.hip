@@ -19,9 +19,15 @@
 #include <vector>
 
 #include "../../include/lumen_hip.h"
+#include "adjoint.h"
 #include "nav.h"
 #include "optics.h"
 #include "philox.h"
+#include "unified.h"
+
+namespace lh_host {
+const std::vector<float> &unified_tables();
+}
 
 namespace lh {
 
@@ -35,6 +41,7 @@ constexpr int MAX_FACETS = 256;
 constexpr int MAX_FACET_ATTEMPTS = 4096;
 constexpr int BLOCK = 256;
 constexpr int DEFAULT_CHUNK = 256;
+constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 
 struct Args {
   Geo g;
@@ -54,6 +61,7 @@ struct Args {
   int S, K;
   double *__restrict__ health;
   unsigned long long *__restrict__ queue;
+  const float *__restrict__ ut;  // the unified facet tables (unified_tables.cpp)
 };
 
 struct Photon {
@@ -67,6 +75,13 @@ struct Photon {
   // the per-photon moment: up to MOMENT_SLOTS sensors and the photon's summed deposits at each
   int ms[MOMENT_SLOTS];
   float mv[MOMENT_SLOTS];
+  // the adjoint's walk: the launch index (to replay), the pass (0 totals, 1 accumulates), the photon's total of
+  // cotangent-weighted deposits T (from pass 0), their running sum, and the rates' derivative caches
+  long long gi;
+  int pass;
+  float T, Dsum;
+  bool hasdep;
+  RateD da, dr;
 };
 
 struct Acc {  // per-lane health accumulators, flushed once per launch
@@ -222,8 +237,13 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
 }
 
 // Geant4's unified facet path (facets.py:318-397): facets drawn for the current direction and mirrored about
-// until the direction leaves the surface; the polarisation mirrored about each facet's half vector
-__device__ inline float3 facet_path(const Photon &p, float sigma, float3 wi, float3 n, float3 &er, float3 &ei) {
+// until the direction leaves the surface; the polarisation mirrored about each facet's half vector.  The path is
+// drawn at the reference width `sigma`; with `ratio` its log density ratio live / reference is added to logw
+// (facets.py `facet_logpdf` per facet, surface_event.py:92-112), and with `deriv` the live log density's derivative
+// in the width to dlogw.
+__device__ inline float3 facet_path(const Photon &p, float sigma, float sigma_live, const float *__restrict__ T,
+                                    bool ratio, bool deriv, float3 wi, float3 n, float3 &er, float3 &ei, float &logw,
+                                    float &dlogw) {
   float3 V = to_local_frame(mul3(-1.0f, wi), n);
   float3 w = wi;
   const float fmax = fminf(1.0f, 4.0f * sigma);
@@ -245,6 +265,12 @@ __device__ inline float3 facet_path(const Photon &p, float sigma, float3 wi, flo
     __sincosf(a, &sa, &ca);
     __sincosf(ph, &sp, &cp);
     const float3 m = mk3(sa * cp, sa * sp, ca);
+    if (ratio || deriv) {
+      float dl, dr;
+      const float lp = unified_lp(T, sigma_live, V, m, dl);
+      if (ratio) logw += lp - unified_lp(T, sigma, V, m, dr);
+      if (deriv) dlogw += dl;
+    }
     const float3 Lo = fma3(2.0f * dot3(V, m), m, mul3(-1.0f, V));
     const float3 wo = to_world(Lo, n);
     pol_mirror(er, ei, unit3(sub3(wo, w)));
@@ -255,18 +281,48 @@ __device__ inline float3 facet_path(const Photon &p, float sigma, float3 wi, flo
   return w;
 }
 
-__device__ inline void medium_cache(const Args &a, Photon &p) {
+template <bool ADJ>
+__device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) {
   if (p.mcls == p.cls) return;
   const int m = class_medium(a, p.cls);
   p.ml = medium_at(a.L, a.P, m, p.lam);
   p.mr = a.has_ref ? medium_at(a.L, a.Pr, m, p.lam) : p.ml;
   p.mcls = p.cls;
+  if (ADJ) {
+    p.da = RateD{-1, -1, 0.0f, 0.0f};
+    p.dr = RateD{-1, -1, 0.0f, 0.0f};
+    if (m >= 0) {
+      const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * m;
+      if (!(M[8] & 1)) {
+        const float E = HC / p.lam;
+        if (M[5] > 0) p.da = rate_d(a.P, b->slots, M[4], M[5], E);
+        if (M[7] > 0) p.dr = rate_d(a.P, b->slots, M[6], M[7], E);
+      }
+    }
+  }
+}
+
+// the cotangent of one deposit at sensor s: dLoss/d(arrival) plus the charge cells' at the photon's wavelength
+__device__ inline float deposit_cotangent(const Args &a, const AdjArgs *b, const Photon &p, int s) {
+  const double *c = b->cot + (long long)p.event * a.tally_stride;
+  double cw = c[s];
+  if (a.K <= 1) {
+    cw += c[3 * a.S + s];
+  } else {
+    const TabPos tp = tab_pos(a.P, a.L[L_QE_TABLE], a.K, p.lam);
+    cw += (1.0 - tp.f) * c[3 * a.S + s * a.K + tp.i];
+    if (tp.f > 0.0f) cw += tp.f * c[3 * a.S + s * a.K + tp.i + 1];
+  }
+  return (float)cw;
 }
 
 __device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.0f ? live / ref : 0.0f; }
 
-// one step of one photon (transport/step.py:516-571); returns whether it goes on
-__device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
+// one step of one photon (transport/step.py:516-571); returns whether it goes on.  ADJ: the adjoint's walk
+// (no tallies; its pass 0 totals the photon's cotangent-weighted deposits, its pass 1 adds the derivative terms
+// into the block's accumulators sg)
+template <bool ADJ>
+__device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b, double *sg) {
   const Geo &g = a.g;
   float b0 = band_at(maxabs3(p.x));
   if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {  // G4Navigator's stuck-track push (step.py:269-282 `_recover`)
@@ -282,7 +338,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
   p.rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
   float3 er0 = p.er, ei0 = p.ei;
   pol_fill(p.d, er0, ei0, ul[3]);
-  medium_cache(a, p);
+  medium_cache<ADJ>(a, p, b);
   const MediumAt ml = p.ml, mr = p.mr;
   const float mu = ml.mu_ray, mu_r = mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
@@ -345,8 +401,31 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
     // the deposit: the expected detection at the crossing (step.py:522-524)
     const float dep_q = p.w * reach_live * att * sl.detect;
     const int sensor = g.vinfo[cr.vol_after].w;
-    if (sensor >= 0 && !cont) deposit(a, p, acc, sensor, dep_q, p.t + dist / v_g);
+    if (sensor >= 0 && !cont) {
+      if (!ADJ) {
+        deposit(a, p, acc, sensor, dep_q, p.t + dist / v_g);
+      } else {
+        const float D = dep_q * p.w0 * deposit_cotangent(a, b, p, sensor);
+        if (D != 0.0f) {
+          p.hasdep = true;
+          if (p.pass == 1) {  // the deposit's own factors: reach, attenuation, the detected share
+            grad_add(sg, p.da.s0, -dist * p.da.d0 * D);
+            grad_add(sg, p.da.s1, -dist * p.da.d1 * D);
+            grad_add(sg, p.dr.s0, -dist * p.dr.d0 * D);
+            grad_add(sg, p.dr.s1, -dist * p.dr.d1 * D);
+            float dsh[N_COL], ddet[N_COL];
+            const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
+            surface_dlogs(Rw, a.P, HC / p.lam, -1, dsh, ddet);
+            add_columns(sg, Rw, a.P, b->slots, HC / p.lam, ddet, D);
+          }
+          p.Dsum += D;
+        }
+      }
+    }
   }
+  // the deposits still ahead of this step's event (the adjoint's pass 1)
+  const float ahead = ADJ ? p.T - p.Dsum : 0.0f;
+  const bool adj_event = ADJ && p.pass == 1 && ahead != 0.0f;
   bool ok;
   float w_new, wr_new;
   if (scatter) {  // a bulk event at the free path's end (step.py:397-436): Rayleigh, the one channel here
@@ -356,6 +435,13 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
     w_new = w_at * att_s;
     wr_new = p.wr * (a.has_ref ? __expf(-mr.mu_abs * s) : att_s);
     acc.bulk += w_at * (1.0f - att_s) * p.w0;
+    if (adj_event) {  // the free path's density mu exp(-mu s) and the absorption exp(-mu_abs s)
+      const float k = (mu > 0.0f ? 1.0f / mu : 0.0f) - s;
+      grad_add(sg, p.dr.s0, k * p.dr.d0 * ahead);
+      grad_add(sg, p.dr.s1, k * p.dr.d1 * ahead);
+      grad_add(sg, p.da.s0, -s * p.da.d0 * ahead);
+      grad_add(sg, p.da.s1, -s * p.da.d1 * ahead);
+    }
     const float3 dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
     pol_project(er0, ei0, dn);
     p.x = fma3(s, p.d, p.x);
@@ -375,6 +461,18 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
     if (at_if) acc.surf += w_at * att * (1.0f - s_cont) * p.w0;
     ok = at_if && o >= 0;
     if (!at_if) acc.lost += w_at * att * p.w0;
+    if (ok && adj_event) {  // reach, attenuation over the segment, and the drawn outcome's share
+      grad_add(sg, p.da.s0, -dist * p.da.d0 * ahead);
+      grad_add(sg, p.da.s1, -dist * p.da.d1 * ahead);
+      grad_add(sg, p.dr.s0, -dist * p.dr.d0 * ahead);
+      grad_add(sg, p.dr.s1, -dist * p.dr.d1 * ahead);
+      float dsh[N_COL], ddet[N_COL];
+      const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
+      if (!cont) {
+        surface_dlogs(Rw, a.P, HC / p.lam, o, dsh, ddet);
+        add_columns(sg, Rw, a.P, b->slots, HC / p.lam, dsh, ahead);
+      }
+    }
     if (ok) {
       const float w_if = w_at * att * ratio(sl.p[o], pprop[o]);
       const float r_o = ratio(pdraw[o], pprop[o]);
@@ -382,6 +480,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
       float3 dn = p.d;
       const float3 wi = p.d, n = cr.n;
       bool far = false;
+      float w_lobe = 1.0f;  // the drawn lobe's live / reference density (a facet path at another width)
       if (o == STRAIGHT) {
         far = true;
       } else if (o == REFRACT) {
@@ -397,12 +496,21 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc) {
         er0 = mul3(-1.0f, er0);
         ei0 = mul3(-1.0f, ei0);
       } else if (o == GLOSSY) {
-        dn = facet_path(p, sr.sigma, wi, n, er0, ei0);
+        float logw = 0.0f, dlogw = 0.0f;
+        const bool want_ratio = a.has_ref && sl.sigma != sr.sigma;
+        dn = facet_path(p, sr.sigma, sl.sigma, a.ut, want_ratio, adj_event, wi, n, er0, ei0, logw, dlogw);
+        if (want_ratio) w_lobe = __expf(logw);
+        if (adj_event) {
+          const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
+          const TabD t = tab_d(a.P, b->slots, Rw[2 + 2 * COL_SIGMA], Rw[3 + 2 * COL_SIGMA], HC / p.lam);
+          grad_add(sg, t.s0, dlogw * t.w0 * ahead);
+          grad_add(sg, t.s1, dlogw * t.w1 * ahead);
+        }
       } else {  // DIFFUSE
         dn = lambert_sample(n, ul[0], ul[1]);
         pol_mirror(er0, ei0, unit3(sub3(dn, wi)));
       }
-      w_new = w_if;
+      w_new = w_if * w_lobe;
       acc.rew += (w_at * att * s_cont - w_new) * p.w0;
       p.x = fma3(dist, p.d, p.x);
       p.t += dist / v_g;
@@ -478,6 +586,27 @@ __device__ inline void flush_health(const Args &a, const Acc &acc) {
 
 __global__ void reset_queue(unsigned long long *q) { *q = 0ull; }
 
+// the navigator on given rays: the next visible crossing (t, the navigator id entered, the normal) and the volume
+// holding each ray's start
+__global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const float *__restrict__ d,
+                             const int *__restrict__ vol, const int *__restrict__ last, float tmax,
+                             float *__restrict__ t_out, int *__restrict__ vol_out, float *__restrict__ n_out,
+                             int *__restrict__ srf_out, int *__restrict__ located) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float3 xi = mk3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+  const float3 di = unit3(mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+  const int v = vol[i] >= 0 ? vol[i] : locate(g, xi);
+  located[i] = locate(g, xi);
+  const Cross c = query(g, xi, di, v, last[i], tmax);
+  t_out[i] = c.t;
+  vol_out[i] = c.vol_after;
+  n_out[3 * i] = c.n.x;
+  n_out[3 * i + 1] = c.n.y;
+  n_out[3 * i + 2] = c.n.z;
+  srf_out[i] = c.srf;
+}
+
 __global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
   Photon p;
   Acc acc{};
@@ -513,10 +642,71 @@ __global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
     }
     if (!__any(alive) && drained) break;
     if (alive) {
-      alive = step(a, p, acc);
+      alive = step<false>(a, p, acc, nullptr, nullptr);
       if (!alive) photon_end(a, p);
     }
   }
+  flush_health(a, acc);
+}
+
+// the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
+__global__ void __launch_bounds__(BLOCK) adjoint_kernel(Args a, AdjArgs b) {
+  extern __shared__ double sg[];
+  for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
+  __syncthreads();
+  Photon p;
+  Acc acc{};
+  bool alive = false;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  long long cursor = 0, chunk_end = 0;
+  bool drained = false;
+  for (;;) {
+    bool need = !alive;
+    unsigned long long mask = __ballot(need);
+    while (mask != 0ull && !drained) {
+      const long long avail = chunk_end - cursor;
+      if (avail <= 0) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+        base = __shfl(base, 0);
+        if ((long long)base >= a.total) {
+          drained = true;
+          break;
+        }
+        cursor = (long long)base;
+        chunk_end = min((long long)base + a.chunk, a.total);
+        continue;
+      }
+      const int rank = __popcll(mask & lt_mask);
+      const bool take = need && rank < avail;
+      if (take) {
+        p.gi = cursor + rank;
+        alive = emit(a, p.gi, p, acc);
+        p.pass = 0;
+        p.T = p.Dsum = 0.0f;
+        p.hasdep = false;
+      }
+      cursor += min((long long)__popcll(mask), avail);
+      need = need && !take;
+      mask = __ballot(need);
+    }
+    if (!__any(alive) && drained) break;
+    if (alive) {
+      alive = step<true>(a, p, acc, &b, sg);
+      if (!alive && p.pass == 0 && p.hasdep) {  // replay the photon from its counter, now adding the terms
+        const float T = p.Dsum;
+        Acc scratch{};
+        alive = emit(a, p.gi, p, scratch);
+        p.pass = 1;
+        p.T = T;
+        p.Dsum = 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x)
+    if (sg[i] != 0.0) unsafeAtomicAdd(b.grad + i, sg[i]);
   flush_health(a, acc);
 }
 
@@ -525,6 +715,12 @@ __global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
 // ---------------------------------------------------------------------------------------------------- the C ABI
 #define LH_API extern "C" __attribute__((visibility("default")))
 using namespace lh;
+
+struct SrcBuf {
+  lh_source *src = nullptr;
+  long long *src_end = nullptr;
+  int cap = 0;
+};
 
 struct lh_model_s {
   Geo g;
@@ -537,10 +733,18 @@ struct lh_model_s {
   lh_source *src = nullptr;
   long long *src_end = nullptr;
   int src_cap = 0;
-  int grid = 0;
+  int grid = 0, grid_adj = 0;
   long long values_needed = 0;  // the packed values a call must pass (the largest table's end)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int timed = 0;
+  float *ut = nullptr;  // the unified facet tables on the device
+  // the staged launch (lh_stage): its sources' count, photons, largest event; the adjoint's checked slots
+  SrcBuf staged;  // the staged launch's own source buffers (lh_trace never touches them)
+  int staged_nsrc = 0, staged_events = 0;
+  long long staged_total = -1;
+  const int32_t *staged_slots = nullptr;
+  int staged_ngrad = 0;
+  int cus = 0, per_trace = 1, per_adj = 1;
 };
 
 static thread_local std::string g_err;
@@ -756,11 +960,22 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   m->K = layout[5];
   m->tally_stride = 3LL * m->S + (long long)m->S * m->K;
   HIP_TRY(hipMalloc((void **)&m->queue, 256));
+  {
+    const std::vector<float> &ut = lh_host::unified_tables();
+    if ((int)ut.size() != UT_WORDS) return fail(LH_ERR_ARG, "lh_model_create: the unified tables have the wrong size");
+    HIP_TRY(hipMalloc((void **)&m->ut, ut.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(m->ut, ut.data(), ut.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   int dev = 0, cus = 0, per = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_kernel, BLOCK, 0));
   m->grid = cus * (per > 0 ? per : 1);
+  m->cus = cus;
+  m->per_trace = per > 0 ? per : 1;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adjoint_kernel, BLOCK, 8 * 1024));
+  m->grid_adj = cus * (per > 0 ? per : 1);
+  m->per_adj = per > 0 ? per : 1;
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
   *out = m;
@@ -774,6 +989,9 @@ LH_API int lh_model_destroy(lh_model m) {
   hipFree(m->queue);
   hipFree(m->src);
   hipFree(m->src_end);
+  hipFree(m->ut);
+  hipFree(m->staged.src);
+  hipFree(m->staged.src_end);
   if (m->ev0) hipEventDestroy(m->ev0);
   if (m->ev1) hipEventDestroy(m->ev1);
   delete m;
@@ -788,7 +1006,21 @@ LH_API int lh_model_info(lh_model m, int32_t *n_sensors, int32_t *n_qe, int64_t 
   return LH_OK;
 }
 
+// copy the sources and their photon prefix sums into `buf` (grown as needed) and check them
+static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_sources, long long *total,
+                      hipStream_t st);
+
 static int stage_sources(lh_model m, const lh_source *sources, int n_sources, long long *total, hipStream_t st) {
+  SrcBuf b{m->src, m->src_end, m->src_cap};
+  const int rc = stage_into(b, m->n_media, sources, n_sources, total, st);
+  m->src = b.src;
+  m->src_end = b.src_end;
+  m->src_cap = b.cap;
+  return rc;
+}
+
+static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_sources, long long *total,
+                      hipStream_t st) {
   if (!sources || n_sources <= 0) return fail(LH_ERR_ARG, "no sources");
   std::vector<long long> ends(n_sources);
   long long acc = 0;
@@ -797,23 +1029,24 @@ static int stage_sources(lh_model m, const lh_source *sources, int n_sources, lo
     if (s.n < 0) return fail(LH_ERR_ARG, "a source with a negative photon count");
     if (s.kind < 0 || s.kind > 2) return fail(LH_ERR_ARG, "an unknown source kind");
     if (s.event < 0) return fail(LH_ERR_ARG, "a negative event index");
-    if (s.kind == LH_SRC_CHERENKOV && (s.medium < 0 || s.medium >= m->n_media))
+    if (s.kind == LH_SRC_CHERENKOV && (s.medium < 0 || s.medium >= n_media))
       return fail(LH_ERR_ARG, "a Cherenkov source's medium is out of range");
     if (s.kind == LH_SRC_CHERENKOV && !(s.beta > 0.0f)) return fail(LH_ERR_ARG, "a Cherenkov source needs beta > 0");
     acc += s.n;
     ends[i] = acc;
   }
-  if (n_sources > m->src_cap) {
-    hipFree(m->src);
-    hipFree(m->src_end);
-    m->src = nullptr;
-    m->src_end = nullptr;
-    HIP_TRY(hipMalloc((void **)&m->src, sizeof(lh_source) * n_sources));
-    HIP_TRY(hipMalloc((void **)&m->src_end, sizeof(long long) * n_sources));
-    m->src_cap = n_sources;
+  if (n_sources > buf.cap) {
+    hipFree(buf.src);
+    hipFree(buf.src_end);
+    buf.src = nullptr;
+    buf.src_end = nullptr;
+    buf.cap = 0;
+    HIP_TRY(hipMalloc((void **)&buf.src, sizeof(lh_source) * n_sources));
+    HIP_TRY(hipMalloc((void **)&buf.src_end, sizeof(long long) * n_sources));
+    buf.cap = n_sources;
   }
-  HIP_TRY(hipMemcpyAsync(m->src, sources, sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(m->src_end, ends.data(), sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(buf.src, sources, sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(buf.src_end, ends.data(), sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));  // the host staging vector goes out of scope
   *total = acc;
   return LH_OK;
@@ -830,8 +1063,9 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.has_ref = (ref && ref != values) ? 1 : 0;
   a.k0 = (uint32_t)seed;
   a.k1 = (uint32_t)(seed >> 32);
-  lh_controls c{0.01f, 0.0f, 1 << 22, 0};
+  lh_controls c{0.01f, 0.0f, 1 << 22, 0, 0, 0};
   if (ctl) c = *ctl;
+  if (c.blocks_per_cu < 0 || c.blocks_per_cu > 64) return fail(LH_ERR_ARG, "blocks_per_cu out of range (0 .. 64)");
   if (!(c.roulette > 0.0f)) return fail(LH_ERR_ARG, "the roulette is a number > 0");
   a.roulette = c.roulette;
   a.reach = m->g.reach;
@@ -844,9 +1078,16 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.K = m->K;
   a.health = health;
   a.queue = m->queue;
+  a.ut = m->ut;
   a.src = m->src;
   a.src_end = m->src_end;
   return LH_OK;
+}
+
+static int grid_of(lh_model m, const lh_controls *ctl, bool adj) {
+  const int per = adj ? m->per_adj : m->per_trace;
+  const int want = (ctl && ctl->blocks_per_cu > 0) ? std::min(ctl->blocks_per_cu, per) : per;
+  return m->cus * want;
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
@@ -868,19 +1109,137 @@ LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  hipLaunchKernelGGL(trace_kernel, dim3(m->grid), dim3(BLOCK), 0, st, a);
+  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false)), dim3(BLOCK), 0, st, a);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
   m->timed = 1;
   return LH_OK;
 }
 
-LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, const int32_t *slots,
-                          const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *ctl,
-                          const double *cot, double *grad, int32_t n_grad, double *health, void *stream) {
-  (void)m; (void)values; (void)ref; (void)slots; (void)sources; (void)n_sources; (void)seed; (void)ctl; (void)cot;
-  (void)grad; (void)n_grad; (void)health; (void)stream;
-  return fail(LH_ERR_ARG, "lh_adjoint: not built in this version");
+LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t n_values, const int32_t *slots,
+                      const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *ctl,
+                      const double *cot, int32_t n_events, double *grad, int32_t n_grad, double *health,
+                      void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_adjoint: null model");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_adjoint: fewer packed values than the layout reads");
+  if (!slots || !cot || !grad) return fail(LH_ERR_ARG, "lh_adjoint: null slots, cotangent or gradient");
+  if (n_grad < 1 || n_grad > MAX_GRAD) return fail(LH_ERR_ARG, "lh_adjoint: n_grad out of range (1 .. 8192)");
+  for (int i = 0; i < n_sources && sources; ++i)
+    if (sources[i].event >= n_events) return fail(LH_ERR_ARG, "lh_adjoint: a source's event is beyond n_events");
+  hipStream_t st = (hipStream_t)stream;
+  {  // every slot indexes the block's accumulators: checked on the host before the launch
+    std::vector<int32_t> hs((size_t)n_values);
+    HIP_TRY(hipMemcpyAsync(hs.data(), slots, sizeof(int32_t) * (size_t)n_values, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (long long i = 0; i < n_values; ++i)
+      if (hs[i] < -1 || hs[i] >= n_grad) return fail(LH_ERR_ARG, "lh_adjoint: a gradient slot is out of range");
+  }
+  Args a;
+  long long total = 0;
+  int rc = stage_sources(m, sources, n_sources, &total, st);
+  if (rc) return rc;
+  rc = fill_args(m, a, values, ref, ctl, seed, const_cast<double *>(cot), health);
+  if (rc) return rc;
+  a.nsrc = n_sources;
+  a.total = total;
+  AdjArgs b{cot, slots, grad, n_grad};
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m->ev0, st));
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true)), dim3(BLOCK), (size_t)n_grad * sizeof(double), st, a,
+                     b);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m->ev1, st));
+  m->timed = 1;
+  return LH_OK;
+}
+
+LH_API int lh_stage(lh_model m, const lh_source *sources, int32_t n_sources, int64_t *total) {
+  if (!m) return fail(LH_ERR_ARG, "lh_stage: null model");
+  long long tot = 0;
+  int rc = stage_into(m->staged, m->n_media, sources, n_sources, &tot, nullptr);
+  if (rc) return rc;
+  int ev = 0;
+  for (int i = 0; i < n_sources; ++i) ev = std::max(ev, sources[i].event + 1);
+  m->staged_nsrc = n_sources;
+  m->staged_total = tot;
+  m->staged_events = ev;
+  if (total) *total = tot;
+  return LH_OK;
+}
+
+LH_API int lh_stage_slots(lh_model m, const int32_t *slots_dev, int64_t n_values, int32_t n_grad) {
+  if (!m || !slots_dev) return fail(LH_ERR_ARG, "lh_stage_slots: null argument");
+  if (n_grad < 1 || n_grad > MAX_GRAD) return fail(LH_ERR_ARG, "lh_stage_slots: n_grad out of range (1 .. 8192)");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_stage_slots: fewer packed values than the layout reads");
+  std::vector<int32_t> hs((size_t)n_values);
+  HIP_TRY(hipMemcpy(hs.data(), slots_dev, sizeof(int32_t) * (size_t)n_values, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < n_values; ++i)
+    if (hs[i] < -1 || hs[i] >= n_grad) return fail(LH_ERR_ARG, "lh_stage_slots: a gradient slot is out of range");
+  m->staged_slots = slots_dev;
+  m->staged_ngrad = n_grad;
+  return LH_OK;
+}
+
+LH_API int lh_launch(lh_model m, const float *values, const float *ref, int64_t n_values, uint64_t seed,
+                     const lh_controls *ctl, double *tally, int32_t n_events, double *health, void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_launch: null model");
+  if (m->staged_total < 0) return fail(LH_ERR_ARG, "lh_launch: no sources staged (lh_stage)");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_launch: fewer packed values than the layout reads");
+  if (n_events < m->staged_events) return fail(LH_ERR_ARG, "lh_launch: a staged source's event is beyond n_events");
+  hipStream_t st = (hipStream_t)stream;
+  Args a;
+  int rc = fill_args(m, a, values, ref, ctl, seed, tally, health);
+  if (rc) return rc;
+  a.src = m->staged.src;
+  a.src_end = m->staged.src_end;
+  a.nsrc = m->staged_nsrc;
+  a.total = m->staged_total;
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
+  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false)), dim3(BLOCK), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  return LH_OK;
+}
+
+LH_API int lh_launch_adjoint(lh_model m, const float *values, const float *ref, int64_t n_values,
+                             const int32_t *slots_dev, uint64_t seed, const lh_controls *ctl, const double *cot,
+                             int32_t n_events, double *grad, int32_t n_grad, double *health, void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_launch_adjoint: null model");
+  if (m->staged_total < 0) return fail(LH_ERR_ARG, "lh_launch_adjoint: no sources staged (lh_stage)");
+  if (!slots_dev || slots_dev != m->staged_slots || n_grad != m->staged_ngrad)
+    return fail(LH_ERR_ARG, "lh_launch_adjoint: the slots were not checked (lh_stage_slots)");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_launch_adjoint: fewer packed values than the layout reads");
+  if (n_events < m->staged_events) return fail(LH_ERR_ARG, "lh_launch_adjoint: a staged source's event is beyond n_events");
+  if (!cot || !grad) return fail(LH_ERR_ARG, "lh_launch_adjoint: null cotangent or gradient");
+  hipStream_t st = (hipStream_t)stream;
+  Args a;
+  int rc = fill_args(m, a, values, ref, ctl, seed, const_cast<double *>(cot), health);
+  if (rc) return rc;
+  a.src = m->staged.src;
+  a.src_end = m->staged.src_end;
+  a.nsrc = m->staged_nsrc;
+  a.total = m->staged_total;
+  AdjArgs b{cot, slots_dev, grad, n_grad};
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true)), dim3(BLOCK), (size_t)n_grad * sizeof(double), st, a,
+                     b);
+  HIP_TRY(hipGetLastError());
+  return LH_OK;
+}
+
+LH_API int lh_query(lh_model m, int32_t n, const float *x_dev, const float *d_dev, const int32_t *vol_dev,
+                    const int32_t *last_dev, float tmax, float *t_dev, int32_t *vol_after_dev, float *normal_dev,
+                    int32_t *srf_dev, int32_t *located_dev, void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_query: null model");
+  if (n < 0) return fail(LH_ERR_ARG, "lh_query: n < 0");
+  if (n == 0) return LH_OK;
+  if (!x_dev || !d_dev || !vol_dev || !last_dev || !t_dev || !vol_after_dev || !normal_dev || !srf_dev || !located_dev)
+    return fail(LH_ERR_ARG, "lh_query: null buffer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(query_kernel, dim3((n + 255) / 256), dim3(256), 0, st, m->g, n, x_dev, d_dev, vol_dev, last_dev,
+                     tmax > 0.0f ? tmax : m->g.reach, t_dev, vol_after_dev, normal_dev, srf_dev, located_dev);
+  HIP_TRY(hipGetLastError());
+  return LH_OK;
 }
 
 LH_API int lh_last_kernel_ms(lh_model m, float *ms) {

@@ -95,6 +95,13 @@ class SurfaceRow:
         bits = sum(1 << k for k in sup)
         return TYPES[self.type][3] | (FLAG_FAR_OPAQUE if self.far_opaque else 0) | (bits << 8)
 
+    def edge(self) -> bool:
+        """Return whether the row is an edge row: a parameter it reads is keyed and can zero an outcome's
+        probability (the cuts and the reflection split: the defensive role "zero", ``surfaces/types.py:322-331``;
+        decided by structure, never by a value, as ``api.py:797-814`` ``_edge_rows``)."""  # noqa: D205, D209
+        return any(isinstance(self.columns.get(c), str) for c in ("R", "T", "eff", "p_spike", "p_lobe", "p_back")
+                   if c in TYPES[self.type][1])
+
     def column(self, c):
         """Return the column's key, or its constant (the type's fixed value, else the neutral value)."""
         if c in self.columns and c in TYPES[self.type][1]:

@@ -107,13 +107,17 @@ class _Builder:
                           np.atleast_1d(v), key)
 
 
-def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows=()) -> Packed:
+def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="structure") -> Packed:
     """Return the Packed operands of ``det`` at ``params`` (default ``det.params()``).
 
-    ``grad_keys``: the keys whose table values get gradient slots (the adjoint's parameters).  ``edge_rows``: surface
-    rows drawn with the defensive mixture (the reference's edge rows, ``transport/estimator.py:131-153``).
+    ``grad_keys``: the keys whose table values get gradient slots (the adjoint's parameters).  ``edge_rows``: the
+    surface rows drawn with the defensive mixture (``transport/estimator.py:131-153``): "structure" (the default)
+    the reference's rule, every row reading a keyed parameter that can zero an outcome (``SurfaceRow.edge``); or an
+    explicit collection of row indices (empty: none).
     """
     p = det.params() if params is None else params
+    if isinstance(edge_rows, str):
+        edge_rows = {r for r, row in enumerate(det.rows) if row.edge()}
     b = _Builder(grad_keys)
     nm, nc, nr = len(det.media), len(det.class_medium), len(det.rows)
     med = np.zeros((nm, MEDIUM_WORDS), np.int32)

@@ -163,6 +163,10 @@ struct Scene {
   std::vector<std::vector<int>> kids;  // per navigator id, its daughters (brute force: no BVH)
   std::vector<Kind> kinds;
   double reach;
+  // the navigator arrays' BVHs (geometry/bvh.py:1-10: depth first, skip links), for the BVH walk mode
+  std::vector<int> vg0, vg1, g0, g1, skip, start, count, items;
+  std::vector<double> gR, lo, hi;
+  bool bvh = false;  // true: daughters by the BVH walk (the reference's Navigator._node); false: brute force
 };
 
 Scene scene_of(const lh_geometry *G) {
@@ -202,6 +206,17 @@ Scene scene_of(const lh_geometry *G) {
     }
   }
   s.reach = G->reach;
+  s.vg0.assign(G->vg0, G->vg0 + s.n + 1);
+  s.vg1.assign(G->vg1, G->vg1 + s.n + 1);
+  s.g0.assign(G->g0, G->g0 + G->n_groups);
+  s.g1.assign(G->g1, G->g1 + G->n_groups);
+  s.gR.assign(G->gR, G->gR + 9 * G->n_groups);
+  s.skip.assign(G->skip, G->skip + G->n_nodes);
+  s.start.assign(G->start, G->start + G->n_nodes);
+  s.count.assign(G->count, G->count + G->n_nodes);
+  s.lo.assign(G->lo, G->lo + 3 * G->n_nodes);
+  s.hi.assign(G->hi, G->hi + 3 * G->n_nodes);
+  s.items.assign(G->items, G->items + G->n_items);
   return s;
 }
 
@@ -396,12 +411,43 @@ template <class R> Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, lon
       best.t = ch.t; best.vol_after = s.parent[vol]; bv = vol; bl = ch.leaf; bf = ch.face; own = true;
     }
   }
-  for (int u : s.kids[vol]) {
+  auto visit = [&](int u) {
     V3<R> xl, dl;
     to_local(s, u, x, d, xl, dl);
     const Change<R> ch = first_change(s, u, xl, dl, false, last);
     if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && own))) {
       best.t = ch.t; best.vol_after = u; bv = u; bl = ch.leaf; bf = ch.face; own = false;
+    }
+  };
+  if (!s.bvh) {
+    for (int u : s.kids[vol]) visit(u);
+  } else {  // each frame group's BVH, stackless: a node the ray's window misses is skipped with its subtree
+    for (int gi = s.vg0[vol]; gi < s.vg1[vol]; ++gi) {
+      const double *M = &s.gR[9 * gi];
+      const V3<R> xg = v3<R>(R(M[0]) * x.x + R(M[3]) * x.y + R(M[6]) * x.z, R(M[1]) * x.x + R(M[4]) * x.y + R(M[7]) * x.z,
+                             R(M[2]) * x.x + R(M[5]) * x.y + R(M[8]) * x.z);
+      const V3<R> dg = v3<R>(R(M[0]) * d.x + R(M[3]) * d.y + R(M[6]) * d.z, R(M[1]) * d.x + R(M[4]) * d.y + R(M[7]) * d.z,
+                             R(M[2]) * d.x + R(M[5]) * d.y + R(M[8]) * d.z);
+      const R ix = R(1.0) / (FABS(dg.x) > R(1e-30) ? dg.x : R(1e-30)), iy = R(1.0) / (FABS(dg.y) > R(1e-30) ? dg.y : R(1e-30)),
+              iz = R(1.0) / (FABS(dg.z) > R(1e-30) ? dg.z : R(1e-30));
+      int ni = s.g0[gi];
+      while (ni < s.g1[gi]) {
+        const R tx0 = (R(s.lo[3 * ni]) - xg.x) * ix, tx1 = (R(s.hi[3 * ni]) - xg.x) * ix;
+        const R ty0 = (R(s.lo[3 * ni + 1]) - xg.y) * iy, ty1 = (R(s.hi[3 * ni + 1]) - xg.y) * iy;
+        const R tz0 = (R(s.lo[3 * ni + 2]) - xg.z) * iz, tz1 = (R(s.hi[3 * ni + 2]) - xg.z) * iz;
+        const R te = MAX<R>(MAX<R>(MIN<R>(tx0, tx1), MIN<R>(ty0, ty1)), MAX<R>(MIN<R>(tz0, tz1), R(0.0)));
+        const R tl = MIN<R>(MIN<R>(MAX<R>(tx0, tx1), MAX<R>(ty0, ty1)), MIN<R>(MAX<R>(tz0, tz1), best.t));
+        if (te > tl) {
+          ni = s.skip[ni];
+          continue;
+        }
+        if (s.count[ni] > 0) {
+          for (int k = 0; k < s.count[ni]; ++k) visit(s.items[s.start[ni] + k]);
+          ni = s.skip[ni];
+        } else {
+          ni += 1;
+        }
+      }
     }
   }
   if (bv >= 0) {
@@ -698,6 +744,23 @@ template <class R> V3<R> facet_path(const Photon<R> &p, double sigma, V3<R> wi, 
   return w;
 }
 
+// Rayleigh, polarised (phase.py `rayleigh_sample`, `dipole_cos`): the dipole axis er with probability |er|^2, else
+// ei; c = cos(k', axis) from 3/4 (1 - c^2) by its inverse CDF
+template <class R> V3<R> rayleigh_dir(V3<R> er, V3<R> ei, const double ul[4]) {
+  const R a2 = dot(er, er), n2 = a2 + dot(ei, ei);
+  const V3<R> axis = unit(R(ul[2]) * n2 < a2 ? er : ei);
+  const R c = MIN<R>(MAX<R>(R(2.0) * COS((ACOS(MIN<R>(MAX<R>(R(1.0 - 2.0 * ul[0]), R(-1.0)), R(1.0))) - R(TWO_PI)) / R(3.0)),
+                            R(-1.0)), R(1.0));
+  const R sn = SQRT(MAX<R>(R(1.0) - c * c, R(0.0)));
+  return unit(to_world(v3<R>(sn * COS(R(TWO_PI * ul[1])), sn * SIN(R(TWO_PI * ul[1])), c), axis));
+}
+
+// Lambertian about n (lobes.py `_cosine`)
+template <class R> V3<R> lambert_dir(V3<R> n, const double ul[4]) {
+  const R rr = SQRT(R(ul[0]));
+  return to_world(v3<R>(rr * COS(R(TWO_PI * ul[1])), rr * SIN(R(TWO_PI * ul[1])), SQRT(MAX<R>(R(1.0 - ul[0]), R(0.0)))), n);
+}
+
 template <class R> struct Moments {  // a photon's deposits summed per sensor (exact, unbounded)
   std::vector<std::pair<int, double>> m;
   void add(int s, double v) {
@@ -858,13 +921,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
     w_new = w_at * att_s;
     wr_new = p.wr * (r.has_ref ? EXP(-mr.mu_abs * s_free) : att_s);
     T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s)) * p.w0;
-    // Rayleigh (phase.py `rayleigh_sample`): the dipole axis er w.p. |er|^2, c from 3/4 (1 - c^2)
-    const R a2 = dot(er0, er0), n2 = a2 + dot(ei0, ei0);
-    const V3<R> axis = unit(R(ul[2]) * n2 < a2 ? er0 : ei0);
-    const R c = MIN<R>(MAX<R>(R(2.0) * COS((ACOS(MIN<R>(MAX<R>(R(1.0 - 2.0 * ul[0]), R(-1.0)), R(1.0))) - R(TWO_PI)) / R(3.0)),
-                              R(-1.0)), R(1.0));
-    const R sn = SQRT(MAX<R>(R(1.0) - c * c, R(0.0)));
-    const V3<R> dn = unit(to_world(v3<R>(sn * COS(R(TWO_PI * ul[1])), sn * SIN(R(TWO_PI * ul[1])), c), axis));
+    const V3<R> dn = rayleigh_dir(er0, ei0, ul);
     er0 = er0 - dot(dn, er0) * dn;
     ei0 = ei0 - dot(dn, ei0) * dn;
     pol_normalise(er0, ei0);
@@ -909,8 +966,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
       } else if (o == GLOSSY) {
         dn = facet_path(p, val(sr.sigma), wi, n, er0, ei0);
       } else {
-        const R rr = SQRT(R(ul[0]));
-        dn = to_world(v3<R>(rr * COS(R(TWO_PI * ul[1])), rr * SIN(R(TWO_PI * ul[1])), SQRT(MAX<R>(R(1.0 - ul[0]), R(0.0)))), n);
+        dn = lambert_dir(n, ul);
         pol_mirror(er0, ei0, unit(dn - wi));
       }
       w_new = w_if;
@@ -1004,11 +1060,13 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
 }
 
 // Trace every photon of the sources in FP64 on `threads` OpenMP threads; add into tally (n_events * stride) and
-// health (LH_HEALTH_N), both host arrays.
+// health (LH_HEALTH_N), both host arrays.  `use_bvh`: daughters by the BVH walk (the reference's navigator, the CPU
+// baseline's mode); 0: by brute force over every daughter (the independent referee).
 ORC_API int oracle_trace(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
                          const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally,
-                         int n_events, double *health, int threads) {
-  const Scene s = scene_of(G);
+                         int n_events, double *health, int threads, int use_bvh) {
+  Scene s = scene_of(G);
+  s.bvh = use_bvh != 0;
   Run r;
   make_run(r, s, layout, values, ref, src, nsrc, seed, ctl);
   const long long total = r.ends.empty() ? 0 : r.ends.back();
@@ -1031,10 +1089,11 @@ ORC_API int oracle_trace(const lh_geometry *G, const int32_t *layout, const floa
 }
 
 // The algorithmic work of the first n photons: floating-point operations (Counted) and Philox integer operations,
-// and the vertices they took (single thread).
+// and the vertices they took (single thread), with the daughters found by the BVH walk as the reference finds them.
 ORC_API int oracle_count_ops(const lh_geometry *G, const int32_t *layout, const float *values, const lh_source *src,
                              int nsrc, uint64_t seed, const lh_controls *ctl, long long n, double *out3) {
-  const Scene s = scene_of(G);
+  Scene s = scene_of(G);
+  s.bvh = true;  // the reference's algorithm walks the BVH
   Run r;
   make_run(r, s, layout, values, nullptr, src, nsrc, seed, ctl);
   Tally T;
@@ -1101,4 +1160,17 @@ ORC_API void oracle_fresnel(double c, double n1, double n2, double out8[8]) {
   const Fres f = fresnel_amps(c, n1, n2);
   const C2 a[4] = {f.rs, f.rp, f.ts, f.tp};
   for (int i = 0; i < 4; ++i) out8[2 * i] = a[i].re, out8[2 * i + 1] = a[i].im;
+}
+
+// n draws of the Rayleigh direction for the state (er, ei) and of the Lambertian direction about +z
+ORC_API void oracle_lobes(const double *er, const double *ei, long long n, uint64_t seed, double *ray, double *lam) {
+  const Rng g{(uint32_t)seed, (uint32_t)(seed >> 32), 7u, 0u};
+  for (long long i = 0; i < n; ++i) {
+    double ul[4];
+    g.draw4((uint32_t)i, (uint32_t)(i >> 32), ul);
+    const V3<double> r = rayleigh_dir(v3(er[0], er[1], er[2]), v3(ei[0], ei[1], ei[2]), ul);
+    const V3<double> l = lambert_dir(v3(0.0, 0.0, 1.0), ul);
+    ray[3 * i] = r.x, ray[3 * i + 1] = r.y, ray[3 * i + 2] = r.z;
+    lam[3 * i] = l.x, lam[3 * i + 1] = l.y, lam[3 * i + 2] = l.z;
+  }
 }

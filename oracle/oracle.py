@@ -1,0 +1,167 @@
+"""THE ORACLE's Python side -- test infrastructure only (tests/, ``__graft_entry__.smoke()``, bench.py's
+cpu_baseline).  It loads ``oracle/liblumen_oracle.so`` (built by ``oracle/Makefile``), the FP64 C++/OpenMP
+restatement of the reference's photon loop, and runs it on the same packed inputs the engine takes.  The product
+(``lumen_amd``) never imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblumen_oracle.so")
+_LIB = None
+
+
+def build():
+    """Compile the oracle (make in oracle/)."""
+    r = subprocess.run(["make", "-C", HERE, "-B", "liblumen_oracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stderr[-4000:]}")
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from lumen_amd import _lib as E
+
+        L = C.CDLL(LIB_PATH)
+        L.oracle_trace.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(E.LhSource), C.c_int, C.c_uint64, C.POINTER(E.LhControls), C.c_void_p,
+                                   C.c_int, C.c_void_p, C.c_int, C.c_int]
+        L.oracle_count_ops.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.POINTER(E.LhSource),
+                                       C.c_int, C.c_uint64, C.POINTER(E.LhControls), C.c_longlong, C.c_void_p]
+        L.oracle_query.argtypes = [C.POINTER(E.LhGeometry), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_locate.argtypes = [C.POINTER(E.LhGeometry), C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_philox.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_philox.restype = None
+        L.oracle_facet_paths.argtypes = [C.c_double, C.c_void_p, C.c_longlong, C.c_uint64, C.c_void_p]
+        L.oracle_facet_paths.restype = None
+        L.oracle_lobes.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_uint64, C.c_void_p, C.c_void_p]
+        L.oracle_lobes.restype = None
+        L.oracle_fresnel.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.oracle_fresnel.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class OracleRun:
+    """The oracle's view of one detector: its geometry struct and packed layout (kept alive together)."""
+
+    def __init__(self, det):
+        from lumen_amd import _lib as E, pack
+
+        self.det = det
+        self.geo, self._keep = E.geometry_struct(det.nav, det.sensor_of_vol, det.reach)
+        self.S = det.n_sensors
+
+    def trace(self, source, samples=None, *, seed=0, params=None, reference=None, controls=None, threads=0,
+              first_id=0, edge_rows="structure", bvh=False):
+        """Return (tally (n_events, stride) float64, health (16,)) of the oracle's run.  ``bvh``: find daughters by
+        the navigator arrays' BVH (the reference's walk; the CPU baseline), else by brute force (the referee)."""
+        from lumen_amd import api, pack
+
+        p = self.det.params() if params is None else params
+        pk = pack.pack(self.det, p, edge_rows=edge_rows)
+        ref = None if reference is None else pack.pack(self.det, reference, edge_rows=edge_rows).values
+        rows, n_ev = api.source_rows(self.det, source, samples, first_id)
+        stride = 3 * self.S + self.S * int(pk.layout[5])
+        tally = np.zeros((n_ev, stride))
+        health = np.zeros(16)
+        ctl = (controls or api.Controls()).struct()
+        vals = np.ascontiguousarray(pk.values, np.float32)
+        lay = np.ascontiguousarray(pk.layout, np.int32)
+        refv = None if ref is None else np.ascontiguousarray(ref, np.float32)
+        rc = lib().oracle_trace(C.byref(self.geo), _p(lay), _p(vals), None if refv is None else _p(refv), rows,
+                                len(rows), C.c_uint64(int(seed)), C.byref(ctl), _p(tally), n_ev, _p(health),
+                                int(threads), int(bool(bvh)))
+        if rc != 0:
+            raise RuntimeError(f"oracle_trace: {rc}")
+        return tally, health
+
+    def count_ops(self, source, samples, n, *, seed=0, controls=None):
+        """Return (FP ops, Philox integer ops, vertices) of the first n photons (the algorithmic work)."""
+        from lumen_amd import api, pack
+
+        pk = pack.pack(self.det)
+        rows, _ = api.source_rows(self.det, source, samples)
+        ctl = (controls or api.Controls()).struct()
+        out = np.zeros(3)
+        vals = np.ascontiguousarray(pk.values, np.float32)
+        lay = np.ascontiguousarray(pk.layout, np.int32)
+        lib().oracle_count_ops(C.byref(self.geo), _p(lay), _p(vals), rows, len(rows), C.c_uint64(seed),
+                               C.byref(ctl), int(n), _p(out))
+        return out
+
+    def query(self, x, d, vol, last=None, tmax=1e30):
+        """Return (t, vol_after, normal, srf) of rays (FP64 referee)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float64).reshape(-1, 3)
+        n = len(x)
+        vol = np.ascontiguousarray(np.broadcast_to(np.asarray(vol, np.int32), (n,)))
+        last = np.ascontiguousarray(np.full(n, -1, np.int64) if last is None else np.asarray(last, np.int64))
+        t, va, nrm, srf = np.zeros(n), np.zeros(n, np.int32), np.zeros((n, 3)), np.zeros(n, np.int64)
+        lib().oracle_query(C.byref(self.geo), n, _p(x), _p(d), _p(vol), _p(last), float(tmax), _p(t), _p(va),
+                           _p(nrm), _p(srf))
+        return t, va, nrm, srf
+
+    def locate(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        out = np.zeros(len(x), np.int32)
+        lib().oracle_locate(C.byref(self.geo), len(x), _p(x), _p(out))
+        return out
+
+
+def philox(ctr, key):
+    """Return Philox4x32-10 of a counter (4 uint32) and key (2 uint32)."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox(_p(c), _p(k), _p(out))
+    return out
+
+
+def facet_paths(sigma, wi, n, seed=0):
+    """Return n outgoing directions of the unified facet path off a flat surface (normal +z) at incidence wi."""
+    w = np.ascontiguousarray(wi, np.float64)
+    out = np.zeros((int(n), 3))
+    lib().oracle_facet_paths(float(sigma), _p(w), int(n), C.c_uint64(seed), _p(out))
+    return out
+
+
+def lobes(er, ei, n, seed=0):
+    """Return (n Rayleigh directions for the state (er, ei), n Lambertian directions about +z)."""
+    a, b = np.ascontiguousarray(er, np.float64), np.ascontiguousarray(ei, np.float64)
+    ray, lam = np.zeros((int(n), 3)), np.zeros((int(n), 3))
+    lib().oracle_lobes(_p(a), _p(b), int(n), C.c_uint64(seed), _p(ray), _p(lam))
+    return ray, lam
+
+
+def fresnel(c, n1, n2):
+    """Return complex (r_s, r_p, t_s, t_p) at incidence cosine c from n1 into n2."""
+    out = np.zeros(8)
+    lib().oracle_fresnel(float(c), float(n1), float(n2), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def light_of(tally, health, S, K, samples):
+    """Return dict(arrivals, arrivals_error, mean_arrival_time, arrival2, charge) of an oracle tally (one event)."""
+    t = tally[0]
+    arr, arr_t, arr2 = t[:S], t[S:2 * S], t[2 * S:3 * S]
+    n = float(max(samples, 2))
+    err = np.sqrt(np.maximum(n / (n - 1) * (arr2 - arr * arr / n), 0.0))
+    mean_t = np.where(arr > 0, arr_t / np.where(arr > 0, arr, 1.0), 0.0)
+    return dict(arrivals=arr, arrivals_error=err, mean_arrival_time=mean_t, arrival2=arr2,
+                charge=t[3 * S:].reshape(S, K), health=health)

@@ -1,0 +1,198 @@
+"""Parity of the HIP engine with the oracle (FP64 CPU restatement), the reference's published numbers and the
+closed forms, through the C ABI (``lh_trace``, ``lh_query``)."""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import lumen_amd as L
+from lumen_amd.api import engine
+from lumen_amd.detector import example_tank, muon_track, water_tank
+from oracle import oracle as O
+from toys import lambert_closed_form, specular_closed_form, sphere_toy
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+SRC1 = L.sources.Point((0.0, 0.0, 0.1), lam=400.0, n_photons=2e5)
+
+
+@pytest.fixture(scope="module")
+def tank1():
+    return example_tank()
+
+
+@pytest.fixture(scope="module")
+def tank2():
+    return water_tank()
+
+
+def _cherenkov():
+    x0, d = muon_track()
+    return L.sources.CherenkovSegment("Water", start=tuple(x0), direction=tuple(d), length=1.0)
+
+
+def _np(light):
+    return {k: getattr(light, k).detach().cpu().numpy() for k in ("arrivals", "arrivals_error", "mean_arrival_time",
+                                                                    "arrival2", "charge")}
+
+
+def test_example_tank_same_photons_as_oracle(gpu, tank1):
+    """One seed: the engine (float32) and the oracle (FP64) trace the same photons (one Philox contract), so each
+    PMT's arrivals agree far inside their Monte Carlo error."""
+    n = 1 << 18
+    g = _np(L.simulate(tank1, SRC1, samples=n, seed=0))
+    tal, h = O.OracleRun(tank1).trace(SRC1, n, seed=0)
+    o = O.light_of(tal, h, 8, g["charge"].shape[1], n)
+    np.testing.assert_allclose(g["arrivals"], o["arrivals"], rtol=2e-4)
+    np.testing.assert_allclose(g["arrival2"], o["arrival2"], rtol=2e-3)
+    np.testing.assert_allclose(g["mean_arrival_time"], o["mean_arrival_time"], rtol=1e-4)
+
+
+def test_example_tank_published_arrivals(gpu, tank1):
+    tut = json.load(open(os.path.join(GOLDEN, "tutorial_example_tank.json")))
+    g = _np(L.simulate(tank1, SRC1, samples=1 << 18, seed=0))
+    pub, perr = np.array(tut["arrivals"]), np.array(tut["arrivals_error"])
+    pulls = (g["arrivals"] - pub) / np.hypot(g["arrivals_error"], perr)
+    assert stats.chi2.sf(np.sum(pulls**2), 8) > 0.01
+    np.testing.assert_allclose(g["mean_arrival_time"], tut["mean_arrival_time_ns"], atol=0.02)
+    light = L.simulate(tank1, SRC1, samples=1 << 18, seed=0)
+    pe = L.readout.pe(light, tank1.params(), tank1).cpu().numpy()
+    np.testing.assert_allclose(pe / g["arrivals"], 0.245, rtol=1e-6)
+
+
+def _pulls(a, ea, b, eb):
+    ok = (ea > 0) & (eb > 0)
+    return (a[ok] - b[ok]) / np.hypot(ea[ok], eb[ok])
+
+
+def test_water_tank_pulls_against_the_oracle(gpu, tank2):
+    """1,000 PMTs, independent seeds on the two sides, four pairs: the pulls have mean 0 +- 0.05 and width
+    1 +- 0.05, and chi2/ndf p > 0.01.  Both sides trace the same number of photons: each PMT's light is skewed
+    (a few high-weight photons), so unequal sample sizes bias the pulls' mean by itself (0.26 for 2^20 against
+    2^18, measured oracle against oracle)."""
+    src = _cherenkov()
+    orc = O.OracleRun(tank2)
+    pulls = []
+    n = 1 << 20
+    for k in range(4):
+        g = _np(L.simulate(tank2, src, samples=n, seed=1000 + k))
+        tal, h = orc.trace(src, n, seed=2000 + k, bvh=True)  # the BVH walk: brute force's result, 30x faster
+        o = O.light_of(tal, h, 1000, 1, n)
+        pulls.append(_pulls(g["arrivals"], g["arrivals_error"], o["arrivals"], o["arrivals_error"]))
+    p = np.concatenate(pulls)
+    assert len(p) > 3900
+    assert abs(p.mean()) < 0.05, p.mean()
+    assert abs(p.std() - 1.0) < 0.05, p.std()
+    assert stats.chi2.sf(np.sum(p**2), len(p)) > 0.01
+
+
+def test_water_tank_same_photons_as_oracle(gpu, tank2):
+    """One seed, the same photons: totals agree to a small fraction of the Monte Carlo error (the facet path's
+    float32 rejection can send a rare photon another way)."""
+    src = _cherenkov()
+    n = 1 << 17
+    g = _np(L.simulate(tank2, src, samples=n, seed=7))
+    tal, h = O.OracleRun(tank2).trace(src, n, seed=7)
+    o = O.light_of(tal, h, 1000, 1, n)
+    tot_err = math.sqrt(o["arrival2"].sum())
+    assert abs(g["arrivals"].sum() - o["arrivals"].sum()) < 0.05 * tot_err
+    assert np.corrcoef(g["arrivals"], o["arrivals"])[0, 1] > 0.999
+
+
+@pytest.mark.parametrize("which", ["example_tank", "water_tank"])
+def test_navigator_distances_against_fp64(gpu, which, tank1, tank2):
+    """Fixed rays from random interior points: the device navigator's distance to the next boundary agrees with
+    the FP64 referee within 1e-5 relative; at most 0.2 % of rays (grazing ones) meet another volume
+    (tests/unit/test_navigator.py:43-73)."""
+    det = tank1 if which == "example_tank" else tank2
+    rng = np.random.default_rng(11)
+    n = 4096
+    if which == "example_tank":
+        x = rng.uniform(-0.45, 0.45, (n, 3))
+    else:
+        r = 1.9 * np.sqrt(rng.uniform(0, 1, n))
+        ph = rng.uniform(0, 2 * np.pi, n)
+        x = np.stack([r * np.cos(ph), r * np.sin(ph), rng.uniform(-1.4, 1.4, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x32 = x.astype(np.float32).astype(np.float64)
+    d32 = d.astype(np.float32).astype(np.float64)
+    orc = O.OracleRun(det)
+    vol = orc.locate(x32)
+    t64, va64, n64, _ = orc.query(x32, d32, vol)
+    t32, va32, n32, _, loc = engine(det).query(x32, d32, vol)
+    assert np.sum(loc != vol) <= 2
+    same = va32 == va64
+    assert same.mean() >= 0.998
+    rel = np.abs(t32[same] - t64[same]) / t64[same]
+    assert rel.max() < 1e-5, rel.max()
+    cosn = np.sum(n32[same] * n64[same], 1)
+    assert cosn.min() > 1 - 1e-5
+
+
+def test_specular_sphere_closed_form(gpu):
+    a, R, La = 1.0, 0.5, 10.0
+    det = sphere_toy(a=a, R=R, L_abs=La)
+    g = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), samples=1 << 16, seed=2,
+                   controls=L.Controls(roulette=1e-12))
+    q = float(g.arrivals[0])
+    assert abs(q - specular_closed_form(a, R, La)) < 3e-5 * specular_closed_form(a, R, La)
+    assert abs(g.health.balance()) < 1e-5
+
+
+def test_lambert_sphere_and_rayleigh_enclosure(gpu):
+    n = 1 << 20
+    det = sphere_toy(a=1.0, surface="PAINTED", R=0.6, L_abs=4.0)
+    g = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), samples=n, seed=3)
+    assert abs(float(g.arrivals[0]) - lambert_closed_form(1.0, 0.6, 4.0)) < 4 * float(g.arrivals_error[0])
+    det = sphere_toy(a=1.0, R=0.0, L_abs=None, L_ray=0.3)
+    g = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), samples=n, seed=4)
+    assert abs(float(g.arrivals[0]) - 1.0) < max(4 * float(g.arrivals_error[0]), 1e-5)
+    assert g.health.vertices > 3 * n
+
+
+def test_group_velocity_timing(gpu):
+    det = sphere_toy(a=1.5, R=0.0, L_abs=None, n=1.40)
+    g = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), samples=4096, seed=5)
+    assert abs(float(g.mean_arrival_time[0]) / (1.5 / (L.units.C_VACUUM / 1.40)) - 1) < 1e-6
+
+
+def test_photons_do_not_depend_on_the_split(gpu, tank2):
+    """Photon i is the same in one launch or in two (first_id), and two runs of one seed agree (FP64 tallies)."""
+    src = _cherenkov()
+    n = 1 << 18
+    a = L.simulate(tank2, src, samples=n, seed=21)
+    b = L.simulate(tank2, src, samples=n, seed=21)
+    np.testing.assert_allclose(a.arrivals.cpu().numpy(), b.arrivals.cpu().numpy(), rtol=1e-12, atol=1e-15)
+    # halves: the same Monte Carlo photons (ids 0..n/2-1 and n/2..n-1), each half's weight over the whole run
+    half = n // 2
+    rows = [L.sources.CherenkovSegment(**{k: getattr(src, k) for k in ("material", "start", "direction", "length")})]
+    h1 = L.simulate(tank2, rows[0], samples=half, seed=21, first_id=0)
+    h2 = L.simulate(tank2, rows[0], samples=half, seed=21, first_id=half)
+    np.testing.assert_allclose(0.5 * (h1.arrivals + h2.arrivals).cpu().numpy(), a.arrivals.cpu().numpy(),
+                               rtol=1e-9, atol=1e-12)
+
+
+def test_health_balances(gpu, tank2):
+    g = L.simulate(tank2, _cherenkov(), samples=1 << 20, seed=3)
+    assert abs(g.health.balance()) < 1e-5 * g.health.start_weight
+    assert g.health.lost == 0 and g.health.abandoned == 0 and g.health.truncated == 0
+
+
+def test_reference_values_hold_the_draws(gpu, tank1):
+    """reference=params is the plain run; with the reference held the light is smooth in a value (common random
+    numbers): the central difference of the total over ABSLENGTH's scale is close to both one-sided ones."""
+    p0 = tank1.params()
+    n = 1 << 18
+    plain = L.simulate(tank1, SRC1, samples=n, seed=0)
+    same = L.simulate(tank1, SRC1, p0, samples=n, seed=0, reference=p0)
+    np.testing.assert_allclose(plain.arrivals.cpu().numpy(), same.arrivals.cpu().numpy(), rtol=1e-6)
+    tot = lambda s: float(L.simulate(tank1, SRC1, p0.scale("materials/Water/ABSLENGTH", s), samples=n, seed=0,  # noqa
+                                     reference=p0).arrivals.sum())
+    f0, fp, fm = tot(1.0), tot(1.01), tot(0.99)
+    d_c, d_p, d_m = (fp - fm) / 0.02, (fp - f0) / 0.01, (f0 - fm) / 0.01
+    assert d_c > 0 and abs(d_p - d_c) < 0.05 * d_c and abs(d_m - d_c) < 0.05 * d_c

@@ -1,0 +1,145 @@
+"""Host logic on the CPU: the navigator-array format, the BVH, the detectors, the packing, the C ABI's exports."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import lumen_amd as L
+from lumen_amd import _lib, pack
+from lumen_amd.detector import NavArrays, bvh, example_tank, navarrays, solids, water_tank
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_abi_exports_every_declared_symbol():
+    """The library loads (no GPU needed to load it) and exports every function include/lumen_hip.h declares."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("the library is not built (python -m lumen_amd.build)")
+    header = open(os.path.join(ROOT, "include", "lumen_hip.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*\**(lh_\w+)\(", header, re.M))
+    assert {"lh_model_create", "lh_trace", "lh_adjoint", "lh_query", "lh_last_error"} <= declared
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_lib.SYMBOLS) <= declared
+
+
+def test_abi_structs_match_the_header():
+    """ctypes' structs have the header's sizes (lh_source: 4 ints, 14 floats, 2 int64)."""
+    assert ctypes.sizeof(_lib.LhSource) == 4 * 4 + 14 * 4 + 2 * 8
+    assert ctypes.sizeof(_lib.LhControls) == 24
+
+
+def _check_bvh(t, lo, hi):
+    n = len(t["skip"])
+    for i in range(n):
+        assert i < t["skip"][i] <= n
+        if t["count"][i] > 0:
+            assert t["skip"][i] == i + 1
+            for it in t["items"][t["start"][i]: t["start"][i] + t["count"][i]]:
+                assert np.all(lo[it] >= t["lo"][i] - 1e-12) and np.all(hi[it] <= t["hi"][i] + 1e-12)
+        else:  # an inner node's children lie inside it
+            assert np.all(t["lo"][i + 1] >= t["lo"][i] - 1e-12) and np.all(t["hi"][i + 1] <= t["hi"][i] + 1e-12)
+    assert sorted(t["items"]) == list(range(len(lo)))
+
+
+def test_bvh_skip_links_cover_every_box():
+    rng = np.random.default_rng(1)
+    c = rng.uniform(-1, 1, (300, 3))
+    h = rng.uniform(0.01, 0.1, (300, 3))
+    t = bvh.build(c - h, c + h)
+    _check_bvh(t, c - h, c + h)
+
+
+def test_navarrays_round_trip(tmp_path):
+    det = water_tank()
+    det.nav.save(str(tmp_path / "nav"))
+    back = NavArrays.load(str(tmp_path / "nav"))
+    for k, v in det.nav.host.items():
+        np.testing.assert_array_equal(v, back.host[k])
+    for tab in ("T", "K", "G"):
+        for k, v in det.nav.csg[tab].items():
+            np.testing.assert_array_equal(v, back.csg[tab][k])
+    assert back.n == det.nav.n and back.names == det.nav.names
+
+
+def test_example_tank_structure():
+    det = example_tank()
+    assert det.n_sensors == 8 and det.nav.n == 9
+    names, pos = det.sensor_table()
+    np.testing.assert_allclose(pos[1], [0, 0, 0.48], atol=1e-7)
+    np.testing.assert_allclose(pos[6], [-0.25, 0, -0.48], atol=1e-7)
+    assert [r.type for r in det.rows] == ["SPECULAR_METAL", "ABSORBER"]
+    assert set(det.params().keys()) >= {"materials/Water/ABSLENGTH", "materials/Water/RINDEX",
+                                        "surfaces/CathodeSurface/REFLECTIVITY", "sensors/pmt/qe"}
+    p = det.params().scale("materials/Water/ABSLENGTH", 2.0)
+    np.testing.assert_allclose(p["materials/Water/ABSLENGTH"].value, 2 * det.params()["materials/Water/ABSLENGTH"].value)
+    assert p.explain("materials/Water/ABSLENGTH") == "set"
+    assert det.params().info("materials/Water/ABSLENGTH").method == "reweight"
+
+
+def test_water_tank_pmts_inside_and_apart():
+    """1,000 caps, each inside the cylinder (every corner of its box) and no two boxes overlapping."""
+    det = water_tank()
+    assert det.n_sensors == 1000
+    lo, hi = det.nav.lo_w[1:], det.nav.hi_w[1:]
+    cap = solids.sphere_cap(0.10, 0.04)
+    R, T = det.nav.host["R"].astype(float), det.nav.host["t"].astype(float)
+    for k in range(1, det.nav.n):
+        # sample the cap's surface: every point inside the tank
+        u = np.random.default_rng(k).normal(size=(64, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        pts = 0.1 * u
+        pts = pts[pts[:, 2] >= 0.06]
+        w = pts @ R[k].T + T[k]
+        assert np.all(np.hypot(w[:, 0], w[:, 1]) < 2.0) and np.all(np.abs(w[:, 2]) < 1.5)
+    order = np.argsort(lo[:, 0])
+    for a in range(len(order)):  # sweep for overlapping boxes
+        i = order[a]
+        for b in range(a + 1, len(order)):
+            j = order[b]
+            if lo[j, 0] > hi[i, 0]:
+                break
+            assert not (np.all(lo[i] < hi[j]) and np.all(lo[j] < hi[i])), (i, j)
+    assert cap.inside(np.array([0.0, 0.0, 0.08])) and not cap.inside(np.array([0.0, 0.0, 0.05]))
+
+
+def test_pack_layout_and_slots():
+    det = water_tank()
+    pk = pack.pack(det, grad_keys=["materials/Water/ABSLENGTH", "surfaces/Liner/REFLECTIVITY"])
+    assert pk.layout[0] == pack.MAGIC and pk.n_grad == 17
+    assert pk.grad_keys[0] == ("materials/Water/ABSLENGTH", 0, 16)
+    # the slotted values are exactly the table's values
+    absl = det.params()["materials/Water/ABSLENGTH"].value
+    np.testing.assert_allclose(pk.values[pk.slots[:] >= 0][:16], absl, rtol=1e-6)
+    # a new value changes the values, never the layout
+    pk2 = pack.pack(det, det.params().scale("materials/Water/ABSLENGTH", 1.1))
+    np.testing.assert_array_equal(pk.layout, pk2.layout)
+    assert not np.array_equal(pk.values, pk2.values)
+
+
+def test_edge_rows_follow_the_structure():
+    """Rows reading a keyed cut or reflection constant are edge rows whatever its value (api.py:797-814)."""
+    assert [r.edge() for r in example_tank().rows] == [True, False]
+    assert [r.edge() for r in water_tank().rows] == [True, True, False]
+
+
+def test_simulate_refuses_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="no GPU"):
+        L.simulate(example_tank(), L.sources.Point((0, 0, 0), lam=400.0), samples=16, seed=0)
+
+
+def test_csg_program_membership():
+    """A tube with a bore is the outer cylinder minus the inner one."""
+    t = solids.tube(1.0, 0.5, rmin=0.4)
+    assert t.inside(np.array([0.7, 0, 0])) and not t.inside(np.array([0.2, 0, 0]))
+    assert not t.inside(np.array([0.7, 0, 0.6]))
+    tabs = navarrays.csg_tables([t])
+    assert list(tabs["G"]["op"]) == [0, 0, 3]

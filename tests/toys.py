@@ -1,0 +1,61 @@
+"""Toy detectors with closed-form light, built with the engine's own builder (the reference's physics toys,
+tests/physics/test_toys.py:126-182): a water sphere of radius ``a`` inside a sensor shell, the crossing out of the
+water a surface of one type, a point source at the centre.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from lumen_amd import units
+from lumen_amd.detector import navarrays, solids
+from lumen_amd.detector.model import Detector, Medium, SensorGroup, SurfaceRow
+from lumen_amd.params import Params, Table
+
+E_KNOTS = np.array([1.5, 5.0])  # eV: flat tables over every wavelength used here
+
+
+def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n=1.33, eff=1.0):
+    """Return a water sphere (radius a) inside a sensor shell: the crossing out of the water is ``surface``
+    (SPECULAR_METAL: R mirror; PAINTED: R Lambertian) detecting the absorbed share with ``eff``."""  # noqa: D205, D209
+    kinds = [solids.orb(a + 0.05, name="shell"), solids.orb(a, name="water")]
+    vols = [navarrays.Volume("shell", 0, np.eye(3), np.zeros(3), -1, 1),
+            navarrays.Volume("shell/water", 1, np.eye(3), np.zeros(3), 0, 0)]
+    nav = navarrays.build(vols, kinds, outside_cls=2)
+    values = {"materials/Water/RINDEX": Table(E_KNOTS, np.array([n, n]))}
+    absorption = rayleigh = None
+    if L_abs is not None:
+        values["materials/Water/ABSLENGTH"] = Table(E_KNOTS, np.array([L_abs, L_abs]))
+        absorption = "materials/Water/ABSLENGTH"
+    if L_ray is not None:
+        values["materials/Water/RAYLEIGH"] = Table(E_KNOTS, np.array([L_ray, L_ray]))
+        rayleigh = "materials/Water/RAYLEIGH"
+    values["surfaces/Wall/REFLECTIVITY"] = np.float64(R)
+    values["surfaces/Wall/EFFICIENCY"] = np.float64(eff)
+    cols = {"R": "surfaces/Wall/REFLECTIVITY", "eff": "surfaces/Wall/EFFICIENCY"}
+    if surface == "PAINTED":
+        cols["p_spike"] = 0.0
+    rows = [SurfaceRow("Wall", surface, cols), SurfaceRow("absorber", "ABSORBER")]
+    row_of = np.full((3, 3), -1, np.int32)
+    row_of[0, 1] = 0
+    row_of[0, 2] = 1
+    sensor_of_vol = np.array([0, -1, -1], np.int32)
+    return Detector("sphere_toy", nav, [Medium("Water", "materials/Water/RINDEX", absorption, rayleigh)],
+                    np.array([0, -1, -1], np.int32), ["Water", "Shell", "OUT"], rows, row_of, sensor_of_vol,
+                    ["shell"], [SensorGroup("pmt", np.arange(1))], Params(values))
+
+
+def specular_closed_form(a, R, L):
+    """Q/N of the specular sensor sphere: (1 - R) e^{-a/L} / (1 - R e^{-2a/L})."""
+    return (1 - R) * np.exp(-a / L) / (1 - R * np.exp(-2 * a / L))
+
+
+def lambert_closed_form(a, R, L):
+    """Q/N of the Lambertian sensor sphere: (1 - R) e^{-a/L} / (1 - R 2 (1 - (1 + b) e^{-b}) / b^2), b = 2a/L."""
+    b = 2 * a / L
+    return (1 - R) * np.exp(-a / L) / (1 - R * 2 * (1 - (1 + b) * np.exp(-b)) / b**2)
+
+
+def group_velocity(n):
+    """The group velocity of a flat index table (m/ns): c / n."""
+    return units.C_VACUUM / n
