@@ -60,16 +60,22 @@ def pmc_traffic():
     return None
 
 
-def cpu_baseline(det, src, seconds=12.0):
-    """Time the oracle (FP64, OpenMP on every host core) on a bounded sample of the workload."""
+def cpu_baseline(det, src, seconds=15.0):
+    """Time the oracle (FP64, OpenMP on the host cores this process may use) on a bounded sample of the workload:
+    a warm-up, a pilot that sizes the sample to about ``seconds``, then the timed sample."""
     from oracle import oracle as O
 
-    threads = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
     run = O.OracleRun(det)
-    n = 1 << 13
-    t0 = time.perf_counter()
-    run.trace(src, n, seed=99, threads=threads, bvh=True)
-    dt = time.perf_counter() - t0
+    run.trace(src, 1 << 10, seed=98, threads=threads, bvh=True)
+    n = 1 << 14
+    while True:
+        t0 = time.perf_counter()
+        run.trace(src, n, seed=99, threads=threads, bvh=True)
+        dt = time.perf_counter() - t0
+        if dt > 1.0 or n >= 1 << 22:
+            break
+        n *= 4
     n = int(min(1 << 24, max(n, n * seconds / max(dt, 1e-3))))
     t0 = time.perf_counter()
     run.trace(src, n, seed=100, threads=threads, bvh=True)
@@ -78,6 +84,19 @@ def cpu_baseline(det, src, seconds=12.0):
                 sample=f"{n} photons of the same workload (1,000-PMT tank, 1 m Cherenkov track), FP64 oracle with the "
                        "reference's BVH walk, "
                        f"{dt:.1f} s on {threads} threads")
+
+
+def copy_calibration(nbytes=1 << 30, repeats=3):
+    """A plain device-to-device copy of known size, the scale for the HBM byte counters (FETCH_SIZE, WRITE_SIZE)
+    of the photon-loop kernel: run under the same counters, it reads and writes ``nbytes`` per dispatch."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(repeats):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": repeats}), flush=True)
 
 
 def main():
@@ -92,10 +111,15 @@ def main():
     ap.add_argument("--steps-adjoint", type=int, default=3)
     ap.add_argument("--batch-repeats", type=int, default=3)
     ap.add_argument("--work-model", action="store_true", help="recompute and store the op-count work model")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (the scale of the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run (the photon loop shards by photon "
-              "id ranges with one all-reduce of the FP64 tallies; see DESIGN.md)", file=sys.stderr)
+              "id ranges with one all-reduce of the FP64 tallies)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -107,7 +107,7 @@ typedef struct {
   int32_t max_vertices;  /* the guard (steps per photon) */
   int32_t chunk;         /* photons a wavefront takes from the work queue at once (0: the engine's default) */
   int32_t blocks_per_cu; /* resident blocks of 256 per CU of the persistent grid (0: as many as fit) */
-  int32_t reserved;
+  int32_t tally_global;  /* 1: accumulate the tallies with global atomics even where they fit in LDS; else 0 */
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.
@@ -139,19 +139,24 @@ int lh_adjoint(lh_model model, const float *values_dev, const float *ref_values_
                const lh_controls *controls, const double *cotangent_dev, int32_t n_events, double *grad_dev,
                int32_t n_grad, double *health_dev, void *stream);
 
-/* A launch split for graph capture: lh_stage uploads and checks the sources (it may synchronise); lh_launch and
- * lh_launch_adjoint then only enqueue kernels on `stream` (no allocation, no host synchronisation), so a caller
- * can capture them in a HIP graph and replay it with new values, seeds and output buffers.  lh_stage_slots checks
- * the adjoint's slots once.  The staged sources have buffers of their own: lh_trace / lh_adjoint leave them as
- * they are. */
-int lh_stage(lh_model model, const lh_source *sources, int32_t n_sources, int64_t *total_photons);
-int lh_stage_slots(lh_model model, const int32_t *slots_dev, int64_t n_values, int32_t n_grad);
-int lh_launch(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values, uint64_t seed,
-              const lh_controls *controls, double *tally_dev, int32_t n_events, double *health_dev, void *stream);
-int lh_launch_adjoint(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
-                      const int32_t *slots_dev, uint64_t seed, const lh_controls *controls,
-                      const double *cotangent_dev, int32_t n_events, double *grad_dev, int32_t n_grad,
-                      double *health_dev, void *stream);
+/* A staged batch, for launches a caller captures in a HIP graph.  lh_batch_create uploads and checks the sources
+ * (and, for the adjoint, the gradient slots: NULL slots_dev for a forward-only batch); it allocates and may
+ * synchronise.  lh_batch_trace / lh_batch_adjoint then only enqueue kernels on `stream` (no allocation, no host
+ * synchronisation), so a caller can capture them and replay the graph with new values and seeds written in place.
+ * Each batch owns its sources and its device work queue: batches and lh_trace calls do not share state, but one
+ * batch's launches must not overlap each other (they share its queue).  Replaces the reference's per-source
+ * simulate loop over a Batch's rows (src/lumen/api.py:932-934, src/lumen/run.py:93-145) with one persistent
+ * launch; the tallies are those of lh_trace over the same rows. */
+typedef struct lh_batch_s *lh_batch;
+int lh_batch_create(lh_model model, const lh_source *sources, int32_t n_sources, const int32_t *slots_dev,
+                    int64_t n_values, int32_t n_grad, lh_batch *out, int64_t *total_photons);
+int lh_batch_destroy(lh_batch batch);
+int lh_batch_trace(lh_batch batch, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+                   uint64_t seed, const lh_controls *controls, double *tally_dev, int32_t n_events, double *health_dev,
+                   void *stream);
+int lh_batch_adjoint(lh_batch batch, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+                     uint64_t seed, const lh_controls *controls, const double *cotangent_dev, int32_t n_events,
+                     double *grad_dev, int32_t n_grad, double *health_dev, void *stream);
 
 /* The navigator on n rays (device arrays): from x (n*3) along d (n*3) in navigator id vol (-1: located here), the
  * photon sitting on face `last` (-1: none), the next visible crossing within tmax (<= 0: the reach): its distance

@@ -51,13 +51,14 @@ class LhSource(C.Structure):
 
 class LhControls(C.Structure):
     _fields_ = [("roulette", C.c_float), ("max_distance", C.c_float), ("max_vertices", C.c_int32),
-                ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32), ("reserved", C.c_int32)]
+                ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32), ("tally_global", C.c_int32)]
 
 
 _LIB = None
 
 SYMBOLS = ("lh_model_create", "lh_model_destroy", "lh_model_info", "lh_trace", "lh_adjoint", "lh_query",
-           "lh_stage", "lh_stage_slots", "lh_launch", "lh_launch_adjoint", "lh_last_kernel_ms", "lh_last_error")
+           "lh_batch_create", "lh_batch_destroy", "lh_batch_trace", "lh_batch_adjoint", "lh_last_kernel_ms",
+           "lh_last_error")
 
 
 def lib():
@@ -76,13 +77,14 @@ def lib():
                                  C.c_int32, C.c_uint64, C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p,
                                  C.c_int32, C.c_void_p, C.c_void_p]
         L.lh_query.argtypes = [C.c_void_p, C.c_int32] + [C.c_void_p] * 4 + [C.c_float] + [C.c_void_p] * 6
-        L.lh_stage.argtypes = [C.c_void_p, C.POINTER(LhSource), C.c_int32, C.POINTER(C.c_int64)]
-        L.lh_stage_slots.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]
-        L.lh_launch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64, C.POINTER(LhControls),
-                                C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
-        L.lh_launch_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_uint64,
-                                        C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
-                                        C.c_void_p, C.c_void_p]
+        L.lh_batch_create.argtypes = [C.c_void_p, C.POINTER(LhSource), C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+        L.lh_batch_destroy.argtypes = [C.c_void_p]
+        L.lh_batch_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                     C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.lh_batch_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                       C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                       C.c_void_p, C.c_void_p]
         L.lh_last_kernel_ms.argtypes = [C.c_void_p, _f]
         L.lh_last_error.restype = C.c_char_p
         for name in SYMBOLS:

@@ -38,11 +38,14 @@ class BatchRun:
         self.n_grad = max(self.pk.n_grad, 1)
         self.rows, self.n_ev = source_rows(det, batch, None)
         total = _lib.C.c_int64()
-        _lib.check(_lib.lib().lh_stage(self.eng.handle, self.rows, len(self.rows), _lib.C.byref(total)), "lh_stage")
+        handle = _lib.C.c_void_p()
+        slots = _lib.C.c_void_p(self.slots.data_ptr()) if grad_keys else None
+        _lib.check(_lib.lib().lh_batch_create(self.eng.handle, self.rows, len(self.rows), slots,
+                                              int(self.values.numel()), self.n_grad, _lib.C.byref(handle),
+                                              _lib.C.byref(total)), "lh_batch_create")
+        self.handle = handle
         self.total = total.value
-        if grad_keys:
-            _lib.check(_lib.lib().lh_stage_slots(self.eng.handle, _lib.C.c_void_p(self.slots.data_ptr()),
-                                                 int(self.values.numel()), self.n_grad), "lh_stage_slots")
+        self.has_grad = bool(grad_keys)
         self.seed = int(seed)
         self.tally = torch.zeros((self.n_ev, self.eng.stride), dtype=torch.float64, device=dev)
         self.health = torch.zeros(_lib.HEALTH_N, dtype=torch.float64, device=dev)
@@ -55,30 +58,42 @@ class BatchRun:
         self._g_fwd = self._g_both = None
         self._loss = None
 
+    def close(self):
+        """Release the batch's device buffers (its sources and work queue); the captured graphs go with them."""
+        if getattr(self, "handle", None):
+            self._g_fwd = self._g_both = None
+            _lib.lib().lh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
+
     # ------------------------------------------------------------------------------------------ raw launches
     def _launch(self, ctl=None):
         import torch
 
         st = torch.cuda.current_stream(self.eng.device)
         c = (ctl or self.controls).struct()
-        _lib.check(_lib.lib().lh_launch(self.eng.handle, _lib.C.c_void_p(self.values.data_ptr()), None,
-                                        int(self.values.numel()), _lib.C.c_uint64(self.seed), _lib.C.byref(c),
-                                        _lib.C.c_void_p(self.tally.data_ptr()), self.n_ev,
-                                        _lib.C.c_void_p(self.health.data_ptr()), _lib.C.c_void_p(st.cuda_stream)),
-                   "lh_launch")
+        _lib.check(_lib.lib().lh_batch_trace(self.handle, _lib.C.c_void_p(self.values.data_ptr()), None,
+                                             int(self.values.numel()), _lib.C.c_uint64(self.seed), _lib.C.byref(c),
+                                             _lib.C.c_void_p(self.tally.data_ptr()), self.n_ev,
+                                             _lib.C.c_void_p(self.health.data_ptr()), _lib.C.c_void_p(st.cuda_stream)),
+                   "lh_batch_trace")
 
     def _launch_adjoint(self, ctl=None):
         import torch
 
         st = torch.cuda.current_stream(self.eng.device)
         c = (ctl or self.controls).struct()
-        _lib.check(_lib.lib().lh_launch_adjoint(self.eng.handle, _lib.C.c_void_p(self.values.data_ptr()), None,
-                                                int(self.values.numel()), _lib.C.c_void_p(self.slots.data_ptr()),
-                                                _lib.C.c_uint64(self.seed), _lib.C.byref(c),
-                                                _lib.C.c_void_p(self.cot.data_ptr()), self.n_ev,
-                                                _lib.C.c_void_p(self.grad.data_ptr()), self.n_grad,
-                                                _lib.C.c_void_p(self.adj_health.data_ptr()),
-                                                _lib.C.c_void_p(st.cuda_stream)), "lh_launch_adjoint")
+        _lib.check(_lib.lib().lh_batch_adjoint(self.handle, _lib.C.c_void_p(self.values.data_ptr()), None,
+                                               int(self.values.numel()), _lib.C.c_uint64(self.seed), _lib.C.byref(c),
+                                               _lib.C.c_void_p(self.cot.data_ptr()), self.n_ev,
+                                               _lib.C.c_void_p(self.grad.data_ptr()), self.n_grad,
+                                               _lib.C.c_void_p(self.adj_health.data_ptr()),
+                                               _lib.C.c_void_p(st.cuda_stream)), "lh_batch_adjoint")
 
     def _fwd_body(self, ctl=None):
         self.tally.zero_()  # a fill kernel (graph-safe), not a memset

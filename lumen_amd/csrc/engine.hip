@@ -15,6 +15,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,16 @@ constexpr int MAX_FACET_ATTEMPTS = 4096;
 constexpr int BLOCK = 256;
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
+constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
+
+#ifndef LH_WAVES_PER_EU
+#define LH_WAVES_PER_EU 0
+#endif
+#if LH_WAVES_PER_EU > 0
+#define LH_OCC __attribute__((amdgpu_waves_per_eu(LH_WAVES_PER_EU)))
+#else
+#define LH_OCC
+#endif
 
 struct Args {
   Geo g;
@@ -62,6 +73,7 @@ struct Args {
   double *__restrict__ health;
   unsigned long long *__restrict__ queue;
   const float *__restrict__ ut;  // the unified facet tables (unified_tables.cpp)
+  int lds_tally;                 // the block's tallies live in LDS (flushed once at exit), else global atomics
 };
 
 struct Photon {
@@ -89,12 +101,27 @@ struct Acc {  // per-lane health accumulators, flushed once per launch
   int records, spills, vertices, photons;
 };
 
+// a[o] by selects (a register array indexed at run time would live in scratch)
+__device__ __forceinline__ float pick6(const float a[6], int o) {
+  float r = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) r = (k == o) ? a[k] : r;
+  return r;
+}
+
 __device__ __forceinline__ int class_medium(const Args &a, int cls) {
   return cls < 0 ? -1 : a.L[a.L[L_CLASS_MEDIUM_AT] + cls];
 }
 
+extern __shared__ double lds_cells[];  // the block's tallies (lds_tally) or gradient accumulators (the adjoint)
+
 __device__ __forceinline__ void tally_add(const Args &a, int event, int slot, double v) {
-  unsafeAtomicAdd(a.tally + (long long)event * a.tally_stride + slot, v);
+  const long long i = (long long)event * a.tally_stride + slot;
+  if (a.lds_tally) {
+    atomicAdd(lds_cells + i, v);
+  } else {
+    unsafeAtomicAdd(a.tally + i, v);
+  }
 }
 
 __device__ inline void moment_flush_slot(const Args &a, Photon &p, int k) {
@@ -128,8 +155,13 @@ __device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, flo
     free_k = MOMENT_SLOTS - 1;
     acc.spills += 1;
   }
-  p.ms[free_k] = s;
-  p.mv[free_k] = v;
+#pragma unroll
+  for (int k = 0; k < MOMENT_SLOTS; ++k) {  // by selects: a register array indexed at run time spills to memory
+    if (k == free_k) {
+      p.ms[k] = s;
+      p.mv[k] = v;
+    }
+  }
 }
 
 // one deposit at sensor s: arrival, arrival_t, the charge cells at the QE knots, the photon's moment
@@ -474,8 +506,8 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       }
     }
     if (ok) {
-      const float w_if = w_at * att * ratio(sl.p[o], pprop[o]);
-      const float r_o = ratio(pdraw[o], pprop[o]);
+      const float w_if = w_at * att * ratio(pick6(sl.p, o), pick6(pprop, o));
+      const float r_o = ratio(pick6(pdraw, o), pick6(pprop, o));
       wr_new = p.wr * (reach_ref != 0.0f ? 1.0f : 0.0f) * att_r * r_o;
       float3 dn = p.d;
       const float3 wi = p.d, n = cr.n;
@@ -607,7 +639,11 @@ __global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const fl
   srf_out[i] = c.srf;
 }
 
-__global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
+__global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) {
+  if (a.lds_tally) {
+    for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
+    __syncthreads();
+  }
   Photon p;
   Acc acc{};
   bool alive = false;
@@ -647,11 +683,16 @@ __global__ void __launch_bounds__(BLOCK) trace_kernel(Args a) {
     }
   }
   flush_health(a, acc);
+  if (a.lds_tally) {  // one global atomic per touched cell per block
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_lds; i += blockDim.x)
+      if (lds_cells[i] != 0.0) unsafeAtomicAdd(a.tally + i, lds_cells[i]);
+  }
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
-__global__ void __launch_bounds__(BLOCK) adjoint_kernel(Args a, AdjArgs b) {
-  extern __shared__ double sg[];
+__global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b) {
+  double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
   __syncthreads();
   Photon p;
@@ -733,18 +774,13 @@ struct lh_model_s {
   lh_source *src = nullptr;
   long long *src_end = nullptr;
   int src_cap = 0;
-  int grid = 0, grid_adj = 0;
+
   long long values_needed = 0;  // the packed values a call must pass (the largest table's end)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int timed = 0;
   float *ut = nullptr;  // the unified facet tables on the device
-  // the staged launch (lh_stage): its sources' count, photons, largest event; the adjoint's checked slots
-  SrcBuf staged;  // the staged launch's own source buffers (lh_trace never touches them)
-  int staged_nsrc = 0, staged_events = 0;
-  long long staged_total = -1;
-  const int32_t *staged_slots = nullptr;
-  int staged_ngrad = 0;
-  int cus = 0, per_trace = 1, per_adj = 1;
+  int cus = 0, per_trace = 0, per_adj = 0;
+  size_t per_trace_lds = 0, per_adj_lds = 0;
 };
 
 static thread_local std::string g_err;
@@ -966,16 +1002,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     HIP_TRY(hipMalloc((void **)&m->ut, ut.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(m->ut, ut.data(), ut.size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  int dev = 0, cus = 0, per = 0;
+  int dev = 0, cus = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_kernel, BLOCK, 0));
-  m->grid = cus * (per > 0 ? per : 1);
   m->cus = cus;
-  m->per_trace = per > 0 ? per : 1;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adjoint_kernel, BLOCK, 8 * 1024));
-  m->grid_adj = cus * (per > 0 ? per : 1);
-  m->per_adj = per > 0 ? per : 1;
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
   *out = m;
@@ -984,16 +1014,14 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
 
 LH_API int lh_model_destroy(lh_model m) {
   if (!m) return LH_OK;
-  hipFree(m->geo_buf);
-  hipFree(m->layout);
-  hipFree(m->queue);
-  hipFree(m->src);
-  hipFree(m->src_end);
-  hipFree(m->ut);
-  hipFree(m->staged.src);
-  hipFree(m->staged.src_end);
-  if (m->ev0) hipEventDestroy(m->ev0);
-  if (m->ev1) hipEventDestroy(m->ev1);
+  (void)hipFree(m->geo_buf);
+  (void)hipFree(m->layout);
+  (void)hipFree(m->queue);
+  (void)hipFree(m->src);
+  (void)hipFree(m->src_end);
+  (void)hipFree(m->ut);
+  if (m->ev0) (void)hipEventDestroy(m->ev0);
+  if (m->ev1) (void)hipEventDestroy(m->ev1);
   delete m;
   return LH_OK;
 }
@@ -1036,8 +1064,8 @@ static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_
     ends[i] = acc;
   }
   if (n_sources > buf.cap) {
-    hipFree(buf.src);
-    hipFree(buf.src_end);
+    (void)hipFree(buf.src);
+    (void)hipFree(buf.src_end);
     buf.src = nullptr;
     buf.src_end = nullptr;
     buf.cap = 0;
@@ -1079,15 +1107,34 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.health = health;
   a.queue = m->queue;
   a.ut = m->ut;
+  a.lds_tally = 0;
   a.src = m->src;
   a.src_end = m->src_end;
   return LH_OK;
 }
 
-static int grid_of(lh_model m, const lh_controls *ctl, bool adj) {
-  const int per = adj ? m->per_adj : m->per_trace;
+static int grid_of(lh_model m, const lh_controls *ctl, bool adj, size_t lds_bytes) {
+  int &per = adj ? m->per_adj : m->per_trace;  // resident blocks per CU at the last LDS size asked
+  size_t &at = adj ? m->per_adj_lds : m->per_trace_lds;
+  if (per <= 0 || at != lds_bytes) {
+    hipError_t e = adj ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adjoint_kernel, BLOCK, lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_kernel, BLOCK, lds_bytes);
+    if (e != hipSuccess || per < 1) per = 1;
+    at = lds_bytes;
+  }
   const int want = (ctl && ctl->blocks_per_cu > 0) ? std::min(ctl->blocks_per_cu, per) : per;
   return m->cus * want;
+}
+
+// the forward launch: the tallies in LDS when they fit
+static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *ctl, hipStream_t st) {
+  const long long cells = (long long)n_events * m->tally_stride;
+  const bool lds = cells <= LDS_TALLY_CELLS && !(ctl && ctl->tally_global == 1);
+  a.lds_tally = lds ? 1 : 0;
+  const size_t bytes = lds ? (size_t)cells * sizeof(double) : 0;
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
+  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false, bytes)), dim3(BLOCK), bytes, st, a,
+                     (int)(lds ? cells : 0));
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
@@ -1106,10 +1153,8 @@ LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n
   if (rc) return rc;
   a.nsrc = n_sources;
   a.total = total;
-  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
-  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false)), dim3(BLOCK), 0, st, a);
+  launch_trace(m, a, n_events, ctl, st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
   m->timed = 1;
@@ -1146,83 +1191,118 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true)), dim3(BLOCK), (size_t)n_grad * sizeof(double), st, a,
-                     b);
+  a.lds_tally = 0;
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, (size_t)n_grad * sizeof(double))), dim3(BLOCK),
+                     (size_t)n_grad * sizeof(double), st, a, b);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
   m->timed = 1;
   return LH_OK;
 }
 
-LH_API int lh_stage(lh_model m, const lh_source *sources, int32_t n_sources, int64_t *total) {
-  if (!m) return fail(LH_ERR_ARG, "lh_stage: null model");
+// a staged batch (lh_batch_create): its own sources, work queue and checked gradient slots
+struct lh_batch_s {
+  lh_model m = nullptr;
+  SrcBuf src;
+  unsigned long long *queue = nullptr;
+  int nsrc = 0, events = 0;
+  long long total = 0;
+  const int32_t *slots = nullptr;
+  int ngrad = 0;
+};
+
+LH_API int lh_batch_create(lh_model m, const lh_source *sources, int32_t n_sources, const int32_t *slots_dev,
+                           int64_t n_values, int32_t n_grad, lh_batch *out, int64_t *total) {
+  if (!m || !out) return fail(LH_ERR_ARG, "lh_batch_create: null model or output");
+  *out = nullptr;
+  std::vector<int32_t> hs;
+  if (slots_dev) {
+    if (n_grad < 1 || n_grad > MAX_GRAD) return fail(LH_ERR_ARG, "lh_batch_create: n_grad out of range (1 .. 8192)");
+    if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_batch_create: fewer packed values than the layout reads");
+    hs.resize((size_t)n_values);
+    HIP_TRY(hipMemcpy(hs.data(), slots_dev, sizeof(int32_t) * (size_t)n_values, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n_values; ++i)
+      if (hs[i] < -1 || hs[i] >= n_grad) return fail(LH_ERR_ARG, "lh_batch_create: a gradient slot is out of range");
+  }
+  std::unique_ptr<lh_batch_s> b(new lh_batch_s());
+  b->m = m;
   long long tot = 0;
-  int rc = stage_into(m->staged, m->n_media, sources, n_sources, &tot, nullptr);
-  if (rc) return rc;
+  int rc = stage_into(b->src, m->n_media, sources, n_sources, &tot, nullptr);
+  if (rc) {
+    (void)hipFree(b->src.src);
+    (void)hipFree(b->src.src_end);
+    return rc;
+  }
+  if (hipMalloc((void **)&b->queue, 256) != hipSuccess) {
+    (void)hipFree(b->src.src);
+    (void)hipFree(b->src.src_end);
+    return fail(LH_ERR_HIP, "lh_batch_create: hipMalloc of the work queue failed");
+  }
   int ev = 0;
   for (int i = 0; i < n_sources; ++i) ev = std::max(ev, sources[i].event + 1);
-  m->staged_nsrc = n_sources;
-  m->staged_total = tot;
-  m->staged_events = ev;
+  b->nsrc = n_sources;
+  b->total = tot;
+  b->events = ev;
+  b->slots = slots_dev;
+  b->ngrad = slots_dev ? n_grad : 0;
   if (total) *total = tot;
+  *out = b.release();
   return LH_OK;
 }
 
-LH_API int lh_stage_slots(lh_model m, const int32_t *slots_dev, int64_t n_values, int32_t n_grad) {
-  if (!m || !slots_dev) return fail(LH_ERR_ARG, "lh_stage_slots: null argument");
-  if (n_grad < 1 || n_grad > MAX_GRAD) return fail(LH_ERR_ARG, "lh_stage_slots: n_grad out of range (1 .. 8192)");
-  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_stage_slots: fewer packed values than the layout reads");
-  std::vector<int32_t> hs((size_t)n_values);
-  HIP_TRY(hipMemcpy(hs.data(), slots_dev, sizeof(int32_t) * (size_t)n_values, hipMemcpyDeviceToHost));
-  for (long long i = 0; i < n_values; ++i)
-    if (hs[i] < -1 || hs[i] >= n_grad) return fail(LH_ERR_ARG, "lh_stage_slots: a gradient slot is out of range");
-  m->staged_slots = slots_dev;
-  m->staged_ngrad = n_grad;
+LH_API int lh_batch_destroy(lh_batch b) {
+  if (!b) return LH_OK;
+  (void)hipFree(b->src.src);
+  (void)hipFree(b->src.src_end);
+  (void)hipFree(b->queue);
+  delete b;
   return LH_OK;
 }
 
-LH_API int lh_launch(lh_model m, const float *values, const float *ref, int64_t n_values, uint64_t seed,
-                     const lh_controls *ctl, double *tally, int32_t n_events, double *health, void *stream) {
-  if (!m) return fail(LH_ERR_ARG, "lh_launch: null model");
-  if (m->staged_total < 0) return fail(LH_ERR_ARG, "lh_launch: no sources staged (lh_stage)");
-  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_launch: fewer packed values than the layout reads");
-  if (n_events < m->staged_events) return fail(LH_ERR_ARG, "lh_launch: a staged source's event is beyond n_events");
+LH_API int lh_batch_trace(lh_batch b, const float *values, const float *ref, int64_t n_values, uint64_t seed,
+                          const lh_controls *ctl, double *tally, int32_t n_events, double *health, void *stream) {
+  if (!b) return fail(LH_ERR_ARG, "lh_batch_trace: null batch");
+  lh_model m = b->m;
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_batch_trace: fewer packed values than the layout reads");
+  if (n_events < b->events) return fail(LH_ERR_ARG, "lh_batch_trace: a staged source's event is beyond n_events");
   hipStream_t st = (hipStream_t)stream;
   Args a;
   int rc = fill_args(m, a, values, ref, ctl, seed, tally, health);
   if (rc) return rc;
-  a.src = m->staged.src;
-  a.src_end = m->staged.src_end;
-  a.nsrc = m->staged_nsrc;
-  a.total = m->staged_total;
-  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
-  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false)), dim3(BLOCK), 0, st, a);
+  a.src = b->src.src;
+  a.src_end = b->src.src_end;
+  a.nsrc = b->nsrc;
+  a.total = b->total;
+  a.queue = b->queue;
+  launch_trace(m, a, n_events, ctl, st);
   HIP_TRY(hipGetLastError());
   return LH_OK;
 }
 
-LH_API int lh_launch_adjoint(lh_model m, const float *values, const float *ref, int64_t n_values,
-                             const int32_t *slots_dev, uint64_t seed, const lh_controls *ctl, const double *cot,
-                             int32_t n_events, double *grad, int32_t n_grad, double *health, void *stream) {
-  if (!m) return fail(LH_ERR_ARG, "lh_launch_adjoint: null model");
-  if (m->staged_total < 0) return fail(LH_ERR_ARG, "lh_launch_adjoint: no sources staged (lh_stage)");
-  if (!slots_dev || slots_dev != m->staged_slots || n_grad != m->staged_ngrad)
-    return fail(LH_ERR_ARG, "lh_launch_adjoint: the slots were not checked (lh_stage_slots)");
-  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_launch_adjoint: fewer packed values than the layout reads");
-  if (n_events < m->staged_events) return fail(LH_ERR_ARG, "lh_launch_adjoint: a staged source's event is beyond n_events");
-  if (!cot || !grad) return fail(LH_ERR_ARG, "lh_launch_adjoint: null cotangent or gradient");
+LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, int64_t n_values, uint64_t seed,
+                            const lh_controls *ctl, const double *cot, int32_t n_events, double *grad, int32_t n_grad,
+                            double *health, void *stream) {
+  if (!b) return fail(LH_ERR_ARG, "lh_batch_adjoint: null batch");
+  lh_model m = b->m;
+  if (!b->slots) return fail(LH_ERR_ARG, "lh_batch_adjoint: the batch was created without gradient slots");
+  if (n_grad != b->ngrad) return fail(LH_ERR_ARG, "lh_batch_adjoint: n_grad differs from the batch's");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_batch_adjoint: fewer packed values than the layout reads");
+  if (n_events < b->events) return fail(LH_ERR_ARG, "lh_batch_adjoint: a staged source's event is beyond n_events");
+  if (!cot || !grad) return fail(LH_ERR_ARG, "lh_batch_adjoint: null cotangent or gradient");
   hipStream_t st = (hipStream_t)stream;
   Args a;
   int rc = fill_args(m, a, values, ref, ctl, seed, const_cast<double *>(cot), health);
   if (rc) return rc;
-  a.src = m->staged.src;
-  a.src_end = m->staged.src_end;
-  a.nsrc = m->staged_nsrc;
-  a.total = m->staged_total;
-  AdjArgs b{cot, slots_dev, grad, n_grad};
-  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true)), dim3(BLOCK), (size_t)n_grad * sizeof(double), st, a,
-                     b);
+  a.src = b->src.src;
+  a.src_end = b->src.src_end;
+  a.nsrc = b->nsrc;
+  a.total = b->total;
+  a.queue = b->queue;
+  AdjArgs g{cot, b->slots, grad, n_grad};
+  hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
+  a.lds_tally = 0;
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, (size_t)n_grad * sizeof(double))), dim3(BLOCK),
+                     (size_t)n_grad * sizeof(double), st, a, g);
   HIP_TRY(hipGetLastError());
   return LH_OK;
 }

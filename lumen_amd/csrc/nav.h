@@ -237,6 +237,7 @@ struct Change {
   int leaf, face;  // leaf within its kind; -1: none
 };
 
+
 // the first change of the solid's membership along the ray after t = 0, from the side `want_from` (true: the
 // point starts inside and the change is an exit; false: an entry).  Endpoints on the face `skip_code` within
 // `skip_t` of the start count as passed.  `vol` names the volume in the surface codes.

@@ -84,3 +84,21 @@ def test_adjoint_matches_crn_finite_differences_water_tank(gpu):
         for i in rng.choice(len(base), min(3, len(base)), replace=False):
             fd = _crn_fd(det, src, k, base, i, 1 << 20, 3, qe=qe)
             assert abs(g[i] - fd) <= 5e-3 * max(abs(fd), 1e-2 * np.abs(g).max()), (k, i, g[i], fd)
+
+
+def test_sigma_alpha_gradient_matches_crn_finite_differences(gpu):
+    """The liner's facet slope width: its derivative runs through the unified facet path's density ratio and the
+    lobe's F/lz tables, the engine's only density-ratio gradient."""
+    det = water_tank()
+    p0 = det.params()
+    x0, d = muon_track()
+    src = L.sources.CherenkovSegment("Water", start=tuple(x0), direction=tuple(d), length=1.0)
+    qe = torch.as_tensor(p0["sensors/pmt/qe"], device="cuda")
+    k = "surfaces/Liner/sigma_alpha"
+    s = table_tensor(p0, k)
+    light = simulate_diff(det, src, {k: s}, samples=1 << 20, seed=8)
+    (light.charge[..., 0] * qe).sum().backward()
+    g = float(s.grad.cpu().numpy()[0])
+    base = np.atleast_1d(np.asarray(p0[k], np.float64))
+    fd = _crn_fd(det, src, k, base, 0, 1 << 20, 8, h=1e-2, qe=qe)
+    assert abs(g - fd) <= 1e-2 * abs(fd) + 1e-3, (g, fd)
