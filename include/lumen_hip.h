@@ -108,6 +108,8 @@ typedef struct {
   int32_t chunk;         /* photons a wavefront takes from the work queue at once (0: the engine's default) */
   int32_t blocks_per_cu; /* resident blocks of 256 per CU of the persistent grid (0: as many as fit) */
   int32_t tally_global;  /* 1: accumulate the tallies with global atomics even where they fit in LDS; else 0 */
+  int32_t nodes_global;  /* 1: read the BVH nodes from global memory even where they fit in LDS; else 0 */
+  int32_t reserved;      /* 0 */
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.

@@ -30,7 +30,8 @@ class Controls(NamedTuple):
     """The step's controls (``transport/step.py:52-71``): ``max_distance`` (m; None: the geometry's reach),
     ``roulette`` (epsilon > 0), ``max_vertices`` (the guard), ``chunk`` (photons per work-queue claim; 0 the
     engine's default), ``blocks_per_cu`` (the persistent grid's resident blocks per CU; 0 as many as fit),
-    ``tally_global`` (accumulate the tallies with global atomics even where a block's copy fits in LDS)."""  # noqa: D205, D209
+    ``tally_global`` (accumulate the tallies with global atomics even where a block's copy fits in LDS),
+    ``nodes_global`` (read the BVH nodes from global memory even where they fit in the block's LDS)."""  # noqa: D205, D209
 
     max_distance: Any = None
     roulette: float = ROULETTE
@@ -38,13 +39,14 @@ class Controls(NamedTuple):
     chunk: int = 0
     blocks_per_cu: int = 0
     tally_global: bool = False
+    nodes_global: bool = False
 
     def struct(self) -> _lib.LhControls:
         if not (float(self.roulette) > 0.0):
             raise ValueError(f"Controls: the roulette is a number > 0, got {self.roulette!r}")
         return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
                                int(self.max_vertices), int(self.chunk), int(self.blocks_per_cu),
-                               int(bool(self.tally_global)))
+                               int(bool(self.tally_global)), int(bool(self.nodes_global)), 0)
 
 
 class Health(NamedTuple):

@@ -38,12 +38,15 @@ constexpr int ZERO_STEPS_BEFORE_KILL = 25;
 constexpr float PUSH_BANDS = 8.0f;
 constexpr float FINE_STRUCTURE = 1.0f / 137.035999084f;
 constexpr int MOMENT_SLOTS = 4;
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int MAX_FACETS = 256;
 constexpr int MAX_FACET_ATTEMPTS = 4096;
 constexpr int BLOCK = 256;
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
+constexpr size_t LDS_BLOCK_BUDGET = 80 * 1024;  // a block's LDS: two blocks of 256 (8 waves) share a CU's 160 KiB
 
 #ifndef LH_WAVES_PER_EU
 #define LH_WAVES_PER_EU 0
@@ -85,8 +88,9 @@ struct Photon {
   int mcls;
   MediumAt ml, mr;
   // the per-photon moment: up to MOMENT_SLOTS sensors and the photon's summed deposits at each
-  int ms[MOMENT_SLOTS];
-  float mv[MOMENT_SLOTS];
+  // (clang vectors: an index known only at run time selects within registers, where an array would go to memory)
+  i4v ms;
+  f4v mv;
   // the adjoint's walk: the launch index (to replay), the pass (0 totals, 1 accumulates), the photon's total of
   // cotangent-weighted deposits T (from pass 0), their running sum, and the rates' derivative caches
   long long gi;
@@ -113,7 +117,6 @@ __device__ __forceinline__ int class_medium(const Args &a, int cls) {
   return cls < 0 ? -1 : a.L[a.L[L_CLASS_MEDIUM_AT] + cls];
 }
 
-extern __shared__ double lds_cells[];  // the block's tallies (lds_tally) or gradient accumulators (the adjoint)
 
 __device__ __forceinline__ void tally_add(const Args &a, int event, int slot, double v) {
   const long long i = (long long)event * a.tally_stride + slot;
@@ -134,34 +137,22 @@ __device__ inline void moment_flush_slot(const Args &a, Photon &p, int k) {
 }
 
 __device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, float v) {
-  int free_k = -1;
-#pragma unroll
-  for (int k = 0; k < MOMENT_SLOTS; ++k) {
-    if (p.ms[k] == s) {
-      p.mv[k] += v;
-      return;
-    }
-    if (free_k < 0 && p.ms[k] < 0) free_k = k;
+  const i4v hit = p.ms == s;  // lanes of -1 where the slot holds sensor s
+  if (hit.x | hit.y | hit.z | hit.w) {
+    p.mv += hit ? v : 0.0f;
+    return;
   }
+  const i4v freev = p.ms < 0;
+  int free_k = freev.x ? 0 : freev.y ? 1 : freev.z ? 2 : freev.w ? 3 : -1;
   if (free_k < 0) {  // more sensors than slots: square the oldest now (its later deposits start a new sum)
     moment_flush_slot(a, p, 0);
-#pragma unroll
-    for (int k = 0; k + 1 < MOMENT_SLOTS; ++k) {
-      p.ms[k] = p.ms[k + 1];
-      p.mv[k] = p.mv[k + 1];
-    }
-    p.ms[MOMENT_SLOTS - 1] = -1;
-    p.mv[MOMENT_SLOTS - 1] = 0.0f;
+    p.ms = i4v{p.ms.y, p.ms.z, p.ms.w, -1};
+    p.mv = f4v{p.mv.y, p.mv.z, p.mv.w, 0.0f};
     free_k = MOMENT_SLOTS - 1;
     acc.spills += 1;
   }
-#pragma unroll
-  for (int k = 0; k < MOMENT_SLOTS; ++k) {  // by selects: a register array indexed at run time spills to memory
-    if (k == free_k) {
-      p.ms[k] = s;
-      p.mv[k] = v;
-    }
-  }
+  p.ms[free_k] = s;
+  p.mv[free_k] = v;
 }
 
 // one deposit at sensor s: arrival, arrival_t, the charge cells at the QE knots, the photon's moment
@@ -254,11 +245,8 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   p.nz = 0;
   p.vertex = 0;
   p.mcls = -2;
-#pragma unroll
-  for (int k = 0; k < MOMENT_SLOTS; ++k) {
-    p.ms[k] = -1;
-    p.mv[k] = 0.0f;
-  }
+  p.ms = i4v{-1, -1, -1, -1};
+  p.mv = f4v{0.0f, 0.0f, 0.0f, 0.0f};
   acc.photons += 1;
   acc.start_w += p.w0;
   if (p.vol >= a.g.nvol || class_medium(a, p.cls) < 0) {  // emitted outside the detector or in an opaque medium
@@ -588,15 +576,13 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   p.wr = wr_new;
   p.vertex += 1;
   acc.vertices += 1;
-  if (alive && p.nz >= ZERO_STEPS_BEFORE_KILL) {
-    acc.aband += p.w * p.w0;
-    alive = false;
-  }
-  if (alive && p.vertex >= a.max_vert) {
-    acc.trunc += p.w * p.w0;
-    alive = false;
-  }
-  return alive;
+  // written as values, not as a branch on which cell to add to: a pointer chosen at run time puts both in memory
+  const bool abandon = alive && p.nz >= ZERO_STEPS_BEFORE_KILL;
+  const bool truncate = alive && !abandon && p.vertex >= a.max_vert;
+  const float wl = p.w * p.w0;
+  acc.aband += abandon ? wl : 0.0f;
+  acc.trunc += truncate ? wl : 0.0f;
+  return alive && !abandon && !truncate;
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -639,11 +625,12 @@ __global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const fl
   srf_out[i] = c.srf;
 }
 
+// dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
 __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) {
-  if (a.lds_tally) {
+  if (a.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
-    __syncthreads();
-  }
+  stage_nodes(a.g);
+  __syncthreads();
   Photon p;
   Acc acc{};
   bool alive = false;
@@ -694,6 +681,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
 __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
+  stage_nodes(a.g);
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -805,7 +793,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   if (n <= 0 || n >= (1 << 21)) return fail(LH_ERR_ARG, "lh_model_create: n_vol out of range");
   for (int i = 0; i < G->n_nodes; ++i) {
     if (G->count[i] > 7) return fail(LH_ERR_ARG, "lh_model_create: a BVH leaf holds more than 7 items");
-    if (G->skip[i] < 0 || G->skip[i] > G->n_nodes) return fail(LH_ERR_ARG, "lh_model_create: a skip link is out of range");
+    if (G->count[i] < 0) return fail(LH_ERR_ARG, "lh_model_create: a negative leaf count");
+    // every walk moves forward (skip links point past the node's subtree), so it ends
+    if (G->skip[i] <= i || G->skip[i] > G->n_nodes)
+      return fail(LH_ERR_ARG, "lh_model_create: a skip link does not point forward within the nodes");
     if (G->count[i] > 0 && (G->start[i] < 0 || G->start[i] + G->count[i] > G->n_items))
       return fail(LH_ERR_ARG, "lh_model_create: a leaf's items are out of range");
   }
@@ -976,6 +967,9 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.items = (const int *)(buf + o_items);
   g.prog = (const int *)(buf + o_prog);
   g.reach = G->reach;
+  g.n_nodes = G->n_nodes;
+  g.lds_nodes = 0;
+  g.lds_node_off = 0;
   HIP_TRY(hipMalloc((void **)&m->layout, (size_t)layout_words * 4));
   HIP_TRY(hipMemcpy(m->layout, layout, (size_t)layout_words * 4, hipMemcpyHostToDevice));
   m->layout_words = layout_words;
@@ -1091,7 +1085,7 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.has_ref = (ref && ref != values) ? 1 : 0;
   a.k0 = (uint32_t)seed;
   a.k1 = (uint32_t)(seed >> 32);
-  lh_controls c{0.01f, 0.0f, 1 << 22, 0, 0, 0};
+  lh_controls c{0.01f, 0.0f, 1 << 22, 0, 0, 0, 0, 0};
   if (ctl) c = *ctl;
   if (c.blocks_per_cu < 0 || c.blocks_per_cu > 64) return fail(LH_ERR_ARG, "blocks_per_cu out of range (0 .. 64)");
   if (!(c.roulette > 0.0f)) return fail(LH_ERR_ARG, "the roulette is a number > 0");
@@ -1126,12 +1120,25 @@ static int grid_of(lh_model m, const lh_controls *ctl, bool adj, size_t lds_byte
   return m->cus * want;
 }
 
-// the forward launch: the tallies in LDS when they fit
+// the BVH nodes follow `cells_bytes` of a launch's LDS when both fit the block's budget; returns the LDS bytes
+static size_t place_nodes(lh_model m, Args &a, size_t cells_bytes, const lh_controls *ctl) {
+  const size_t off = (cells_bytes + 15) / 16 * 16;
+  const size_t node_bytes = (size_t)m->g.n_nodes * 32;
+  a.g.lds_nodes = 0;
+  a.g.lds_node_off = 0;
+  if (ctl && ctl->nodes_global == 1) return cells_bytes;
+  if (m->g.n_nodes == 0 || off + node_bytes > LDS_BLOCK_BUDGET) return cells_bytes;
+  a.g.lds_nodes = 1;
+  a.g.lds_node_off = (int)(off / 16);
+  return off + node_bytes;
+}
+
+// the forward launch: the tallies in LDS when they fit, then the BVH nodes when they fit too
 static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *ctl, hipStream_t st) {
   const long long cells = (long long)n_events * m->tally_stride;
   const bool lds = cells <= LDS_TALLY_CELLS && !(ctl && ctl->tally_global == 1);
   a.lds_tally = lds ? 1 : 0;
-  const size_t bytes = lds ? (size_t)cells * sizeof(double) : 0;
+  const size_t bytes = place_nodes(m, a, lds ? (size_t)cells * sizeof(double) : 0, ctl);
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
   hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false, bytes)), dim3(BLOCK), bytes, st, a,
                      (int)(lds ? cells : 0));
@@ -1192,8 +1199,8 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
   a.lds_tally = 0;
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, (size_t)n_grad * sizeof(double))), dim3(BLOCK),
-                     (size_t)n_grad * sizeof(double), st, a, b);
+  const size_t bytes = place_nodes(m, a, (size_t)n_grad * sizeof(double), ctl);
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, bytes)), dim3(BLOCK), bytes, st, a, b);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
   m->timed = 1;
@@ -1301,8 +1308,8 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
   a.lds_tally = 0;
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, (size_t)n_grad * sizeof(double))), dim3(BLOCK),
-                     (size_t)n_grad * sizeof(double), st, a, g);
+  const size_t bytes = place_nodes(m, a, (size_t)n_grad * sizeof(double), ctl);
+  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, bytes)), dim3(BLOCK), bytes, st, a, g);
   HIP_TRY(hipGetLastError());
   return LH_OK;
 }

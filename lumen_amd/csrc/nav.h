@@ -27,6 +27,9 @@ constexpr float K_ON = 4.0f;
 constexpr int MAX_CSG_LEAVES = 8;
 constexpr int MAX_GROUPS_WALK = 8;
 
+// the launch's dynamic LDS: a kernel may stage the BVH nodes there (Geo::lds_nodes) after its own cells
+extern __shared__ double lds_cells[];
+
 struct Geo {
   int nvol;
   const float4 *__restrict__ xf;    // 3 per volume: rows of R^T with the offset -R^T t (world -> local)
@@ -41,7 +44,23 @@ struct Geo {
   const int4 *__restrict__ kinds;   // l0, nl, p0, pl
   const int *__restrict__ prog;     // op << 24 | arg
   float reach;
+  int n_nodes;
+  int lds_nodes;     // 1: the kernel staged all BVH nodes in LDS at float4 offset lds_node_off of lds_cells
+  int lds_node_off;
 };
+
+// a BVH node's float4 i (2 per node) from LDS when the kernel staged them, else from global memory
+__device__ __forceinline__ float4 nav_node(const Geo &g, int i) {
+  if (g.lds_nodes) return reinterpret_cast<const float4 *>(lds_cells)[g.lds_node_off + i];
+  return g.node[i];
+}
+
+// copy the BVH nodes into the block's LDS (every thread of the block calls it; a barrier follows)
+__device__ inline void stage_nodes(const Geo &g) {
+  if (!g.lds_nodes) return;
+  float4 *dst = reinterpret_cast<float4 *>(lds_cells) + g.lds_node_off;
+  for (int i = threadIdx.x; i < 2 * g.n_nodes; i += blockDim.x) dst[i] = g.node[i];
+}
 
 __device__ __forceinline__ float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
@@ -367,39 +386,44 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
                            1.0f / (fabsf(dg.z) > 1e-30f ? dg.z : copysignf(1e-30f, dg.z)));
     const int2 nr = g.grange[gi];
     int ni = nr.x;
-    while (ni < nr.y) {
-      const float4 lo = g.node[2 * ni], hi = g.node[2 * ni + 1];
-      const float tx0 = (lo.x - xg.x) * inv.x, tx1 = (hi.x - xg.x) * inv.x;
-      const float ty0 = (lo.y - xg.y) * inv.y, ty1 = (hi.y - xg.y) * inv.y;
-      const float tz0 = (lo.z - xg.z) * inv.z, tz1 = (hi.z - xg.z) * inv.z;
-      const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-      const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
-      const int skip = __float_as_int(lo.w);
-      if (te > tl) {
-        ni = skip;
-        continue;
-      }
-      const int sc = __float_as_int(hi.w);
-      const int cnt = sc & 7;
-      if (cnt > 0) {
-        const int st = sc >> 3;
-        for (int k = 0; k < cnt; ++k) {
-          const int u = g.items[st + k];
-          float3 xl, dl;
-          to_local(g, u, x, d, xl, dl);
-          const Change ch = first_change(g, g.vinfo[u].x, u, xl, dl, false, last, skip_t);
-          if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
-            best.t = ch.t;
-            best.vol_after = u;
-            best.own = 0;
-            best_leaf = ch.leaf;
-            best_face = ch.face;
-            best_vol = u;
-          }
+    // the walk in two loops (traverse to the next leaf, then test its items): a wavefront runs the costly item
+    // tests once for all lanes that found a leaf, instead of on every node step where any one lane sits at a leaf
+    for (;;) {
+      int pst = 0, pcnt = 0;
+      while (ni < nr.y) {
+        const float4 lo = nav_node(g, 2 * ni), hi = nav_node(g, 2 * ni + 1);
+        const float tx0 = (lo.x - xg.x) * inv.x, tx1 = (hi.x - xg.x) * inv.x;
+        const float ty0 = (lo.y - xg.y) * inv.y, ty1 = (hi.y - xg.y) * inv.y;
+        const float tz0 = (lo.z - xg.z) * inv.z, tz1 = (hi.z - xg.z) * inv.z;
+        const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
+        const int skip = __float_as_int(lo.w);
+        const int sc = __float_as_int(hi.w);
+        if (te > tl) {
+          ni = skip;
+        } else if ((sc & 7) > 0) {
+          pst = sc >> 3;
+          pcnt = sc & 7;
+          ni = skip;
+          break;
+        } else {
+          ni += 1;
         }
-        ni = skip;
-      } else {
-        ni += 1;
+      }
+      if (pcnt == 0) break;
+      for (int k = 0; k < pcnt; ++k) {
+        const int u = g.items[pst + k];
+        float3 xl, dl;
+        to_local(g, u, x, d, xl, dl);
+        const Change ch = first_change(g, g.vinfo[u].x, u, xl, dl, false, last, skip_t);
+        if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
+          best.t = ch.t;
+          best.vol_after = u;
+          best.own = 0;
+          best_leaf = ch.leaf;
+          best_face = ch.face;
+          best_vol = u;
+        }
       }
     }
   }
@@ -424,7 +448,7 @@ __device__ inline int locate(const Geo &g, float3 x) {
       const int2 nr = g.grange[gi];
       int ni = nr.x;
       while (ni < nr.y && found < 0) {
-        const float4 lo = g.node[2 * ni], hi = g.node[2 * ni + 1];
+        const float4 lo = nav_node(g, 2 * ni), hi = nav_node(g, 2 * ni + 1);
         const int skip = __float_as_int(lo.w);
         if (xg.x < lo.x || xg.x > hi.x || xg.y < lo.y || xg.y > hi.y || xg.z < lo.z || xg.z > hi.z) {
           ni = skip;

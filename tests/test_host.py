@@ -30,7 +30,7 @@ def test_abi_exports_every_declared_symbol():
 def test_abi_structs_match_the_header():
     """ctypes' structs have the header's sizes (lh_source: 4 ints, 14 floats, 2 int64)."""
     assert ctypes.sizeof(_lib.LhSource) == 4 * 4 + 14 * 4 + 2 * 8
-    assert ctypes.sizeof(_lib.LhControls) == 24
+    assert ctypes.sizeof(_lib.LhControls) == 8 * 4
 
 
 def _check_bvh(t, lo, hi):
