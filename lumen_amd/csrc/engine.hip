@@ -46,7 +46,8 @@ constexpr int BLOCK = 256;
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
-constexpr size_t LDS_BLOCK_BUDGET = 80 * 1024;  // a block's LDS: two blocks of 256 (8 waves) share a CU's 160 KiB
+constexpr size_t LDS_BLOCK_MAX = 64 * 1024;
+constexpr int OCC_CACHE = 8;  // the most dynamic LDS one block of the persistent grid takes
 
 #ifndef LH_WAVES_PER_EU
 #define LH_WAVES_PER_EU 0
@@ -83,7 +84,7 @@ struct Photon {
   float3 x, d, er, ei;
   float t, lam, w, wr, w0;
   int vol, srf, nz, vertex, cls, event;
-  lh_rng rng;
+  uint32_t id_lo, id_hi;  // the photon's id: its Philox counter words (the key is the launch's seed)
   // the medium cache: the class it was evaluated for, live and reference values
   int mcls;
   MediumAt ml, mr;
@@ -195,10 +196,12 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   const lh_source &S = a.src[j];
   const long long local = gi - (j > 0 ? a.src_end[j - 1] : 0);
   const unsigned long long id = (unsigned long long)(S.first_id + local);
-  p.rng = lh_rng{a.k0, a.k1, (uint32_t)id, (uint32_t)(id >> 32)};
+  p.id_lo = (uint32_t)id;
+  p.id_hi = (uint32_t)(id >> 32);
+  const lh_rng rng{a.k0, a.k1, p.id_lo, p.id_hi};
   float u[4], v[4];
-  p.rng.draw4(LH_SOURCE_VERTEX, 0, u);
-  p.rng.draw4(LH_SOURCE_VERTEX, 1, v);
+  rng.draw4(LH_SOURCE_VERTEX, 0, u);
+  rng.draw4(LH_SOURCE_VERTEX, 1, v);
   p.x = mk3(S.pos[0], S.pos[1], S.pos[2]);
   p.t = S.time;
   p.w0 = S.weight;
@@ -261,7 +264,7 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
 // drawn at the reference width `sigma`; with `ratio` its log density ratio live / reference is added to logw
 // (facets.py `facet_logpdf` per facet, surface_event.py:92-112), and with `deriv` the live log density's derivative
 // in the width to dlogw.
-__device__ inline float3 facet_path(const Photon &p, float sigma, float sigma_live, const float *__restrict__ T,
+__device__ inline float3 facet_path(const Photon &p, const lh_rng &rng, float sigma, float sigma_live, const float *__restrict__ T,
                                     bool ratio, bool deriv, float3 wi, float3 n, float3 &er, float3 &ei, float &logw,
                                     float &dlogw) {
   float3 V = to_local_frame(mul3(-1.0f, wi), n);
@@ -271,7 +274,7 @@ __device__ inline float3 facet_path(const Photon &p, float sigma, float sigma_li
     float a = 0.0f, u2 = 0.5f;
     for (int t = 0; t < MAX_FACET_ATTEMPTS; ++t) {
       float u[4];
-      p.rng.draw4(p.vertex, LH_FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
+      rng.draw4(p.vertex, LH_FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
       const float al = sigma * sqrtf(-2.0f * __logf(u[0]));
       if (al < HALF_PI && u[1] * al * PI_F < fminf(__sinf(al), fmax) * front_arc(V, al)) {
         a = al;
@@ -301,12 +304,12 @@ __device__ inline float3 facet_path(const Photon &p, float sigma, float sigma_li
   return w;
 }
 
-template <bool ADJ>
+template <bool ADJ, bool REF>
 __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) {
   if (p.mcls == p.cls) return;
   const int m = class_medium(a, p.cls);
   p.ml = medium_at(a.L, a.P, m, p.lam);
-  p.mr = a.has_ref ? medium_at(a.L, a.Pr, m, p.lam) : p.ml;
+  if (REF) p.mr = medium_at(a.L, a.Pr, m, p.lam);
   p.mcls = p.cls;
   if (ADJ) {
     p.da = RateD{-1, -1, 0.0f, 0.0f};
@@ -341,7 +344,7 @@ __device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.
 // one step of one photon (transport/step.py:516-571); returns whether it goes on.  ADJ: the adjoint's walk
 // (no tallies; its pass 0 totals the photon's cotangent-weighted deposits, its pass 1 adds the derivative terms
 // into the block's accumulators sg)
-template <bool ADJ>
+template <bool ADJ, bool REF>
 __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b, double *sg) {
   const Geo &g = a.g;
   float b0 = band_at(maxabs3(p.x));
@@ -354,12 +357,13 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     b0 = band_at(maxabs3(p.x));
   }
   float u[4], ul[4];
-  p.rng.draw4(p.vertex, 0, u);   // path, channel, outcome, roulette
-  p.rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
+  const lh_rng rng{a.k0, a.k1, p.id_lo, p.id_hi};
+  rng.draw4(p.vertex, 0, u);   // path, channel, outcome, roulette
+  rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
   float3 er0 = p.er, ei0 = p.ei;
   pol_fill(p.d, er0, ei0, ul[3]);
-  medium_cache<ADJ>(a, p, b);
-  const MediumAt ml = p.ml, mr = p.mr;
+  medium_cache<ADJ, REF>(a, p, b);
+  const MediumAt ml = p.ml, mr = REF ? p.mr : p.ml;
   const float mu = ml.mu_ray, mu_r = mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
@@ -373,8 +377,8 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const bool cont = hit && zero && row < 0;  // a zero-length crossing no surface joins: a straight pass
   if (cont) row = 0;
   const bool scatter = mu_r > 0.0f && s < dist;
-  const float reach_live = __expf(-mu * dist), reach_ref = a.has_ref ? __expf(-mu_r * dist) : reach_live;
-  const float att = __expf(-ml.mu_abs * dist), att_r = a.has_ref ? __expf(-mr.mu_abs * dist) : att;
+  const float reach_live = __expf(-mu * dist), reach_ref = REF ? __expf(-mu_r * dist) : reach_live;
+  const float att = __expf(-ml.mu_abs * dist), att_r = REF ? __expf(-mr.mu_abs * dist) : att;
   const float v_g = C_VACUUM / ml.ng;
   SurfEval sl, sr;
   int o = -1;
@@ -389,7 +393,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       sl.p[STRAIGHT] = 1.0f;
       sl.detect = 0.0f;
     }
-    if (a.has_ref) {
+    if (REF) {
       const float n2r = index_at(a.L, a.Pr, m2, p.lam);
       sr = surface_eval(a.L, a.Pr, row, p.lam, p.d, cr.n, mr.n, n2r, er0, ei0);
       if (cont) {
@@ -453,7 +457,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     const float att_s = __expf(-ml.mu_abs * s);
     const float w_at = p.w * w_path;
     w_new = w_at * att_s;
-    wr_new = p.wr * (a.has_ref ? __expf(-mr.mu_abs * s) : att_s);
+    wr_new = p.wr * (REF ? __expf(-mr.mu_abs * s) : att_s);
     acc.bulk += w_at * (1.0f - att_s) * p.w0;
     if (adj_event) {  // the free path's density mu exp(-mu s) and the absorption exp(-mu_abs s)
       const float k = (mu > 0.0f ? 1.0f / mu : 0.0f) - s;
@@ -517,8 +521,8 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
         ei0 = mul3(-1.0f, ei0);
       } else if (o == GLOSSY) {
         float logw = 0.0f, dlogw = 0.0f;
-        const bool want_ratio = a.has_ref && sl.sigma != sr.sigma;
-        dn = facet_path(p, sr.sigma, sl.sigma, a.ut, want_ratio, adj_event, wi, n, er0, ei0, logw, dlogw);
+        const bool want_ratio = REF && sl.sigma != sr.sigma;
+        dn = facet_path(p, rng, sr.sigma, sl.sigma, a.ut, want_ratio, adj_event, wi, n, er0, ei0, logw, dlogw);
         if (want_ratio) w_lobe = __expf(logw);
         if (adj_event) {
           const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
@@ -626,6 +630,7 @@ __global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const fl
 }
 
 // dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
+template <bool REF>
 __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) {
   if (a.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
     }
     if (!__any(alive) && drained) break;
     if (alive) {
-      alive = step<false>(a, p, acc, nullptr, nullptr);
+      alive = step<false, REF>(a, p, acc, nullptr, nullptr);
       if (!alive) photon_end(a, p);
     }
   }
@@ -678,6 +683,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
+template <bool REF>
 __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
@@ -722,7 +728,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b
     }
     if (!__any(alive) && drained) break;
     if (alive) {
-      alive = step<true>(a, p, acc, &b, sg);
+      alive = step<true, REF>(a, p, acc, &b, sg);
       if (!alive && p.pass == 0 && p.hasdep) {  // replay the photon from its counter, now adding the terms
         const float T = p.Dsum;
         Acc scratch{};
@@ -767,8 +773,11 @@ struct lh_model_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int timed = 0;
   float *ut = nullptr;  // the unified facet tables on the device
-  int cus = 0, per_trace = 0, per_adj = 0;
-  size_t per_trace_lds = 0, per_adj_lds = 0;
+  int cus = 0, lds_per_cu = 0;
+  const void *occ_fn[OCC_CACHE] = {};  // the occupancy cache of grid_of
+  size_t occ_lds[OCC_CACHE] = {};
+  int occ_per[OCC_CACHE] = {};
+  int occ_next = 0;
 };
 
 static thread_local std::string g_err;
@@ -999,6 +1008,9 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   int dev = 0, cus = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int lds_cu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  m->lds_per_cu = lds_cu > 0 ? lds_cu : 64 * 1024;
   m->cus = cus;
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
@@ -1107,41 +1119,75 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   return LH_OK;
 }
 
-static int grid_of(lh_model m, const lh_controls *ctl, bool adj, size_t lds_bytes) {
-  int &per = adj ? m->per_adj : m->per_trace;  // resident blocks per CU at the last LDS size asked
-  size_t &at = adj ? m->per_adj_lds : m->per_trace_lds;
-  if (per <= 0 || at != lds_bytes) {
-    hipError_t e = adj ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adjoint_kernel, BLOCK, lds_bytes)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_kernel, BLOCK, lds_bytes);
-    if (e != hipSuccess || per < 1) per = 1;
-    at = lds_bytes;
+// the kernel of a launch: forward or adjoint, with or without separate reference values
+static const void *kernel_of(bool adj, bool ref) {
+  if (adj) return ref ? (const void *)adjoint_kernel<true> : (const void *)adjoint_kernel<false>;
+  return ref ? (const void *)trace_kernel<true> : (const void *)trace_kernel<false>;
+}
+
+static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t lds_bytes) {
+  // resident blocks per CU of (kernel, LDS bytes), remembered in a small cache
+  int per = 0;
+  for (int i = 0; i < OCC_CACHE; ++i)
+    if (m->occ_fn[i] == fn && m->occ_lds[i] == lds_bytes) per = m->occ_per[i];
+  if (per <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, lds_bytes) != hipSuccess || per < 1) per = 1;
+    const int i = m->occ_next++ % OCC_CACHE;
+    m->occ_fn[i] = fn;
+    m->occ_lds[i] = lds_bytes;
+    m->occ_per[i] = per;
   }
   const int want = (ctl && ctl->blocks_per_cu > 0) ? std::min(ctl->blocks_per_cu, per) : per;
   return m->cus * want;
 }
 
-// the BVH nodes follow `cells_bytes` of a launch's LDS when both fit the block's budget; returns the LDS bytes
-static size_t place_nodes(lh_model m, Args &a, size_t cells_bytes, const lh_controls *ctl) {
-  const size_t off = (cells_bytes + 15) / 16 * 16;
+// a launch's dynamic LDS: its cells (the adjoint's gradient accumulators: required; the forward's tallies:
+// optional, `cells_optional`), then the BVH nodes.  The budget is the CU's LDS over the blocks its registers let
+// reside, so LDS never lowers the occupancy; the nodes (a load on every walk step) come before optional cells.
+// Sets a.lds_tally and the nodes' placement; returns the LDS bytes.
+static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes, bool cells_optional,
+                        const lh_controls *ctl) {
+  const int per_regs = std::max(1, grid_of(m, nullptr, fn, 0) / std::max(m->cus, 1));
+  const size_t budget = std::min<size_t>(LDS_BLOCK_MAX, (size_t)m->lds_per_cu / (size_t)per_regs) / 16 * 16;
   const size_t node_bytes = (size_t)m->g.n_nodes * 32;
-  a.g.lds_nodes = 0;
-  a.g.lds_node_off = 0;
-  if (ctl && ctl->nodes_global == 1) return cells_bytes;
-  if (m->g.n_nodes == 0 || off + node_bytes > LDS_BLOCK_BUDGET) return cells_bytes;
-  a.g.lds_nodes = 1;
+  const bool want_nodes = m->g.n_nodes > 0 && !(ctl && ctl->nodes_global == 1);
+  const bool want_cells = cells_bytes > 0 && !(cells_optional && ctl && ctl->tally_global == 1);
+  const size_t cells_al = (cells_bytes + 15) / 16 * 16;
+  bool cells = want_cells && (!cells_optional || cells_al <= budget);
+  bool nodes = want_nodes && (cells ? cells_al : 0) + node_bytes <= budget;
+  if (cells_optional && cells && want_nodes && !nodes && node_bytes <= budget) {  // nodes first
+    cells = false;
+    nodes = true;
+  }
+  a.lds_tally = (cells_optional && cells) ? 1 : 0;
+  a.g.lds_nodes = nodes ? 1 : 0;
+  const size_t off = cells ? cells_al : 0;
   a.g.lds_node_off = (int)(off / 16);
-  return off + node_bytes;
+  return off + (nodes ? node_bytes : 0);
 }
 
-// the forward launch: the tallies in LDS when they fit, then the BVH nodes when they fit too
+static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
+                           hipStream_t st) {
+  const dim3 grid(grid_of(m, ctl, kernel_of(true, a.has_ref), bytes));
+  if (a.has_ref)
+    hipLaunchKernelGGL(adjoint_kernel<true>, grid, dim3(BLOCK), bytes, st, a, b);
+  else
+    hipLaunchKernelGGL(adjoint_kernel<false>, grid, dim3(BLOCK), bytes, st, a, b);
+}
+
+// the forward launch: the BVH nodes and the block's tallies in LDS as they fit
 static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *ctl, hipStream_t st) {
   const long long cells = (long long)n_events * m->tally_stride;
-  const bool lds = cells <= LDS_TALLY_CELLS && !(ctl && ctl->tally_global == 1);
-  a.lds_tally = lds ? 1 : 0;
-  const size_t bytes = place_nodes(m, a, lds ? (size_t)cells * sizeof(double) : 0, ctl);
+  const void *fn = kernel_of(false, a.has_ref);
+  const size_t bytes =
+      place_lds(m, a, fn, cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0, true, ctl);
+  const int n_lds = a.lds_tally ? (int)cells : 0;
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  hipLaunchKernelGGL(trace_kernel, dim3(grid_of(m, ctl, false, bytes)), dim3(BLOCK), bytes, st, a,
-                     (int)(lds ? cells : 0));
+  const dim3 grid(grid_of(m, ctl, fn, bytes));
+  if (a.has_ref)
+    hipLaunchKernelGGL(trace_kernel<true>, grid, dim3(BLOCK), bytes, st, a, n_lds);
+  else
+    hipLaunchKernelGGL(trace_kernel<false>, grid, dim3(BLOCK), bytes, st, a, n_lds);
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
@@ -1198,9 +1244,8 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  a.lds_tally = 0;
-  const size_t bytes = place_nodes(m, a, (size_t)n_grad * sizeof(double), ctl);
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, bytes)), dim3(BLOCK), bytes, st, a, b);
+  const size_t bytes = place_lds(m, a, kernel_of(true, a.has_ref), (size_t)n_grad * sizeof(double), false, ctl);
+  launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
   m->timed = 1;
@@ -1307,9 +1352,8 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   a.queue = b->queue;
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  a.lds_tally = 0;
-  const size_t bytes = place_nodes(m, a, (size_t)n_grad * sizeof(double), ctl);
-  hipLaunchKernelGGL(adjoint_kernel, dim3(grid_of(m, ctl, true, bytes)), dim3(BLOCK), bytes, st, a, g);
+  const size_t bytes = place_lds(m, a, kernel_of(true, a.has_ref), (size_t)n_grad * sizeof(double), false, ctl);
+  launch_adjoint(m, a, g, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   return LH_OK;
 }
