@@ -111,6 +111,9 @@ def _function():
 _PL = None
 
 
+NO_ADJOINT = ("WLSCOMPONENT", "WLSMEANNUMBERPHOTONS", "WLSTIMECONSTANT")
+
+
 def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: Params | None = None,
                   reference: Params | None = None, controls: Controls | None = None, first_id=0,
                   edge_rows="structure", device="cuda") -> Light:
@@ -130,6 +133,9 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
         if k.endswith("/RINDEX"):
             raise ValueError(f"simulate_diff: {k} is calibrated by value only (the reference raises "
                              "GradientNotAvailable for it)")
+        if k.rsplit("/", 1)[-1] in NO_ADJOINT:
+            raise ValueError(f"simulate_diff: {k}: the adjoint differentiates the path's rates and surface shares; "
+                             "a re-emission's spectrum, yield and delay are calibrated by value (finite differences)")
     pk = packing.pack(det, p, grad_keys=keys, edge_rows=edge_rows)
     values = torch.as_tensor(pk.values, device=eng.device)
     slots = torch.as_tensor(pk.slots, device=eng.device)
@@ -152,8 +158,8 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
     for s0, n, t in parts:
         slot_vals = slot_vals.index_put((torch.arange(s0, s0 + n, device=eng.device),), t)
     vals = values.index_put((value_index,), slot_vals[slot_of_value])
-    if reference is None:
-        ref_values = vals.detach().clone()
+    if reference is None:  # the draws at the values themselves: no separate reference operand (the kernel without it)
+        ref_values = None
     else:
         ref_values = torch.as_tensor(packing.pack(det, reference, edge_rows=edge_rows).values, device=eng.device)
     rows, n_ev = source_rows(det, source, samples, first_id)

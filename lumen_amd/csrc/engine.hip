@@ -47,16 +47,16 @@ constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
 constexpr size_t LDS_BLOCK_MAX = 64 * 1024;
-constexpr int OCC_CACHE = 8;  // the most dynamic LDS one block of the persistent grid takes
+constexpr int OCC_CACHE = 8;
+constexpr unsigned F_REF = 1, F_WLS = 2;  // a kernel's features (template parameter)  // the most dynamic LDS one block of the persistent grid takes
 
+// The forward kernel's register budget: 4 wavefronts per SIMD (128 VGPRs) hide the walk's load latency better
+// than the 2-3 its natural allocation allows, spills included (measured on the 1,000-PMT tank).  The adjoint and
+// the kernels with separate reference values keep their natural allocation.
 #ifndef LH_WAVES_PER_EU
-#define LH_WAVES_PER_EU 0
+#define LH_WAVES_PER_EU 4
 #endif
-#if LH_WAVES_PER_EU > 0
-#define LH_OCC __attribute__((amdgpu_waves_per_eu(LH_WAVES_PER_EU)))
-#else
-#define LH_OCC
-#endif
+#define LH_OCC_FWD(F) __attribute__((amdgpu_waves_per_eu(((F) & 1u) ? 1 : LH_WAVES_PER_EU)))
 
 struct Args {
   Geo g;
@@ -98,7 +98,7 @@ struct Photon {
   int pass;
   float T, Dsum;
   bool hasdep;
-  RateD da, dr;
+  RateD da, dr, dw;
 };
 
 struct Acc {  // per-lane health accumulators, flushed once per launch
@@ -304,8 +304,9 @@ __device__ inline float3 facet_path(const Photon &p, const lh_rng &rng, float si
   return w;
 }
 
-template <bool ADJ, bool REF>
+template <bool ADJ, unsigned F>
 __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) {
+  constexpr bool REF = F & F_REF, WLS = F & F_WLS;
   if (p.mcls == p.cls) return;
   const int m = class_medium(a, p.cls);
   p.ml = medium_at(a.L, a.P, m, p.lam);
@@ -314,12 +315,14 @@ __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) 
   if (ADJ) {
     p.da = RateD{-1, -1, 0.0f, 0.0f};
     p.dr = RateD{-1, -1, 0.0f, 0.0f};
+    p.dw = RateD{-1, -1, 0.0f, 0.0f};
     if (m >= 0) {
       const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * m;
-      if (!(M[8] & 1)) {
+      if (!(M[8] & MED_OPAQUE)) {
         const float E = HC / p.lam;
         if (M[5] > 0) p.da = rate_d(a.P, b->slots, M[4], M[5], E);
         if (M[7] > 0) p.dr = rate_d(a.P, b->slots, M[6], M[7], E);
+        if (WLS && (M[8] & MED_WLS)) p.dw = rate_d(a.P, b->slots, M[9], M[10], E);
       }
     }
   }
@@ -344,8 +347,9 @@ __device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.
 // one step of one photon (transport/step.py:516-571); returns whether it goes on.  ADJ: the adjoint's walk
 // (no tallies; its pass 0 totals the photon's cotangent-weighted deposits, its pass 1 adds the derivative terms
 // into the block's accumulators sg)
-template <bool ADJ, bool REF>
+template <bool ADJ, unsigned F>
 __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b, double *sg) {
+  constexpr bool REF = F & F_REF, WLS = F & F_WLS;
   const Geo &g = a.g;
   float b0 = band_at(maxabs3(p.x));
   if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {  // G4Navigator's stuck-track push (step.py:269-282 `_recover`)
@@ -362,9 +366,9 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
   float3 er0 = p.er, ei0 = p.ei;
   pol_fill(p.d, er0, ei0, ul[3]);
-  medium_cache<ADJ, REF>(a, p, b);
+  medium_cache<ADJ, F>(a, p, b);
   const MediumAt ml = p.ml, mr = REF ? p.mr : p.ml;
-  const float mu = ml.mu_ray, mu_r = mr.mu_ray;
+  const float mu = ml.mu_ray + (WLS ? ml.mu_wls : 0.0f), mu_r = mr.mu_ray + (WLS ? mr.mu_wls : 0.0f);  // totals
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
   const Cross cr = query(g, p.x, p.d, p.vol, p.srf, max_dist);
@@ -437,6 +441,8 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
             grad_add(sg, p.da.s1, -dist * p.da.d1 * D);
             grad_add(sg, p.dr.s0, -dist * p.dr.d0 * D);
             grad_add(sg, p.dr.s1, -dist * p.dr.d1 * D);
+            grad_add(sg, p.dw.s0, -dist * p.dw.d0 * D);
+            grad_add(sg, p.dw.s1, -dist * p.dw.d1 * D);
             float dsh[N_COL], ddet[N_COL];
             const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
             surface_dlogs(Rw, a.P, HC / p.lam, -1, dsh, ddet);
@@ -452,27 +458,54 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const bool adj_event = ADJ && p.pass == 1 && ahead != 0.0f;
   bool ok;
   float w_new, wr_new;
-  if (scatter) {  // a bulk event at the free path's end (step.py:397-436): Rayleigh, the one channel here
-    const float w_path = ratio(mu * __expf(-mu * s), mu_r * __expf(-mu_r * s));
+  if (scatter) {  // a bulk event at the free path's end (step.py:397-436, `_bulk`)
+    // the channel drawn from the reference rates in Geant4's order (Rayleigh, WLS), weighted by the live /
+    // reference density of the free path in it, mu_c exp(-mu s), times the absorption over the path
+    const bool wls = WLS && !(u[1] * mu_r < mr.mu_ray);
+    const float mc = wls ? ml.mu_wls : ml.mu_ray, mc_r = wls ? mr.mu_wls : mr.mu_ray;
+    const float w_path = ratio(mc * __expf(-mu * s), mc_r * __expf(-mu_r * s));
     const float att_s = __expf(-ml.mu_abs * s);
     const float w_at = p.w * w_path;
-    w_new = w_at * att_s;
-    wr_new = p.wr * (REF ? __expf(-mr.mu_abs * s) : att_s);
-    acc.bulk += w_at * (1.0f - att_s) * p.w0;
-    if (adj_event) {  // the free path's density mu exp(-mu s) and the absorption exp(-mu_abs s)
-      const float k = (mu > 0.0f ? 1.0f / mu : 0.0f) - s;
-      grad_add(sg, p.dr.s0, k * p.dr.d0 * ahead);
-      grad_add(sg, p.dr.s1, k * p.dr.d1 * ahead);
+    const float att_s_r = REF ? __expf(-mr.mu_abs * s) : att_s;
+    if (adj_event) {  // the free path's density mu_c exp(-mu s) and the absorption exp(-mu_abs s)
+      const float kc = mc > 0.0f ? 1.0f / mc : 0.0f;
+      const float kr = (wls ? 0.0f : kc) - s, kw = (wls ? kc : 0.0f) - s;
+      grad_add(sg, p.dr.s0, kr * p.dr.d0 * ahead);
+      grad_add(sg, p.dr.s1, kr * p.dr.d1 * ahead);
+      grad_add(sg, p.dw.s0, kw * p.dw.d0 * ahead);
+      grad_add(sg, p.dw.s1, kw * p.dw.d1 * ahead);
       grad_add(sg, p.da.s0, -s * p.da.d0 * ahead);
       grad_add(sg, p.da.s1, -s * p.da.d1 * ahead);
     }
-    const float3 dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
-    pol_project(er0, ei0, dn);
     p.x = fma3(s, p.d, p.x);
     p.t += s / v_g;
-    p.d = unit3(dn);
-    p.er = er0;
-    p.ei = ei0;
+    if (!wls) {  // Rayleigh: the polarised dipole lobe (phase.py:45-75)
+      w_new = w_at * att_s;
+      wr_new = p.wr * att_s_r;
+      acc.bulk += w_at * (1.0f - att_s) * p.w0;
+      const float3 dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
+      pol_project(er0, ei0, dn);
+      p.d = unit3(dn);
+      p.er = er0;
+      p.ei = ei0;
+    } else {  // a wavelength shifter (G4OpWLS): re-emitted isotropic, unpolarised, after the record's delay
+      float ur[4];
+      rng.draw4(p.vertex, 2, ur);  // (wavelength, delay, -, -)
+      const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * class_medium(a, p.cls);
+      const WlsDraw em = wls_emit(a.P, REF ? a.Pr : a.P, M, p.lam, ur[0], ur[1]);
+      w_new = w_at * att_s * em.w;
+      wr_new = p.wr * att_s_r * em.wr;
+      acc.bulk += w_at * (1.0f - att_s * em.w) * p.w0;
+      const float c = 1.0f - 2.0f * ul[0];
+      const float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float sp, cp;
+      __sincosf(TWO_PI * ul[1], &sp, &cp);
+      p.d = unit3(to_world(mk3(sn * cp, sn * sp, c), p.d));
+      p.er = p.ei = mk3(0.f, 0.f, 0.f);
+      p.lam = em.lam;
+      p.t += em.delay;
+      p.mcls = -2;  // the medium's values at the new wavelength
+    }
     const bool held = s <= ZERO_STEP_BANDS * b0 && p.srf >= 0;
     if (!held) p.srf = -1;
     ok = true;
@@ -490,6 +523,8 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       grad_add(sg, p.da.s1, -dist * p.da.d1 * ahead);
       grad_add(sg, p.dr.s0, -dist * p.dr.d0 * ahead);
       grad_add(sg, p.dr.s1, -dist * p.dr.d1 * ahead);
+      grad_add(sg, p.dw.s0, -dist * p.dw.d0 * ahead);
+      grad_add(sg, p.dw.s1, -dist * p.dw.d1 * ahead);
       float dsh[N_COL], ddet[N_COL];
       const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
       if (!cont) {
@@ -629,9 +664,14 @@ __global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const fl
   srf_out[i] = c.srf;
 }
 
+// the number of lanes below this one set in a wavefront mask
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
 // dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
-template <bool REF>
-__global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) {
+template <unsigned F>
+__global__ void __launch_bounds__(BLOCK) LH_OCC_FWD(F) trace_kernel(Args a, int n_lds) {
   if (a.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
   stage_nodes(a.g);
@@ -640,7 +680,6 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
   Acc acc{};
   bool alive = false;
   const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   long long cursor = 0, chunk_end = 0;
   bool drained = false;
   for (;;) {
@@ -661,7 +700,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
         chunk_end = min((long long)base + a.chunk, a.total);
         continue;
       }
-      const int rank = __popcll(mask & lt_mask);
+      const int rank = lanes_below(mask);
       const bool take = need && rank < avail;
       if (take) alive = emit(a, cursor + rank, p, acc);
       cursor += min((long long)__popcll(mask), avail);
@@ -670,7 +709,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
     }
     if (!__any(alive) && drained) break;
     if (alive) {
-      alive = step<false, REF>(a, p, acc, nullptr, nullptr);
+      alive = step<false, F>(a, p, acc, nullptr, nullptr);
       if (!alive) photon_end(a, p);
     }
   }
@@ -683,8 +722,8 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC trace_kernel(Args a, int n_lds) 
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
-template <bool REF>
-__global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b) {
+template <unsigned F>
+__global__ void __launch_bounds__(BLOCK) adjoint_kernel(Args a, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
   stage_nodes(a.g);
@@ -693,7 +732,6 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b
   Acc acc{};
   bool alive = false;
   const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   long long cursor = 0, chunk_end = 0;
   bool drained = false;
   for (;;) {
@@ -713,7 +751,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b
         chunk_end = min((long long)base + a.chunk, a.total);
         continue;
       }
-      const int rank = __popcll(mask & lt_mask);
+      const int rank = lanes_below(mask);
       const bool take = need && rank < avail;
       if (take) {
         p.gi = cursor + rank;
@@ -728,7 +766,7 @@ __global__ void __launch_bounds__(BLOCK) LH_OCC adjoint_kernel(Args a, AdjArgs b
     }
     if (!__any(alive) && drained) break;
     if (alive) {
-      alive = step<true, REF>(a, p, acc, &b, sg);
+      alive = step<true, F>(a, p, acc, &b, sg);
       if (!alive && p.pass == 0 && p.hasdep) {  // replay the photon from its counter, now adding the terms
         const float T = p.Dsum;
         Acc scratch{};
@@ -774,6 +812,7 @@ struct lh_model_s {
   int timed = 0;
   float *ut = nullptr;  // the unified facet tables on the device
   int cus = 0, lds_per_cu = 0;
+  bool has_wls = false;  // a medium shifts wavelengths
   const void *occ_fn[OCC_CACHE] = {};  // the occupancy cache of grid_of
   size_t occ_lds[OCC_CACHE] = {};
   int occ_per[OCC_CACHE] = {};
@@ -868,9 +907,16 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     }
     for (int mi = 0; mi < nm; ++mi) {
       const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
-      if (!(M[8] & 1) && (M[1] < 1 || M[3] < 1)) return fail(LH_ERR_LAYOUT, "lh_model_create: a medium without RINDEX");
+      if (!(M[8] & MED_OPAQUE) && (M[1] < 1 || M[3] < 1))
+        return fail(LH_ERR_LAYOUT, "lh_model_create: a medium without RINDEX");
+      if (M[8] & ~(MED_OPAQUE | MED_WLS | MED_WLS_EXP)) return fail(LH_ERR_LAYOUT, "lh_model_create: unknown medium flags");
       for (int j = 0; j < 8; j += 2)
         if (M[j] < 0 || M[j + 1] < 0) return fail(LH_ERR_LAYOUT, "lh_model_create: a negative table offset");
+      if (M[8] & MED_WLS) {  // WLSABSLENGTH, WLSCOMPONENT (two knots or more), the yield and the time constant
+        for (int j = 9; j < 17; j += 2)
+          if (M[j] < 0 || M[j + 1] < 1) return fail(LH_ERR_LAYOUT, "lh_model_create: a WLS table is empty");
+        if (M[12] < 2) return fail(LH_ERR_LAYOUT, "lh_model_create: a WLS emission spectrum needs two knots");
+      }
     }
   }
 
@@ -987,12 +1033,16 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     for (int mi = 0; mi < layout[1]; ++mi) {
       const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
       for (int j = 0; j < 8; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
+      if (M[8] & MED_WLS)
+        for (int j = 9; j < 17; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
     }
     for (int r = 0; r < layout[3]; ++r) {
       const int32_t *R = layout + layout[9] + ROW_WORDS * r;
       for (int c = 0; c < N_COL; ++c) need = std::max(need, (long long)R[2 + 2 * c] + 2LL * R[3 + 2 * c]);
     }
     m->values_needed = need;
+    for (int mi = 0; mi < layout[1]; ++mi)
+      if (layout[layout[6] + MEDIUM_WORDS * mi + 8] & MED_WLS) m->has_wls = true;
   }
   m->n_media = layout[1];
   m->S = layout[4];
@@ -1119,10 +1169,40 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   return LH_OK;
 }
 
-// the kernel of a launch: forward or adjoint, with or without separate reference values
-static const void *kernel_of(bool adj, bool ref) {
-  if (adj) return ref ? (const void *)adjoint_kernel<true> : (const void *)adjoint_kernel<false>;
-  return ref ? (const void *)trace_kernel<true> : (const void *)trace_kernel<false>;
+// the kernel of a launch: forward or adjoint, specialised on its features (F_REF: separate reference values;
+// F_WLS: a medium shifts wavelengths), so a run without them carries none of their registers
+template <unsigned F>
+static const void *kernel_f(bool adj) {
+  return adj ? (const void *)adjoint_kernel<F> : (const void *)trace_kernel<F>;
+}
+
+static unsigned features_of(lh_model m, const Args &a) { return (a.has_ref ? F_REF : 0u) | (m->has_wls ? F_WLS : 0u); }
+
+static const void *kernel_of(bool adj, unsigned f) {
+  switch (f) {
+    case 0: return kernel_f<0>(adj);
+    case F_REF: return kernel_f<F_REF>(adj);
+    case F_WLS: return kernel_f<F_WLS>(adj);
+    default: return kernel_f<F_REF | F_WLS>(adj);
+  }
+}
+
+template <unsigned F>
+static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Args &a, const AdjArgs &b, int n_lds) {
+  if (adj)
+    hipLaunchKernelGGL(adjoint_kernel<F>, grid, dim3(BLOCK), bytes, st, a, b);
+  else
+    hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(BLOCK), bytes, st, a, n_lds);
+}
+
+static void launch_kernel(bool adj, unsigned f, dim3 grid, size_t bytes, hipStream_t st, const Args &a,
+                          const AdjArgs &b, int n_lds) {
+  switch (f) {
+    case 0: launch_f<0>(adj, grid, bytes, st, a, b, n_lds); break;
+    case F_REF: launch_f<F_REF>(adj, grid, bytes, st, a, b, n_lds); break;
+    case F_WLS: launch_f<F_WLS>(adj, grid, bytes, st, a, b, n_lds); break;
+    default: launch_f<F_REF | F_WLS>(adj, grid, bytes, st, a, b, n_lds); break;
+  }
 }
 
 static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t lds_bytes) {
@@ -1168,26 +1248,20 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes,
 
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
                            hipStream_t st) {
-  const dim3 grid(grid_of(m, ctl, kernel_of(true, a.has_ref), bytes));
-  if (a.has_ref)
-    hipLaunchKernelGGL(adjoint_kernel<true>, grid, dim3(BLOCK), bytes, st, a, b);
-  else
-    hipLaunchKernelGGL(adjoint_kernel<false>, grid, dim3(BLOCK), bytes, st, a, b);
+  const unsigned f = features_of(m, a);
+  launch_kernel(true, f, dim3(grid_of(m, ctl, kernel_of(true, f), bytes)), bytes, st, a, b, 0);
 }
 
 // the forward launch: the BVH nodes and the block's tallies in LDS as they fit
 static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *ctl, hipStream_t st) {
   const long long cells = (long long)n_events * m->tally_stride;
-  const void *fn = kernel_of(false, a.has_ref);
+  const unsigned f = features_of(m, a);
+  const void *fn = kernel_of(false, f);
   const size_t bytes =
       place_lds(m, a, fn, cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0, true, ctl);
   const int n_lds = a.lds_tally ? (int)cells : 0;
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const dim3 grid(grid_of(m, ctl, fn, bytes));
-  if (a.has_ref)
-    hipLaunchKernelGGL(trace_kernel<true>, grid, dim3(BLOCK), bytes, st, a, n_lds);
-  else
-    hipLaunchKernelGGL(trace_kernel<false>, grid, dim3(BLOCK), bytes, st, a, n_lds);
+  launch_kernel(false, f, dim3(grid_of(m, ctl, fn, bytes)), bytes, st, a, AdjArgs{}, n_lds);
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
@@ -1244,7 +1318,7 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  const size_t bytes = place_lds(m, a, kernel_of(true, a.has_ref), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a)), (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
@@ -1352,7 +1426,7 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   a.queue = b->queue;
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const size_t bytes = place_lds(m, a, kernel_of(true, a.has_ref), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a)), (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, g, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   return LH_OK;

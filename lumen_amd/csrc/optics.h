@@ -20,6 +20,7 @@ constexpr float HALF_PI = 1.5707963267948966f;
 constexpr float FLOOR32 = 1e-30f;
 constexpr float TINY_NORMAL = 1.17549435e-38f;
 constexpr float DEFENSIVE = 0.01f;
+constexpr float WLS_TRIES = 100.0f;  // G4OpWLS redraws an emission energy above the absorbed one up to 100 times
 constexpr int N_OUT = 6;
 enum { STRAIGHT = 0, REFRACT = 1, SPECULAR = 2, BACKSCATTER = 3, GLOSSY = 4, DIFFUSE = 5 };
 enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1 };
@@ -28,7 +29,8 @@ enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_
 // the packed layout's header words (lumen_amd/pack.py)
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
        L_ROWS_AT, L_QE_TABLE };
-constexpr int MEDIUM_WORDS = 9;
+constexpr int MEDIUM_WORDS = 17;
+enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4 };  // a medium's flag word (its word 8)
 constexpr int ROW_WORDS = 2 + 2 * N_COL;
 
 struct TabPos {
@@ -59,14 +61,14 @@ __device__ __forceinline__ float tab(const float *__restrict__ P, int off, int n
 __device__ __forceinline__ float rate_of(float L) { return L >= TINY_NORMAL ? 1.0f / L : 1.0f / TINY_NORMAL; }
 
 struct MediumAt {
-  float n, ng, mu_abs, mu_ray;
+  float n, ng, mu_abs, mu_ray, mu_wls;  // the event channels' rates in Geant4's order: Rayleigh, WLS
 };
 
 __device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__restrict__ P, int m, float lam) {
-  MediumAt r{1.0f, 1.0f, 0.0f, 0.0f};
+  MediumAt r{1.0f, 1.0f, 0.0f, 0.0f, 0.0f};
   if (m < 0) return r;
   const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
-  if (M[8] & 1) return r;
+  if (M[8] & MED_OPAQUE) return r;
   const float E = HC / lam;
   r.n = tab(P, M[0], M[1], E);
   const TabPos pg = tab_pos(P, M[2], M[3], E);  // Geant4 interpolates the group velocity (1 / n_g) in energy
@@ -75,17 +77,119 @@ __device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__r
   r.ng = 1.0f / inv;
   if (M[5] > 0) r.mu_abs = rate_of(tab(P, M[4], M[5], E));
   if (M[7] > 0) r.mu_ray = rate_of(tab(P, M[6], M[7], E));
+  if (M[8] & MED_WLS) r.mu_wls = rate_of(tab(P, M[9], M[10], E));
+  return r;
+}
+
+__device__ __forceinline__ float clip01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// ------------------------------------------------------------------------------------------- WLS emission
+// Geant4's sampler of an emission spectrum (media/spectra.py `g4_arrays`): each knot interval drawn with its
+// trapezoid mass, uniform in energy within it; optionally under the one defensive mixture over the intervals
+// without mass (`covered`: share `cover` uniform over them).  Walked interval by interval: spectra are short.
+struct Spectrum {
+  const float *E, *S;  // knots (eV) and intensities
+  int n;
+  float inv_tot, cover, inv_zero;
+  __device__ float dens(int k) const {  // the density (1/eV) on interval k
+    const float m = S[k] + S[k + 1];
+    float d = 0.5f * m * inv_tot;
+    if (cover > 0.0f) d = (1.0f - cover) * d + ((m <= 0.0f && E[k + 1] > E[k]) ? cover * inv_zero : 0.0f);
+    return d;
+  }
+};
+
+__device__ inline Spectrum spectrum_of(const float *E, const float *S, int n, float cover) {
+  float tot = 0.0f, zero = 0.0f;
+  for (int k = 0; k + 1 < n; ++k) {
+    const float h = E[k + 1] - E[k];
+    tot += 0.5f * (S[k] + S[k + 1]) * h;
+    if (S[k] + S[k + 1] <= 0.0f && h > 0.0f) zero += h;
+  }
+  return Spectrum{E, S, n, tot > 0.0f ? 1.0f / tot : 0.0f, zero > 0.0f ? cover : 0.0f,
+                  zero > 0.0f ? 1.0f / zero : 0.0f};
+}
+
+// the mass at energies up to e (spectra.py `cumulative`)
+__device__ inline float spectrum_cdf(const Spectrum &q, float e) {
+  if (e < q.E[0]) return 0.0f;
+  float c = 0.0f, u = 0.0f;
+  for (int k = 0; k + 1 < q.n; ++k) {
+    const float h = q.E[k + 1] - q.E[k], d = q.dens(k);
+    if (q.E[k] <= e || k == 0) u = c + d * fminf(fmaxf(e - q.E[k], 0.0f), h);  // the last interval starting <= e
+    c += d * h;
+  }
+  return clip01(u);
+}
+
+// the energy at a uniform r (spectra.py `invert`): the exact inverse of the cumulative mass
+__device__ inline float spectrum_invert(const Spectrum &q, float r) {
+  float c = 0.0f, ck = 0.0f;
+  int k = 0;
+  for (int j = 0; j + 1 < q.n; ++j) {
+    if (c <= r) {
+      k = j;
+      ck = c;
+    }
+    c += q.dens(j) * (q.E[j + 1] - q.E[j]);
+  }
+  const float d = q.dens(k), h = q.E[k + 1] - q.E[k];
+  const float x = d > 0.0f ? fmaxf(r - ck, 0.0f) / d : 0.0f;
+  return q.E[k] + fminf(fmaxf(x, 0.0f), h);
+}
+
+// the density of the interval holding e, 0 outside the knots (spectra.py `on_intervals`)
+__device__ inline float spectrum_density(const Spectrum &q, float e) {
+  if (!(e >= q.E[0] && e <= q.E[q.n - 1])) return 0.0f;
+  int k = 0;
+  for (int j = 0; j + 1 < q.n; ++j)
+    if (q.E[j] <= e) k = j;
+  return q.dens(k);
+}
+
+// G4OpWLS's chance that one of its 100 draws lands at or below the absorbed energy, f that of one draw
+// (spectra.py `kept`)
+__device__ __forceinline__ float wls_kept(float f) {
+  return -expm1f(WLS_TRIES * log1pf(-fminf(fmaxf(f, 0.0f), 1.0f - EPS32)));
+}
+
+struct WlsDraw {
+  float lam, delay, w, wr;  // the emitted wavelength (nm), the delay (ns), the weight and the reference weight
+};
+
+__device__ __forceinline__ float safe_ratio(float a, float b) { return b != 0.0f ? a / b : 0.0f; }
+
+// a re-emission of a photon absorbed at lam_in (media/emission.py `EmissionRecord.sample`): the wavelength drawn
+// from the reference spectrum (under the defensive mixture) truncated at the absorbed energy by its exact inverse
+// CDF; the weight the yield times Geant4's chance of an energy below the absorbed one times the live / drawn
+// density; the reference weight the same at the reference values; the delay from the live time constant
+__device__ inline WlsDraw wls_emit(const float *__restrict__ P, const float *__restrict__ Pr, const int *M, float lam_in,
+                                   float u_lam, float u_time) {
+  const int off = M[11], n = M[12];
+  const Spectrum q = spectrum_of(Pr + off, Pr + off + n, n, DEFENSIVE);
+  const Spectrum l = spectrum_of(P + off, P + off + n, n, 0.0f);
+  const float e_in = HC / lam_in;
+  const float f_q = spectrum_cdf(q, e_in);
+  const float e = spectrum_invert(q, u_lam * f_q);
+  const float qd = spectrum_density(q, e);
+  const float dens_q = safe_ratio(qd, f_q);
+  const float f_l = spectrum_cdf(l, e_in);
+  const float y = P[M[13] + M[14]], y_r = Pr[M[13] + M[14]], tau = P[M[15] + M[16]];
+  WlsDraw r;
+  r.lam = HC / e;
+  r.w = safe_ratio(y * wls_kept(f_l) * safe_ratio(spectrum_density(l, e), f_l), dens_q);
+  r.wr = safe_ratio(y_r * wls_kept(f_q) * safe_ratio(qd, f_q), dens_q);
+  r.delay = (M[8] & MED_WLS_EXP) ? -tau * __logf(u_time) : tau;
   return r;
 }
 
 __device__ __forceinline__ float index_at(const int *__restrict__ L, const float *__restrict__ P, int m, float lam) {
   if (m < 0) return 1.0f;
   const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
-  if (M[8] & 1) return 1.0f;
+  if (M[8] & MED_OPAQUE) return 1.0f;
   return tab(P, M[0], M[1], HC / lam);
 }
 
-__device__ __forceinline__ float clip01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 // ------------------------------------------------------------------------------------------- polarisation
 __device__ __forceinline__ void basis(float3 n, float3 &t1, float3 &t2) {  // Duff et al., JCGT 2017

@@ -5,6 +5,7 @@ from pathlib import Path
 from .gdml import Group, Recipe, load
 from .model import Detector
 from .navarrays import NavArrays
+from .lar import lar_box, voxel_points
 from .tanks import muon_track, water_tank
 
 DATA = Path(__file__).resolve().parents[1] / "data"
@@ -17,4 +18,5 @@ def example_tank() -> Detector:
     return load(str(DATA / "example_tank.gdml"), EXAMPLE_TANK)
 
 
-__all__ = ["Detector", "NavArrays", "Recipe", "Group", "load", "example_tank", "water_tank", "muon_track"]
+__all__ = ["Detector", "NavArrays", "Recipe", "Group", "load", "example_tank", "water_tank", "muon_track",
+           "lar_box", "voxel_points"]

@@ -61,14 +61,37 @@ TYPES = {
 }
 
 
+WLS_PROFILES = ("delta", "exponential")  # G4OpWLS's time profiles (delta: Geant4's default)
+
+
 @dataclass
 class Medium:
-    """A bulk medium: its keys (None: absent).  ``rayleigh`` may be ``"water_rule"``: Geant4's table from RINDEX."""
+    """A bulk medium: its keys (None: absent).  ``rayleigh`` may be ``"water_rule"``: Geant4's table from RINDEX.
+
+    A wavelength shifter (G4OpWLS, the reference's ``media/bulk.py`` channel "wls" with ``media/emission.py``'s
+    record): ``wls_length`` the WLSABSLENGTH key; ``wls_spectrum`` the WLSCOMPONENT key (a Table of emission
+    intensity over photon energy); ``wls_yield`` WLSMEANNUMBERPHOTONS (a key or a constant; 1 when absent: the
+    expected photons per absorbed one, a weight); ``wls_tau`` WLSTIMECONSTANT (ns; a key or a constant, 0 when
+    absent) under the time profile ``wls_profile``.  The emission is isotropic and unpolarised, at or below the
+    absorbed energy.
+    """
 
     name: str
     rindex: str | None
     absorption: str | None = None
     rayleigh: str | None = None
+    wls_length: str | None = None
+    wls_spectrum: str | None = None
+    wls_yield: str | float = 1.0
+    wls_tau: str | float = 0.0
+    wls_profile: str = "delta"
+
+    def __post_init__(self):
+        if self.wls_length is not None and self.wls_spectrum is None:
+            raise ValueError(f"medium {self.name!r}: WLSABSLENGTH without WLSCOMPONENT (G4OpWLS absorbs and "
+                             "emits nothing)")
+        if self.wls_profile not in WLS_PROFILES:
+            raise ValueError(f"medium {self.name!r}: wls_profile {self.wls_profile!r} is not one of {WLS_PROFILES}")
 
 
 @dataclass

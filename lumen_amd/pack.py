@@ -8,7 +8,9 @@ differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` decl
 
     [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
     [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)
-    media:  per medium 9 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
+    media:  per medium 17 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
+            (1 opaque, 2 a wavelength shifter, 4 its exponential time profile); (offset, knots) of WLSABSLENGTH,
+            of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of WLSTIMECONSTANT
     rows:   per row 18 words: model, flags, then (offset, knots) of R, T, eff, p_spike, p_lobe, p_back, sigma_alpha,
             polish
 
@@ -28,7 +30,8 @@ from .params import Params, Table
 
 MAGIC = 0x4C484F31
 HEADER = 11
-MEDIUM_WORDS = 9
+MEDIUM_WORDS = 17
+MED_OPAQUE, MED_WLS, MED_WLS_EXP = 1, 2, 4
 ROW_WORDS = 2 + 2 * len(COLUMNS)
 
 
@@ -123,7 +126,7 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
     med = np.zeros((nm, MEDIUM_WORDS), np.int32)
     for i, m in enumerate(det.media):
         if m.rindex is None:
-            med[i, 8] = 1  # opaque
+            med[i, 8] = MED_OPAQUE
             continue
         rt = p[m.rindex]
         med[i, 0:2] = b.table(rt.energy, rt.value, m.rindex)
@@ -135,6 +138,17 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
             med[i, 6:8] = b.table(rt.energy, water_rayleigh(rt.energy, rt.value))
         elif m.rayleigh is not None:
             med[i, 6:8] = b.value(p, m.rayleigh)
+        if m.wls_length is not None:
+            med[i, 8] |= MED_WLS | (MED_WLS_EXP if m.wls_profile == "exponential" else 0)
+            med[i, 9:11] = b.value(p, m.wls_length)
+            spec = p[m.wls_spectrum]
+            e, v = np.asarray(spec.energy, np.float64), np.asarray(spec.value, np.float64)
+            if len(e) < 2 or np.any(v < 0) or not np.sum(0.5 * (v[1:] + v[:-1]) * np.diff(e)) > 0:
+                raise ValueError(f"pack: {m.wls_spectrum}: an emission spectrum needs two or more knots and a "
+                                 "non-negative intensity with positive integral")
+            med[i, 11:13] = b.table(e, v, m.wls_spectrum)
+            for j, c in ((13, m.wls_yield), (15, m.wls_tau)):
+                med[i, j:j + 2] = b.value(p, c) if isinstance(c, str) else b.table([0.0], [float(c)])
     rows = np.zeros((nr, ROW_WORDS), np.int32)
     for r, row in enumerate(det.rows):
         rows[r, 0] = row.model

@@ -8,6 +8,8 @@
 //                         :397-436 (_bulk), :464-492 (_after), :495-513 (_advance), :516-571 (step)
 //   estimator             src/lumen/transport/estimator.py:50-153 (proposal, choose, defensive, edge_floor)
 //   media                 src/lumen/media/bulk.py:61-110, src/lumen/media/laws.py:97-181 (tables, GROUPVEL)
+//   wavelength shifter    src/lumen/media/emission.py:120-159 (EmissionRecord.sample), src/lumen/media/spectra.py:37-137
+//                         (g4_arrays, covered, cumulative, invert, on_intervals, kept), src/lumen/rules/optics.py:319-350
 //   surfaces              src/lumen/surfaces/types.py:211-296, :352-389 (chained_cuts, _none, _real, kernel),
 //                         src/lumen/surfaces/fresnel.py:25-45, src/lumen/surfaces/outcomes.py:36-75,
 //                         src/lumen/surfaces/polar.py:39-118
@@ -31,6 +33,9 @@
 //   whole loop        pinned: the tutorial's published arrivals (example_tank), the specular-sphere closed form
 //   Cherenkov weight  pinned: Frank-Tamm photon yield per metre in closed form
 //   moment error      pinned: brute force on a small run
+//   WLS emission      pinned (tests/test_oracle_wls.py): the trapezoid sampler's interval masses (chi2), the truncation
+//                     at the absorbed energy with weight yield (1 - (1 - F)^100), the live/reference reweighting
+//                     (importance-weighted mean energy), the exponential delay (KS), an enclosing sensor: Q/N = yield
 #include <omp.h>
 
 #include <algorithm>
@@ -513,18 +518,18 @@ struct Optics {
   }
   int cls_medium(int c) const { return c < 0 ? -1 : L[L[7] + c]; }
   int row_of(int a, int b) const { return L[L[8] + a * L[2] + b]; }
-  const int32_t *medium(int m) const { return L + L[6] + 9 * m; }
+  const int32_t *medium(int m) const { return L + L[6] + 17 * m; }
   const int32_t *row(int r) const { return L + L[9] + 18 * r; }
 };
 
 inline double rate(double len) { return len >= TINY_NORMAL ? 1.0 / len : 1.0 / TINY_NORMAL; }
 
 template <class R> struct Med {
-  R n, ng, mu_abs, mu_ray;
+  R n, ng, mu_abs, mu_ray, mu_wls;
 };
 
 template <class R> Med<R> medium_at(const Optics &o, int m, double lam) {
-  Med<R> r{R(1.0), R(1.0), R(0.0), R(0.0)};
+  Med<R> r{R(1.0), R(1.0), R(0.0), R(0.0), R(0.0)};
   if (m < 0) return r;
   const int32_t *M = o.medium(m);
   if (M[8] & 1) return r;
@@ -538,6 +543,81 @@ template <class R> Med<R> medium_at(const Optics &o, int m, double lam) {
   r.ng = R(1.0) / inv;
   if (M[5] > 0) r.mu_abs = R(rate(o.tab(M[4], M[5], E)));
   if (M[7] > 0) r.mu_ray = R(rate(o.tab(M[6], M[7], E)));
+  if (M[8] & 2) r.mu_wls = R(rate(o.tab(M[9], M[10], E)));
+  return r;
+}
+
+// G4OpWLS's emission (src/lumen/media/emission.py:120-159 `EmissionRecord.sample`; the sampler of
+// src/lumen/media/spectra.py:37-137: g4_arrays, covered, cumulative, invert, on_intervals, kept), FP64
+struct Spec {
+  std::vector<double> e, d, c;  // interval starts+end (eV), densities (1/eV), CDF at the starts
+};
+inline Spec spec_of(const float *E, const float *S, int n, double cover) {
+  Spec q;
+  q.e.assign(E, E + n);
+  double tot = 0.0, zero = 0.0;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double h = (double)E[k + 1] - E[k];
+    tot += 0.5 * ((double)S[k] + S[k + 1]) * h;
+    if ((double)S[k] + S[k + 1] <= 0.0 && h > 0.0) zero += h;
+  }
+  double c = 0.0;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double h = (double)E[k + 1] - E[k], m = (double)S[k] + S[k + 1];
+    double d = tot > 0.0 ? 0.5 * m / tot : 0.0;
+    if (zero > 0.0) d = (1.0 - cover) * d + ((m <= 0.0 && h > 0.0) ? cover / zero : 0.0);
+    q.d.push_back(d);
+    q.c.push_back(c);
+    c += d * h;
+  }
+  return q;
+}
+inline int spec_interval(const Spec &q, double e) {  // the last interval starting at or below e (clipped)
+  int k = 0;
+  for (int j = 0; j + 1 < (int)q.e.size(); ++j)
+    if (q.e[j] <= e) k = j;
+  return k;
+}
+inline double spec_cdf(const Spec &q, double e) {
+  if (e < q.e[0]) return 0.0;
+  const int k = spec_interval(q, e);
+  const double u = q.c[k] + q.d[k] * std::min(std::max(e - q.e[k], 0.0), q.e[k + 1] - q.e[k]);
+  return std::min(std::max(u, 0.0), 1.0);
+}
+inline double spec_invert(const Spec &q, double r) {
+  int k = 0;
+  for (int j = 0; j < (int)q.c.size(); ++j)
+    if (q.c[j] <= r) k = j;
+  const double x = q.d[k] > 0.0 ? std::max(r - q.c[k], 0.0) / q.d[k] : 0.0;
+  return q.e[k] + std::min(std::max(x, 0.0), q.e[k + 1] - q.e[k]);
+}
+inline double spec_density(const Spec &q, double e) {
+  if (!(e >= q.e.front() && e <= q.e.back())) return 0.0;
+  return q.d[spec_interval(q, e)];
+}
+inline double wls_kept(double f) {
+  return -std::expm1(100.0 * std::log1p(-std::min(std::max(f, 0.0), 1.0 - 1.1920928955078125e-07)));
+}
+inline double safe_ratio(double a, double b) { return b != 0.0 ? a / b : 0.0; }
+struct WlsDraw {
+  double lam, delay, w, wr;
+};
+inline WlsDraw wls_emit(const Optics &live, const Optics &ref, const int32_t *M, double lam_in, double u_lam,
+                        double u_time) {
+  const int off = M[11], n = M[12];
+  const Spec q = spec_of(ref.P + off, ref.P + off + n, n, DEFENSIVE);
+  const Spec l = spec_of(live.P + off, live.P + off + n, n, 0.0);
+  const double e_in = HC / lam_in;
+  const double f_q = spec_cdf(q, e_in);
+  const double e = spec_invert(q, u_lam * f_q);
+  const double qd = spec_density(q, e), dens_q = safe_ratio(qd, f_q);
+  const double f_l = spec_cdf(l, e_in);
+  const double y = live.P[M[13] + M[14]], y_r = ref.P[M[13] + M[14]], tau = live.P[M[15] + M[16]];
+  WlsDraw r;
+  r.lam = HC / e;
+  r.w = safe_ratio(y * wls_kept(f_l) * safe_ratio(spec_density(l, e), f_l), dens_q);
+  r.wr = safe_ratio(y_r * wls_kept(f_q) * safe_ratio(qd, f_q), dens_q);
+  r.delay = (M[8] & 4) ? -tau * std::log(u_time) : tau;
   return r;
 }
 
@@ -848,7 +928,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   const int m = r.live.cls_medium(p.cls);
   const Med<R> ml = medium_at<R>(r.live, m, p.lam);
   const Med<R> mr = r.has_ref ? medium_at<R>(r.ref, m, p.lam) : ml;
-  const R mu = ml.mu_ray, mu_r = mr.mu_ray;
+  const R mu = ml.mu_ray + ml.mu_wls, mu_r = mr.mu_ray + mr.mu_wls;  // the channels' total rates
   const R s_free = -LOG(R(u[0])) / MAX<R>(mu_r, R(FLOOR32));
   const Cross<R> cr = query<R>(s, p.x, p.d, p.vol, p.srf, R(r.max_dist));
   const bool hit = cr.vol_after >= 0;
@@ -914,22 +994,39 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   }
   bool ok;
   R w_new = R(0.0), wr_new = R(0.0);
-  if (scatter) {
-    const R w_path = ratio<R>(mu * EXP(-mu * s_free), mu_r * EXP(-mu_r * s_free));
+  if (scatter) {  // the channel from the reference rates (Rayleigh, WLS), step.py:397-436 `_bulk`
+    const bool wls = !(R(u[1]) * mu_r < mr.mu_ray);
+    const R mc = wls ? ml.mu_wls : ml.mu_ray, mc_r = wls ? mr.mu_wls : mr.mu_ray;
+    const R w_path = ratio<R>(mc * EXP(-mu * s_free), mc_r * EXP(-mu_r * s_free));
     const R att_s = EXP(-ml.mu_abs * s_free);
     const R w_at = p.w * w_path;
-    w_new = w_at * att_s;
-    wr_new = p.wr * (r.has_ref ? EXP(-mr.mu_abs * s_free) : att_s);
-    T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s)) * p.w0;
-    const V3<R> dn = rayleigh_dir(er0, ei0, ul);
-    er0 = er0 - dot(dn, er0) * dn;
-    ei0 = ei0 - dot(dn, ei0) * dn;
-    pol_normalise(er0, ei0);
+    const R att_s_r = r.has_ref ? EXP(-mr.mu_abs * s_free) : att_s;
     p.x = p.x + s_free * p.d;
     p.t = p.t + s_free / v_g;
-    p.d = dn;
-    p.er = er0;
-    p.ei = ei0;
+    if (!wls) {
+      w_new = w_at * att_s;
+      wr_new = p.wr * att_s_r;
+      T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s)) * p.w0;
+      const V3<R> dn = rayleigh_dir(er0, ei0, ul);
+      er0 = er0 - dot(dn, er0) * dn;
+      ei0 = ei0 - dot(dn, ei0) * dn;
+      pol_normalise(er0, ei0);
+      p.d = dn;
+      p.er = er0;
+      p.ei = ei0;
+    } else {  // G4OpWLS: isotropic, unpolarised, at the record's wavelength after its delay
+      double ur[4];
+      p.rng.draw4(p.vertex, 2, ur);
+      const WlsDraw em = wls_emit(r.live, r.has_ref ? r.ref : r.live, r.live.medium(m), p.lam, ur[0], ur[1]);
+      w_new = w_at * att_s * R(em.w);
+      wr_new = p.wr * att_s_r * R(em.wr);
+      T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s * R(em.w))) * p.w0;
+      const double c = 1.0 - 2.0 * ul[0], sn = std::sqrt(std::max(1.0 - c * c, 0.0));
+      p.d = unit(to_world(v3<R>(R(sn * std::cos(TWO_PI * ul[1])), R(sn * std::sin(TWO_PI * ul[1])), R(c)), p.d));
+      p.er = p.ei = v3<R>(R(0.0), R(0.0), R(0.0));
+      p.lam = em.lam;
+      p.t = p.t + R(em.delay);
+    }
     if (!(s_free <= R(ZERO_STEP) && p.srf >= 0)) p.srf = -1;
     ok = true;
   } else if (hit) {
@@ -1173,4 +1270,18 @@ ORC_API void oracle_lobes(const double *er, const double *ei, long long n, uint6
     ray[3 * i] = r.x, ray[3 * i + 1] = r.y, ray[3 * i + 2] = r.z;
     lam[3 * i] = l.x, lam[3 * i + 1] = l.y, lam[3 * i + 2] = l.z;
   }
+}
+
+// n re-emissions by medium m's wavelength shifter of photons absorbed at lam_in (uniforms u_lam, u_time: n each);
+// out (n*4): the emitted wavelength (nm), the delay (ns), the weight and the reference weight
+ORC_API int oracle_wls_emit(const int32_t *layout, const float *values, const float *ref, int m, double lam_in,
+                            long long n, const double *u_lam, const double *u_time, double *out) {
+  const Optics live{layout, values}, rf{layout, ref ? ref : values};
+  const int32_t *M = live.medium(m);
+  if (!(M[8] & 2)) return 1;
+  for (long long i = 0; i < n; ++i) {
+    const WlsDraw e = wls_emit(live, rf, M, lam_in, u_lam[i], u_time[i]);
+    out[4 * i] = e.lam, out[4 * i + 1] = e.delay, out[4 * i + 2] = e.w, out[4 * i + 3] = e.wr;
+  }
+  return 0;
 }

@@ -49,6 +49,8 @@ def lib():
         L.oracle_lobes.restype = None
         L.oracle_fresnel.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p]
         L.oracle_fresnel.restype = None
+        L.oracle_wls_emit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_longlong,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -154,6 +156,19 @@ def fresnel(c, n1, n2):
     out = np.zeros(8)
     lib().oracle_fresnel(float(c), float(n1), float(n2), _p(out))
     return out[0::2] + 1j * out[1::2]
+
+
+def wls_emit(packed, medium, lam_in, u_lam, u_time, ref_packed=None):
+    """Return (n, 4) re-emissions by ``medium``'s wavelength shifter of photons absorbed at ``lam_in`` nm: the
+    wavelength (nm), delay (ns), weight and reference weight, from the uniforms ``u_lam``, ``u_time`` (n each)."""  # noqa: D205, D209
+    ul, ut = np.ascontiguousarray(u_lam, np.float64), np.ascontiguousarray(u_time, np.float64)
+    out = np.zeros((len(ul), 4))
+    ref = None if ref_packed is None else _p(ref_packed.values)
+    rc = lib().oracle_wls_emit(_p(packed.layout), _p(packed.values), ref, int(medium), float(lam_in), len(ul), _p(ul),
+                               _p(ut), _p(out))
+    if rc:
+        raise ValueError(f"medium {medium} shifts no wavelengths")
+    return out
 
 
 def light_of(tally, health, S, K, samples):

@@ -15,9 +15,17 @@ from lumen_amd.params import Params, Table
 E_KNOTS = np.array([1.5, 5.0])  # eV: flat tables over every wavelength used here
 
 
-def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n=1.33, eff=1.0):
+WLS_KNOTS = np.array([1.5, 2.95, 3.05, 8.0])  # eV: a shifter absorbing above 3 eV only
+SPEC_E = np.array([2.2, 2.4, 2.6, 2.8])  # eV: its emission (443-564 nm), below its absorption edge
+SPEC_S = np.array([0.0, 1.0, 0.6, 0.0])
+
+
+def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n=1.33, eff=1.0, L_wls=None,
+               wls_yield=0.7, wls_tau=0.0, wls_profile="delta"):
     """Return a water sphere (radius a) inside a sensor shell: the crossing out of the water is ``surface``
-    (SPECULAR_METAL: R mirror; PAINTED: R Lambertian) detecting the absorbed share with ``eff``."""  # noqa: D205, D209
+    (SPECULAR_METAL: R mirror; PAINTED: R Lambertian) detecting the absorbed share with ``eff``.  With ``L_wls``
+    the water holds a wavelength shifter absorbing above 3 eV with mean free path L_wls (m) and emitting on
+    2.2-2.8 eV with yield ``wls_yield``."""  # noqa: D205, D209
     kinds = [solids.orb(a + 0.05, name="shell"), solids.orb(a, name="water")]
     vols = [navarrays.Volume("shell", 0, np.eye(3), np.zeros(3), -1, 1),
             navarrays.Volume("shell/water", 1, np.eye(3), np.zeros(3), 0, 0)]
@@ -30,6 +38,15 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
     if L_ray is not None:
         values["materials/Water/RAYLEIGH"] = Table(E_KNOTS, np.array([L_ray, L_ray]))
         rayleigh = "materials/Water/RAYLEIGH"
+    wls = {}
+    if L_wls is not None:
+        values["materials/Water/WLSABSLENGTH"] = Table(WLS_KNOTS, np.array([1e30, 1e30, L_wls, L_wls]))
+        values["materials/Water/WLSCOMPONENT"] = Table(SPEC_E, SPEC_S)
+        values["materials/Water/WLSMEANNUMBERPHOTONS"] = np.float64(wls_yield)
+        values["materials/Water/WLSTIMECONSTANT"] = np.float64(wls_tau)
+        wls = dict(wls_length="materials/Water/WLSABSLENGTH", wls_spectrum="materials/Water/WLSCOMPONENT",
+                   wls_yield="materials/Water/WLSMEANNUMBERPHOTONS", wls_tau="materials/Water/WLSTIMECONSTANT",
+                   wls_profile=wls_profile)
     values["surfaces/Wall/REFLECTIVITY"] = np.float64(R)
     values["surfaces/Wall/EFFICIENCY"] = np.float64(eff)
     cols = {"R": "surfaces/Wall/REFLECTIVITY", "eff": "surfaces/Wall/EFFICIENCY"}
@@ -40,7 +57,7 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
     row_of[0, 1] = 0
     row_of[0, 2] = 1
     sensor_of_vol = np.array([0, -1, -1], np.int32)
-    return Detector("sphere_toy", nav, [Medium("Water", "materials/Water/RINDEX", absorption, rayleigh)],
+    return Detector("sphere_toy", nav, [Medium("Water", "materials/Water/RINDEX", absorption, rayleigh, **wls)],
                     np.array([0, -1, -1], np.int32), ["Water", "Shell", "OUT"], rows, row_of, sensor_of_vol,
                     ["shell"], [SensorGroup("pmt", np.arange(1))], Params(values))
 
