@@ -95,6 +95,9 @@ class Light:
     samples: int
     provenance: dict = field(default_factory=dict)
     kernel_ms: float = float("nan")
+    time: Any = None  # (S, nt) the time histogram's node masses (a timed detector: ``Detector.timed``)
+    late: Any = None  # (S, 2) (q, q t) beyond the last node
+    early: Any = None  # (S, 2) (q, q t) before the first node
 
     def to_dlpack(self, name="arrivals"):
         """Return a DLPack capsule of one output (device memory: ``jax.dlpack.from_dlpack`` takes it)."""
@@ -129,6 +132,10 @@ class Engine:
         _lib.check(_lib.lib().lh_model_info(h, _lib.C.byref(S), _lib.C.byref(K), _lib.C.byref(stride)),
                    "lh_model_info")
         self.S, self.K, self.stride = S.value, K.value, stride.value
+        tg = getattr(det, "time_grid", None)
+        self.nt = 0 if tg is None else tg.nt
+        if self.stride != 3 * self.S + self.S * self.K + (self.S * (self.nt + 4) if self.nt else 0):
+            raise _lib.EngineError(f"the engine's tally layout ({self.stride} cells per event) is not the detector's")
         weakref.finalize(self, _lib.lib().lh_model_destroy, h)
 
     def query(self, x, d, vol=None, last=None, tmax=0.0):
@@ -248,8 +255,10 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
                                    _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
     light = light_of(eng, tally, health.cpu().numpy(), samples if not isinstance(source, Batch) else None, source)
     light.kernel_ms = eng.kernel_ms()
+    tg = getattr(det, "time_grid", None)
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, controls=controls._asdict(),
-                            engine="liblumen_hip (gfx950)", reference=reference is not None)
+                            engine="liblumen_hip (gfx950)", reference=reference is not None,
+                            time_grid=None if tg is None else (tg.t0, tg.dt, tg.nt))
     return light
 
 
@@ -259,7 +268,13 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
 
     S, K = eng.S, eng.K
     arr, arr_t, arr2 = tally[:, :S], tally[:, S:2 * S], tally[:, 2 * S:3 * S]
-    charge = tally[:, 3 * S:].reshape(-1, S, K)
+    charge = tally[:, 3 * S:3 * S + S * K].reshape(-1, S, K)
+    time = late = early = None
+    if eng.nt > 0:
+        t0 = 3 * S + S * K
+        time = tally[:, t0:t0 + S * eng.nt].reshape(-1, S, eng.nt)
+        ov = tally[:, t0 + S * eng.nt:t0 + S * (eng.nt + 4)].reshape(-1, S, 2, 2)
+        late, early = ov[:, :, 0], ov[:, :, 1]
     if isinstance(source, Batch):
         n = torch.as_tensor([max(int(x), 2) for x in source.samples], dtype=torch.float64, device=tally.device)
         n = n[:, None] if source.events is None else torch.full_like(arr[:, :1], float(max(source.samples)))
@@ -269,4 +284,6 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
     mean_t = torch.where(arr > 0, arr_t / torch.where(arr > 0, arr, torch.ones_like(arr)), torch.zeros_like(arr))
     if tally.shape[0] == 1 and not isinstance(source, Batch):
         arr, err, mean_t, charge, arr2 = arr[0], err[0], mean_t[0], charge[0], arr2[0]
-    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0)
+        if time is not None:
+            time, late, early = time[0], late[0], early[0]
+    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0, time=time, late=late, early=early)

@@ -13,6 +13,8 @@
 // "step" schedule, src/lumen/run.py:81) and a launch of many sources with uneven photon counts keeps every lane busy.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -48,7 +50,7 @@ constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64
 constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
 constexpr size_t LDS_BLOCK_MAX = 64 * 1024;
 constexpr int OCC_CACHE = 8;
-constexpr unsigned F_REF = 1, F_WLS = 2;  // a kernel's features (template parameter)  // the most dynamic LDS one block of the persistent grid takes
+constexpr unsigned F_REF = 1, F_WLS = 2, F_TIME = 4;  // a kernel's features (template parameter)  // the most dynamic LDS one block of the persistent grid takes
 
 // The forward kernel's register budget: 4 wavefronts per SIMD (128 VGPRs) hide the walk's load latency better
 // than the 2-3 its natural allocation allows, spills included (measured on the 1,000-PMT tank).  The adjoint and
@@ -74,6 +76,9 @@ struct Args {
   double *__restrict__ tally;
   long long tally_stride;  // doubles per event
   int S, K;
+  int nt;               // the time histogram's nodes (0: none); its cells start at time_at of each event's tally
+  long long time_at;
+  float t0, dt;
   double *__restrict__ health;
   unsigned long long *__restrict__ queue;
   const float *__restrict__ ut;  // the unified facet tables (unified_tables.cpp)
@@ -119,7 +124,7 @@ __device__ __forceinline__ int class_medium(const Args &a, int cls) {
 }
 
 
-__device__ __forceinline__ void tally_add(const Args &a, int event, int slot, double v) {
+__device__ __forceinline__ void tally_add(const Args &a, int event, long long slot, double v) {
   const long long i = (long long)event * a.tally_stride + slot;
   if (a.lds_tally) {
     atomicAdd(lds_cells + i, v);
@@ -157,6 +162,7 @@ __device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, flo
 }
 
 // one deposit at sensor s: arrival, arrival_t, the charge cells at the QE knots, the photon's moment
+template <unsigned F>
 __device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float q, float time) {
   acc.records += 1;
   const float v = q * p.w0;
@@ -171,6 +177,41 @@ __device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float 
     const TabPos tp = tab_pos(a.P, qoff, a.K, p.lam);
     tally_add(a, p.event, 3 * a.S + s * a.K + tp.i, dv * (double)(1.0f - tp.f));
     if (tp.f > 0.0f) tally_add(a, p.event, 3 * a.S + s * a.K + tp.i + 1, dv * (double)tp.f);
+  }
+  if ((F & F_TIME) && a.nt > 0) {  // the time histogram (readout/grid.py `time_cells`): cubic Lagrange weights on the four nodes
+    // around the time, the shares beyond the last / before the first node kept as (q, q t) per sensor
+    double dq = dv;
+    if (a.K > 1) {  // weighted by the acceptance at the deposit's wavelength: the detected light's time shape
+      const TabPos tp = tab_pos(a.P, qoff, a.K, p.lam);
+      dq *= (double)tab_at(a.P, qoff, a.K, tp);
+    }
+    const float x = fminf(fmaxf((time - a.t0) / a.dt, -4.0f), (float)a.nt + 4.0f);
+    const float jf = floorf(x), g = x - jf;
+    const int j = (int)jf;
+    const float wk[4] = {-g * (g - 1.0f) * (g - 2.0f) / 6.0f, (g + 1.0f) * (g - 1.0f) * (g - 2.0f) / 2.0f,
+                         -(g + 1.0f) * g * (g - 2.0f) / 2.0f, (g + 1.0f) * g * (g - 1.0f) / 6.0f};
+    const long long tb = a.time_at + (long long)s * a.nt;
+    float late = 0.0f, early = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int n = j - 1 + k;
+      if (n >= a.nt) {
+        late += wk[k];
+      } else if (n < 0) {
+        early += wk[k];
+      } else {
+        tally_add(a, p.event, tb + n, dq * (double)wk[k]);
+      }
+    }
+    const long long ob = a.time_at + (long long)a.S * a.nt + 4LL * s;
+    if (late != 0.0f) {
+      tally_add(a, p.event, ob, dq * (double)late);
+      tally_add(a, p.event, ob + 1, dq * (double)late * (double)time);
+    }
+    if (early != 0.0f) {
+      tally_add(a, p.event, ob + 2, dq * (double)early);
+      tally_add(a, p.event, ob + 3, dq * (double)early * (double)time);
+    }
   }
   moment_add(a, p, acc, s, v);
 }
@@ -431,7 +472,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     const int sensor = g.vinfo[cr.vol_after].w;
     if (sensor >= 0 && !cont) {
       if (!ADJ) {
-        deposit(a, p, acc, sensor, dep_q, p.t + dist / v_g);
+        deposit<F>(a, p, acc, sensor, dep_q, p.t + dist / v_g);
       } else {
         const float D = dep_q * p.w0 * deposit_cotangent(a, b, p, sensor);
         if (D != 0.0f) {
@@ -802,6 +843,8 @@ struct lh_model_s {
   int layout_words = 0;
   int S = 0, K = 1, n_media = 0;
   long long tally_stride = 0;
+  int NT = 0;  // the time histogram's nodes t0 + j dt (ns), j < NT (0: none)
+  float t0 = 0.0f, dt = 1.0f;
   unsigned long long *queue = nullptr;
   lh_source *src = nullptr;
   long long *src_end = nullptr;
@@ -836,7 +879,15 @@ LH_API const char *lh_last_error(void) { return g_err.c_str(); }
 
 LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t layout_words, lh_model *out) {
   if (!G || !layout || !out) return fail(LH_ERR_ARG, "lh_model_create: null argument");
-  if (layout_words < 11 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  if (layout_words < 14 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  {  // the time histogram's grid
+    float t0 = 0.0f, dt = 0.0f;
+    memcpy(&t0, &layout[12], 4);
+    memcpy(&dt, &layout[13], 4);
+    if (layout[11] < 0 || layout[11] > (1 << 20) || (layout[11] > 0 && (!(dt > 0.0f) || !std::isfinite(dt) ||
+                                                                          !std::isfinite(t0))))
+      return fail(LH_ERR_LAYOUT, "lh_model_create: a time histogram needs 0 < nt <= 2^20 nodes, dt > 0, finite t0");
+  }
   const int n = G->n_vol;
   if (n <= 0 || n >= (1 << 21)) return fail(LH_ERR_ARG, "lh_model_create: n_vol out of range");
   for (int i = 0; i < G->n_nodes; ++i) {
@@ -1047,7 +1098,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   m->n_media = layout[1];
   m->S = layout[4];
   m->K = layout[5];
-  m->tally_stride = 3LL * m->S + (long long)m->S * m->K;
+  m->NT = layout[11];
+  memcpy(&m->t0, &layout[12], 4);
+  memcpy(&m->dt, &layout[13], 4);
+  m->tally_stride = 3LL * m->S + (long long)m->S * m->K + (m->NT > 0 ? (long long)m->S * (m->NT + 4) : 0);
   HIP_TRY(hipMalloc((void **)&m->queue, 256));
   {
     const std::vector<float> &ut = lh_host::unified_tables();
@@ -1158,6 +1212,10 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.chunk = c.chunk > 0 ? c.chunk : DEFAULT_CHUNK;
   a.tally = tally;
   a.tally_stride = m->tally_stride;
+  a.nt = m->NT;
+  a.time_at = 3LL * m->S + (long long)m->S * m->K;
+  a.t0 = m->t0;
+  a.dt = m->dt;
   a.S = m->S;
   a.K = m->K;
   a.health = health;
@@ -1173,24 +1231,29 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
 // F_WLS: a medium shifts wavelengths), so a run without them carries none of their registers
 template <unsigned F>
 static const void *kernel_f(bool adj) {
-  return adj ? (const void *)adjoint_kernel<F> : (const void *)trace_kernel<F>;
+  return adj ? (const void *)adjoint_kernel<F & (F_REF | F_WLS)> : (const void *)trace_kernel<F>;
 }
 
-static unsigned features_of(lh_model m, const Args &a) { return (a.has_ref ? F_REF : 0u) | (m->has_wls ? F_WLS : 0u); }
+static unsigned features_of(lh_model m, const Args &a, bool adj) {
+  return (a.has_ref ? F_REF : 0u) | (m->has_wls ? F_WLS : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u);
+}
+
+#define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
 static const void *kernel_of(bool adj, unsigned f) {
   switch (f) {
-    case 0: return kernel_f<0>(adj);
-    case F_REF: return kernel_f<F_REF>(adj);
-    case F_WLS: return kernel_f<F_WLS>(adj);
-    default: return kernel_f<F_REF | F_WLS>(adj);
+#define LH_KCASE(F) \
+  case F: return kernel_f<F>(adj);
+    LH_FEATURES(LH_KCASE)
+#undef LH_KCASE
   }
+  return nullptr;
 }
 
 template <unsigned F>
 static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Args &a, const AdjArgs &b, int n_lds) {
   if (adj)
-    hipLaunchKernelGGL(adjoint_kernel<F>, grid, dim3(BLOCK), bytes, st, a, b);
+    hipLaunchKernelGGL(adjoint_kernel<F & (F_REF | F_WLS)>, grid, dim3(BLOCK), bytes, st, a, b);
   else
     hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(BLOCK), bytes, st, a, n_lds);
 }
@@ -1198,10 +1261,10 @@ static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Ar
 static void launch_kernel(bool adj, unsigned f, dim3 grid, size_t bytes, hipStream_t st, const Args &a,
                           const AdjArgs &b, int n_lds) {
   switch (f) {
-    case 0: launch_f<0>(adj, grid, bytes, st, a, b, n_lds); break;
-    case F_REF: launch_f<F_REF>(adj, grid, bytes, st, a, b, n_lds); break;
-    case F_WLS: launch_f<F_WLS>(adj, grid, bytes, st, a, b, n_lds); break;
-    default: launch_f<F_REF | F_WLS>(adj, grid, bytes, st, a, b, n_lds); break;
+#define LH_LCASE(F) \
+  case F: launch_f<F>(adj, grid, bytes, st, a, b, n_lds); break;
+    LH_FEATURES(LH_LCASE)
+#undef LH_LCASE
   }
 }
 
@@ -1248,14 +1311,14 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes,
 
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
                            hipStream_t st) {
-  const unsigned f = features_of(m, a);
+  const unsigned f = features_of(m, a, true);
   launch_kernel(true, f, dim3(grid_of(m, ctl, kernel_of(true, f), bytes)), bytes, st, a, b, 0);
 }
 
 // the forward launch: the BVH nodes and the block's tallies in LDS as they fit
 static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *ctl, hipStream_t st) {
   const long long cells = (long long)n_events * m->tally_stride;
-  const unsigned f = features_of(m, a);
+  const unsigned f = features_of(m, a, false);
   const void *fn = kernel_of(false, f);
   const size_t bytes =
       place_lds(m, a, fn, cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0, true, ctl);
@@ -1318,7 +1381,7 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a)), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
@@ -1426,7 +1489,7 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   a.queue = b->queue;
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a)), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, g, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   return LH_OK;

@@ -132,6 +132,21 @@ class SurfaceRow:
         return TYPES[self.type][2].get(c, NEUTRAL[c])
 
 
+@dataclass(frozen=True)
+class TimeGrid:
+    """The time histogram's nodes t0 + j dt (ns), j < nt: each deposit split over its four nearest nodes with cubic
+    Lagrange weights; what falls beyond the last node (late) or before the first (early) kept as (q, q t) per sensor
+    (the reference's ``readout/grid.py`` ``Grid`` t0, dt, nt and ``time_cells``)."""  # noqa: D205, D209
+
+    t0: float
+    dt: float
+    nt: int
+
+    def __post_init__(self):
+        if self.nt < 1 or not self.dt > 0:
+            raise ValueError(f"TimeGrid: nt >= 1 and dt > 0, got nt={self.nt}, dt={self.dt}")
+
+
 @dataclass
 class SensorGroup:
     """A sensor group: its sensors (indices into the detector's sensors), QE and efficiency keys."""
@@ -158,6 +173,13 @@ class Detector:
     groups: list
     values: Params
     source_box: tuple = None
+    time_grid: "TimeGrid | None" = None  # the time histogram's static nodes (None: no time histogram)
+
+    def timed(self, t0: float, dt: float, nt: int) -> "Detector":
+        """Return this detector with a time histogram on the nodes t0 + j dt, j < nt (ns)."""
+        import dataclasses
+
+        return dataclasses.replace(self, time_grid=TimeGrid(float(t0), float(dt), int(nt)))
 
     def params(self) -> Params:
         """Return the detector's parameter values."""

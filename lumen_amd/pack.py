@@ -7,7 +7,8 @@ table of one knot.  ``slots`` (int32, one per ``values`` entry) names the gradie
 differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` declares the same):
 
     [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
-    [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)
+    [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)  [11] time nodes nt (0: no time
+    histogram)  [12] t0 (ns, float32 bits)  [13] dt (ns, float32 bits)
     media:  per medium 17 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
             (1 opaque, 2 a wavelength shifter, 4 its exponential time profile); (offset, knots) of WLSABSLENGTH,
             of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of WLSTIMECONSTANT
@@ -29,7 +30,7 @@ from .detector.model import COLUMNS, FLAG_EDGE, Detector
 from .params import Params, Table
 
 MAGIC = 0x4C484F31
-HEADER = 11
+HEADER = 14
 MEDIUM_WORDS = 17
 MED_OPAQUE, MED_WLS, MED_WLS_EXP = 1, 2, 4
 ROW_WORDS = 2 + 2 * len(COLUMNS)
@@ -162,7 +163,10 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
         qe_off, n_qe = b.table(q.energy, q.value)
     else:  # no QE table, or one QE per sensor: a single charge cell per sensor
         qe_off, n_qe = b.table([0.0], [1.0])
-    L = [MAGIC, nm, nc, nr, det.n_sensors, n_qe, 0, 0, 0, 0, qe_off]
+    tg = getattr(det, "time_grid", None)
+    f32bits = lambda x: int(np.array([x], np.float32).view(np.int32)[0])  # noqa: E731
+    L = [MAGIC, nm, nc, nr, det.n_sensors, n_qe, 0, 0, 0, 0, qe_off,
+         0 if tg is None else tg.nt, 0 if tg is None else f32bits(tg.t0), 0 if tg is None else f32bits(tg.dt)]
     L[6] = len(L)
     L += list(med.ravel())
     L[7] = len(L)

@@ -34,3 +34,61 @@ def pe(light, params: Params, det=None, group: int = 0):
         return eff * (charge * qe).sum(-1)
     qe = torch.as_tensor(np.asarray(q, np.float64), device=dev, dtype=dt)
     return eff * qe * charge[..., 0]
+
+
+def pe_error(light, params: Params, det=None, group: int = 0):
+    """Return the Monte Carlo error of ``pe`` per sensor at the values the run was given (the reference's
+    ``readout/detection.py`` ``pe_error``): the pe per arrival times the arrivals' error (exact when the sensor's
+    acceptance does not vary over the wavelengths it sees: a QE per sensor, or monochromatic light)."""  # noqa: D205, D209
+    import torch
+
+    p = pe(light, params, det, group)
+    a = light.arrivals
+    return torch.where(a > 0, p / torch.where(a > 0, a, torch.ones_like(a)), torch.zeros_like(a)) * light.arrivals_error
+
+
+def time_masses(light, params: Params, det=None, group: int = 0):
+    """Return the time histogram as photoelectrons (node masses (S, nt), late (S, 2), early (S, 2); (q, q t) in each
+    overflow), scaled per sensor so nodes and overflow sum to ``pe`` (``readout/detection.py`` ``block_masses``)."""  # noqa: D205, D209
+    import torch
+
+    if light.time is None:
+        raise ValueError("time_masses: no time histogram: simulate a timed detector (Detector.timed)")
+    H, late, early = light.time, light.late, light.early
+    total = H.sum(-1) + late[..., 0] + early[..., 0]
+    scale = pe(light, params, det, group) / torch.where(total != 0.0, total, torch.ones_like(total))
+    return H * scale[..., None], late * scale[..., None], early * scale[..., None]
+
+
+def pe_density(light, params: Params, gate, bin_width: float, tts_sigma: float, det=None, group: int = 0):
+    """Return the expected photoelectrons per time bin (S, n_bins) over ``gate`` = (g0, g1) ns, bins of
+    ``bin_width``: the node masses (``time_masses``), each spread by a Gaussian transit-time spread of width
+    ``tts_sigma`` ns and integrated over the bins (its CDF differences); the overflow masses at their mean times
+    (``readout/detection.py`` ``density`` with a Gaussian ``spread``).  ``light.provenance`` carries the grid."""  # noqa: D205, D209
+    import math
+
+    import torch
+
+    if not tts_sigma > 0:
+        raise ValueError("pe_density: tts_sigma > 0")
+    H, late, early = time_masses(light, params, det, group)
+    tg = light.provenance.get("time_grid") if light.provenance else None
+    if tg is None:
+        raise ValueError("pe_density: the Light does not carry its time grid")
+    t0, dt, nt = tg
+    dev, dtype = H.device, H.dtype
+    n_bins = int(math.ceil((gate[1] - gate[0]) / bin_width - 1e-9))
+    edges = gate[0] + bin_width * torch.arange(n_bins + 1, device=dev, dtype=dtype)
+    nodes = t0 + dt * torch.arange(nt, device=dev, dtype=dtype)
+
+    def spread(times):  # (..., m) -> (..., m, n_bins): the Gaussian's mass in each bin
+        z = (edges - times[..., None]) / (tts_sigma * math.sqrt(2.0))
+        c = 0.5 * (1.0 + torch.erf(z))
+        return c[..., 1:] - c[..., :-1]
+
+    out = H @ spread(nodes)
+    for ov in (late, early):
+        q, qt = ov[..., 0], ov[..., 1]
+        tm = torch.where(q != 0.0, qt / torch.where(q != 0.0, q, torch.ones_like(q)), torch.zeros_like(q))
+        out = out + q[..., None] * spread(tm[..., None])[..., 0, :]
+    return out

@@ -770,6 +770,9 @@ struct Run {
   int max_vert;
   int S, K;
   long long stride;
+  int nt;  // the time histogram's nodes t0 + j dt (0: none), its cells from time_at of each event's tally
+  double t0, dt;
+  long long time_at;
 };
 
 struct Tally {
@@ -988,6 +991,27 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
           tal[3 * r.S + sensor * r.K + i] += v * (1 - f);
           if (f > 0) tal[3 * r.S + sensor * r.K + i + 1] += v * f;
         }
+        if (r.nt > 0) {  // the time histogram (readout/grid.py:190-225 `_time_weights`, `time_cells`)
+          double dq = v;
+          if (r.K > 1) dq *= r.live.tab(r.live.L[10], r.K, p.lam);  // weighted by the acceptance at the wavelength
+          const double x = std::min(std::max((time - r.t0) / r.dt, -4.0), r.nt + 4.0);
+          const double jf = std::floor(x), g = x - jf;
+          const int j = (int)jf;
+          const double wk[4] = {-g * (g - 1) * (g - 2) / 6, (g + 1) * (g - 1) * (g - 2) / 2,
+                                -(g + 1) * g * (g - 2) / 2, (g + 1) * g * (g - 1) / 6};
+          double late = 0, early = 0;
+          for (int k = 0; k < 4; ++k) {
+            const int nn = j - 1 + k;
+            if (nn >= r.nt) late += wk[k];
+            else if (nn < 0) early += wk[k];
+            else tal[r.time_at + (long long)sensor * r.nt + nn] += dq * wk[k];
+          }
+          double *ov = tal + r.time_at + (long long)r.S * r.nt + 4LL * sensor;
+          ov[0] += dq * late;
+          ov[1] += dq * late * time;
+          ov[2] += dq * early;
+          ov[3] += dq * early * time;
+        }
         mom.add(sensor, v);
       }
     }
@@ -1152,7 +1176,14 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
   r.max_vert = (ctl && ctl->max_vertices > 0) ? ctl->max_vertices : (1 << 22);
   r.S = layout[4];
   r.K = layout[5];
-  r.stride = 3LL * r.S + (long long)r.S * r.K;
+  r.nt = layout[11];
+  float t0f, dtf;
+  std::memcpy(&t0f, &layout[12], 4);
+  std::memcpy(&dtf, &layout[13], 4);
+  r.t0 = t0f;
+  r.dt = dtf;
+  r.time_at = 3LL * r.S + (long long)r.S * r.K;
+  r.stride = r.time_at + (r.nt > 0 ? (long long)r.S * (r.nt + 4) : 0);
   return 0;
 }
 

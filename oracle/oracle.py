@@ -79,7 +79,8 @@ class OracleRun:
         pk = pack.pack(self.det, p, edge_rows=edge_rows)
         ref = None if reference is None else pack.pack(self.det, reference, edge_rows=edge_rows).values
         rows, n_ev = api.source_rows(self.det, source, samples, first_id)
-        stride = 3 * self.S + self.S * int(pk.layout[5])
+        nt = int(pk.layout[11])
+        stride = 3 * self.S + self.S * int(pk.layout[5]) + (self.S * (nt + 4) if nt else 0)
         tally = np.zeros((n_ev, stride))
         health = np.zeros(16)
         ctl = (controls or api.Controls()).struct()
@@ -178,5 +179,12 @@ def light_of(tally, health, S, K, samples):
     n = float(max(samples, 2))
     err = np.sqrt(np.maximum(n / (n - 1) * (arr2 - arr * arr / n), 0.0))
     mean_t = np.where(arr > 0, arr_t / np.where(arr > 0, arr, 1.0), 0.0)
-    return dict(arrivals=arr, arrivals_error=err, mean_arrival_time=mean_t, arrival2=arr2,
-                charge=t[3 * S:].reshape(S, K), health=health)
+    out = dict(arrivals=arr, arrivals_error=err, mean_arrival_time=mean_t, arrival2=arr2,
+               charge=t[3 * S:3 * S + S * K].reshape(S, K), health=health)
+    rest = t[3 * S + S * K:]
+    if rest.size:  # a timed detector: the time histogram (S, nt) and its overflow (S, 2 late/early, 2 q/qt)
+        nt = rest.size // S - 4
+        out["time"] = rest[:S * nt].reshape(S, nt)
+        ov = rest[S * nt:].reshape(S, 2, 2)
+        out["late"], out["early"] = ov[:, 0], ov[:, 1]
+    return out

@@ -10,6 +10,7 @@ import pytest
 from scipy import stats
 
 import lumen_amd as L
+from lumen_amd import units
 from lumen_amd.api import engine
 from lumen_amd.detector import example_tank, muon_track, water_tank
 from oracle import oracle as O
@@ -196,3 +197,37 @@ def test_reference_values_hold_the_draws(gpu, tank1):
     f0, fp, fm = tot(1.0), tot(1.01), tot(0.99)
     d_c, d_p, d_m = (fp - fm) / 0.02, (fp - f0) / 0.01, (f0 - fm) / 0.01
     assert d_c > 0 and abs(d_p - d_c) < 0.05 * d_c and abs(d_m - d_c) < 0.05 * d_c
+
+
+def test_time_histogram_against_oracle(gpu, tank2):
+    """The timed 1,000-PMT tank: the engine's time histogram and overflow cells against the oracle's (the same
+    photons), and the readout's density integrating to the pe over a gate that holds them."""
+    det = tank2.timed(0.0, 0.5, 80)
+    src = _cherenkov()
+    n = 1 << 16
+    g = L.simulate(det, src, samples=n, seed=4)
+    tal, h = O.OracleRun(det).trace(src, n, seed=4, bvh=True)
+    o = O.light_of(tal, h, det.n_sensors, 1, n)
+    Hg, Ho = g.time.cpu().numpy(), o["time"]
+    assert Hg.sum() > 0
+    np.testing.assert_allclose(Hg.sum(), Ho.sum(), rtol=2e-3)
+    np.testing.assert_allclose(g.late.cpu().numpy().sum(0), o["late"].sum(0), rtol=2e-3, atol=1e-9)
+    tot = Hg.sum(-1) + g.late.cpu().numpy()[:, 0] + g.early.cpu().numpy()[:, 0]
+    np.testing.assert_allclose(tot, g.arrivals.cpu().numpy(), rtol=1e-6, atol=1e-12)
+    p = det.params()
+    dens = L.readout.pe_density(g, p, gate=(-10.0, 200.0), bin_width=1.0, tts_sigma=1.3, det=det)
+    pe = L.readout.pe(g, p, det)
+    np.testing.assert_allclose(dens.sum(-1).cpu().numpy(), pe.cpu().numpy(), rtol=1e-6, atol=1e-12)
+
+
+def test_time_histogram_moments_on_device(gpu):
+    """Every photon of the sensor sphere arrives at T = a / v_g: the node masses' first three moments are T's."""
+    a, n_idx = 1.5, 1.40
+    T = a / (units.C_VACUUM / n_idx)
+    det = sphere_toy(a=a, R=0.0, L_abs=None, n=n_idx).timed(0.0, 0.37, 40)
+    g = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), samples=1 << 14, seed=2)
+    H, tn = g.time.cpu().numpy()[0], 0.37 * np.arange(40)
+    s = float(g.arrivals[0])
+    np.testing.assert_allclose(H.sum(), s, rtol=1e-6)
+    for k in (1, 2, 3):
+        np.testing.assert_allclose((H * tn**k).sum() / s, T**k, rtol=2e-5)  # float32 times

@@ -321,3 +321,42 @@ def test_bvh_walk_equals_brute_force():
     b, hb = o.trace(src, 1 << 12, seed=3, bvh=False)
     np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
     assert ha[12] == hb[12]
+
+
+def _timed_sphere(t0, dt, nt, a=1.5, n_idx=1.40):
+    return sphere_toy(a=a, R=0.0, L_abs=None, n=n_idx).timed(t0, dt, nt)
+
+
+def test_time_histogram_reproduces_cubic_moments():
+    """Every photon arrives at T = a / v_g: the cubic Lagrange split of a point on the nodes keeps the moments of
+    order 0 to 3 (readout/grid.py:190-199), so the node masses sum to the arrivals and their first three time moments
+    are T, T^2, T^3; nothing overflows."""
+    a, n_idx = 1.5, 1.40
+    T = a / (units.C_VACUUM / n_idx)
+    det = _timed_sphere(0.0, 0.37, 40, a, n_idx)
+    tal, h = O.OracleRun(det).trace(L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0), 512, seed=2)
+    o = O.light_of(tal, h, 1, 1, 512)
+    H, tn = o["time"][0], 0.37 * np.arange(40)
+    s = o["arrivals"][0]
+    np.testing.assert_allclose(H.sum(), s, rtol=1e-12)
+    for k in (1, 2, 3):
+        np.testing.assert_allclose((H * tn**k).sum() / s, T**k, rtol=2e-7)  # T from float32 tables
+    assert np.all(o["late"] == 0) and np.all(o["early"] == 0)
+
+
+def test_time_histogram_overflow_keeps_totals():
+    """A grid ending before the arrivals puts them all in the late cell as (q, q T); one ending within two nodes of
+    them splits them with signed weights, nodes plus overflow still summing to the arrivals."""
+    a, n_idx = 1.5, 1.40
+    T = a / (units.C_VACUUM / n_idx)
+    src = L.sources.Point((0, 0, 0), lam=400.0, n_photons=1.0)
+    det = _timed_sphere(0.0, 0.5, 8, a, n_idx)  # nodes 0 .. 3.5 ns, T = 7.0 ns
+    o = O.light_of(*O.OracleRun(det).trace(src, 256, seed=3), 1, 1, 256)
+    s = o["arrivals"][0]
+    assert np.all(o["time"] == 0)
+    np.testing.assert_allclose(o["late"][0], [s, s * T], rtol=2e-7)
+    det = _timed_sphere(T - 1.75, 0.5, 4, a, n_idx)  # T half a node past the last node
+    o = O.light_of(*O.OracleRun(det).trace(src, 256, seed=3), 1, 1, 256)
+    tot = o["time"][0].sum() + o["late"][0, 0] + o["early"][0, 0]
+    np.testing.assert_allclose(tot, s, rtol=1e-12)
+    assert o["late"][0, 0] != 0
