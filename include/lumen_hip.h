@@ -121,7 +121,9 @@ int lh_model_destroy(lh_model model);
 int lh_model_info(lh_model model, int32_t *n_sensors, int32_t *n_qe, int64_t *tally_per_event);
 
 /* Trace every photon of `sources` (host array of n_sources) and add into `tally_dev` (n_events blocks of
- * [arrival S][arrival_t S][arrival2 S][charge S x n_qe], FP64) and `health_dev` (LH_HEALTH_N FP64).  Adds: zero
+ * [arrival S][arrival_t S][arrival2 S][charge S x n_qe], and with a time grid in the layout (its words 11-13: nt,
+ * t0, dt) [time S x nt][overflow S x (late q, late q t, early q, early q t)], FP64) and `health_dev` (LH_HEALTH_N
+ * FP64).  Adds: zero
  * them first for a fresh run.  `values_dev`: the packed values (float32, device, n_values of them, each table
  * within them); `ref_values_dev`: the reference values every random choice is made at (NULL: values_dev); every
  * source's event is below n_events.  Replaces run.trace + run.rebuild (src/lumen/run.py:93-145). */

@@ -974,7 +974,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   // the device layout: float4 / int4 arrays in one allocation
   std::vector<float4> xf(3 * (size_t)n), gxf(3 * (size_t)std::max(G->n_groups, 1)), node(2 * (size_t)std::max(G->n_nodes, 1)),
       leaf(4 * (size_t)G->n_leaves), plane((size_t)std::max(G->n_planes, 1));
-  std::vector<int4> vinfo((size_t)n + 1), kinds((size_t)G->n_kinds);
+  std::vector<int4> vinfo((size_t)n + 1), kinds((size_t)G->n_kinds), vkind((size_t)n + 1);
   std::vector<int2> vgrp((size_t)n + 1), grange((size_t)std::max(G->n_groups, 1));
   std::vector<int> items((size_t)std::max(G->n_items, 1)), prog((size_t)std::max(G->n_prog, 1));
   for (int v = 0; v < n; ++v) {
@@ -1026,6 +1026,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     }
   }
   for (int k = 0; k < G->n_kinds; ++k) kinds[k] = make_int4(G->kind_l0[k], G->kind_nl[k], G->kind_p0[k], G->kind_pl[k]);
+  for (int v = 0; v <= n; ++v) vkind[v] = v < n ? kinds[G->kind[v]] : make_int4(0, 0, 0, 0);
   for (int j = 0; j < G->n_prog; ++j) prog[j] = (G->prog_op[j] << 24) | (G->prog_arg[j] & 0xFFFFFF);
 
   size_t off = 0;
@@ -1036,7 +1037,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   };
   const size_t o_xf = place(xf.size() * 16), o_gxf = place(gxf.size() * 16), o_node = place(node.size() * 16),
                o_leaf = place(leaf.size() * 16), o_plane = place(plane.size() * 16), o_vinfo = place(vinfo.size() * 16),
-               o_kinds = place(kinds.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
+               o_kinds = place(kinds.size() * 16), o_vkind = place(vkind.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
                o_items = place(items.size() * 4), o_prog = place(prog.size() * 4);
   lh_model m = new lh_model_s();
   char *buf = nullptr;
@@ -1054,6 +1055,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   memcpy(host.data() + o_plane, plane.data(), plane.size() * 16);
   memcpy(host.data() + o_vinfo, vinfo.data(), vinfo.size() * 16);
   memcpy(host.data() + o_kinds, kinds.data(), kinds.size() * 16);
+  memcpy(host.data() + o_vkind, vkind.data(), vkind.size() * 16);
   memcpy(host.data() + o_vgrp, vgrp.data(), vgrp.size() * 8);
   memcpy(host.data() + o_grange, grange.data(), grange.size() * 8);
   memcpy(host.data() + o_items, items.data(), items.size() * 4);
@@ -1068,6 +1070,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.plane = (const float4 *)(buf + o_plane);
   g.vinfo = (const int4 *)(buf + o_vinfo);
   g.kinds = (const int4 *)(buf + o_kinds);
+  g.vkind = (const int4 *)(buf + o_vkind);
   g.vgrp = (const int2 *)(buf + o_vgrp);
   g.grange = (const int2 *)(buf + o_grange);
   g.items = (const int *)(buf + o_items);

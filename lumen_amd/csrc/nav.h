@@ -42,6 +42,7 @@ struct Geo {
   const float4 *__restrict__ leaf;  // 4 per leaf: (q, c), (A00 A01 A02 A11), (A12 A22 b0 b1), (b2, typ, p0, np)
   const float4 *__restrict__ plane; // (n, h) relative to the leaf's q
   const int4 *__restrict__ kinds;   // l0, nl, p0, pl
+  const int4 *__restrict__ vkind;   // per navigator id: its kind's (l0, nl, p0, pl), read beside its transform
   const int *__restrict__ prog;     // op << 24 | arg
   float reach;
   int n_nodes;
@@ -243,8 +244,7 @@ __device__ __forceinline__ bool run_program(const Geo &g, int p0, int pl, uint32
   return (st & 1u) != 0u;
 }
 
-__device__ inline bool solid_inside(const Geo &g, int kind, float3 x) {
-  const int4 K = g.kinds[kind];
+__device__ inline bool solid_inside(const Geo &g, int4 K, float3 x) {
   if (K.y == 1) return leaf_inside(g, K.x, x);
   uint32_t bits = 0;
   for (int k = 0; k < K.y && k < 32; ++k) bits |= (uint32_t)leaf_inside(g, K.x + k, x) << k;
@@ -260,9 +260,8 @@ struct Change {
 // the first change of the solid's membership along the ray after t = 0, from the side `want_from` (true: the
 // point starts inside and the change is an exit; false: an entry).  Endpoints on the face `skip_code` within
 // `skip_t` of the start count as passed.  `vol` names the volume in the surface codes.
-__device__ inline Change first_change(const Geo &g, int kind, int vol, float3 o, float3 d, bool exit_query,
+__device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, float3 d, bool exit_query,
                                       int skip_code, float skip_t) {
-  const int4 K = g.kinds[kind];
   Change ch{HUGEF, -1, -1};
   if (K.y == 1) {  // one leaf: its intervals are the solid's
     LeafIv iv;
@@ -339,7 +338,7 @@ struct Cross {
 __device__ inline float3 crossing_normal(const Geo &g, int v, int leaf_local, int face, float3 x, float3 d, float t) {
   float3 xl, dl;
   to_local(g, v, x, d, xl, dl);
-  const int li = g.kinds[g.vinfo[v].x].x + leaf_local;
+  const int li = g.vkind[v].x + leaf_local;
   const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
   float3 nl;
   if (face == 0) {
@@ -363,7 +362,7 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
   if (vol < g.nvol) {
     float3 xl, dl;
     to_local(g, vol, x, d, xl, dl);
-    const Change ch = first_change(g, g.vinfo[vol].x, vol, xl, dl, true, last, skip_t);
+    const Change ch = first_change(g, g.vkind[vol], vol, xl, dl, true, last, skip_t);
     if (ch.leaf >= 0 && ch.t <= best.t) {
       best.t = ch.t;
       best.vol_after = g.vinfo[vol].y;
@@ -415,7 +414,7 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
         const int u = g.items[pst + k];
         float3 xl, dl;
         to_local(g, u, x, d, xl, dl);
-        const Change ch = first_change(g, g.vinfo[u].x, u, xl, dl, false, last, skip_t);
+        const Change ch = first_change(g, g.vkind[u], u, xl, dl, false, last, skip_t);
         if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
           best.t = ch.t;
           best.vol_after = u;
@@ -461,7 +460,7 @@ __device__ inline int locate(const Geo &g, float3 x) {
             const int u = g.items[(sc >> 3) + k];
             float3 xl, dl;
             to_local(g, u, x, mk3(0.f, 0.f, 1.f), xl, dl);
-            if (solid_inside(g, g.vinfo[u].x, xl)) { found = u; break; }
+            if (solid_inside(g, g.vkind[u], xl)) { found = u; break; }
           }
           ni = skip;
         } else {
