@@ -174,6 +174,12 @@ int lh_query(lh_model model, int32_t n, const float *x_dev, const float *d_dev, 
 /* The time of the last lh_trace / lh_adjoint's photon-loop kernel on its stream, in ms (HIP events). */
 int lh_last_kernel_ms(lh_model model, float *ms);
 
+/* The shape of the model's last launch (8 int32): grid blocks, resident blocks per CU, dynamic LDS bytes per
+ * block, tallies in LDS (0/1), BVH nodes in LDS (0/1), the LDS budget per block, the CU's LDS bytes, the kernel's
+ * feature bits (1 separate reference values, 2 a wavelength shifter, 4 a time histogram; 0x100 the adjoint).
+ * Diagnostics: no counterpart in the reference. */
+int lh_last_launch(lh_model model, int32_t *info);
+
 const char *lh_last_error(void);
 
 #ifdef __cplusplus

@@ -138,6 +138,14 @@ class Engine:
             raise _lib.EngineError(f"the engine's tally layout ({self.stride} cells per event) is not the detector's")
         weakref.finalize(self, _lib.lib().lh_model_destroy, h)
 
+    def last_launch(self) -> dict:
+        """Return the shape of the last launch on this model (grid, residency, LDS use, kernel features)."""
+        out = np.zeros(8, np.int32)
+        _lib.check(_lib.lib().lh_last_launch(self.handle, out.ctypes.data_as(_lib._i)), "lh_last_launch")
+        keys = ("grid", "blocks_per_cu", "lds_bytes", "tally_in_lds", "nodes_in_lds", "lds_budget", "lds_per_cu",
+                "features")
+        return dict(zip(keys, (int(v) for v in out)))
+
     def query(self, x, d, vol=None, last=None, tmax=0.0):
         """Return (t, vol_after, normal, srf, located) of rays x, d (n, 3) on the device navigator (float32)."""
         import torch

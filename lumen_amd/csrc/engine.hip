@@ -856,6 +856,9 @@ struct lh_model_s {
   float *ut = nullptr;  // the unified facet tables on the device
   int cus = 0, lds_per_cu = 0;
   bool has_wls = false;  // a medium shifts wavelengths
+  // the last launch's shape: grid blocks, blocks per CU, LDS bytes per block, tallies in LDS, nodes in LDS, the
+  // LDS budget per block, the CU's LDS, the kernel's feature bits
+  int32_t last_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const void *occ_fn[OCC_CACHE] = {};  // the occupancy cache of grid_of
   size_t occ_lds[OCC_CACHE] = {};
   int occ_per[OCC_CACHE] = {};
@@ -1027,6 +1030,13 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   }
   for (int k = 0; k < G->n_kinds; ++k) kinds[k] = make_int4(G->kind_l0[k], G->kind_nl[k], G->kind_p0[k], G->kind_pl[k]);
   for (int v = 0; v <= n; ++v) vkind[v] = v < n ? kinds[G->kind[v]] : make_int4(0, 0, 0, 0);
+  std::vector<float4> ixf((size_t)3 * std::max(G->n_items, 1));
+  std::vector<int4> ikind((size_t)std::max(G->n_items, 1));
+  for (int i = 0; i < G->n_items; ++i) {  // the items' volume data in the items' order
+    const int u = G->items[i];
+    for (int r = 0; r < 3; ++r) ixf[3 * i + r] = xf[3 * u + r];
+    ikind[i] = vkind[u];
+  }
   for (int j = 0; j < G->n_prog; ++j) prog[j] = (G->prog_op[j] << 24) | (G->prog_arg[j] & 0xFFFFFF);
 
   size_t off = 0;
@@ -1037,7 +1047,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   };
   const size_t o_xf = place(xf.size() * 16), o_gxf = place(gxf.size() * 16), o_node = place(node.size() * 16),
                o_leaf = place(leaf.size() * 16), o_plane = place(plane.size() * 16), o_vinfo = place(vinfo.size() * 16),
-               o_kinds = place(kinds.size() * 16), o_vkind = place(vkind.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
+               o_kinds = place(kinds.size() * 16), o_vkind = place(vkind.size() * 16),
+               o_ixf = place(ixf.size() * 16), o_ikind = place(ikind.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
                o_items = place(items.size() * 4), o_prog = place(prog.size() * 4);
   lh_model m = new lh_model_s();
   char *buf = nullptr;
@@ -1056,6 +1067,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   memcpy(host.data() + o_vinfo, vinfo.data(), vinfo.size() * 16);
   memcpy(host.data() + o_kinds, kinds.data(), kinds.size() * 16);
   memcpy(host.data() + o_vkind, vkind.data(), vkind.size() * 16);
+  memcpy(host.data() + o_ixf, ixf.data(), ixf.size() * 16);
+  memcpy(host.data() + o_ikind, ikind.data(), ikind.size() * 16);
   memcpy(host.data() + o_vgrp, vgrp.data(), vgrp.size() * 8);
   memcpy(host.data() + o_grange, grange.data(), grange.size() * 8);
   memcpy(host.data() + o_items, items.data(), items.size() * 4);
@@ -1071,6 +1084,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.vinfo = (const int4 *)(buf + o_vinfo);
   g.kinds = (const int4 *)(buf + o_kinds);
   g.vkind = (const int4 *)(buf + o_vkind);
+  g.ixf = (const float4 *)(buf + o_ixf);
+  g.ikind = (const int4 *)(buf + o_ikind);
   g.vgrp = (const int2 *)(buf + o_vgrp);
   g.grange = (const int2 *)(buf + o_grange);
   g.items = (const int *)(buf + o_items);
@@ -1309,13 +1324,22 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes,
   a.g.lds_nodes = nodes ? 1 : 0;
   const size_t off = cells ? cells_al : 0;
   a.g.lds_node_off = (int)(off / 16);
+  m->last_launch[3] = a.lds_tally;
+  m->last_launch[4] = a.g.lds_nodes;
+  m->last_launch[5] = (int32_t)budget;
+  m->last_launch[6] = m->lds_per_cu;
   return off + (nodes ? node_bytes : 0);
 }
 
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
                            hipStream_t st) {
   const unsigned f = features_of(m, a, true);
-  launch_kernel(true, f, dim3(grid_of(m, ctl, kernel_of(true, f), bytes)), bytes, st, a, b, 0);
+  const int grid = grid_of(m, ctl, kernel_of(true, f), bytes);
+  m->last_launch[0] = grid;
+  m->last_launch[1] = grid / std::max(m->cus, 1);
+  m->last_launch[2] = (int32_t)bytes;
+  m->last_launch[7] = (int32_t)f | 0x100;
+  launch_kernel(true, f, dim3(grid), bytes, st, a, b, 0);
 }
 
 // the forward launch: the BVH nodes and the block's tallies in LDS as they fit
@@ -1327,7 +1351,12 @@ static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *c
       place_lds(m, a, fn, cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0, true, ctl);
   const int n_lds = a.lds_tally ? (int)cells : 0;
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  launch_kernel(false, f, dim3(grid_of(m, ctl, fn, bytes)), bytes, st, a, AdjArgs{}, n_lds);
+  const int grid = grid_of(m, ctl, fn, bytes);
+  m->last_launch[0] = grid;
+  m->last_launch[1] = grid / std::max(m->cus, 1);
+  m->last_launch[2] = (int32_t)bytes;
+  m->last_launch[7] = (int32_t)f;
+  launch_kernel(false, f, dim3(grid), bytes, st, a, AdjArgs{}, n_lds);
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
@@ -1510,6 +1539,12 @@ LH_API int lh_query(lh_model m, int32_t n, const float *x_dev, const float *d_de
   hipLaunchKernelGGL(query_kernel, dim3((n + 255) / 256), dim3(256), 0, st, m->g, n, x_dev, d_dev, vol_dev, last_dev,
                      tmax > 0.0f ? tmax : m->g.reach, t_dev, vol_after_dev, normal_dev, srf_dev, located_dev);
   HIP_TRY(hipGetLastError());
+  return LH_OK;
+}
+
+LH_API int lh_last_launch(lh_model m, int32_t *info) {
+  if (!m || !info) return fail(LH_ERR_ARG, "lh_last_launch: null argument");
+  for (int i = 0; i < 8; ++i) info[i] = m->last_launch[i];
   return LH_OK;
 }
 

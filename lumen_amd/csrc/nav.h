@@ -43,6 +43,8 @@ struct Geo {
   const float4 *__restrict__ plane; // (n, h) relative to the leaf's q
   const int4 *__restrict__ kinds;   // l0, nl, p0, pl
   const int4 *__restrict__ vkind;   // per navigator id: its kind's (l0, nl, p0, pl), read beside its transform
+  const float4 *__restrict__ ixf;   // per BVH item (in items' order): its volume's 3 transform rows
+  const int4 *__restrict__ ikind;   // per BVH item: its volume's kind record
   const int *__restrict__ prog;     // op << 24 | arg
   float reach;
   int n_nodes;
@@ -87,6 +89,17 @@ __device__ __forceinline__ int srf_code(int vol, int leaf_local, int face) {
   return (vol << 10) | (leaf_local << 5) | face;
 }
 
+// world -> a frame given by its three rows (R^T | -R^T t)
+__device__ __forceinline__ void rows_to_local(const float4 *__restrict__ rows, float3 x, float3 d, float3 &xl,
+                                              float3 &dl) {
+  const float4 r0 = rows[0], r1 = rows[1], r2 = rows[2];
+  xl = mk3(fmaf(r0.x, x.x, fmaf(r0.y, x.y, fmaf(r0.z, x.z, r0.w))),
+           fmaf(r1.x, x.x, fmaf(r1.y, x.y, fmaf(r1.z, x.z, r1.w))),
+           fmaf(r2.x, x.x, fmaf(r2.y, x.y, fmaf(r2.z, x.z, r2.w))));
+  dl = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+           r2.x * d.x + r2.y * d.y + r2.z * d.z);
+}
+
 // world -> a volume's frame
 __device__ __forceinline__ void to_local(const Geo &g, int v, float3 x, float3 d, float3 &xl, float3 &dl) {
   const float4 r0 = g.xf[3 * v], r1 = g.xf[3 * v + 1], r2 = g.xf[3 * v + 2];
@@ -95,6 +108,12 @@ __device__ __forceinline__ void to_local(const Geo &g, int v, float3 x, float3 d
            fmaf(r2.x, x.x, fmaf(r2.y, x.y, fmaf(r2.z, x.z, r2.w))));
   dl = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
            r2.x * d.x + r2.y * d.y + r2.z * d.z);
+}
+
+__device__ __forceinline__ float3 rows_normal_to_world(const float4 *__restrict__ rows, float3 n) {
+  const float4 r0 = rows[0], r1 = rows[1], r2 = rows[2];
+  return mk3(r0.x * n.x + r1.x * n.y + r2.x * n.z, r0.y * n.x + r1.y * n.y + r2.y * n.z,
+             r0.z * n.x + r1.z * n.y + r2.z * n.z);
 }
 
 __device__ __forceinline__ float3 normal_to_world(const Geo &g, int v, float3 n) {
@@ -254,7 +273,21 @@ __device__ inline bool solid_inside(const Geo &g, int4 K, float3 x) {
 struct Change {
   float t;
   int leaf, face;  // leaf within its kind; -1: none
+  float3 n;        // the face's normal there, in the solid's frame (not normalised)
 };
+
+// the normal of leaf li's face `face` (0 the quadric: A y + b at y = o + t d - q; k >= 1 plane k) in its frame; the
+// leaf's data was just read by its interval test, so these reads hit the cache
+__device__ __forceinline__ float3 leaf_normal(const Geo &g, int li, int face, float3 o, float3 d, float t) {
+  const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
+  if (face == 0) {
+    const float3 y = sub3(fma3(t, d, o), mk3(L0.x, L0.y, L0.z));
+    return mk3(L1.x * y.x + L1.y * y.y + L1.z * y.z + L2.z, L1.y * y.x + L1.w * y.y + L2.x * y.z + L2.w,
+               L1.z * y.x + L2.x * y.y + L2.y * y.z + L3.x);
+  }
+  const float4 P = g.plane[__float_as_int(L3.z) + face - 1];
+  return mk3(P.x, P.y, P.z);
+}
 
 
 // the first change of the solid's membership along the ray after t = 0, from the side `want_from` (true: the
@@ -262,7 +295,7 @@ struct Change {
 // `skip_t` of the start count as passed.  `vol` names the volume in the surface codes.
 __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, float3 d, bool exit_query,
                                       int skip_code, float skip_t) {
-  Change ch{HUGEF, -1, -1};
+  Change ch{HUGEF, -1, -1, mk3(0.f, 0.f, 1.f)};
   if (K.y == 1) {  // one leaf: its intervals are the solid's
     LeafIv iv;
     leaf_intervals(g, K.x, o, d, iv);
@@ -283,6 +316,7 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
       ch.leaf = 0;
       ch.face = 0;
     }
+    if (ch.leaf >= 0) ch.n = leaf_normal(g, K.x, ch.face, o, d, ch.t);
     return ch;
   }
   // several leaves: scan the endpoints in order for the first change of the program's value
@@ -305,7 +339,7 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
     return run_program(g, K.z, K.w, bits);
   };
   const bool start = member(0.0f);
-  if (exit_query && !start) return Change{0.0f, 0, 0};
+  if (exit_query && !start) return Change{0.0f, 0, 0, leaf_normal(g, K.x, 0, o, d, 0.0f)};
   if (!exit_query && start) return ch;  // inside a daughter already (rounding): no entry ahead
   float cur = 0.0f;
   for (int it = 0; it < 4 * nl + 5; ++it) {
@@ -319,10 +353,10 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
         }
       }
     if (nl_ < 0 || nxt >= HUGEF) break;
-    if (member(nxt) != start) return Change{nxt, nl_, nf};
+    if (member(nxt) != start) return Change{nxt, nl_, nf, leaf_normal(g, K.x + nl_, nf, o, d, nxt)};
     cur = nxt;
   }
-  if (exit_query) return Change{0.0f, 0, 0};
+  if (exit_query) return Change{0.0f, 0, 0, leaf_normal(g, K.x, 0, o, d, 0.0f)};
   return ch;
 }
 
@@ -333,26 +367,6 @@ struct Cross {
   int own;        // 1: the exit of the photon's own volume
   float3 n;       // unit normal facing the photon
 };
-
-// the surface normal of a crossing (world frame, facing the photon)
-__device__ inline float3 crossing_normal(const Geo &g, int v, int leaf_local, int face, float3 x, float3 d, float t) {
-  float3 xl, dl;
-  to_local(g, v, x, d, xl, dl);
-  const int li = g.vkind[v].x + leaf_local;
-  const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
-  float3 nl;
-  if (face == 0) {
-    const float3 y = sub3(fma3(t, dl, xl), mk3(L0.x, L0.y, L0.z));
-    nl = mk3(L1.x * y.x + L1.y * y.y + L1.z * y.z + L2.z, L1.y * y.x + L1.w * y.y + L2.x * y.z + L2.w,
-             L1.z * y.x + L2.x * y.y + L2.y * y.z + L3.x);
-  } else {
-    const float4 P = g.plane[__float_as_int(L3.z) + face - 1];
-    nl = mk3(P.x, P.y, P.z);
-  }
-  float3 n = unit3(normal_to_world(g, v, nl));
-  if (dot3(n, d) > 0.0f) n = mul3(-1.0f, n);
-  return n;
-}
 
 // the next visible crossing from x along d in navigator id `vol`, within tmax; `last` the face the photon sits on
 __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
@@ -370,6 +384,7 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
       best_leaf = ch.leaf;
       best_face = ch.face;
       best_vol = vol;
+      best.n = normal_to_world(g, vol, ch.n);
     }
   }
   const int2 gr = g.vgrp[vol];
@@ -410,11 +425,12 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
         }
       }
       if (pcnt == 0) break;
-      for (int k = 0; k < pcnt; ++k) {
-        const int u = g.items[pst + k];
+      for (int k = 0; k < pcnt; ++k) {  // the item's transform and kind by item index: no wait on its volume id
+        const int it = pst + k;
+        const int u = g.items[it];
         float3 xl, dl;
-        to_local(g, u, x, d, xl, dl);
-        const Change ch = first_change(g, g.vkind[u], u, xl, dl, false, last, skip_t);
+        rows_to_local(g.ixf + 3 * it, x, d, xl, dl);
+        const Change ch = first_change(g, g.ikind[it], u, xl, dl, false, last, skip_t);
         if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
           best.t = ch.t;
           best.vol_after = u;
@@ -422,13 +438,15 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
           best_leaf = ch.leaf;
           best_face = ch.face;
           best_vol = u;
+          best.n = rows_normal_to_world(g.ixf + 3 * it, ch.n);
         }
       }
     }
   }
-  if (best_vol >= 0) {
+  if (best_vol >= 0) {  // the winner's normal, computed by its own test: unit, facing the photon
     best.srf = srf_code(best_vol, best_leaf, best_face);
-    best.n = crossing_normal(g, best_vol, best_leaf, best_face, x, d, best.t);
+    best.n = unit3(best.n);
+    if (dot3(best.n, d) > 0.0f) best.n = mul3(-1.0f, best.n);
   }
   return best;
 }
