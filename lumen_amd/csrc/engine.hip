@@ -44,11 +44,17 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int MAX_FACETS = 256;
 constexpr int MAX_FACET_ATTEMPTS = 4096;
-constexpr int BLOCK = 256;
+constexpr int BLOCK = 256;  // the adjoint's and the reference-value kernels' workgroup
+// The plain forward kernels run one workgroup of 16 wavefronts per CU (4 per SIMD at their 128-VGPR budget): the
+// BVH nodes and the tally copy are staged once per CU instead of once per 4 wavefronts, leaving room for both.
+#ifndef LH_FWD_BLOCK
+#define LH_FWD_BLOCK 1024
+#endif
+template <unsigned F>
+constexpr int block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
-constexpr long long LDS_TALLY_CELLS = 8192;  // tallies of at most this many cells (64 KiB) accumulate in LDS
-constexpr size_t LDS_BLOCK_MAX = 64 * 1024;
+constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
 constexpr int OCC_CACHE = 8;
 constexpr unsigned F_REF = 1, F_WLS = 2, F_TIME = 4;  // a kernel's features (template parameter)  // the most dynamic LDS one block of the persistent grid takes
 
@@ -712,7 +718,7 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 
 // dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
 template <unsigned F>
-__global__ void __launch_bounds__(BLOCK) LH_OCC_FWD(F) trace_kernel(Args a, int n_lds) {
+__global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a, int n_lds) {
   if (a.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
   stage_nodes(a.g);
@@ -854,7 +860,7 @@ struct lh_model_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int timed = 0;
   float *ut = nullptr;  // the unified facet tables on the device
-  int cus = 0, lds_per_cu = 0;
+  int cus = 0, lds_per_cu = 0, lds_per_block = 0;
   bool has_wls = false;  // a medium shifts wavelengths
   // the last launch's shape: grid blocks, blocks per CU, LDS bytes per block, tallies in LDS, nodes in LDS, the
   // LDS budget per block, the CU's LDS, the kernel's feature bits
@@ -1133,6 +1139,9 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   int lds_cu = 0;
   HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   m->lds_per_cu = lds_cu > 0 ? lds_cu : 64 * 1024;
+  int lds_blk = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  m->lds_per_block = lds_blk > 0 ? lds_blk : 64 * 1024;
   m->cus = cus;
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
@@ -1258,6 +1267,9 @@ static unsigned features_of(lh_model m, const Args &a, bool adj) {
 
 #define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
+// the workgroup size of a launch's kernel
+static int block_for(bool adj, unsigned f) { return adj || (f & F_REF) ? BLOCK : LH_FWD_BLOCK; }
+
 static const void *kernel_of(bool adj, unsigned f) {
   switch (f) {
 #define LH_KCASE(F) \
@@ -1273,7 +1285,7 @@ static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Ar
   if (adj)
     hipLaunchKernelGGL(adjoint_kernel<F & (F_REF | F_WLS)>, grid, dim3(BLOCK), bytes, st, a, b);
   else
-    hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(BLOCK), bytes, st, a, n_lds);
+    hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(block_of<F>()), bytes, st, a, n_lds);
 }
 
 static void launch_kernel(bool adj, unsigned f, dim3 grid, size_t bytes, hipStream_t st, const Args &a,
@@ -1286,13 +1298,13 @@ static void launch_kernel(bool adj, unsigned f, dim3 grid, size_t bytes, hipStre
   }
 }
 
-static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t lds_bytes) {
+static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t lds_bytes, int block) {
   // resident blocks per CU of (kernel, LDS bytes), remembered in a small cache
   int per = 0;
   for (int i = 0; i < OCC_CACHE; ++i)
     if (m->occ_fn[i] == fn && m->occ_lds[i] == lds_bytes) per = m->occ_per[i];
   if (per <= 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BLOCK, lds_bytes) != hipSuccess || per < 1) per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, lds_bytes) != hipSuccess || per < 1) per = 1;
     const int i = m->occ_next++ % OCC_CACHE;
     m->occ_fn[i] = fn;
     m->occ_lds[i] = lds_bytes;
@@ -1306,10 +1318,11 @@ static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t ld
 // optional, `cells_optional`), then the BVH nodes.  The budget is the CU's LDS over the blocks its registers let
 // reside, so LDS never lowers the occupancy; the nodes (a load on every walk step) come before optional cells.
 // Sets a.lds_tally and the nodes' placement; returns the LDS bytes.
-static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes, bool cells_optional,
+static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t cells_bytes, bool cells_optional,
                         const lh_controls *ctl) {
-  const int per_regs = std::max(1, grid_of(m, nullptr, fn, 0) / std::max(m->cus, 1));
-  const size_t budget = std::min<size_t>(LDS_BLOCK_MAX, (size_t)m->lds_per_cu / (size_t)per_regs) / 16 * 16;
+  const int per_regs = std::max(1, grid_of(m, nullptr, fn, 0, block) / std::max(m->cus, 1));
+  const size_t budget =
+      std::min<size_t>((size_t)m->lds_per_block, (size_t)m->lds_per_cu / (size_t)per_regs) / 16 * 16;
   const size_t node_bytes = (size_t)m->g.n_nodes * 32;
   const bool want_nodes = m->g.n_nodes > 0 && !(ctl && ctl->nodes_global == 1);
   const bool want_cells = cells_bytes > 0 && !(cells_optional && ctl && ctl->tally_global == 1);
@@ -1334,7 +1347,7 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, size_t cells_bytes,
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
                            hipStream_t st) {
   const unsigned f = features_of(m, a, true);
-  const int grid = grid_of(m, ctl, kernel_of(true, f), bytes);
+  const int grid = grid_of(m, ctl, kernel_of(true, f), bytes, block_for(true, f));
   m->last_launch[0] = grid;
   m->last_launch[1] = grid / std::max(m->cus, 1);
   m->last_launch[2] = (int32_t)bytes;
@@ -1348,10 +1361,11 @@ static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *c
   const unsigned f = features_of(m, a, false);
   const void *fn = kernel_of(false, f);
   const size_t bytes =
-      place_lds(m, a, fn, cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0, true, ctl);
+      place_lds(m, a, fn, block_for(false, f), cells <= LDS_TALLY_CELLS ? (size_t)cells * sizeof(double) : 0,
+                true, ctl);
   const int n_lds = a.lds_tally ? (int)cells : 0;
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const int grid = grid_of(m, ctl, fn, bytes);
+  const int grid = grid_of(m, ctl, fn, bytes, block_for(false, f));
   m->last_launch[0] = grid;
   m->last_launch[1] = grid / std::max(m->cus, 1);
   m->last_launch[2] = (int32_t)bytes;
@@ -1413,7 +1427,8 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev0, st));
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), BLOCK,
+                                 (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(m->ev1, st));
@@ -1521,7 +1536,8 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   a.queue = b->queue;
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), (size_t)n_grad * sizeof(double), false, ctl);
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), BLOCK,
+                                 (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, g, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
   return LH_OK;
