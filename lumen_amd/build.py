@@ -2,8 +2,9 @@
 
     python -m lumen_amd.build        # or lumen_amd.build.build()
 
-Every HIP source is compiled with packed FP32 instructions disabled (``-target-feature -packed-fp32-ops``) and
-hardware FP64 atomics; the library exports the C ABI of ``include/lumen_hip.h`` only.
+Every HIP source is compiled with packed FP32 instructions disabled (``-target-feature -packed-fp32-ops``),
+hardware FP64 atomics, and FP32 division / square root by the hardware's approximations with denormals flushed; the
+library exports the C ABI of ``include/lumen_hip.h`` only.
 """
 
 from __future__ import annotations
@@ -21,9 +22,11 @@ DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("nav.h", "optics.h", "
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LUMEN_ARCH", "gfx950")
 
+# FP32 division and square root by the hardware's reciprocal and root (<= 2.5 ulp, not correctly rounded) with
+# float32 denormals flushed: about 10 instructions fewer per division on a VALU-bound loop; FP64 is untouched
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-fvisibility=hidden",
-         "-Wno-unused-result"]
+         "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero", "-Wno-unused-result"]
 
 
 def build(verbose: bool = False, extra=()) -> str:
