@@ -330,7 +330,7 @@ __device__ inline float3 facet_path(const Photon &p, const lh_rng &rng, float si
       }
     }
     const float psi = (2.0f * u2 - 1.0f) * front_arc(V, a);
-    const float ph = atan2f(V.y, V.x) + psi;
+    const float ph = atan2_as(V.y, V.x) + psi;
     float sa, ca, sp, cp;
     __sincosf(a, &sa, &ca);
     __sincosf(ph, &sp, &cp);

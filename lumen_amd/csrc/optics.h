@@ -382,12 +382,47 @@ __device__ __forceinline__ int choose6(const float q[N_OUT], float u) {
 }
 
 // ------------------------------------------------------------------------------------------- directions
+// arccos by Abramowitz & Stegun 4.4.46 (Hastings' minimax form sqrt(1 - x) P7(x)): |error| <= 2.2e-8 on [-1, 1]
+__device__ __forceinline__ float acos_as(float x) {
+  const float ax = fminf(fabsf(x), 1.0f);
+  float p = -0.0012624911f;
+  p = fmaf(p, ax, 0.0066700901f);
+  p = fmaf(p, ax, -0.0170881256f);
+  p = fmaf(p, ax, 0.0308918810f);
+  p = fmaf(p, ax, -0.0501743046f);
+  p = fmaf(p, ax, 0.0889789874f);
+  p = fmaf(p, ax, -0.2145988016f);
+  p = fmaf(p, ax, 1.5707963050f);
+  const float r = sqrtf(1.0f - ax) * p;
+  return x >= 0.0f ? r : PI_F - r;
+}
+
+// arctan2 by Abramowitz & Stegun 4.4.49 (arctan z = z P8(z^2) on [0, 1], |error| <= 1.4e-8) on the octants
+__device__ __forceinline__ float atan2_as(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float z = mx > 0.0f ? mn / mx : 0.0f, z2 = z * z;
+  float p = 0.0028662257f;
+  p = fmaf(p, z2, -0.0161657367f);
+  p = fmaf(p, z2, 0.0429096138f);
+  p = fmaf(p, z2, -0.0752896400f);
+  p = fmaf(p, z2, 0.1065626393f);
+  p = fmaf(p, z2, -0.1420889944f);
+  p = fmaf(p, z2, 0.1999355085f);
+  p = fmaf(p, z2, -0.3333314528f);
+  p = fmaf(p, z2, 1.0f);
+  float r = z * p;
+  if (ay > ax) r = HALF_PI - r;
+  if (x < 0.0f) r = PI_F - r;
+  return copysignf(r, y);
+}
+
 // Rayleigh, polarised: the dipole axis er with probability |er|^2, else ei; c from p(c) = 3/4 (1 - c^2)
 // (phase.py `rayleigh_sample`, `dipole_cos`)
 __device__ inline float3 rayleigh_sample(float3 er, float3 ei, float u0, float u1, float u2) {
   const float a2 = dot3(er, er), norm2 = a2 + dot3(ei, ei);
   const float3 axis = unit3(u2 * norm2 < a2 ? er : ei);
-  const float c = fminf(fmaxf(2.0f * cosf((acosf(fminf(fmaxf(1.0f - 2.0f * u0, -1.0f), 1.0f)) - TWO_PI) / 3.0f),
+  const float c = fminf(fmaxf(2.0f * __cosf((acos_as(fminf(fmaxf(1.0f - 2.0f * u0, -1.0f), 1.0f)) - TWO_PI) / 3.0f),
                               -1.0f), 1.0f);
   const float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
   float sp, cp;
@@ -419,7 +454,7 @@ __device__ __forceinline__ float front_arc(float3 V, float a) {
   float sa, ca;
   __sincosf(a, &sa, &ca);
   const float ratio = V.z * ca / fmaxf(sv * sa, FLOOR32);
-  return acosf(-fminf(ratio, 1.0f));
+  return acos_as(-fminf(ratio, 1.0f));
 }
 
 }  // namespace lh

@@ -74,13 +74,13 @@ __device__ inline void ut_lz(const float *__restrict__ T, float y, float &lz, fl
 // dlp: its derivative in sigma
 __device__ inline float unified_lp(const float *__restrict__ T, float sigma, float3 V, float3 m, float &dlp) {
   const float s = sqrtf(m.x * m.x + m.y * m.y), c = m.z;
-  const float al = atan2f(s, c);
+  const float al = atan2_as(s, c);
   const float fmax = fminf(1.0f, 4.0f * sigma);
   const float y = __logf(sigma);
   float lz, dlz, F, Fx, Fy;
   ut_lz(T, y, lz, dlz);
   const float vs = sqrtf(V.x * V.x + V.y * V.y);
-  const float x = atan2f(V.z, sigma * vs) / HALF_PI;
+  const float x = atan2_as(V.z, sigma * vs) / HALF_PI;
   ut_front(T, x, y, F, Fx, Fy);
   F = fmaxf(F, FLOOR32);
   const float lf = fminf(0.0f, __logf(4.0f * sigma)), ls = __logf(fmaxf(s, fmax));
