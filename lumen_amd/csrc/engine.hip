@@ -56,7 +56,8 @@ constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
 constexpr int OCC_CACHE = 8;
-constexpr unsigned F_REF = 1, F_WLS = 2, F_TIME = 4;  // a kernel's features (template parameter)  // the most dynamic LDS one block of the persistent grid takes
+constexpr unsigned F_REF = 1, F_WLS = 2, F_TIME = 4, F_GNODES = 8;  // a kernel's features (template parameter)
+constexpr unsigned F_ADJ_MASK = F_REF | F_WLS | F_GNODES;  // the features an adjoint kernel is specialised on
 
 // The forward kernel's register budget: 4 wavefronts per SIMD (128 VGPRs) hide the walk's load latency better
 // than the 2-3 its natural allocation allows, spills included (measured on the 1,000-PMT tank).  The adjoint and
@@ -418,7 +419,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const float mu = ml.mu_ray + (WLS ? ml.mu_wls : 0.0f), mu_r = mr.mu_ray + (WLS ? mr.mu_wls : 0.0f);  // totals
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
-  const Cross cr = query(g, p.x, p.d, p.vol, p.srf, max_dist);
+  const Cross cr = query_t<!(F & F_GNODES)>(g, p.x, p.d, p.vol, p.srf, max_dist);
   const bool hit = cr.vol_after >= 0;
   const float dist = cr.t;
   const bool zero = dist <= ZERO_STEP_BANDS * b0;
@@ -1258,14 +1259,14 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
 // F_WLS: a medium shifts wavelengths), so a run without them carries none of their registers
 template <unsigned F>
 static const void *kernel_f(bool adj) {
-  return adj ? (const void *)adjoint_kernel<F & (F_REF | F_WLS)> : (const void *)trace_kernel<F>;
+  return adj ? (const void *)adjoint_kernel<F & F_ADJ_MASK> : (const void *)trace_kernel<F>;
 }
 
 static unsigned features_of(lh_model m, const Args &a, bool adj) {
   return (a.has_ref ? F_REF : 0u) | (m->has_wls ? F_WLS : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u);
 }
 
-#define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
 
 // the workgroup size of a launch's kernel
 static int block_for(bool adj, unsigned f) { return adj || (f & F_REF) ? BLOCK : LH_FWD_BLOCK; }
@@ -1283,7 +1284,7 @@ static const void *kernel_of(bool adj, unsigned f) {
 template <unsigned F>
 static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Args &a, const AdjArgs &b, int n_lds) {
   if (adj)
-    hipLaunchKernelGGL(adjoint_kernel<F & (F_REF | F_WLS)>, grid, dim3(BLOCK), bytes, st, a, b);
+    hipLaunchKernelGGL(adjoint_kernel<F & F_ADJ_MASK>, grid, dim3(BLOCK), bytes, st, a, b);
   else
     hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(block_of<F>()), bytes, st, a, n_lds);
 }
@@ -1346,7 +1347,7 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
 
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
                            hipStream_t st) {
-  const unsigned f = features_of(m, a, true);
+  const unsigned f = features_of(m, a, true) | (a.g.lds_nodes ? 0u : F_GNODES);
   const int grid = grid_of(m, ctl, kernel_of(true, f), bytes, block_for(true, f));
   m->last_launch[0] = grid;
   m->last_launch[1] = grid / std::max(m->cus, 1);
@@ -1365,12 +1366,13 @@ static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *c
                 true, ctl);
   const int n_lds = a.lds_tally ? (int)cells : 0;
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const int grid = grid_of(m, ctl, fn, bytes, block_for(false, f));
+  const unsigned fk = f | (a.g.lds_nodes ? 0u : F_GNODES);  // the walk reads its nodes from LDS or from memory
+  const int grid = grid_of(m, ctl, kernel_of(false, fk), bytes, block_for(false, fk));
   m->last_launch[0] = grid;
   m->last_launch[1] = grid / std::max(m->cus, 1);
   m->last_launch[2] = (int32_t)bytes;
-  m->last_launch[7] = (int32_t)f;
-  launch_kernel(false, f, dim3(grid), bytes, st, a, AdjArgs{}, n_lds);
+  m->last_launch[7] = (int32_t)fk;
+  launch_kernel(false, fk, dim3(grid), bytes, st, a, AdjArgs{}, n_lds);
 }
 
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,

@@ -52,9 +52,11 @@ struct Geo {
   int lds_node_off;
 };
 
-// a BVH node's float4 i (2 per node) from LDS when the kernel staged them, else from global memory
+// a BVH node's float4 i (2 per node): from LDS where the kernel staged them (LDS: a compile-time choice, so each
+// walk compiles to LDS reads or to global reads, never to a generic load that decides per access)
+template <bool LDS>
 __device__ __forceinline__ float4 nav_node(const Geo &g, int i) {
-  if (g.lds_nodes) return reinterpret_cast<const float4 *>(lds_cells)[g.lds_node_off + i];
+  if (LDS) return reinterpret_cast<const float4 *>(lds_cells)[g.lds_node_off + i];
   return g.node[i];
 }
 
@@ -369,7 +371,8 @@ struct Cross {
 };
 
 // the next visible crossing from x along d in navigator id `vol`, within tmax; `last` the face the photon sits on
-__device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
+template <bool LDS>
+__device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
   const float skip_t = K_ON * band_at(maxabs3(x));
   Cross best{tmax, -1, -1, 0, mk3(0.f, 0.f, 1.f)};
   int best_leaf = -1, best_face = -1, best_vol = -1;
@@ -405,7 +408,7 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
     for (;;) {
       int pst = 0, pcnt = 0;
       while (ni < nr.y) {
-        const float4 lo = nav_node(g, 2 * ni), hi = nav_node(g, 2 * ni + 1);
+        const float4 lo = nav_node<LDS>(g, 2 * ni), hi = nav_node<LDS>(g, 2 * ni + 1);
         const float tx0 = (lo.x - xg.x) * inv.x, tx1 = (hi.x - xg.x) * inv.x;
         const float ty0 = (lo.y - xg.y) * inv.y, ty1 = (hi.y - xg.y) * inv.y;
         const float tz0 = (lo.z - xg.z) * inv.z, tz1 = (hi.z - xg.z) * inv.z;
@@ -451,6 +454,10 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
   return best;
 }
 
+__device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
+  return query_t<false>(g, x, d, vol, last, tmax);
+}
+
 // the deepest navigator id holding x (the outside when no volume does)
 __device__ inline int locate(const Geo &g, float3 x) {
   int cur = g.nvol;
@@ -465,7 +472,7 @@ __device__ inline int locate(const Geo &g, float3 x) {
       const int2 nr = g.grange[gi];
       int ni = nr.x;
       while (ni < nr.y && found < 0) {
-        const float4 lo = nav_node(g, 2 * ni), hi = nav_node(g, 2 * ni + 1);
+        const float4 lo = g.node[2 * ni], hi = g.node[2 * ni + 1];
         const int skip = __float_as_int(lo.w);
         if (xg.x < lo.x || xg.x > hi.x || xg.y < lo.y || xg.y > hi.y || xg.z < lo.z || xg.z > hi.z) {
           ni = skip;

@@ -67,11 +67,13 @@ def test_graph_forward_adjoint_matches_eager(gpu, det):
 
 
 def test_tally_in_lds_equals_global(gpu, det):
-    """The block-local LDS tallies (flushed once per block) and direct global atomics sum the same deposits."""
+    """The block-local LDS tallies (flushed once per block) and direct global atomics sum the same deposits; the walk
+    over LDS-staged BVH nodes and over nodes in memory (another kernel) finds the same crossings."""
     src = L.sources.Point((0.3, -0.2, 0.4), lam=420.0, n_photons=1e5)
     a = L.simulate(det, src, samples=1 << 18, seed=21)
-    b = L.simulate(det, src, samples=1 << 18, seed=21, controls=Controls(tally_global=True))
-    for k in ("arrivals", "arrival2", "charge", "mean_arrival_time"):
-        np.testing.assert_allclose(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy(), rtol=1e-9,
-                                   atol=1e-12, err_msg=k)
-    np.testing.assert_allclose(np.asarray(a.health), np.asarray(b.health), rtol=1e-9)
+    for ctl in (Controls(tally_global=True), Controls(nodes_global=True), Controls(tally_global=True, nodes_global=True)):
+        b = L.simulate(det, src, samples=1 << 18, seed=21, controls=ctl)
+        for k in ("arrivals", "arrival2", "charge", "mean_arrival_time"):
+            np.testing.assert_allclose(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy(), rtol=1e-9,
+                                       atol=1e-12, err_msg=f"{k} {ctl}")
+        np.testing.assert_allclose(np.asarray(a.health), np.asarray(b.health), rtol=1e-9)
