@@ -402,30 +402,26 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
                            1.0f / (fabsf(dg.y) > 1e-30f ? dg.y : copysignf(1e-30f, dg.y)),
                            1.0f / (fabsf(dg.z) > 1e-30f ? dg.z : copysignf(1e-30f, dg.z)));
     const int2 nr = g.grange[gi];
+    const float3 xi = mk3(xg.x * inv.x, xg.y * inv.y, xg.z * inv.z);  // the slabs as one fma each
     int ni = nr.x;
     // the walk in two loops (traverse to the next leaf, then test its items): a wavefront runs the costly item
-    // tests once for all lanes that found a leaf, instead of on every node step where any one lane sits at a leaf
+    // tests once for all lanes that found a leaf, instead of on every node step where any one lane sits at a leaf.
+    // A node step is branch-free: a missed box or a leaf moves on to the skip link, an inner box to its first child.
     for (;;) {
       int pst = 0, pcnt = 0;
-      while (ni < nr.y) {
+      while (ni < nr.y && pcnt == 0) {
         const float4 lo = nav_node<LDS>(g, 2 * ni), hi = nav_node<LDS>(g, 2 * ni + 1);
-        const float tx0 = (lo.x - xg.x) * inv.x, tx1 = (hi.x - xg.x) * inv.x;
-        const float ty0 = (lo.y - xg.y) * inv.y, ty1 = (hi.y - xg.y) * inv.y;
-        const float tz0 = (lo.z - xg.z) * inv.z, tz1 = (hi.z - xg.z) * inv.z;
+        const float tx0 = fmaf(lo.x, inv.x, -xi.x), tx1 = fmaf(hi.x, inv.x, -xi.x);
+        const float ty0 = fmaf(lo.y, inv.y, -xi.y), ty1 = fmaf(hi.y, inv.y, -xi.y);
+        const float tz0 = fmaf(lo.z, inv.z, -xi.z), tz1 = fmaf(hi.z, inv.z, -xi.z);
         const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
-        const int skip = __float_as_int(lo.w);
         const int sc = __float_as_int(hi.w);
-        if (te > tl) {
-          ni = skip;
-        } else if ((sc & 7) > 0) {
-          pst = sc >> 3;
-          pcnt = sc & 7;
-          ni = skip;
-          break;
-        } else {
-          ni += 1;
-        }
+        const bool in = !(te > tl);
+        const int cnt = in ? (sc & 7) : 0;
+        pst = sc >> 3;
+        pcnt = cnt;
+        ni = (in && cnt == 0) ? ni + 1 : __float_as_int(lo.w);
       }
       if (pcnt == 0) break;
       for (int k = 0; k < pcnt; ++k) {  // the item's transform and kind by item index: no wait on its volume id
