@@ -16,21 +16,15 @@ struct lh_u4 {
   uint32_t x, y, z, w;
 };
 
-LH_HD uint32_t lh_mulhi(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __umulhi(a, b);
-#else
-  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
-#endif
-}
-
 // ten rounds; round constants and key bumps as published (M0 = 0xD2511F53, M1 = 0xCD9E8D57, W0 = 0x9E3779B9,
 // W1 = 0xBB67AE85)
 LH_HD lh_u4 lh_philox(lh_u4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = lh_mulhi(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = lh_mulhi(0xCD9E8D57u, c.z);
+    // each product's low and high words from one 32 x 32 -> 64 multiply (one v_mad_u64_u32 on CDNA)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     c = lh_u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;

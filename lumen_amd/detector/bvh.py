@@ -10,7 +10,9 @@ from __future__ import annotations
 
 import numpy as np
 
-LEAF_SIZE = 2
+# one box per leaf: for rays from inside a container to daughters on its walls it visits as many nodes as two
+# per leaf and tests half the daughters (the costly part of a walk)
+LEAF_SIZE = 1
 BINS = 16
 
 
