@@ -71,8 +71,14 @@ def test_tally_in_lds_equals_global(gpu, det):
     over LDS-staged BVH nodes and over nodes in memory (another kernel) finds the same crossings."""
     src = L.sources.Point((0.3, -0.2, 0.4), lam=420.0, n_photons=1e5)
     a = L.simulate(det, src, samples=1 << 18, seed=21)
+    info = engine(det).last_launch()
+    assert info["nodes_in_lds"] == 1 and info["tally_in_lds"] == 1 and info["features"] == 0, info
     for ctl in (Controls(tally_global=True), Controls(nodes_global=True), Controls(tally_global=True, nodes_global=True)):
         b = L.simulate(det, src, samples=1 << 18, seed=21, controls=ctl)
+        info = engine(det).last_launch()
+        assert info["nodes_in_lds"] == (0 if ctl.nodes_global else 1), info
+        assert info["tally_in_lds"] == (0 if ctl.tally_global else 1), info
+        assert bool(info["features"] & 8) == bool(ctl.nodes_global), info  # the walk over nodes in memory
         for k in ("arrivals", "arrival2", "charge", "mean_arrival_time"):
             np.testing.assert_allclose(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy(), rtol=1e-9,
                                        atol=1e-12, err_msg=f"{k} {ctl}")
