@@ -56,8 +56,10 @@ constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
 constexpr int OCC_CACHE = 8;
-constexpr unsigned F_REF = 1, F_WLS = 2, F_TIME = 4, F_GNODES = 8;  // a kernel's features (template parameter)
-constexpr unsigned F_ADJ_MASK = F_REF | F_WLS | F_GNODES;  // the features an adjoint kernel is specialised on
+// a kernel's features (template parameter): separate reference values, bulk channels beyond Rayleigh (Mie,
+// WLS), the time histogram, BVH nodes read from global memory
+constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8;
+constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES;  // the features an adjoint kernel is specialised on
 
 // The forward kernel's register budget: 4 wavefronts per SIMD (128 VGPRs) hide the walk's load latency better
 // than the 2-3 its natural allocation allows, spills included (measured on the 1,000-PMT tank).  The adjoint and
@@ -110,13 +112,22 @@ struct Photon {
   int pass;
   float T, Dsum;
   bool hasdep;
-  RateD da, dr, dw;
+  RateD da, dr, dm, dw;
 };
 
-struct Acc {  // per-lane health accumulators, flushed once per launch
-  float bulk, surf, esc, rou, rew, lost, dcut, aband, trunc, pushed, start_w;
-  int records, spills, vertices, photons;
+struct Acc {  // per-lane health accumulators of the frequent fates, flushed once per launch
+  float bulk, surf, rou, rew, start_w;
+  int records, vertices, photons;
 };
+
+// the rare fates (a photon's end, a stuck push, a moment spill) go to the block's LDS cells by atomics: registers
+// held all launch long for them would crowd the step's working set
+enum { RF_LOST = 0, RF_ESC, RF_DCUT, RF_ABAND, RF_TRUNC, RF_PUSHED, RF_SPILLS, RF_N };
+__shared__ float s_rare[RF_N];
+
+__device__ __forceinline__ void rare_add(int cell, float v) {
+  if (v != 0.0f) atomicAdd(&s_rare[cell], v);
+}
 
 // a[o] by selects (a register array indexed at run time would live in scratch)
 __device__ __forceinline__ float pick6(const float a[6], int o) {
@@ -149,7 +160,7 @@ __device__ inline void moment_flush_slot(const Args &a, Photon &p, int k) {
   }
 }
 
-__device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, float v) {
+__device__ inline void moment_add(const Args &a, Photon &p, int s, float v) {
   const i4v hit = p.ms == s;  // lanes of -1 where the slot holds sensor s
   if (hit.x | hit.y | hit.z | hit.w) {
     p.mv += hit ? v : 0.0f;
@@ -162,7 +173,7 @@ __device__ inline void moment_add(const Args &a, Photon &p, Acc &acc, int s, flo
     p.ms = i4v{p.ms.y, p.ms.z, p.ms.w, -1};
     p.mv = f4v{p.mv.y, p.mv.z, p.mv.w, 0.0f};
     free_k = MOMENT_SLOTS - 1;
-    acc.spills += 1;
+    rare_add(RF_SPILLS, 1.0f);
   }
   p.ms[free_k] = s;
   p.mv[free_k] = v;
@@ -220,7 +231,7 @@ __device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float 
       tally_add(a, p.event, ob + 3, dq * (double)early * (double)time);
     }
   }
-  moment_add(a, p, acc, s, v);
+  moment_add(a, p, s, v);
 }
 
 __device__ inline void photon_end(const Args &a, Photon &p) {
@@ -301,7 +312,7 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   acc.photons += 1;
   acc.start_w += p.w0;
   if (p.vol >= a.g.nvol || class_medium(a, p.cls) < 0) {  // emitted outside the detector or in an opaque medium
-    acc.lost += p.w0;
+    rare_add(RF_LOST, p.w0);
     return false;
   }
   return true;
@@ -354,7 +365,7 @@ __device__ inline float3 facet_path(const Photon &p, const lh_rng &rng, float si
 
 template <bool ADJ, unsigned F>
 __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) {
-  constexpr bool REF = F & F_REF, WLS = F & F_WLS;
+  constexpr bool REF = F & F_REF, CHAN = F & F_CHAN;
   if (p.mcls == p.cls) return;
   const int m = class_medium(a, p.cls);
   p.ml = medium_at(a.L, a.P, m, p.lam);
@@ -363,6 +374,7 @@ __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) 
   if (ADJ) {
     p.da = RateD{-1, -1, 0.0f, 0.0f};
     p.dr = RateD{-1, -1, 0.0f, 0.0f};
+    p.dm = RateD{-1, -1, 0.0f, 0.0f};
     p.dw = RateD{-1, -1, 0.0f, 0.0f};
     if (m >= 0) {
       const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * m;
@@ -370,7 +382,8 @@ __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) 
         const float E = HC / p.lam;
         if (M[5] > 0) p.da = rate_d(a.P, b->slots, M[4], M[5], E);
         if (M[7] > 0) p.dr = rate_d(a.P, b->slots, M[6], M[7], E);
-        if (WLS && (M[8] & MED_WLS)) p.dw = rate_d(a.P, b->slots, M[9], M[10], E);
+        if (CHAN && (M[8] & MED_MIE)) p.dm = rate_d(a.P, b->slots, M[17], M[18], E);
+        if (CHAN && (M[8] & MED_WLS)) p.dw = rate_d(a.P, b->slots, M[9], M[10], E);
       }
     }
   }
@@ -392,16 +405,33 @@ __device__ inline float deposit_cotangent(const Args &a, const AdjArgs *b, const
 
 __device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.0f ? live / ref : 0.0f; }
 
+// the derivative terms of the rates over one segment: k_c d(mu_c)/d(value) v per rate (absorption, Rayleigh, Mie,
+// WLS), through the photon's cached table derivatives
+template <bool CHAN>
+__device__ __forceinline__ void rates_add(double *sg, const Photon &p, float ka, float kr, float km, float kw,
+                                          float v) {
+  grad_add(sg, p.da.s0, ka * p.da.d0 * v);
+  grad_add(sg, p.da.s1, ka * p.da.d1 * v);
+  grad_add(sg, p.dr.s0, kr * p.dr.d0 * v);
+  grad_add(sg, p.dr.s1, kr * p.dr.d1 * v);
+  if (CHAN) {
+    grad_add(sg, p.dm.s0, km * p.dm.d0 * v);
+    grad_add(sg, p.dm.s1, km * p.dm.d1 * v);
+    grad_add(sg, p.dw.s0, kw * p.dw.d0 * v);
+    grad_add(sg, p.dw.s1, kw * p.dw.d1 * v);
+  }
+}
+
 // one step of one photon (transport/step.py:516-571); returns whether it goes on.  ADJ: the adjoint's walk
 // (no tallies; its pass 0 totals the photon's cotangent-weighted deposits, its pass 1 adds the derivative terms
 // into the block's accumulators sg)
 template <bool ADJ, unsigned F>
 __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b, double *sg) {
-  constexpr bool REF = F & F_REF, WLS = F & F_WLS;
+  constexpr bool REF = F & F_REF, CHAN = F & F_CHAN;
   const Geo &g = a.g;
   float b0 = band_at(maxabs3(p.x));
   if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {  // G4Navigator's stuck-track push (step.py:269-282 `_recover`)
-    if (p.nz == ZERO_STEPS_BEFORE_PUSH) acc.pushed += p.w * p.w0;
+    if (p.nz == ZERO_STEPS_BEFORE_PUSH) rare_add(RF_PUSHED, p.w * p.w0);
     p.x = fma3(PUSH_BANDS * b0, p.d, p.x);
     p.vol = locate(g, p.x);
     p.cls = g.vinfo[p.vol].z;
@@ -416,7 +446,9 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   pol_fill(p.d, er0, ei0, ul[3]);
   medium_cache<ADJ, F>(a, p, b);
   const MediumAt ml = p.ml, mr = REF ? p.mr : p.ml;
-  const float mu = ml.mu_ray + (WLS ? ml.mu_wls : 0.0f), mu_r = mr.mu_ray + (WLS ? mr.mu_wls : 0.0f);  // totals
+  // the total rates, summed in Geant4's channel order (Rayleigh, Mie, WLS)
+  const float mu = CHAN ? (ml.mu_ray + ml.mu_mie) + ml.mu_wls : ml.mu_ray;
+  const float mu_r = CHAN ? (mr.mu_ray + mr.mu_mie) + mr.mu_wls : mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
   const Cross cr = query_t<!(F & F_GNODES)>(g, p.x, p.d, p.vol, p.srf, max_dist);
@@ -485,12 +517,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
         if (D != 0.0f) {
           p.hasdep = true;
           if (p.pass == 1) {  // the deposit's own factors: reach, attenuation, the detected share
-            grad_add(sg, p.da.s0, -dist * p.da.d0 * D);
-            grad_add(sg, p.da.s1, -dist * p.da.d1 * D);
-            grad_add(sg, p.dr.s0, -dist * p.dr.d0 * D);
-            grad_add(sg, p.dr.s1, -dist * p.dr.d1 * D);
-            grad_add(sg, p.dw.s0, -dist * p.dw.d0 * D);
-            grad_add(sg, p.dw.s1, -dist * p.dw.d1 * D);
+            rates_add<CHAN>(sg, p, -dist, -dist, -dist, -dist, D);
             float dsh[N_COL], ddet[N_COL];
             const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
             surface_dlogs(Rw, a.P, HC / p.lam, -1, dsh, ddet);
@@ -507,31 +534,54 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   bool ok;
   float w_new, wr_new;
   if (scatter) {  // a bulk event at the free path's end (step.py:397-436, `_bulk`)
-    // the channel drawn from the reference rates in Geant4's order (Rayleigh, WLS), weighted by the live /
-    // reference density of the free path in it, mu_c exp(-mu s), times the absorption over the path
-    const bool wls = WLS && !(u[1] * mu_r < mr.mu_ray);
-    const float mc = wls ? ml.mu_wls : ml.mu_ray, mc_r = wls ? mr.mu_wls : mr.mu_ray;
+    // the channel drawn from the reference rates in Geant4's order (Rayleigh, Mie, WLS; estimator.py `choose`:
+    // the first whose running sum exceeds u times the total, Rayleigh if rounding leaves none), weighted by the
+    // live / reference density of the free path in it, mu_c exp(-mu s), times the absorption over the path
+    int ch = 0;
+    if (CHAN) {
+      const float xc = u[1] * mu_r, c1 = mr.mu_ray + mr.mu_mie;
+      ch = xc < mr.mu_ray ? 0 : xc < c1 ? 1 : xc < mu_r ? 2 : 0;
+    }
+    const float mc = ch == 0 ? ml.mu_ray : ch == 1 ? ml.mu_mie : ml.mu_wls;
+    const float mc_r = ch == 0 ? mr.mu_ray : ch == 1 ? mr.mu_mie : mr.mu_wls;
     const float w_path = ratio(mc * __expf(-mu * s), mc_r * __expf(-mu_r * s));
     const float att_s = __expf(-ml.mu_abs * s);
     const float w_at = p.w * w_path;
     const float att_s_r = REF ? __expf(-mr.mu_abs * s) : att_s;
     if (adj_event) {  // the free path's density mu_c exp(-mu s) and the absorption exp(-mu_abs s)
       const float kc = mc > 0.0f ? 1.0f / mc : 0.0f;
-      const float kr = (wls ? 0.0f : kc) - s, kw = (wls ? kc : 0.0f) - s;
-      grad_add(sg, p.dr.s0, kr * p.dr.d0 * ahead);
-      grad_add(sg, p.dr.s1, kr * p.dr.d1 * ahead);
-      grad_add(sg, p.dw.s0, kw * p.dw.d0 * ahead);
-      grad_add(sg, p.dw.s1, kw * p.dw.d1 * ahead);
-      grad_add(sg, p.da.s0, -s * p.da.d0 * ahead);
-      grad_add(sg, p.da.s1, -s * p.da.d1 * ahead);
+      rates_add<CHAN>(sg, p, -s, (ch == 0 ? kc : 0.0f) - s, (ch == 1 ? kc : 0.0f) - s, (ch == 2 ? kc : 0.0f) - s,
+                      ahead);
     }
     p.x = fma3(s, p.d, p.x);
     p.t += s / v_g;
-    if (!wls) {  // Rayleigh: the polarised dipole lobe (phase.py:45-75)
-      w_new = w_at * att_s;
+    if (ch != 2) {  // a scattering: Rayleigh's polarised dipole lobe (phase.py:45-75) or Mie's two HG lobes
+      float3 dn;
+      float w_lobe = 1.0f;
+      if (ch == 0) {
+        dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
+      } else {  // G4OpMieHG, drawn at the reference's lobes under the cover of its forward ratio, weighted by the
+                // live mixture density over the drawn one (transport/medium.py:96-133 `channel_draw`)
+        const int *M = a.L + a.L[L_MEDIA_AT] + MEDIUM_WORDS * class_medium(a, p.cls);
+        const MieLobe lv = mie_lobe(a.P, M), lr = REF ? mie_lobe(a.Pr, M) : lv;
+        const float rc = mie_cover(lr.r);
+        dn = mie_sample(p.d, lr.gf, lr.gb, rc, ul[0], ul[1], ul[2]);
+        const float3 dmi = sub3(dn, p.d), dpl = add3(dn, p.d);
+        const float cm = 0.5f * dot3(dmi, dmi), cp = 0.5f * dot3(dpl, dpl);
+        const float f_l = hg_density(lv.gf, cm, cp), b_l = hg_density(-lv.gb, cm, cp);
+        const float live = lv.r * f_l + (1.0f - lv.r) * b_l;
+        const float drawn = (REF || rc != lv.r) ? mie_density(lr.gf, lr.gb, rc, cm, cp) : live;
+        w_lobe = ratio(live, drawn);
+        if (adj_event && live > 0.0f) {  // the live mixture's log density in (g_f, g_b, r)
+          const float k = ahead / live;
+          grad_add(sg, b->slots[M[19] + M[20]], lv.r * f_l * hg_dlog_g(lv.gf, cm, cp) * k);
+          grad_add(sg, b->slots[M[21] + M[22]], -(1.0f - lv.r) * b_l * hg_dlog_g(-lv.gb, cm, cp) * k);
+          grad_add(sg, b->slots[M[23] + M[24]], (f_l - b_l) * k);
+        }
+      }
+      w_new = w_at * att_s * w_lobe;
       wr_new = p.wr * att_s_r;
       acc.bulk += w_at * (1.0f - att_s) * p.w0;
-      const float3 dn = rayleigh_sample(er0, ei0, ul[0], ul[1], ul[2]);
       pol_project(er0, ei0, dn);
       p.d = unit3(dn);
       p.er = er0;
@@ -565,14 +615,9 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     acc.bulk += w_at * (1.0f - att) * p.w0;
     if (at_if) acc.surf += w_at * att * (1.0f - s_cont) * p.w0;
     ok = at_if && o >= 0;
-    if (!at_if) acc.lost += w_at * att * p.w0;
+    if (!at_if) rare_add(RF_LOST, w_at * att * p.w0);
     if (ok && adj_event) {  // reach, attenuation over the segment, and the drawn outcome's share
-      grad_add(sg, p.da.s0, -dist * p.da.d0 * ahead);
-      grad_add(sg, p.da.s1, -dist * p.da.d1 * ahead);
-      grad_add(sg, p.dr.s0, -dist * p.dr.d0 * ahead);
-      grad_add(sg, p.dr.s1, -dist * p.dr.d1 * ahead);
-      grad_add(sg, p.dw.s0, -dist * p.dw.d0 * ahead);
-      grad_add(sg, p.dw.s1, -dist * p.dw.d1 * ahead);
+      rates_add<CHAN>(sg, p, -dist, -dist, -dist, -dist, ahead);
       float dsh[N_COL], ddet[N_COL];
       const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
       if (!cont) {
@@ -636,7 +681,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     const float d_end = fminf(dist, max_dist);
     const float w_far = p.w * ratio(__expf(-mu * d_end), __expf(-mu_r * d_end));
     const float att_far = __expf(-ml.mu_abs * d_end);
-    if (max_dist < a.reach) acc.dcut += w_far * att_far * p.w0; else acc.esc += w_far * att_far * p.w0;
+    rare_add(max_dist < a.reach ? RF_DCUT : RF_ESC, w_far * att_far * p.w0);
     acc.bulk += w_far * (1.0f - att_far) * p.w0;
     ok = false;
     w_new = wr_new = 0.0f;
@@ -667,8 +712,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const bool abandon = alive && p.nz >= ZERO_STEPS_BEFORE_KILL;
   const bool truncate = alive && !abandon && p.vertex >= a.max_vert;
   const float wl = p.w * p.w0;
-  acc.aband += abandon ? wl : 0.0f;
-  acc.trunc += truncate ? wl : 0.0f;
+  if (abandon || truncate) rare_add(abandon ? RF_ABAND : RF_TRUNC, wl);
   return alive && !abandon && !truncate;
 }
 
@@ -677,15 +721,26 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// every thread of the block calls it (a barrier inside)
 __device__ inline void flush_health(const Args &a, const Acc &acc) {
-  const float vals[LH_HEALTH_N] = {acc.bulk, acc.surf, acc.lost, acc.esc, acc.dcut, acc.aband, acc.trunc, acc.rou,
-                                   acc.rew, acc.pushed, (float)acc.records, (float)acc.spills,
-                                   (float)acc.vertices, (float)acc.photons, acc.start_w, 0.0f};
+  const float vals[LH_HEALTH_N] = {acc.bulk, acc.surf, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, acc.rou, acc.rew, 0.0f,
+                                   (float)acc.records, 0.0f, (float)acc.vertices, (float)acc.photons, acc.start_w,
+                                   0.0f};
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < LH_HEALTH_N; ++k) {
+    if (k == LH_H_LOST || k == LH_H_ESCAPED || k == LH_H_DISTANCE_CUT || k == LH_H_ABANDONED ||
+        k == LH_H_TRUNCATED || k == LH_H_PUSHED || k == LH_H_MOMENT_SPILLS || k == 15)
+      continue;
     const float s = wave_sum(vals[k]);
     if (lane == 0 && s != 0.0f) unsafeAtomicAdd(a.health + k, (double)s);
+  }
+  __syncthreads();
+  if (threadIdx.x < RF_N) {
+    const int cell[RF_N] = {LH_H_LOST, LH_H_ESCAPED, LH_H_DISTANCE_CUT, LH_H_ABANDONED, LH_H_TRUNCATED, LH_H_PUSHED,
+                            LH_H_MOMENT_SPILLS};
+    const float v = s_rare[threadIdx.x];
+    if (v != 0.0f) unsafeAtomicAdd(a.health + cell[threadIdx.x], (double)v);
   }
 }
 
@@ -722,6 +777,7 @@ template <unsigned F>
 __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a, int n_lds) {
   if (a.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
+  if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   stage_nodes(a.g);
   __syncthreads();
   Photon p;
@@ -774,6 +830,7 @@ template <unsigned F>
 __global__ void __launch_bounds__(BLOCK) adjoint_kernel(Args a, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
+  if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   stage_nodes(a.g);
   __syncthreads();
   Photon p;
@@ -862,7 +919,7 @@ struct lh_model_s {
   int timed = 0;
   float *ut = nullptr;  // the unified facet tables on the device
   int cus = 0, lds_per_cu = 0, lds_per_block = 0;
-  bool has_wls = false;  // a medium shifts wavelengths
+  bool has_chan = false;  // a medium has a Mie or WLS channel
   // the last launch's shape: grid blocks, blocks per CU, LDS bytes per block, tallies in LDS, nodes in LDS, the
   // LDS budget per block, the CU's LDS, the kernel's feature bits
   int32_t last_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -970,13 +1027,20 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
       const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
       if (!(M[8] & MED_OPAQUE) && (M[1] < 1 || M[3] < 1))
         return fail(LH_ERR_LAYOUT, "lh_model_create: a medium without RINDEX");
-      if (M[8] & ~(MED_OPAQUE | MED_WLS | MED_WLS_EXP)) return fail(LH_ERR_LAYOUT, "lh_model_create: unknown medium flags");
+      if (M[8] & ~(MED_OPAQUE | MED_WLS | MED_WLS_EXP | MED_MIE))
+        return fail(LH_ERR_LAYOUT, "lh_model_create: unknown medium flags");
       for (int j = 0; j < 8; j += 2)
         if (M[j] < 0 || M[j + 1] < 0) return fail(LH_ERR_LAYOUT, "lh_model_create: a negative table offset");
       if (M[8] & MED_WLS) {  // WLSABSLENGTH, WLSCOMPONENT (two knots or more), the yield and the time constant
         for (int j = 9; j < 17; j += 2)
           if (M[j] < 0 || M[j + 1] < 1) return fail(LH_ERR_LAYOUT, "lh_model_create: a WLS table is empty");
         if (M[12] < 2) return fail(LH_ERR_LAYOUT, "lh_model_create: a WLS emission spectrum needs two knots");
+      }
+      if (M[8] & MED_MIE) {  // MIEHG (a table), MIEHG_FORWARD, MIEHG_BACKWARD, MIEHG_FORWARD_RATIO (one knot each)
+        if (M[17] < 0 || M[18] < 1) return fail(LH_ERR_LAYOUT, "lh_model_create: a MIEHG table is empty");
+        for (int j = 19; j < 25; j += 2)
+          if (M[j] < 0 || M[j + 1] != 1)
+            return fail(LH_ERR_LAYOUT, "lh_model_create: a Mie lobe constant is not one value");
       }
     }
   }
@@ -1111,6 +1175,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
       for (int j = 0; j < 8; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
       if (M[8] & MED_WLS)
         for (int j = 9; j < 17; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
+      if (M[8] & MED_MIE)
+        for (int j = 17; j < 25; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
     }
     for (int r = 0; r < layout[3]; ++r) {
       const int32_t *R = layout + layout[9] + ROW_WORDS * r;
@@ -1118,7 +1184,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     }
     m->values_needed = need;
     for (int mi = 0; mi < layout[1]; ++mi)
-      if (layout[layout[6] + MEDIUM_WORDS * mi + 8] & MED_WLS) m->has_wls = true;
+      if (layout[layout[6] + MEDIUM_WORDS * mi + 8] & (MED_WLS | MED_MIE)) m->has_chan = true;
   }
   m->n_media = layout[1];
   m->S = layout[4];
@@ -1256,14 +1322,14 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
 }
 
 // the kernel of a launch: forward or adjoint, specialised on its features (F_REF: separate reference values;
-// F_WLS: a medium shifts wavelengths), so a run without them carries none of their registers
+// F_CHAN: a medium has a Mie or WLS channel), so a run without them carries none of their registers
 template <unsigned F>
 static const void *kernel_f(bool adj) {
   return adj ? (const void *)adjoint_kernel<F & F_ADJ_MASK> : (const void *)trace_kernel<F>;
 }
 
 static unsigned features_of(lh_model m, const Args &a, bool adj) {
-  return (a.has_ref ? F_REF : 0u) | (m->has_wls ? F_WLS : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u);
+  return (a.has_ref ? F_REF : 0u) | (m->has_chan ? F_CHAN : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u);
 }
 
 #define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)

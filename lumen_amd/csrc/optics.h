@@ -29,8 +29,9 @@ enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_
 // the packed layout's header words (lumen_amd/pack.py)
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
        L_ROWS_AT, L_QE_TABLE };
-constexpr int MEDIUM_WORDS = 17;
-enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4 };  // a medium's flag word (its word 8)
+constexpr int MEDIUM_WORDS = 25;
+enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4, MED_MIE = 8 };  // a medium's flag word (its word 8)
+constexpr float MIE_G_MAX = 0.999f;  // the largest |g| of a Mie lobe
 constexpr int ROW_WORDS = 2 + 2 * N_COL;
 
 struct TabPos {
@@ -61,11 +62,11 @@ __device__ __forceinline__ float tab(const float *__restrict__ P, int off, int n
 __device__ __forceinline__ float rate_of(float L) { return L >= TINY_NORMAL ? 1.0f / L : 1.0f / TINY_NORMAL; }
 
 struct MediumAt {
-  float n, ng, mu_abs, mu_ray, mu_wls;  // the event channels' rates in Geant4's order: Rayleigh, WLS
+  float n, ng, mu_abs, mu_ray, mu_mie, mu_wls;  // the event channels' rates in Geant4's order: Rayleigh, Mie, WLS
 };
 
 __device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__restrict__ P, int m, float lam) {
-  MediumAt r{1.0f, 1.0f, 0.0f, 0.0f, 0.0f};
+  MediumAt r{1.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (m < 0) return r;
   const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
   if (M[8] & MED_OPAQUE) return r;
@@ -77,6 +78,7 @@ __device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__r
   r.ng = 1.0f / inv;
   if (M[5] > 0) r.mu_abs = rate_of(tab(P, M[4], M[5], E));
   if (M[7] > 0) r.mu_ray = rate_of(tab(P, M[6], M[7], E));
+  if (M[8] & MED_MIE) r.mu_mie = rate_of(tab(P, M[17], M[18], E));
   if (M[8] & MED_WLS) r.mu_wls = rate_of(tab(P, M[9], M[10], E));
   return r;
 }
@@ -428,6 +430,71 @@ __device__ inline float3 rayleigh_sample(float3 er, float3 ei, float u0, float u
   float sp, cp;
   __sincosf(TWO_PI * u1, &sp, &cp);
   return to_world(mk3(s * cp, s * sp, c), axis);
+}
+
+// ------------------------------------------------------------------------------------------------- Mie
+// G4OpMieHG (media/phase.py:98-190): a forward Henyey-Greenstein lobe HG(g_f) with probability r, else the
+// backward one HG(-g_b); the constants MIEHG_FORWARD, MIEHG_BACKWARD, MIEHG_FORWARD_RATIO are the medium's words
+// 19-24 (tables of one knot).  Cosines travel as (1 - mu, 1 + mu), both sums of positive terms, so neither cancels.
+struct MieLobe {
+  float gf, gb, r;
+};
+
+__device__ __forceinline__ MieLobe mie_lobe(const float *__restrict__ P, const int *M) {
+  return MieLobe{P[M[19] + M[20]], P[M[21] + M[22]], P[M[23] + M[24]]};
+}
+
+// Geant4's inverse CDF mu = 2u(1+g)^2(1-g+gu)/(1-g+2gu)^2 - 1 as (1 - mu, 1 + mu) through the identity
+// D^2 - u(1+g)^2(1-g+gu) = (1-g)^2(1-u)(1+gu), D = 1 - g + 2gu, each factor written without cancellation for
+// either sign of g (phase.py `mie_hg_cos`)
+__device__ __forceinline__ void hg_cos(float g, float u, float &cm, float &cp) {
+  const bool up = g >= 0.0f;
+  const float d = up ? (1.0f - g) + 2.0f * g * u : (1.0f - g) * (1.0f - u) + (1.0f + g) * u;
+  const float one_gu = up ? 1.0f + g * u : (1.0f - u) + (1.0f + g) * u;
+  const float inv_d2 = 1.0f / (d * d);
+  cm = 2.0f * (1.0f - g) * (1.0f - g) * (1.0f - u) * one_gu * inv_d2;
+  cp = 2.0f * u * (1.0f + g) * (1.0f + g) * ((1.0f - g) + g * u) * inv_d2;
+}
+
+// HG(mu; g) per steradian from cm = 1 - mu, cp = 1 + mu: 1 + g^2 - 2 g mu as (1-g)^2 + 2 g cm (g >= 0) or
+// (1+g)^2 - 2 g cp (g < 0) (phase.py `hg_density`)
+__device__ __forceinline__ float hg_density(float g, float cm, float cp) {
+  const float q = g >= 0.0f ? (1.0f - g) * (1.0f - g) + 2.0f * g * cm : (1.0f + g) * (1.0f + g) - 2.0f * g * cp;
+  return (1.0f - g) * (1.0f + g) / (2.0f * TWO_PI * q * sqrtf(q));
+}
+
+// d HG(mu; g) / dg over HG: -2g / (1 - g^2) - 3 (g - mu) / (1 + g^2 - 2 g mu)
+__device__ __forceinline__ float hg_dlog_g(float g, float cm, float cp) {
+  const float q = g >= 0.0f ? (1.0f - g) * (1.0f - g) + 2.0f * g * cm : (1.0f + g) * (1.0f + g) - 2.0f * g * cp;
+  const float mu = cm < cp ? 1.0f - cm : cp - 1.0f;
+  return -2.0f * g / ((1.0f - g) * (1.0f + g)) - 3.0f * (g - mu) / q;
+}
+
+// the lobe choice's forward ratio the draw is made at: the one defensive mixture over (r, 1 - r) (phase.py
+// `_mie_cover`): e at r = 0, 1 - e at r = 1, r itself between
+__device__ __forceinline__ float mie_cover(float r) { return r <= 0.0f ? DEFENSIVE : r >= 1.0f ? 1.0f - DEFENSIVE : r; }
+
+// a direction about wi from the Mie lobes at (g_f, g_b, forward ratio rc): uniforms lobe, cosine, azimuth
+// (phase.py `mie_hg_sample`, `hg_direction`)
+__device__ inline float3 mie_sample(float3 wi, float gf, float gb, float rc, float u0, float u1, float u2) {
+  const bool fwd = u0 <= rc;
+  float cm, cp;
+  hg_cos(fwd ? gf : gb, u1, cm, cp);
+  if (!fwd) {
+    const float t = cm;
+    cm = cp;
+    cp = t;
+  }
+  const float mu = cm < cp ? 1.0f - cm : cp - 1.0f;
+  const float s = sqrtf(fmaxf(cm * cp, 0.0f));
+  float sp, cph;
+  __sincosf(TWO_PI * u2, &sp, &cph);
+  return to_world(mk3(s * cph, s * sp, mu), wi);
+}
+
+// the two lobes' mixture density per steradian at (cm, cp) of the outgoing direction (phase.py `mie_hg_logpdf`)
+__device__ __forceinline__ float mie_density(float gf, float gb, float r, float cm, float cp) {
+  return r * hg_density(gf, cm, cp) + (1.0f - r) * hg_density(-gb, cm, cp);
 }
 
 // Lambertian about n (lobes.py `_cosine`)

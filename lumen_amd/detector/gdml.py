@@ -124,6 +124,15 @@ def _parse(path: str) -> _Doc:
                                          _num(el.get("rmin", "0"), d.consts) * u, name=name)
         elif el.tag == "orb":
             d.solids[name] = solids.orb(_num(el.get("r"), d.consts) * _lunit(el), name=name)
+        elif el.tag == "sphere":
+            u, au = _lunit(el), _aunit(el)
+            dphi = _num(el.get("deltaphi", "0"), d.consts) * au if el.get("deltaphi") else 2 * math.pi
+            if abs(dphi - 2 * math.pi) > 1e-9:
+                raise ValueError(f"gdml: sphere {name!r} with a phi cut is not supported")
+            t0 = _num(el.get("starttheta", "0"), d.consts) * au
+            dt = _num(el.get("deltatheta"), d.consts) * au if el.get("deltatheta") else math.pi
+            d.solids[name] = solids.sphere(_num(el.get("rmin", "0"), d.consts) * u, _num(el.get("rmax"), d.consts) * u,
+                                           t0, dt, name=name)
         elif el.tag in ("union", "subtraction", "intersection"):
             a, b = d.solids[el.find("first").get("ref")], d.solids[el.find("second").get("ref")]
             pos = el.find("position")

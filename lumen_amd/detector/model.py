@@ -74,12 +74,21 @@ class Medium:
     expected photons per absorbed one, a weight); ``wls_tau`` WLSTIMECONSTANT (ns; a key or a constant, 0 when
     absent) under the time profile ``wls_profile``.  The emission is isotropic and unpolarised, at or below the
     absorbed energy.
+
+    Mie scattering (G4OpMieHG, ``media/bulk.py`` ``mie``, ``media/phase.py:98-190``): ``mie_length`` the MIEHG key
+    (a mean free path over photon energy); ``mie_forward``, ``mie_backward``, ``mie_ratio`` MIEHG_FORWARD,
+    MIEHG_BACKWARD, MIEHG_FORWARD_RATIO (keys or constants): the forward lobe HG(g_f) with probability r, else the
+    backward one HG(-g_b); |g| <= 0.999, 0 <= r <= 1.
     """
 
     name: str
     rindex: str | None
     absorption: str | None = None
     rayleigh: str | None = None
+    mie_length: str | None = None
+    mie_forward: str | float = 0.0
+    mie_backward: str | float = 0.0
+    mie_ratio: str | float = 1.0
     wls_length: str | None = None
     wls_spectrum: str | None = None
     wls_yield: str | float = 1.0

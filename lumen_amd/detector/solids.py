@@ -126,6 +126,50 @@ def sphere_cap(r: float, height: float, name="cap") -> Solid:
     return Solid([lf], [(0, 0)], np.array([-rc, -rc, z0]), np.array([rc, rc, r]), name, dict(r=r, height=height))
 
 
+def polar_cone(theta: float, r: float, name="cone") -> Solid:
+    """Return the points whose polar angle (from +z, at the origin) is at most ``theta`` (0 < theta < pi), boxed by
+    +-r.  Below pi/2 one leaf: the cone (x^2 + y^2) cos^2 - z^2 sin^2 <= 0 cut by z >= 0; at pi/2 the half-space
+    z >= 0; above it the union of that half-space and the outside of the cone of half-angle pi - theta."""  # noqa: D205, D209
+    h = np.full(3, float(r))
+    up = np.array([[0.0, 0.0, -1.0]])
+    half = _planes_leaf(up, [0.0])
+    if abs(theta - 0.5 * np.pi) < 1e-15:
+        return Solid([half], [(0, 0)], -h, h, name, dict(theta=theta))
+    if theta < 0.5 * np.pi:
+        c2, s2 = np.cos(theta) ** 2, np.sin(theta) ** 2
+        lf = Leaf(1, _z3(), _z3(), np.diag([c2, c2, -s2]), _z3(), 0.0, up, np.array([0.0]))
+        return Solid([lf], [(0, 0)], -h, h, name, dict(theta=theta))
+    a = np.pi - theta
+    c2, s2 = np.cos(a) ** 2, np.sin(a) ** 2
+    out = Leaf(1, _z3(), _z3(), np.diag([-c2, -c2, s2]), _z3(), 0.0, np.zeros((0, 3)), np.zeros(0))
+    return Solid([half, out], [(0, 0), (0, 1), (OPS["union"], 0)], -h, h, name, dict(theta=theta))
+
+
+def sphere(rmin: float, rmax: float, theta0: float = 0.0, dtheta: float = np.pi, name="sphere") -> Solid:
+    """Return G4Sphere without a phi cut: radii [rmin, rmax], polar angles [theta0, theta0 + dtheta] (radians).
+
+    The shell (one orb minus another), intersected with the cone of the upper polar angle and less the cone of the
+    lower one where they cut; its box is exact: z over the corners (r, theta), the radius in xy at most rmax times
+    the largest sin theta of the range.
+    """
+    t1 = min(theta0 + dtheta, np.pi)
+    if not (0.0 <= rmin < rmax and 0.0 <= theta0 < t1):
+        raise ValueError(f"sphere {name!r}: 0 <= rmin < rmax and 0 <= theta0 < theta0 + dtheta, got "
+                         f"{rmin}, {rmax}, {theta0}, {dtheta}")
+    s = orb(rmax, name=name)
+    if rmin > 0.0:
+        s = boolean("sub", s, orb(rmin), name=name)
+    if t1 < np.pi - 1e-12:
+        s = boolean("inter", s, polar_cone(t1, rmax), name=name)
+    if theta0 > 1e-12:
+        s = boolean("sub", s, polar_cone(theta0, rmax), name=name)
+    zs = [r * np.cos(t) for r in (rmin, rmax) for t in (theta0, t1)]
+    rho = rmax * (1.0 if theta0 <= 0.5 * np.pi <= t1 else max(np.sin(theta0), np.sin(t1)))
+    s.lo, s.hi = np.array([-rho, -rho, min(zs)]), np.array([rho, rho, max(zs)])
+    s.params = dict(rmin=rmin, rmax=rmax, theta0=theta0, dtheta=dtheta)
+    return s
+
+
 def _box_corners(lo, hi):
     return np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
 

@@ -9,9 +9,10 @@ differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` decl
     [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
     [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)  [11] time nodes nt (0: no time
     histogram)  [12] t0 (ns, float32 bits)  [13] dt (ns, float32 bits)
-    media:  per medium 17 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
-            (1 opaque, 2 a wavelength shifter, 4 its exponential time profile); (offset, knots) of WLSABSLENGTH,
-            of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of WLSTIMECONSTANT
+    media:  per medium 25 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
+            (1 opaque, 2 a wavelength shifter, 4 its exponential time profile, 8 Mie scattering); (offset, knots)
+            of WLSABSLENGTH, of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of
+            WLSTIMECONSTANT; of MIEHG, of MIEHG_FORWARD, of MIEHG_BACKWARD, of MIEHG_FORWARD_RATIO
     rows:   per row 18 words: model, flags, then (offset, knots) of R, T, eff, p_spike, p_lobe, p_back, sigma_alpha,
             polish
 
@@ -31,8 +32,9 @@ from .params import Params, Table
 
 MAGIC = 0x4C484F31
 HEADER = 14
-MEDIUM_WORDS = 17
-MED_OPAQUE, MED_WLS, MED_WLS_EXP = 1, 2, 4
+MEDIUM_WORDS = 25
+MED_OPAQUE, MED_WLS, MED_WLS_EXP, MED_MIE = 1, 2, 4, 8
+MIE_G_MAX = 0.999  # the largest |g| of a Mie lobe (the reference's rules/choices.py MIE_G_MAX)
 ROW_WORDS = 2 + 2 * len(COLUMNS)
 
 
@@ -150,6 +152,24 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
             med[i, 11:13] = b.table(e, v, m.wls_spectrum)
             for j, c in ((13, m.wls_yield), (15, m.wls_tau)):
                 med[i, j:j + 2] = b.value(p, c) if isinstance(c, str) else b.table([0.0], [float(c)])
+        if m.mie_length is not None:
+            med[i, 8] |= MED_MIE
+            lt = p[m.mie_length]
+            lv = np.atleast_1d(np.asarray(lt.value if isinstance(lt, Table) else lt, np.float64))
+            if np.any(lv <= 0.0):
+                raise ValueError(f"pack: {m.mie_length}: a scattering length of 0 (Geant4 scatters such a photon "
+                                 "where it stands and never ends it)")
+            med[i, 17:19] = b.value(p, m.mie_length)
+            lobe = []
+            for j, c in ((19, m.mie_forward), (21, m.mie_backward), (23, m.mie_ratio)):
+                v = np.asarray(p[c] if isinstance(c, str) else c, np.float64)
+                if v.size != 1:
+                    raise ValueError(f"pack: medium {m.name!r}: a Mie lobe constant is one value, got {v.size}")
+                lobe.append(float(v.reshape(-1)[0]))
+                med[i, j:j + 2] = b.table([0.0], [lobe[-1]], c if isinstance(c, str) else None)
+            if abs(lobe[0]) > MIE_G_MAX or abs(lobe[1]) > MIE_G_MAX or not 0.0 <= lobe[2] <= 1.0:
+                raise ValueError(f"pack: medium {m.name!r}: Mie lobes need |g_f|, |g_b| <= {MIE_G_MAX} and a forward "
+                                 f"ratio in [0, 1], got {lobe}")
     rows = np.zeros((nr, ROW_WORDS), np.int32)
     for r, row in enumerate(det.rows):
         rows[r, 0] = row.model

@@ -13,7 +13,8 @@
 //   surfaces              src/lumen/surfaces/types.py:211-296, :352-389 (chained_cuts, _none, _real, kernel),
 //                         src/lumen/surfaces/fresnel.py:25-45, src/lumen/surfaces/outcomes.py:36-75,
 //                         src/lumen/surfaces/polar.py:39-118
-//   lobes                 src/lumen/media/phase.py:45-75 (Rayleigh), src/lumen/lobes.py:60-78 (Lambert),
+//   lobes                 src/lumen/media/phase.py:45-75 (Rayleigh), :98-190 (Mie: G4OpMieHG's two Henyey-Greenstein
+//                         lobes), src/lumen/transport/medium.py:96-133 (channel_draw), src/lumen/lobes.py:60-78 (Lambert),
 //                         src/lumen/surfaces/facets.py:318-397 (unified facet path)
 //   sources               src/lumen/sources/point.py:73-86, src/lumen/sources/cherenkov.py:33-66
 //   readout               src/lumen/run.py:483-528 (_moment_adds: a photon's deposits per sensor summed, then
@@ -36,6 +37,10 @@
 //   WLS emission      pinned (tests/test_oracle_wls.py): the trapezoid sampler's interval masses (chi2), the truncation
 //                     at the absorbed energy with weight yield (1 - (1 - F)^100), the live/reference reweighting
 //                     (importance-weighted mean energy), the exponential delay (KS), an enclosing sensor: Q/N = yield
+//   Mie lobes         pinned (tests/test_oracle_mie.py): the scattering angle against the mixture's closed-form CDF in
+//                     equal-mass bins (chi2), the Henyey-Greenstein moments <P_l> = r g_f^l + (1 - r)(-g_b)^l, the
+//                     Geant4 10.7.4 / 11.2.2 single-scatter histograms (tests/golden/g4_*_mie*_hist.npz), the density's
+//                     normalisation and the live/drawn weight's unit mean
 #include <omp.h>
 
 #include <algorithm>
@@ -518,18 +523,18 @@ struct Optics {
   }
   int cls_medium(int c) const { return c < 0 ? -1 : L[L[7] + c]; }
   int row_of(int a, int b) const { return L[L[8] + a * L[2] + b]; }
-  const int32_t *medium(int m) const { return L + L[6] + 17 * m; }
+  const int32_t *medium(int m) const { return L + L[6] + 25 * m; }
   const int32_t *row(int r) const { return L + L[9] + 18 * r; }
 };
 
 inline double rate(double len) { return len >= TINY_NORMAL ? 1.0 / len : 1.0 / TINY_NORMAL; }
 
 template <class R> struct Med {
-  R n, ng, mu_abs, mu_ray, mu_wls;
+  R n, ng, mu_abs, mu_ray, mu_mie, mu_wls;  // the channels in Geant4's order: Rayleigh, Mie, WLS
 };
 
 template <class R> Med<R> medium_at(const Optics &o, int m, double lam) {
-  Med<R> r{R(1.0), R(1.0), R(0.0), R(0.0), R(0.0)};
+  Med<R> r{R(1.0), R(1.0), R(0.0), R(0.0), R(0.0), R(0.0)};
   if (m < 0) return r;
   const int32_t *M = o.medium(m);
   if (M[8] & 1) return r;
@@ -543,6 +548,7 @@ template <class R> Med<R> medium_at(const Optics &o, int m, double lam) {
   r.ng = R(1.0) / inv;
   if (M[5] > 0) r.mu_abs = R(rate(o.tab(M[4], M[5], E)));
   if (M[7] > 0) r.mu_ray = R(rate(o.tab(M[6], M[7], E)));
+  if (M[8] & 8) r.mu_mie = R(rate(o.tab(M[17], M[18], E)));
   if (M[8] & 2) r.mu_wls = R(rate(o.tab(M[9], M[10], E)));
   return r;
 }
@@ -838,6 +844,43 @@ template <class R> V3<R> rayleigh_dir(V3<R> er, V3<R> ei, const double ul[4]) {
   return unit(to_world(v3<R>(sn * COS(R(TWO_PI * ul[1])), sn * SIN(R(TWO_PI * ul[1])), c), axis));
 }
 
+// G4OpMieHG (src/lumen/media/phase.py:98-190), FP64: the forward lobe HG(g_f) with probability r, else the backward
+// one, mirrored (theta -> pi - theta); the lobe constants are the medium's words 19-24 (one value each).  The
+// cosine is Geant4's inverse CDF as PostStepDoIt writes it: mu = 2u(1+g)^2(1-g+gu)/(1-g+2gu)^2 - 1 (2u - 1 at g = 0).
+struct MieConsts {
+  double gf, gb, r;
+};
+inline MieConsts mie_consts(const Optics &o, const int32_t *M) {
+  return {o.P[M[19] + M[20]], o.P[M[21] + M[22]], o.P[M[23] + M[24]]};
+}
+// the forward ratio the lobe is drawn at: the defensive mixture over (r, 1 - r) (phase.py `_mie_cover`,
+// estimator.py `defensive_q`): a ratio of 0 (1) draws the absent lobe with the share e
+inline double mie_cover(double r) {
+  if (r > 0.0 && r < 1.0) return r;
+  const double eps = DEFENSIVE / (1.0 - DEFENSIVE);
+  return r <= 0.0 ? eps / (eps + 1.0) : 1.0 / (1.0 + eps);
+}
+inline double hg_mu(double g, double u) {
+  if (g == 0.0) return 2.0 * u - 1.0;
+  const double d = 1.0 - g + 2.0 * g * u;
+  return std::min(std::max(2.0 * u * (1.0 + g) * (1.0 + g) * (1.0 - g + g * u) / (d * d) - 1.0, -1.0), 1.0);
+}
+// HG(mu; g) per steradian: (1 - g^2) / (4 pi (1 + g^2 - 2 g mu)^(3/2))
+inline double hg_pdf(double g, double mu) {
+  const double q = 1.0 + g * g - 2.0 * g * mu;
+  return (1.0 - g * g) / (4.0 * PI * q * std::sqrt(q));
+}
+inline double mie_pdf(const MieConsts &c, double r, double mu) {
+  return r * hg_pdf(c.gf, mu) + (1.0 - r) * hg_pdf(-c.gb, mu);
+}
+template <class R> V3<R> mie_dir(V3<R> wi, const MieConsts &c, double rc, const double ul[4]) {
+  const bool fwd = ul[0] <= rc;
+  double mu = hg_mu(fwd ? c.gf : c.gb, ul[1]);
+  if (!fwd) mu = -mu;
+  const double sn = std::sqrt(std::max(1.0 - mu * mu, 0.0));
+  return unit(to_world(v3<R>(R(sn * std::cos(TWO_PI * ul[2])), R(sn * std::sin(TWO_PI * ul[2])), R(mu)), wi));
+}
+
 // Lambertian about n (lobes.py `_cosine`)
 template <class R> V3<R> lambert_dir(V3<R> n, const double ul[4]) {
   const R rr = SQRT(R(ul[0]));
@@ -931,7 +974,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   const int m = r.live.cls_medium(p.cls);
   const Med<R> ml = medium_at<R>(r.live, m, p.lam);
   const Med<R> mr = r.has_ref ? medium_at<R>(r.ref, m, p.lam) : ml;
-  const R mu = ml.mu_ray + ml.mu_wls, mu_r = mr.mu_ray + mr.mu_wls;  // the channels' total rates
+  const R mu = (ml.mu_ray + ml.mu_mie) + ml.mu_wls, mu_r = (mr.mu_ray + mr.mu_mie) + mr.mu_wls;  // total rates
   const R s_free = -LOG(R(u[0])) / MAX<R>(mu_r, R(FLOOR32));
   const Cross<R> cr = query<R>(s, p.x, p.d, p.vol, p.srf, R(r.max_dist));
   const bool hit = cr.vol_after >= 0;
@@ -1018,9 +1061,13 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   }
   bool ok;
   R w_new = R(0.0), wr_new = R(0.0);
-  if (scatter) {  // the channel from the reference rates (Rayleigh, WLS), step.py:397-436 `_bulk`
-    const bool wls = !(R(u[1]) * mu_r < mr.mu_ray);
-    const R mc = wls ? ml.mu_wls : ml.mu_ray, mc_r = wls ? mr.mu_wls : mr.mu_ray;
+  if (scatter) {  // the channel from the reference rates (Rayleigh, Mie, WLS), step.py:397-436 `_bulk`
+    // estimator.py `choose`: the first channel whose running sum exceeds u times the total (the first if none)
+    const R xc = R(u[1]) * mu_r;
+    const int ch = xc < mr.mu_ray ? 0 : xc < mr.mu_ray + mr.mu_mie ? 1 : xc < mu_r ? 2 : 0;
+    const bool wls = ch == 2;
+    const R mc = ch == 0 ? ml.mu_ray : ch == 1 ? ml.mu_mie : ml.mu_wls;
+    const R mc_r = ch == 0 ? mr.mu_ray : ch == 1 ? mr.mu_mie : mr.mu_wls;
     const R w_path = ratio<R>(mc * EXP(-mu * s_free), mc_r * EXP(-mu_r * s_free));
     const R att_s = EXP(-ml.mu_abs * s_free);
     const R w_at = p.w * w_path;
@@ -1028,10 +1075,22 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
     p.x = p.x + s_free * p.d;
     p.t = p.t + s_free / v_g;
     if (!wls) {
-      w_new = w_at * att_s;
+      R w_lobe = R(1.0);
+      V3<R> dn;
+      if (ch == 0) {
+        dn = rayleigh_dir(er0, ei0, ul);
+      } else {  // Mie: drawn at the reference's lobes under the cover of its ratio, weighted live / drawn density
+        const int32_t *M = r.live.medium(m);
+        const MieConsts lv = mie_consts(r.live, M), lr = mie_consts(r.has_ref ? r.ref : r.live, M);
+        const double rc = mie_cover(lr.r);
+        dn = mie_dir(p.d, lr, rc, ul);
+        const double c = std::min(std::max(val(dot(p.d, dn)), -1.0), 1.0);
+        const double drawn = mie_pdf(lr, rc, c);
+        w_lobe = R(drawn > 0.0 ? mie_pdf(lv, lv.r, c) / drawn : 0.0);
+      }
+      w_new = w_at * att_s * w_lobe;
       wr_new = p.wr * att_s_r;
       T.h[LH_H_ABSORBED_BULK] += val(w_at * (R(1.0) - att_s)) * p.w0;
-      const V3<R> dn = rayleigh_dir(er0, ei0, ul);
       er0 = er0 - dot(dn, er0) * dn;
       ei0 = ei0 - dot(dn, ei0) * dn;
       pol_normalise(er0, ei0);
@@ -1301,6 +1360,30 @@ ORC_API void oracle_lobes(const double *er, const double *ei, long long n, uint6
     ray[3 * i] = r.x, ray[3 * i + 1] = r.y, ray[3 * i + 2] = r.z;
     lam[3 * i] = l.x, lam[3 * i + 1] = l.y, lam[3 * i + 2] = l.z;
   }
+}
+
+// n Mie directions about wi drawn at the constants (gf, gb, r) under the cover of r, and the weight of the live
+// constants (lf, lb, lr) over the drawn density: out (n*4) = direction, weight
+ORC_API void oracle_mie(const double *wi, double gf, double gb, double r, double lf, double lb, double lr,
+                        long long n, uint64_t seed, double *out) {
+  const Rng g{(uint32_t)seed, (uint32_t)(seed >> 32), 11u, 0u};
+  const MieConsts drawn{gf, gb, r}, live{lf, lb, lr};
+  const double rc = mie_cover(r);
+  const V3<double> w = unit(v3(wi[0], wi[1], wi[2]));
+  for (long long i = 0; i < n; ++i) {
+    double ul[4];
+    g.draw4((uint32_t)i, (uint32_t)(i >> 32), ul);
+    const V3<double> d = mie_dir(w, drawn, rc, ul);
+    const double c = std::min(std::max(dot(w, d), -1.0), 1.0);
+    out[4 * i] = d.x, out[4 * i + 1] = d.y, out[4 * i + 2] = d.z;
+    out[4 * i + 3] = mie_pdf(live, live.r, c) / mie_pdf(drawn, rc, c);
+  }
+}
+
+// the Mie mixture density per steradian at the cosines mu (n of them)
+ORC_API void oracle_mie_pdf(double gf, double gb, double r, long long n, const double *mu, double *out) {
+  const MieConsts c{gf, gb, r};
+  for (long long i = 0; i < n; ++i) out[i] = mie_pdf(c, r, mu[i]);
 }
 
 // n re-emissions by medium m's wavelength shifter of photons absorbed at lam_in (uniforms u_lam, u_time: n each);

@@ -51,6 +51,10 @@ def lib():
         L.oracle_fresnel.restype = None
         L.oracle_wls_emit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_longlong,
                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_mie.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.c_longlong, C.c_uint64, C.c_void_p]
+        L.oracle_mie.restype = None
+        L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
+        L.oracle_mie_pdf.restype = None
         _LIB = L
     return _LIB
 
@@ -169,6 +173,25 @@ def wls_emit(packed, medium, lam_in, u_lam, u_time, ref_packed=None):
                                _p(ut), _p(out))
     if rc:
         raise ValueError(f"medium {medium} shifts no wavelengths")
+    return out
+
+
+def mie(wi, gf, gb, r, n, seed=0, live=None):
+    """Return (directions (n, 3), weights (n,)) of n Mie draws about ``wi`` at the constants (gf, gb, r), drawn under
+    the cover of r, weighted by the density of ``live`` = (gf, gb, r) (default: the drawn constants) over the drawn
+    one."""  # noqa: D205, D209
+    lf, lb, lr = (gf, gb, r) if live is None else live
+    out = np.zeros((int(n), 4))
+    w = np.ascontiguousarray(wi, np.float64)
+    lib().oracle_mie(_p(w), float(gf), float(gb), float(r), float(lf), float(lb), float(lr), int(n), int(seed), _p(out))
+    return out[:, :3], out[:, 3]
+
+
+def mie_pdf(gf, gb, r, mu):
+    """Return the Mie mixture density per steradian at the cosines ``mu``."""
+    m = np.ascontiguousarray(mu, np.float64)
+    out = np.zeros(len(m))
+    lib().oracle_mie_pdf(float(gf), float(gb), float(r), len(m), _p(m), _p(out))
     return out
 
 

@@ -62,6 +62,55 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
                     ["shell"], [SensorGroup("pmt", np.arange(1))], Params(values))
 
 
+MIE_CASES = {"A": (0.6, 0.3, 0.7), "B": (0.0, 0.8, 0.5), "water": (0.95, 0.3, 1.0)}  # (g_f, g_b, r)
+MIE_R, MIE_L, MIE_N, MIE_E = 1.0, 20.0, 1.33, 3.0  # sphere radius (m), MIEHG (m), RINDEX, photon energy (eV)
+
+
+def mie_toy(case="A", rings=1, length=MIE_L):
+    """Return the Mie toy of the reference's Mie transport tests: a Mie-only medium (MIEHG ``length``, RINDEX 1.33,
+    no absorption) in a world orb of 1.2 m, inside a counting shell r in [1, 1.01] m cut into ``rings`` polar rings
+    of equal angle (each its own sensor, every photon absorbed and counted where it enters one), the lobe constants
+    of ``case`` (MIE_CASES) as parameter keys."""  # noqa: D205, D209
+    gf, gb, r = MIE_CASES[case]
+    d = np.pi / rings
+    kinds = [solids.orb(1.2, name="world")] + [solids.sphere(MIE_R, 1.01, i * d, d, name=f"ring{i}")
+                                              for i in range(rings)]
+    vols = [navarrays.Volume("world", 0, np.eye(3), np.zeros(3), -1, 0)]
+    vols += [navarrays.Volume(f"world/ring{i}", 1 + i, np.eye(3), np.zeros(3), 0, 1) for i in range(rings)]
+    nav = navarrays.build(vols, kinds, outside_cls=2)
+    m = "materials/MieLike/"
+    values = {m + "RINDEX": Table(E_KNOTS, np.array([MIE_N, MIE_N])),
+              m + "MIEHG": Table(E_KNOTS, np.array([length, length])),
+              m + "MIEHG_FORWARD": np.float64(gf), m + "MIEHG_BACKWARD": np.float64(gb),
+              m + "MIEHG_FORWARD_RATIO": np.float64(r),
+              "materials/SensLike/RINDEX": Table(E_KNOTS, np.array([MIE_N, MIE_N]))}
+    media = [Medium("MieLike", m + "RINDEX", mie_length=m + "MIEHG", mie_forward=m + "MIEHG_FORWARD",
+                    mie_backward=m + "MIEHG_BACKWARD", mie_ratio=m + "MIEHG_FORWARD_RATIO"),
+             Medium("SensLike", "materials/SensLike/RINDEX")]
+    rows = [SurfaceRow("ring", "SPECULAR_METAL", {"R": 0.0, "eff": 1.0}), SurfaceRow("absorber", "ABSORBER")]
+    row_of = np.full((3, 3), -1, np.int32)
+    row_of[0, 1] = 0
+    row_of[0, 2] = row_of[1, 0] = row_of[1, 2] = 1
+    sensor_of_vol = np.array([-1] + list(range(rings)) + [-1], np.int32)
+    return Detector(f"mie_{case}_{rings}", nav, media, np.array([0, 1, -1], np.int32), ["MieLike", "SensLike", "OUT"],
+                    rows, row_of, sensor_of_vol, [f"ring{i}" for i in range(rings)],
+                    [SensorGroup("rings", np.arange(rings))], Params(values))
+
+
+def mie_hg_cdf_angle(theta, g):
+    """P(scattering angle <= theta) of HG(g), float64 (the textbook CDF in 1 - cos theta = 2 sin^2(theta/2))."""
+    s2 = np.sin(0.5 * np.asarray(theta, np.float64)) ** 2
+    if g == 0.0:
+        return s2
+    q = (1.0 - g) ** 2 + 4.0 * g * s2
+    return (1.0 - g * g) / (2.0 * g) * (1.0 / (1.0 - g) - 1.0 / np.sqrt(q))
+
+
+def mie_cdf_angle(theta, gf, gb, r):
+    """P(scattering angle <= theta) of the mixture r HG(g_f) + (1 - r) HG(-g_b)."""
+    return r * mie_hg_cdf_angle(theta, gf) + (1.0 - r) * mie_hg_cdf_angle(theta, -gb)
+
+
 def specular_closed_form(a, R, L):
     """Q/N of the specular sensor sphere: (1 - R) e^{-a/L} / (1 - R e^{-2a/L})."""
     return (1 - R) * np.exp(-a / L) / (1 - R * np.exp(-2 * a / L))
