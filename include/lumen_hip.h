@@ -79,6 +79,10 @@ typedef struct {
   const int32_t *kind_l0, *kind_nl, *kind_p0, *kind_pl; /* n_kinds */
   const int32_t *prog_op, *prog_arg;                    /* n_prog */
   float reach; /* the diagonal of the box holding every volume (m) */
+  /* n_vol+1, or NULL: per volume the CSG leaf of its daughter-free core (-1 none), an acceleration with no
+   * counterpart in the reference's arrays: no daughter meets that leaf's solid, so the walk skips every BVH box the
+   * ray crosses inside it (lumen_amd/detector/navarrays.py skin_core) */
+  const int32_t *skin;
 } lh_geometry;
 
 /* One emission source.  Photon i of it (0 <= i < n) has the id first_id + i: its random numbers are a function of

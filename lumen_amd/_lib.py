@@ -39,7 +39,7 @@ class LhGeometry(C.Structure):
         ("gR", _f), ("g0", _i), ("g1", _i), ("lo", _f), ("hi", _f), ("skip", _i), ("start", _i), ("count", _i),
         ("items", _i), ("leaf_typ", _i), ("leaf_p0", _i), ("leaf_np", _i), ("leaf_o", _f), ("leaf_q", _f),
         ("leaf_A", _f), ("leaf_b", _f), ("leaf_c", _f), ("plane_n", _f), ("plane_h", _f), ("kind_l0", _i),
-        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float)]
+        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float), ("skin", _i)]
 
 
 class LhSource(C.Structure):
@@ -125,6 +125,8 @@ def geometry_struct(nav, sensor_of_vol, reach: float):
         plane_n=f32(np.stack([T["n0"], T["n1"], T["n2"]], 1)).reshape(-1), plane_h=f32(T["h"]),
         kind_l0=i32(K["l0"]), kind_nl=i32(K["nl"]), kind_p0=i32(K["p0"]), kind_pl=i32(K["pl"]),
         prog_op=i32(G["op"]), prog_arg=i32(G["arg"]))
+    if "skin" in h:  # the daughter-free cores (absent from the reference's arrays: no acceleration)
+        keep["skin"] = i32(h["skin"])
     n_nodes = len(keep["skip"]) if nav.stats.get("groups", 1) else 0
     g = LhGeometry(n_vol=nav.n, n_nodes=n_nodes, n_items=len(keep["items"]),
                    n_groups=int(nav.stats.get("groups", len(keep["g0"]))), n_leaves=n_leaves,

@@ -998,6 +998,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   for (int gi = 0; gi < G->n_groups; ++gi)
     if (G->g0[gi] < 0 || G->g1[gi] > G->n_nodes || G->g0[gi] > G->g1[gi])
       return fail(LH_ERR_ARG, "lh_model_create: a group's nodes are out of range");
+  if (G->skin)
+    for (int v = 0; v <= n; ++v)
+      if (G->skin[v] < -1 || G->skin[v] >= G->n_leaves || (v == n && G->skin[v] != -1))
+        return fail(LH_ERR_ARG, "lh_model_create: a daughter-free core's leaf is out of range");
   const int nc = layout[2];
   for (int v = 0; v <= n; ++v) {
     if (G->cls[v] < 0 || G->cls[v] >= nc) return fail(LH_ERR_ARG, "lh_model_create: a class is out of range");
@@ -1109,6 +1113,43 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     ikind[i] = vkind[u];
   }
   for (int j = 0; j < G->n_prog; ++j) prog[j] = (G->prog_op[j] << 24) | (G->prog_arg[j] & 0xFFFFFF);
+  auto int_bits = [](int i) {
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+  };
+  // the daughter-free cores as the navigator reads them: an axis-aligned box about a centre, cut by a cylinder
+  // about z (the shrunk tube or box of a tank), else the leaf itself
+  std::vector<float4> vcore(3 * ((size_t)n + 1), make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int v = 0; v < n && G->skin; ++v) {
+    const int li = G->skin[v];
+    if (li < 0) continue;
+    const float *A = G->leaf_A + 9 * li, *b = G->leaf_b + 3 * li, *o = G->leaf_o + 3 * li, *q = G->leaf_q + 3 * li;
+    const int typ = G->leaf_typ[li], p0 = G->leaf_p0[li], np = G->leaf_np[li];
+    bool simple = typ == 0 || (typ == 1 && A[0] == 1.0f && A[4] == 1.0f && A[8] == 0.0f && A[1] == 0.0f &&
+                               A[2] == 0.0f && A[3] == 0.0f && A[5] == 0.0f && A[6] == 0.0f && A[7] == 0.0f &&
+                               b[0] == 0.0f && b[1] == 0.0f && b[2] == 0.0f);
+    const float *c = typ == 1 ? q : o;
+    float lo3[3] = {-HUGEF, -HUGEF, -HUGEF}, hi3[3] = {HUGEF, HUGEF, HUGEF};
+    for (int k = 0; k < np && simple; ++k) {
+      const float *nn = G->plane_n + 3 * (p0 + k);
+      int ax = -1;
+      for (int j = 0; j < 3; ++j)
+        if (std::fabs(nn[j]) == 1.0f && nn[(j + 1) % 3] == 0.0f && nn[(j + 2) % 3] == 0.0f) ax = j;
+      if (ax < 0) { simple = false; break; }
+      const float h = G->plane_h[p0 + k] - (nn[0] * (c[0] - o[0]) + nn[1] * (c[1] - o[1]) + nn[2] * (c[2] - o[2]));
+      if (nn[ax] > 0.0f) hi3[ax] = std::min(hi3[ax], h);
+      else lo3[ax] = std::max(lo3[ax], -h);
+    }
+    if (simple) {
+      vcore[3 * v] = make_float4(c[0], c[1], c[2], typ == 1 ? -G->leaf_c[li] : HUGEF);
+      vcore[3 * v + 1] = make_float4(lo3[0], lo3[1], lo3[2], int_bits(1));
+      vcore[3 * v + 2] = make_float4(hi3[0], hi3[1], hi3[2], 0.0f);
+    } else {
+      vcore[3 * v + 1] = make_float4(0.f, 0.f, 0.f, int_bits(2));
+      vcore[3 * v + 2] = make_float4(0.f, 0.f, 0.f, int_bits(li));
+    }
+  }
 
   size_t off = 0;
   auto place = [&](size_t bytes) {
@@ -1120,7 +1161,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
                o_leaf = place(leaf.size() * 16), o_plane = place(plane.size() * 16), o_vinfo = place(vinfo.size() * 16),
                o_kinds = place(kinds.size() * 16), o_vkind = place(vkind.size() * 16),
                o_ixf = place(ixf.size() * 16), o_ikind = place(ikind.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
-               o_items = place(items.size() * 4), o_prog = place(prog.size() * 4);
+               o_items = place(items.size() * 4), o_prog = place(prog.size() * 4),
+               o_vcore = place(vcore.size() * 16);
   lh_model m = new lh_model_s();
   char *buf = nullptr;
   hipError_t e = hipMalloc((void **)&buf, off);
@@ -1144,6 +1186,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   memcpy(host.data() + o_grange, grange.data(), grange.size() * 8);
   memcpy(host.data() + o_items, items.data(), items.size() * 4);
   memcpy(host.data() + o_prog, prog.data(), prog.size() * 4);
+  memcpy(host.data() + o_vcore, vcore.data(), vcore.size() * 16);
   HIP_TRY(hipMemcpy(buf, host.data(), off, hipMemcpyHostToDevice));
   Geo &g = m->g;
   g.nvol = n;
@@ -1161,6 +1204,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.grange = (const int2 *)(buf + o_grange);
   g.items = (const int *)(buf + o_items);
   g.prog = (const int *)(buf + o_prog);
+  g.vcore = (const float4 *)(buf + o_vcore);
   g.reach = G->reach;
   g.n_nodes = G->n_nodes;
   g.lds_nodes = 0;

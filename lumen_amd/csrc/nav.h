@@ -25,6 +25,7 @@ constexpr float BAND_ULPS = 8.0f;
 constexpr float BAND_FLOOR = 1e-9f;
 constexpr float K_ON = 4.0f;
 constexpr int MAX_CSG_LEAVES = 8;
+constexpr int WALK_PENDING = 3;  // leaves a lane collects on its walk before the wavefront tests their items
 constexpr int MAX_GROUPS_WALK = 8;
 
 // the launch's dynamic LDS: a kernel may stage the BVH nodes there (Geo::lds_nodes) after its own cells
@@ -46,6 +47,7 @@ struct Geo {
   const float4 *__restrict__ ixf;   // per BVH item (in items' order): its volume's 3 transform rows
   const int4 *__restrict__ ikind;   // per BVH item: its volume's kind record
   const int *__restrict__ prog;     // op << 24 | arg
+  const float4 *__restrict__ vcore;  // 3 per navigator id: its daughter-free core (core_interval)
   float reach;
   int n_nodes;
   int lds_nodes;     // 1: the kernel staged all BVH nodes in LDS at float4 offset lds_node_off of lds_cells
@@ -371,14 +373,63 @@ struct Cross {
 };
 
 // the next visible crossing from x along d in navigator id `vol`, within tmax; `last` the face the photon sits on
+// the stretch [lo, hi] of the ray xl + t dl (the volume's frame) inside the volume's daughter-free core: a box
+// (lo, hi about the centre c, +-HUGEF where open) cut by a cylinder about the z axis through c (radius^2 r2, HUGEF
+// none): records (c, r2), (lo, code), (hi, leaf); code 0 no core, 1 this form, 2 any other leaf (its intervals)
+__device__ inline void core_interval(const Geo &g, int vol, float3 xl, float3 dl, float &lo, float &hi) {
+  const float4 C1 = g.vcore[3 * vol + 1];
+  const int code = __float_as_int(C1.w);
+  lo = HUGEF;
+  hi = -HUGEF;
+  if (code == 0) return;
+  const float4 C0 = g.vcore[3 * vol], C2 = g.vcore[3 * vol + 2];
+  if (code == 2) {
+    LeafIv iv;
+    leaf_intervals(g, __float_as_int(C2.w), xl, dl, iv);
+    lo = iv.lo[0];
+    hi = iv.hi[0];
+    return;
+  }
+  const float3 y = sub3(xl, mk3(C0.x, C0.y, C0.z));
+  const float ix = 1.0f / (fabsf(dl.x) > 1e-30f ? dl.x : copysignf(1e-30f, dl.x));
+  const float iy = 1.0f / (fabsf(dl.y) > 1e-30f ? dl.y : copysignf(1e-30f, dl.y));
+  const float iz = 1.0f / (fabsf(dl.z) > 1e-30f ? dl.z : copysignf(1e-30f, dl.z));
+  const float tx0 = (C1.x - y.x) * ix, tx1 = (C2.x - y.x) * ix;
+  const float ty0 = (C1.y - y.y) * iy, ty1 = (C2.y - y.y) * iy;
+  const float tz0 = (C1.z - y.z) * iz, tz1 = (C2.z - y.z) * iz;
+  float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+  float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+  if (C0.w < HUGEF) {  // the cylinder: (y + t d)_xy^2 <= r2
+    const float a = dl.x * dl.x + dl.y * dl.y, b = y.x * dl.x + y.y * dl.y;
+    const float c = y.x * y.x + y.y * y.y - C0.w;
+    const float disc = b * b - a * c;
+    if (a > 0.0f && disc > 0.0f) {
+      const float q = -(b + copysignf(sqrtf(disc), b));
+      const float r0 = q / a, r1 = q != 0.0f ? c / q : r0;
+      t0 = fmaxf(t0, fminf(r0, r1));
+      t1 = fminf(t1, fmaxf(r0, r1));
+    } else if (!(a == 0.0f && c <= 0.0f)) {  // misses the cylinder (or runs along it outside)
+      t1 = -HUGEF;
+    }
+  }
+  if (t0 < t1) {
+    lo = t0;
+    hi = t1;
+  }
+}
+
 template <bool LDS>
 __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
   const float skip_t = K_ON * band_at(maxabs3(x));
   Cross best{tmax, -1, -1, 0, mk3(0.f, 0.f, 1.f)};
   int best_leaf = -1, best_face = -1, best_vol = -1;
+  // the ray's stretch [core_lo, core_hi] inside the volume's daughter-free core: a box the ray crosses only there
+  // holds nothing it can hit (the parameter t is the same in every frame: the transforms are rigid)
+  float core_lo = HUGEF, core_hi = -HUGEF;
   if (vol < g.nvol) {
     float3 xl, dl;
     to_local(g, vol, x, d, xl, dl);
+    core_interval(g, vol, xl, dl, core_lo, core_hi);
     const Change ch = first_change(g, g.vkind[vol], vol, xl, dl, true, last, skip_t);
     if (ch.leaf >= 0 && ch.t <= best.t) {
       best.t = ch.t;
@@ -404,12 +455,13 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
     const int2 nr = g.grange[gi];
     const float3 xi = mk3(xg.x * inv.x, xg.y * inv.y, xg.z * inv.z);  // the slabs as one fma each
     int ni = nr.x;
-    // the walk in two loops (traverse to the next leaf, then test its items): a wavefront runs the costly item
-    // tests once for all lanes that found a leaf, instead of on every node step where any one lane sits at a leaf.
-    // A node step is branch-free: a missed box or a leaf moves on to the skip link, an inner box to its first child.
+    // the walk in two loops (traverse collecting up to WALK_PENDING leaves, then test their items): a wavefront
+    // runs the costly item tests once for all lanes that found leaves, instead of on every node step where any one
+    // lane sits at a leaf, and a lane that found a leaf walks on while others look for theirs.  A node step is
+    // branch-free: a missed box or a leaf moves on to the skip link, an inner box to its first child.
     for (;;) {
-      int pst = 0, pcnt = 0;
-      while (ni < nr.y && pcnt == 0) {
+      int npend = 0, pl0 = 0, pl1 = 0, pl2 = 0;  // the leaves found (start * 8 + count), the latest in pl0
+      while (ni < nr.y && npend < WALK_PENDING) {
         const float4 lo = nav_node<LDS>(g, 2 * ni), hi = nav_node<LDS>(g, 2 * ni + 1);
         const float tx0 = fmaf(lo.x, inv.x, -xi.x), tx1 = fmaf(hi.x, inv.x, -xi.x);
         const float ty0 = fmaf(lo.y, inv.y, -xi.y), ty1 = fmaf(hi.y, inv.y, -xi.y);
@@ -417,27 +469,33 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
         const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
         const int sc = __float_as_int(hi.w);
-        const bool in = !(te > tl);
-        const int cnt = in ? (sc & 7) : 0;
-        pst = sc >> 3;
-        pcnt = cnt;
-        ni = (in && cnt == 0) ? ni + 1 : __float_as_int(lo.w);
+        const bool in = !(te > tl) && !(te >= core_lo && tl <= core_hi);
+        const bool leaf = in && (sc & 7) != 0;
+        pl2 = leaf ? pl1 : pl2;
+        pl1 = leaf ? pl0 : pl1;
+        pl0 = leaf ? sc : pl0;
+        npend += leaf ? 1 : 0;
+        ni = (in && !leaf) ? ni + 1 : __float_as_int(lo.w);
       }
-      if (pcnt == 0) break;
-      for (int k = 0; k < pcnt; ++k) {  // the item's transform and kind by item index: no wait on its volume id
-        const int it = pst + k;
-        const int u = g.items[it];
-        float3 xl, dl;
-        rows_to_local(g.ixf + 3 * it, x, d, xl, dl);
-        const Change ch = first_change(g, g.ikind[it], u, xl, dl, false, last, skip_t);
-        if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
-          best.t = ch.t;
-          best.vol_after = u;
-          best.own = 0;
-          best_leaf = ch.leaf;
-          best_face = ch.face;
-          best_vol = u;
-          best.n = rows_normal_to_world(g.ixf + 3 * it, ch.n);
+      if (npend == 0) break;
+      for (int q = npend - 1; q >= 0; --q) {  // oldest first: the walk's order
+        const int sc = q == 2 ? pl2 : q == 1 ? pl1 : pl0;
+        const int pst = sc >> 3, pcnt = sc & 7;
+        for (int k = 0; k < pcnt; ++k) {  // the item's transform and kind by item index: no wait on its volume id
+          const int it = pst + k;
+          const int u = g.items[it];
+          float3 xl, dl;
+          rows_to_local(g.ixf + 3 * it, x, d, xl, dl);
+          const Change ch = first_change(g, g.ikind[it], u, xl, dl, false, last, skip_t);
+          if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
+            best.t = ch.t;
+            best.vol_after = u;
+            best.own = 0;
+            best_leaf = ch.leaf;
+            best_face = ch.face;
+            best_vol = u;
+            best.n = rows_normal_to_world(g.ixf + 3 * it, ch.n);
+          }
         }
       }
     }

@@ -8,6 +8,12 @@ reference's column names.  ``save``/``load`` write and read ``arrays.npz`` plus 
 The builder here covers the solids this engine navigates (quadric-and-plane leaves; no meshes, no tori), volumes that
 lie inside their mother (no overlaps, so no ownership links) and world-frame daughter groups (``gR`` the identity).
 Navigator ids: volumes 0..n-1, the outside n.  ``R``, ``t`` map a volume's frame to the world: x_w = R x + t.
+
+One array is this build's own, an acceleration the reference's format does not hold (absent, nothing changes):
+``host["skin"]`` (n+1,), per volume the CSG leaf of its daughter-free core, or -1.  A volume whose solid is one
+convex leaf (a box, a full tube, an orb) and whose daughters all sit near its boundary (sensors on a tank's walls)
+gets the leaf shrunk by the depth D of its deepest daughter plus a margin: no daughter's box meets the core, so a ray
+skips every BVH box over the part of its path inside the core.  ``skin_core`` says how D is found.
 """
 
 from __future__ import annotations
@@ -19,7 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import bvh
-from .solids import Solid, program_depth
+from .solids import Leaf, Solid, program_depth
 
 HUGE = 1e30
 BOX_PAD_REL = 1e-6
@@ -129,6 +135,78 @@ def csg_tables(kinds: list[Solid], dtype=np.float32) -> dict:
                 kind_box=[(0.5 * (k.lo + k.hi), 0.5 * (k.hi - k.lo)) for k in kinds])
 
 
+# ---------------------------------------------------------------------------------------------- the skin core
+SKIN_MARGIN_REL = 1e-3  # the core's clearance from the deepest daughter box, relative to the solid's half-size
+SKIN_MAX_REL = 0.5  # no core when the daughters reach deeper than this share of the solid's smallest half-size
+
+
+def _dist_to_hull_2d(pts):
+    """Return the distance from the origin to the convex hull of 2D points (0 when inside it)."""
+    from scipy.spatial import ConvexHull
+
+    try:
+        hull = ConvexHull(pts)
+    except Exception:  # degenerate (collinear) points: the distance to their segments
+        hull = None
+    if hull is not None and np.all(hull.equations[:, 2] <= 0.0):  # the origin satisfies every facet: inside
+        return 0.0
+    best = np.inf
+    for i in range(len(pts)):
+        for j in range(i + 1, len(pts)):
+            a, b = pts[i], pts[j]
+            ab = b - a
+            den = float(ab @ ab)
+            s = 0.0 if den == 0.0 else min(max(-float(a @ ab) / den, 0.0), 1.0)
+            best = min(best, float(np.linalg.norm(a + s * ab)))
+    return best
+
+
+def skin_core(kd: Solid, boxes) -> Solid | None:
+    """Return the daughter-free core of solid ``kd`` (in its own frame) for daughters given as oriented boxes
+    (centre C, axes U (columns), half-sizes h) in that frame, or None.
+
+    ``kd`` must be one convex leaf: half-spaces n.(x - o) <= h, with or without a quadric that is a ball or a round
+    cylinder about its centre (A's eigenvalues 0 or 1, b = 0, c = -r^2).  A box B lies outside the core shrunk by D
+    when it lies beyond one shrunk face: a plane with min_B n.(x - o) >= h - D (the box's support), or the quadric
+    with the least distance of B from the ball's centre (the box point nearest it) or from the cylinder's axis (the
+    box projected along the axis: the distance to that convex polygon) at least r - D.  D is the largest over the
+    daughters of the least such D over the faces, plus the margin.
+    """
+    if len(kd.leaves) != 1 or kd.program != [(0, 0)] or len(boxes) == 0:
+        return None
+    lf = kd.leaves[0]
+    half = 0.5 * float(np.min(kd.hi - kd.lo))
+    axes = None
+    if lf.typ == 1:
+        w, V = np.linalg.eigh(lf.A)
+        unit, zero = np.abs(w - 1.0) < 1e-9, np.abs(w) < 1e-9
+        if not (np.all(unit | zero) and unit.sum() >= 2 and np.allclose(lf.b, 0.0) and lf.c < 0.0):
+            return None
+        r, q, axes = float(np.sqrt(-lf.c)), lf.origin + lf.qo, V[:, unit]
+    D = 0.0
+    for C, U, h in boxes:
+        need = np.inf
+        for k in range(len(lf.N)):  # the box's least n.(x - o): its centre's less the support
+            s = float(lf.N[k] @ (C - lf.origin)) - float(np.abs(lf.N[k] @ U) @ h)
+            need = min(need, lf.H[k] - s)
+        if axes is not None:
+            if axes.shape[1] == 3:  # a ball: the box point nearest its centre
+                s = U.T @ (q - C)
+                dist = float(np.linalg.norm(s - np.clip(s, -h, h)))
+            else:  # a cylinder: the box's corners projected on the plane across the axis
+                corners = np.array([C + U @ (h * np.array([sx, sy, sz]))
+                                    for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+                dist = _dist_to_hull_2d((corners - q) @ axes)
+            need = min(need, r - dist)
+        D = max(D, need)
+    D += SKIN_MARGIN_REL * half
+    if not (0.0 < D < SKIN_MAX_REL * half):
+        return None
+    core = Leaf(lf.typ, lf.origin.copy(), lf.qo.copy(), lf.A.copy(), lf.b.copy(),
+                -(np.sqrt(-lf.c) - D) ** 2 if lf.typ == 1 else 0.0, lf.N.copy(), lf.H - D)
+    return Solid([core], [(0, 0)], kd.lo + D, kd.hi - D, kd.name + "/core")
+
+
 # ---------------------------------------------------------------------------------------------- the builder
 def _world_box(kd: Solid, R, t):
     c, h = 0.5 * (kd.lo + kd.hi), 0.5 * (kd.hi - kd.lo)
@@ -148,13 +226,27 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
     for i, p in enumerate(parent):
         if p < n and p >= i:
             raise ValueError("navarrays.build: volumes must come after their mother (depth-first order)")
-    csg = csg_tables(kinds, fdt)
     lo_w, hi_w = np.zeros((n, 3)), np.zeros((n, 3))
     for i, v in enumerate(volumes):
         lo_w[i], hi_w[i] = _world_box(kinds[v.kind], v.R, v.t)
     kids = [[] for _ in range(n + 1)]
     for i in range(n):
         kids[parent[i]].append(i)
+    # the daughter-free cores: extra kinds (no volume places them) whose one leaf each volume's "skin" names
+    cores, core_of = [], np.full(n + 1, -1, np.int64)
+    for u in range(n):
+        if kids[u]:
+            v = volumes[u]
+            boxes = []
+            for k in kids[u]:  # each daughter's own box, oriented, in the mother's frame
+                kk, vk = kinds[volumes[k].kind], volumes[k]
+                boxes.append((v.R.T @ (vk.R @ (0.5 * (kk.lo + kk.hi)) + vk.t - v.t), v.R.T @ vk.R,
+                              0.5 * (kk.hi - kk.lo)))
+            c = skin_core(kinds[v.kind], boxes)
+            if c is not None:
+                core_of[u] = len(kinds) + len(cores)
+                cores.append(c)
+    csg = csg_tables(list(kinds) + cores, fdt)
     nodes = dict(lo=[], hi=[], skip=[], start=[], count=[])
     items, g0, g1, vg0, vg1 = [], [], [], np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
     depth = 0
@@ -198,7 +290,8 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
         size=np.concatenate([size, [0]]).astype(np.int32),
         piece=np.full(n + 1, -1, np.int32),
         ub=np.concatenate([[0], np.cumsum(nl)]).astype(np.int32),
+        skin=np.where(core_of >= 0, csg["K"]["l0"][np.maximum(core_of, 0)], -1).astype(np.int32),
     )
     stats = dict(volumes=n, kinds=len(kinds), groups=G, bvh_depth=depth,
-                 max_daughters=max(len(k) for k in kids))
+                 max_daughters=max(len(k) for k in kids), skin_cores=len(cores))
     return NavArrays(host, csg, n, tuple(v.name for v in volumes), lo_w, hi_w, 0, precision, stats)

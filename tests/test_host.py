@@ -31,6 +31,74 @@ def test_abi_structs_match_the_header():
     """ctypes' structs have the header's sizes (lh_source: 4 ints, 14 floats, 2 int64)."""
     assert ctypes.sizeof(_lib.LhSource) == 4 * 4 + 14 * 4 + 2 * 8
     assert ctypes.sizeof(_lib.LhControls) == 8 * 4
+    assert _lib.LhGeometry.skin.offset == _lib.LhGeometry.reach.offset + 8  # the float, padding, the pointer
+
+
+def _leaf(T, li, c=None, h_shift=0.0):
+    """Leaf ``li`` of the CSG table T as a solids.Leaf (optionally with another c, and its planes moved out)."""
+    pl = slice(int(T["p0"][li]), int(T["p0"][li] + T["np_"][li]))
+    return solids.Leaf(int(T["typ"][li]), np.array([T[f"o{a}"][li] for a in range(3)], float),
+                       np.array([T[f"q{a}"][li] - T[f"o{a}"][li] for a in range(3)], float),
+                       np.array([[T[f"A{a}{b}"][li] for b in range(3)] for a in range(3)], float),
+                       np.array([T[f"b{a}"][li] for a in range(3)], float), float(T["c"][li] if c is None else c),
+                       np.stack([T["n0"], T["n1"], T["n2"]], 1)[pl].astype(float),
+                       np.asarray(T["h"][pl], float) + h_shift)
+
+
+def test_daughter_free_cores_hold_no_daughter():
+    """Each volume's core (its skin leaf) holds no point of any daughter: points drawn in every daughter's solid
+    are outside the core (the mother's frame); a core a quarter of the skin thicker does hold some (the check
+    can fail), and the core holds the mother's centre."""
+    from lumen_amd.detector import lar_box
+
+    rng = np.random.default_rng(0)
+    for det in (water_tank(), example_tank(), lar_box()):
+        nav, h = det.nav, det.nav.host
+        T = nav.csg["T"]
+        assert nav.stats["skin_cores"] == 1
+        v = int(np.flatnonzero(h["skin"] >= 0)[0])
+        li = int(h["skin"][v])
+        core = _leaf(T, li)
+        # the outer solid's leaf: the core's depth D from the plane offsets of the two
+        outer_li = int(nav.csg["K"]["l0"][int(h["kind"][v])])
+        D = float(T["h"][T["p0"][outer_li]] - T["h"][T["p0"][li]])
+        c_big = -(np.sqrt(-T["c"][li]) + 0.25 * D) ** 2 if T["typ"][li] == 1 else None
+        bigger = _leaf(T, li, c=c_big, h_shift=0.25 * D)
+        Rm, tm = h["R"][v].astype(float), h["t"][v].astype(float)
+        assert core.inside(np.zeros((1, 3)) @ Rm - tm @ Rm)[0]
+        n_pts = n_big = 0
+        for k in np.flatnonzero(h["parent"][:nav.n] == v):
+            x = rng.uniform(nav.lo_w[k], nav.hi_w[k], size=(2000, 3))
+            Rk, tk = h["R"][k].astype(float), h["t"][k].astype(float)
+            xin = x[_inside_kind(nav, int(h["kind"][k]), (x - tk) @ Rk)]
+            n_pts += len(xin)
+            assert not np.any(core.inside((xin - tm) @ Rm)), (det.name, int(k))
+            n_big += int(np.sum(bigger.inside((xin - tm) @ Rm)))
+        assert n_pts > 1000 and n_big > 0, (det.name, n_pts, n_big)
+
+
+def _inside_kind(nav, kind, x):
+    """Membership of points x (kind frame) in kind ``kind`` from the CSG tables."""
+    T, K, G = nav.csg["T"], nav.csg["K"], nav.csg["G"]
+    l0, nl, p0, pl = (int(K[k][kind]) for k in ("l0", "nl", "p0", "pl"))
+    vals = []
+    for li in range(l0, l0 + nl):
+        lf = solids.Leaf(int(T["typ"][li]), np.array([T[f"o{a}"][li] for a in range(3)], float),
+                         np.array([T[f"q{a}"][li] - T[f"o{a}"][li] for a in range(3)], float),
+                         np.array([[T[f"A{a}{b}"][li] for b in range(3)] for a in range(3)], float),
+                         np.array([T[f"b{a}"][li] for a in range(3)], float), float(T["c"][li]),
+                         np.stack([T["n0"], T["n1"], T["n2"]], 1)[T["p0"][li]:T["p0"][li] + T["np_"][li]].astype(float),
+                         np.asarray(T["h"][T["p0"][li]:T["p0"][li] + T["np_"][li]], float))
+        vals.append(lf.inside(x))
+    st = []
+    for j in range(p0, p0 + pl):
+        op, arg = int(G["op"][j]), int(G["arg"][j])
+        if op == 0:
+            st.append(vals[arg])
+        else:
+            b, a = st.pop(), st.pop()
+            st.append(a | b if op == 1 else a & b if op == 2 else a & ~b)
+    return st[-1]
 
 
 def _check_bvh(t, lo, hi):
