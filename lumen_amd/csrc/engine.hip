@@ -45,10 +45,10 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int MAX_FACETS = 256;
 constexpr int MAX_FACET_ATTEMPTS = 4096;
 constexpr int BLOCK = 256;  // the adjoint's and the reference-value kernels' workgroup
-// The plain forward kernels run one workgroup of 16 wavefronts per CU (4 per SIMD at their 128-VGPR budget): the
+// The plain forward kernels run one workgroup of 12 wavefronts per CU (3 per SIMD at their 168-VGPR budget): the
 // BVH nodes and the tally copy are staged once per CU instead of once per 4 wavefronts, leaving room for both.
 #ifndef LH_FWD_BLOCK
-#define LH_FWD_BLOCK 1024
+#define LH_FWD_BLOCK 768
 #endif
 template <unsigned F>
 constexpr int block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
@@ -61,11 +61,12 @@ constexpr int OCC_CACHE = 8;
 constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8;
 constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES;  // the features an adjoint kernel is specialised on
 
-// The forward kernel's register budget: 4 wavefronts per SIMD (128 VGPRs) hide the walk's load latency better
-// than the 2-3 its natural allocation allows, spills included (measured on the 1,000-PMT tank).  The adjoint and
+// The forward kernel's register budget: 3 wavefronts per SIMD (168 VGPRs) hold the step without spills; at 4
+// (128 VGPRs) about 77 VGPRs spill to scratch, several GB of scratch traffic per launch, and the 1,000-PMT tank
+// runs 4 % slower (10.7 against 10.3 ms for 2^24 photons); at 2 the walk's load latency shows.  The adjoint and
 // the kernels with separate reference values keep their natural allocation.
 #ifndef LH_WAVES_PER_EU
-#define LH_WAVES_PER_EU 4
+#define LH_WAVES_PER_EU 3
 #endif
 #define LH_OCC_FWD(F) __attribute__((amdgpu_waves_per_eu(((F) & 1u) ? 1 : LH_WAVES_PER_EU)))
 
