@@ -121,11 +121,13 @@ typedef struct {
 int lh_model_create(const lh_geometry *geometry, const int32_t *layout, int32_t layout_words, lh_model *out);
 int lh_model_destroy(lh_model model);
 
-/* Query the model: number of sensors, of QE knots (charge cells per sensor) and tally doubles per event. */
+/* Query the model: number of sensors, of charge cells per sensor (the QE wavelength knots times the CE theta knots,
+ * the layout's words 5 and 14) and tally doubles per event. */
 int lh_model_info(lh_model model, int32_t *n_sensors, int32_t *n_qe, int64_t *tally_per_event);
 
 /* Trace every photon of `sources` (host array of n_sources) and add into `tally_dev` (n_events blocks of
- * [arrival S][arrival_t S][arrival2 S][charge S x n_qe], and with a time grid in the layout (its words 11-13: nt,
+ * [arrival S][arrival_t S][arrival2 S][charge S x n_qe (wavelength knot major, theta knot minor)], and with a time
+ * grid in the layout (its words 11-13: nt,
  * t0, dt) [time S x nt][overflow S x (late q, late q t, early q, early q t)], FP64) and `health_dev` (LH_HEALTH_N
  * FP64).  Adds: zero
  * them first for a fresh run.  `values_dev`: the packed values (float32, device, n_values of them, each table

@@ -84,7 +84,8 @@ class Health(NamedTuple):
 class Light:
     """What ``simulate`` returns (the reference's ``api.py:515-537``): per sensor (with ``events``: per event and
     sensor) the expected arrivals, their Monte Carlo error, the mean arrival time; the charge cells (S, K) on the QE
-    knots; the raw tallies; the health."""  # noqa: D205, D209
+    wavelength knots times the CE theta knots (``charge_axes`` = their counts, the wavelength's major); the raw
+    tallies; the health."""  # noqa: D205, D209
 
     arrivals: Any
     arrivals_error: Any
@@ -98,6 +99,7 @@ class Light:
     time: Any = None  # (S, nt) the time histogram's node masses (a timed detector: ``Detector.timed``)
     late: Any = None  # (S, 2) (q, q t) beyond the last node
     early: Any = None  # (S, 2) (q, q t) before the first node
+    charge_axes: tuple = (1, 1)  # (wavelength knots, theta knots) of the charge cells
 
     def to_dlpack(self, name="arrivals"):
         """Return a DLPack capsule of one output (device memory: ``jax.dlpack.from_dlpack`` takes it)."""
@@ -132,6 +134,8 @@ class Engine:
         _lib.check(_lib.lib().lh_model_info(h, _lib.C.byref(S), _lib.C.byref(K), _lib.C.byref(stride)),
                    "lh_model_info")
         self.S, self.K, self.stride = S.value, K.value, stride.value
+        self.Kt = int(layout[14])  # the charge cells: K = (QE wavelength knots) x (CE theta knots)
+        self.Kl = self.K // self.Kt
         tg = getattr(det, "time_grid", None)
         self.nt = 0 if tg is None else tg.nt
         if self.stride != 3 * self.S + self.S * self.K + (self.S * (self.nt + 4) if self.nt else 0):
@@ -294,4 +298,5 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
         arr, err, mean_t, charge, arr2 = arr[0], err[0], mean_t[0], charge[0], arr2[0]
         if time is not None:
             time, late, early = time[0], late[0], early[0]
-    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0, time=time, late=late, early=early)
+    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0, time=time, late=late, early=early,
+                 charge_axes=(eng.Kl, eng.Kt))

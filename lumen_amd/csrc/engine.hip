@@ -85,7 +85,8 @@ struct Args {
   int max_vert, chunk;
   double *__restrict__ tally;
   long long tally_stride;  // doubles per event
-  int S, K;
+  int S, K;  // sensors; charge cells per sensor, Kl wavelength (QE) knots x Kt theta (CE) knots
+  int Kl, Kt;
   int nt;               // the time histogram's nodes (0: none); its cells start at time_at of each event's tally
   long long time_at;
   float t0, dt;
@@ -180,30 +181,62 @@ __device__ inline void moment_add(const Args &a, Photon &p, int s, float v) {
   p.mv[free_k] = v;
 }
 
-// one deposit at sensor s: arrival, arrival_t, the charge cells at the QE knots, the photon's moment
+// the cathode position angle of a crossing point x (world) on sensor volume v: the polar angle about the volume's
+// +z axis through its origin (the reference's sensors/features.py `theta`, WCSim's collection-efficiency angle)
+__device__ inline float cathode_theta(const Geo &g, int v, float3 x) {
+  const float4 r0 = g.xf[3 * v], r1 = g.xf[3 * v + 1], r2 = g.xf[3 * v + 2];
+  const float lx = fmaf(r0.x, x.x, fmaf(r0.y, x.y, fmaf(r0.z, x.z, r0.w)));
+  const float ly = fmaf(r1.x, x.x, fmaf(r1.y, x.y, fmaf(r1.z, x.z, r1.w)));
+  const float lz = fmaf(r2.x, x.x, fmaf(r2.y, x.y, fmaf(r2.z, x.z, r2.w)));
+  return atan2_as(sqrtf(lx * lx + ly * ly), lz);
+}
+
+// a deposit's charge cells: hat weights on the QE wavelength knots and on the CE theta knots (linear between
+// knots, held beyond the ends; the reference's readout/grid.py `hats`, `cells`): cell il * Kt + it and its
+// neighbours, weights (1 - fl)(1 - ft), fl (1 - ft), (1 - fl) ft, fl ft; and the acceptance QE(lambda) CE(theta)
+struct Cells {
+  int base;      // the first cell (il * Kt + it)
+  float fl, ft;  // the weights of the next wavelength and theta knots
+  float acc;     // QE(lambda) CE(theta) (1 without tables)
+};
+
+__device__ inline Cells charge_cells(const Args &a, float lam, float theta) {
+  Cells c{0, 0.0f, 0.0f, 1.0f};
+  if (a.Kl > 1) {
+    const TabPos tp = tab_pos(a.P, a.L[L_QE_TABLE], a.Kl, lam);
+    c.base = tp.i * a.Kt;
+    c.fl = tp.f;
+    c.acc = tab_at(a.P, a.L[L_QE_TABLE], a.Kl, tp);
+  }
+  if (a.Kt > 1) {
+    const TabPos tp = tab_pos(a.P, a.L[L_CE_TABLE], a.Kt, theta);
+    c.base += tp.i;
+    c.ft = tp.f;
+    c.acc *= tab_at(a.P, a.L[L_CE_TABLE], a.Kt, tp);
+  }
+  return c;
+}
+
+// one deposit at sensor s: arrival, arrival_t, the charge cells on the QE and CE knots, the photon's moment
 template <unsigned F>
-__device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float q, float time) {
+__device__ inline void deposit(const Args &a, Photon &p, Acc &acc, int s, float q, float time, float theta) {
   acc.records += 1;
   const float v = q * p.w0;
   if (v == 0.0f) return;
   const double dv = (double)v;
   tally_add(a, p.event, s, dv);
   tally_add(a, p.event, a.S + s, dv * (double)time);
-  const int qoff = a.L[L_QE_TABLE];
-  if (a.K <= 1) {
-    tally_add(a, p.event, 3 * a.S + s, dv);
-  } else {  // hat weights on the QE wavelength knots (linear in wavelength, held beyond the ends)
-    const TabPos tp = tab_pos(a.P, qoff, a.K, p.lam);
-    tally_add(a, p.event, 3 * a.S + s * a.K + tp.i, dv * (double)(1.0f - tp.f));
-    if (tp.f > 0.0f) tally_add(a, p.event, 3 * a.S + s * a.K + tp.i + 1, dv * (double)tp.f);
+  const Cells c = charge_cells(a, p.lam, theta);
+  const long long cb = 3LL * a.S + (long long)s * a.K + c.base;
+  tally_add(a, p.event, cb, dv * (double)((1.0f - c.fl) * (1.0f - c.ft)));
+  if (c.fl > 0.0f) tally_add(a, p.event, cb + a.Kt, dv * (double)(c.fl * (1.0f - c.ft)));
+  if (c.ft > 0.0f) {
+    tally_add(a, p.event, cb + 1, dv * (double)((1.0f - c.fl) * c.ft));
+    if (c.fl > 0.0f) tally_add(a, p.event, cb + a.Kt + 1, dv * (double)(c.fl * c.ft));
   }
   if ((F & F_TIME) && a.nt > 0) {  // the time histogram (readout/grid.py `time_cells`): cubic Lagrange weights on the four nodes
     // around the time, the shares beyond the last / before the first node kept as (q, q t) per sensor
-    double dq = dv;
-    if (a.K > 1) {  // weighted by the acceptance at the deposit's wavelength: the detected light's time shape
-      const TabPos tp = tab_pos(a.P, qoff, a.K, p.lam);
-      dq *= (double)tab_at(a.P, qoff, a.K, tp);
-    }
+    const double dq = dv * (double)c.acc;  // weighted by the acceptance: the detected light's time shape
     const float x = fminf(fmaxf((time - a.t0) / a.dt, -4.0f), (float)a.nt + 4.0f);
     const float jf = floorf(x), g = x - jf;
     const int j = (int)jf;
@@ -390,16 +423,16 @@ __device__ inline void medium_cache(const Args &a, Photon &p, const AdjArgs *b) 
   }
 }
 
-// the cotangent of one deposit at sensor s: dLoss/d(arrival) plus the charge cells' at the photon's wavelength
-__device__ inline float deposit_cotangent(const Args &a, const AdjArgs *b, const Photon &p, int s) {
+// the cotangent of one deposit at sensor s: dLoss/d(arrival) plus the charge cells' at its wavelength and angle
+__device__ inline float deposit_cotangent(const Args &a, const AdjArgs *b, const Photon &p, int s, float theta) {
   const double *c = b->cot + (long long)p.event * a.tally_stride;
-  double cw = c[s];
-  if (a.K <= 1) {
-    cw += c[3 * a.S + s];
-  } else {
-    const TabPos tp = tab_pos(a.P, a.L[L_QE_TABLE], a.K, p.lam);
-    cw += (1.0 - tp.f) * c[3 * a.S + s * a.K + tp.i];
-    if (tp.f > 0.0f) cw += tp.f * c[3 * a.S + s * a.K + tp.i + 1];
+  const Cells cl = charge_cells(a, p.lam, theta);
+  const long long cb = 3LL * a.S + (long long)s * a.K + cl.base;
+  double cw = c[s] + (double)((1.0f - cl.fl) * (1.0f - cl.ft)) * c[cb];
+  if (cl.fl > 0.0f) cw += (double)(cl.fl * (1.0f - cl.ft)) * c[cb + a.Kt];
+  if (cl.ft > 0.0f) {
+    cw += (double)((1.0f - cl.fl) * cl.ft) * c[cb + 1];
+    if (cl.fl > 0.0f) cw += (double)(cl.fl * cl.ft) * c[cb + a.Kt + 1];
   }
   return (float)cw;
 }
@@ -511,10 +544,11 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     const float dep_q = p.w * reach_live * att * sl.detect;
     const int sensor = g.vinfo[cr.vol_after].w;
     if (sensor >= 0 && !cont) {
+      const float theta = a.Kt > 1 ? cathode_theta(g, cr.vol_after, fma3(dist, p.d, p.x)) : 0.0f;
       if (!ADJ) {
-        deposit<F>(a, p, acc, sensor, dep_q, p.t + dist / v_g);
+        deposit<F>(a, p, acc, sensor, dep_q, p.t + dist / v_g, theta);
       } else {
-        const float D = dep_q * p.w0 * deposit_cotangent(a, b, p, sensor);
+        const float D = dep_q * p.w0 * deposit_cotangent(a, b, p, sensor, theta);
         if (D != 0.0f) {
           p.hasdep = true;
           if (p.pass == 1) {  // the deposit's own factors: reach, attenuation, the detected share
@@ -906,7 +940,7 @@ struct lh_model_s {
   void *geo_buf = nullptr;
   int *layout = nullptr;
   int layout_words = 0;
-  int S = 0, K = 1, n_media = 0;
+  int S = 0, K = 1, Kl = 1, Kt = 1, n_media = 0;  // K = Kl wavelength knots x Kt theta knots
   long long tally_stride = 0;
   int NT = 0;  // the time histogram's nodes t0 + j dt (ns), j < NT (0: none)
   float t0 = 0.0f, dt = 1.0f;
@@ -947,7 +981,9 @@ LH_API const char *lh_last_error(void) { return g_err.c_str(); }
 
 LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t layout_words, lh_model *out) {
   if (!G || !layout || !out) return fail(LH_ERR_ARG, "lh_model_create: null argument");
-  if (layout_words < 14 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  if (layout_words < 16 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  if (layout[14] < 1 || layout[14] > 4096 || layout[15] < 0 || (long long)layout[5] * layout[14] > (1 << 20))
+    return fail(LH_ERR_LAYOUT, "lh_model_create: the charge cells (wavelength x theta knots) are out of range");
   {  // the time histogram's grid
     float t0 = 0.0f, dt = 0.0f;
     memcpy(&t0, &layout[12], 4);
@@ -1214,7 +1250,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   HIP_TRY(hipMemcpy(m->layout, layout, (size_t)layout_words * 4, hipMemcpyHostToDevice));
   m->layout_words = layout_words;
   {
-    long long need = (long long)layout[10] + 2LL * layout[5];
+    long long need = std::max((long long)layout[10] + 2LL * layout[5], (long long)layout[15] + 2LL * layout[14]);
     for (int mi = 0; mi < layout[1]; ++mi) {
       const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
       for (int j = 0; j < 8; j += 2) need = std::max(need, (long long)M[j] + 2LL * M[j + 1]);
@@ -1233,7 +1269,9 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   }
   m->n_media = layout[1];
   m->S = layout[4];
-  m->K = layout[5];
+  m->Kl = layout[5];
+  m->Kt = layout[14];
+  m->K = m->Kl * m->Kt;
   m->NT = layout[11];
   memcpy(&m->t0, &layout[12], 4);
   memcpy(&m->dt, &layout[13], 4);
@@ -1357,6 +1395,8 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.dt = m->dt;
   a.S = m->S;
   a.K = m->K;
+  a.Kl = m->Kl;
+  a.Kt = m->Kt;
   a.health = health;
   a.queue = m->queue;
   a.ut = m->ut;

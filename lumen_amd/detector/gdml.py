@@ -315,7 +315,7 @@ def load(path: str, recipe: Recipe) -> Detector:
                 sensor_of_vol[i] = len(names)
                 ids.append(len(names))
                 names.append("/".join(_path(placed, i)))
-        qe = eff = None
+        qe = eff = ce = None
         if g.values is not None:
             with open(g.values) as f:
                 vs = json.load(f)
@@ -323,12 +323,16 @@ def load(path: str, recipe: Recipe) -> Detector:
                 values[f"sensors/{g.name}/qe"] = Table(np.asarray(vs["qe"]["wavelength_nm"], float),
                                                        np.asarray(vs["qe"]["values"], float))
                 qe = f"sensors/{g.name}/qe"
+            if vs.get("ce"):  # CE over the cathode position angle: theta_deg knots in the file, radians here
+                values[f"sensors/{g.name}/ce"] = Table(np.radians(np.asarray(vs["ce"]["theta_deg"], float)),
+                                                       np.asarray(vs["ce"]["values"], float))
+                ce = f"sensors/{g.name}/ce"
             values[f"sensors/{g.name}/efficiency"] = np.float64(vs.get("efficiency", 1.0))
             eff = f"sensors/{g.name}/efficiency"
-        groups.append(SensorGroup(g.name, np.asarray(ids, np.int64), qe, eff))
+        groups.append(SensorGroup(g.name, np.asarray(ids, np.int64), qe, eff, ce))
     origin = {k: "detector file" for k in values}
     for g in groups:
-        for k in (g.qe, g.efficiency):
+        for k in (g.qe, g.efficiency, g.ce):
             if k is not None:
                 origin[k] = "sensor model defaults"
     return Detector(recipe.name, nav, media, class_medium, cls_names, srows, row_of, sensor_of_vol, names, groups,

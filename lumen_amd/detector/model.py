@@ -158,12 +158,17 @@ class TimeGrid:
 
 @dataclass
 class SensorGroup:
-    """A sensor group: its sensors (indices into the detector's sensors), QE and efficiency keys."""
+    """A sensor group: its sensors (indices into the detector's sensors), QE, efficiency and collection-efficiency
+    keys.  ``ce``: a Table of the collection efficiency over the cathode position angle theta (rad: the polar angle
+    of the crossing point about the sensor volume's +z axis, WCSim's angle; the reference's ``sensors/features.py``
+    ``theta``): the charge histogram then gains a theta axis on its knots (hat weights) and ``readout.pe`` multiplies
+    CE(theta) in, as QE(lambda)."""  # noqa: D205, D209
 
     name: str
     sensors: np.ndarray
     qe: str | None = None
     efficiency: str | None = None
+    ce: str | None = None
 
 
 @dataclass

@@ -8,7 +8,9 @@ differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` decl
 
     [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
     [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)  [11] time nodes nt (0: no time
-    histogram)  [12] t0 (ns, float32 bits)  [13] dt (ns, float32 bits)
+    histogram)  [12] t0 (ns, float32 bits)  [13] dt (ns, float32 bits)  [14] theta knots (1: no theta axis)
+    [15] ce table (offset of its theta knots, radians, in values)
+    The charge cells per sensor are (qe knots) x (theta knots), row-major (the wavelength's knot major).
     media:  per medium 25 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
             (1 opaque, 2 a wavelength shifter, 4 its exponential time profile, 8 Mie scattering); (offset, knots)
             of WLSABSLENGTH, of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of
@@ -31,7 +33,7 @@ from .detector.model import COLUMNS, FLAG_EDGE, Detector
 from .params import Params, Table
 
 MAGIC = 0x4C484F31
-HEADER = 14
+HEADER = 16
 MEDIUM_WORDS = 25
 MED_OPAQUE, MED_WLS, MED_WLS_EXP, MED_MIE = 1, 2, 4, 8
 MIE_G_MAX = 0.999  # the largest |g| of a Mie lobe (the reference's rules/choices.py MIE_G_MAX)
@@ -183,10 +185,16 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
         qe_off, n_qe = b.table(q.energy, q.value)
     else:  # no QE table, or one QE per sensor: a single charge cell per sensor
         qe_off, n_qe = b.table([0.0], [1.0])
+    ce = p[det.groups[0].ce] if det.groups and getattr(det.groups[0], "ce", None) is not None else None
+    if isinstance(ce, Table) and len(ce.energy) > 1:  # the charge histogram's theta knots (rad); CE read after tracking
+        ce_off, n_ce = b.table(ce.energy, ce.value)
+    else:
+        ce_off, n_ce = b.table([0.0], [1.0])
     tg = getattr(det, "time_grid", None)
     f32bits = lambda x: int(np.array([x], np.float32).view(np.int32)[0])  # noqa: E731
     L = [MAGIC, nm, nc, nr, det.n_sensors, n_qe, 0, 0, 0, 0, qe_off,
-         0 if tg is None else tg.nt, 0 if tg is None else f32bits(tg.t0), 0 if tg is None else f32bits(tg.dt)]
+         0 if tg is None else tg.nt, 0 if tg is None else f32bits(tg.t0), 0 if tg is None else f32bits(tg.dt),
+         n_ce, ce_off]
     L[6] = len(L)
     L += list(med.ravel())
     L[7] = len(L)

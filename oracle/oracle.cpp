@@ -774,7 +774,7 @@ struct Run {
   uint32_t k0, k1;
   double roulette, max_dist;
   int max_vert;
-  int S, K;
+  int S, K, Kl, Kt;  // sensors; charge cells per sensor = Kl wavelength (QE) knots x Kt theta (CE) knots
   long long stride;
   int nt;  // the time histogram's nodes t0 + j dt (0: none), its cells from time_at of each event's tally
   double t0, dt;
@@ -1025,18 +1025,29 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
       if (v != 0.0) {
         tal[sensor] += v;
         tal[r.S + sensor] += v * time;
-        if (r.K <= 1) {
-          tal[3 * r.S + sensor] += v;
-        } else {
-          int i;
-          double f;
-          r.live.tab(r.live.L[10], r.K, p.lam, &i, &f);
-          tal[3 * r.S + sensor * r.K + i] += v * (1 - f);
-          if (f > 0) tal[3 * r.S + sensor * r.K + i + 1] += v * f;
+        // the charge cells: hat weights on the QE wavelength knots and on the CE knots of the cathode position
+        // angle (readout/grid.py:161-185 `hats`, `cells`, `charge_cells`; sensors/features.py `theta`: the polar
+        // angle of the crossing point about the sensor volume's +z axis), the wavelength's knot major
+        int il = 0, it = 0;
+        double fl = 0.0, ft = 0.0, accept = 1.0;
+        if (r.Kl > 1) {
+          r.live.tab(r.live.L[10], r.Kl, p.lam, &il, &fl);
+          accept *= r.live.tab(r.live.L[10], r.Kl, p.lam);
         }
+        if (r.Kt > 1) {
+          V3<R> xl, dl;
+          to_local(s, cr.vol_after, p.x + dist * p.d, p.d, xl, dl);
+          const double th = std::atan2(std::sqrt(val(xl.x) * val(xl.x) + val(xl.y) * val(xl.y)), val(xl.z));
+          r.live.tab(r.live.L[15], r.Kt, th, &it, &ft);
+          accept *= r.live.tab(r.live.L[15], r.Kt, th);
+        }
+        double *cc = &tal[3 * r.S + (long long)sensor * r.K + il * r.Kt + it];
+        cc[0] += v * (1 - fl) * (1 - ft);
+        if (fl > 0) cc[r.Kt] += v * fl * (1 - ft);
+        if (ft > 0) cc[1] += v * (1 - fl) * ft;
+        if (fl > 0 && ft > 0) cc[r.Kt + 1] += v * fl * ft;
         if (r.nt > 0) {  // the time histogram (readout/grid.py:190-225 `_time_weights`, `time_cells`)
-          double dq = v;
-          if (r.K > 1) dq *= r.live.tab(r.live.L[10], r.K, p.lam);  // weighted by the acceptance at the wavelength
+          const double dq = v * accept;  // weighted by the acceptance: the detected light's time shape
           const double x = std::min(std::max((time - r.t0) / r.dt, -4.0), r.nt + 4.0);
           const double jf = std::floor(x), g = x - jf;
           const int j = (int)jf;
@@ -1234,7 +1245,9 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
   r.max_dist = (ctl && ctl->max_distance > 0) ? ctl->max_distance : s.reach;
   r.max_vert = (ctl && ctl->max_vertices > 0) ? ctl->max_vertices : (1 << 22);
   r.S = layout[4];
-  r.K = layout[5];
+  r.Kl = layout[5];
+  r.Kt = layout[14];
+  r.K = r.Kl * r.Kt;
   r.nt = layout[11];
   float t0f, dtf;
   std::memcpy(&t0f, &layout[12], 4);

@@ -84,7 +84,7 @@ class OracleRun:
         ref = None if reference is None else pack.pack(self.det, reference, edge_rows=edge_rows).values
         rows, n_ev = api.source_rows(self.det, source, samples, first_id)
         nt = int(pk.layout[11])
-        stride = 3 * self.S + self.S * int(pk.layout[5]) + (self.S * (nt + 4) if nt else 0)
+        stride = 3 * self.S + self.S * int(pk.layout[5]) * int(pk.layout[14]) + (self.S * (nt + 4) if nt else 0)
         tally = np.zeros((n_ev, stride))
         health = np.zeros(16)
         ctl = (controls or api.Controls()).struct()

@@ -231,3 +231,59 @@ def test_time_histogram_moments_on_device(gpu):
     np.testing.assert_allclose(H.sum(), s, rtol=1e-6)
     for k in (1, 2, 3):
         np.testing.assert_allclose((H * tn**k).sum() / s, T**k, rtol=2e-5)  # float32 times
+
+
+CE_KNOTS = np.array([0.0, 0.5, 1.2, 2.0, 2.6, np.pi])  # rad
+CE_VALUES = np.array([1.0, 0.9, 0.7, 0.5, 0.3, 0.2])
+
+
+def test_theta_axis_against_oracle_and_readout(gpu):
+    """The charge histogram's theta axis (the cathode position angle, hats on the CE knots): a reflecting sensor
+    sphere with absorption traced by the engine and the oracle on the same photons (cells within 1e-4 of the total);
+    readout.pe multiplies CE(theta) in (sum_j cell_j CE_j); the time histogram is weighted by CE."""
+    from lumen_amd.params import Table
+    from lumen_amd.readout import pe, time_masses
+
+    det = sphere_toy(a=1.0, R=0.3, L_abs=2.0, ce=Table(CE_KNOTS, CE_VALUES)).timed(0.0, 0.5, 60)
+    src = L.sources.Point((0.2, -0.1, 0.3), lam=400.0, n_photons=1.0)
+    n = 1 << 18
+    g = L.simulate(det, src, samples=n, seed=4)
+    assert g.charge_axes == (1, len(CE_KNOTS))
+    tal, h = O.OracleRun(det).trace(src, n, seed=4)
+    o = O.light_of(tal, h, 1, len(CE_KNOTS), n)
+    gc = g.charge.cpu().numpy().reshape(-1)
+    assert np.all(np.abs(gc - o["charge"][0]) < 1e-4 * o["arrivals"][0]), (gc, o["charge"][0])
+    assert abs(gc.sum() - float(g.arrivals[0])) < 1e-6 * float(g.arrivals[0])
+    p = pe(g, det.params(), det)
+    want = float(gc @ CE_VALUES.astype(np.float32))
+    assert float(p[0]) == pytest.approx(want, rel=1e-6)
+    H, late, early = time_masses(g, det.params(), det)
+    assert float(H.sum() + late[..., 0].sum() + early[..., 0].sum()) == pytest.approx(want, rel=1e-5)
+    assert float(H.sum()) > 0.9 * want  # the grid holds most of the light
+
+
+def test_theta_axis_adjoint(gpu):
+    """d pe / d(absorption scale) with a CE table: the adjoint's cotangent of each deposit reads its theta cells;
+    against common-random-number central differences."""
+    from lumen_amd.autograd import simulate_diff, table_tensor
+    from lumen_amd.params import Table
+    from lumen_amd.readout import pe
+
+    det = sphere_toy(a=1.0, R=0.3, L_abs=2.0, ce=Table(CE_KNOTS, CE_VALUES))
+    src = L.sources.Point((0.2, -0.1, 0.3), lam=400.0, n_photons=1.0)
+    p0 = det.params()
+    key = "materials/Water/ABSLENGTH"
+    n = 1 << 18
+    t = table_tensor(p0, key)
+    light = simulate_diff(det, src, {key: t}, samples=n, seed=6)
+    pe(light, p0, det).sum().backward()
+    base = np.asarray(p0[key].value, np.float64)
+    g = float((t.grad.cpu().numpy() * base).sum())
+
+    def tot(scale):
+        p = p0.set(key, L.params.Table(p0[key].energy, base * scale))
+        return float(pe(L.simulate(det, src, p, samples=n, seed=6, reference=p0), p, det).sum())
+
+    h = 1e-3
+    fd = (tot(1 + h) - tot(1 - h)) / (2 * h)
+    assert abs(g - fd) <= 0.01 * abs(fd) + 1e-5 and abs(fd) > 1e-2, (g, fd)

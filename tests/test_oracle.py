@@ -360,3 +360,47 @@ def test_time_histogram_overflow_keeps_totals():
     tot = o["time"][0].sum() + o["late"][0, 0] + o["early"][0, 0]
     np.testing.assert_allclose(tot, s, rtol=1e-12)
     assert o["late"][0, 0] != 0
+
+
+CE_KNOTS = np.array([0.0, 0.5, 1.2, 2.0, 2.6, np.pi])  # rad: uneven, both poles included
+CE_VALUES = np.array([1.0, 0.9, 0.7, 0.5, 0.3, 0.2])
+
+
+def _hat_means(knots):
+    """E[hat_j(theta)] and E[hat_j(theta)^2] for theta of an isotropic direction (density sin(theta) / 2)."""
+    th = np.linspace(0.0, np.pi, 200001)
+    w = 0.5 * np.sin(th)
+    m1, m2 = [], []
+    for j in range(len(knots)):
+        h = np.interp(th, knots, np.eye(len(knots))[j])
+        m1.append(np.trapezoid(h * w, th))
+        m2.append(np.trapezoid(h * h * w, th))
+    return np.array(m1), np.array(m2)
+
+
+def test_charge_cells_on_theta_knots_closed_form():
+    """The charge histogram's theta axis (the cathode position angle): a centred isotropic source in an absorbing,
+    detecting sensor sphere puts each photon's unit deposit at the polar angle of its direction, so cell j holds
+    N E[hat_j(theta)] with theta of density sin(theta) / 2 (quadrature), and sum_j cell_j CE_j = N E[CE(theta)];
+    the time histogram is weighted by CE(theta) (its total is N E[CE])."""
+    from lumen_amd.params import Table
+    from toys import sphere_toy
+
+    det = sphere_toy(a=1.0, R=0.0, L_abs=None, ce=Table(CE_KNOTS, CE_VALUES)).timed(0.0, 0.5, 40)
+    n = 1 << 17
+    tal, h = O.OracleRun(det).trace(L.sources.Point((0, 0, 0), lam=400.0, n_photons=n), n, seed=21)
+    light = O.light_of(tal, h, 1, len(CE_KNOTS), n)
+    cells = light["charge"][0]
+    m1, m2 = _hat_means(CE_KNOTS)
+    z = (cells - n * m1) / np.sqrt(n * (m2 - m1**2))
+    assert np.all(np.abs(z) < 5.0), z
+    assert abs(cells.sum() - n) < 1e-6 * n  # the hats sum to one
+    pe_ce = float(cells @ CE_VALUES)
+    want = n * float(m1 @ CE_VALUES)
+    assert abs(pe_ce - want) < 5.0 * np.sqrt(n * 0.1), (pe_ce, want)
+    time_total = light["time"][0].sum() + light["late"][0, 0] + light["early"][0, 0]
+    ce32 = CE_VALUES.astype(np.float32).astype(np.float64)  # the packed (float32) values the time weighting reads
+    assert time_total == pytest.approx(float(cells @ ce32), rel=1e-9)
+    # a wrong axis (the azimuth instead of the polar angle) fails the same check
+    mz, _ = _hat_means(CE_KNOTS * 2.0)
+    assert np.any(np.abs((cells - n * mz) / np.sqrt(n * (m2 + 1e-3))) > 10.0)

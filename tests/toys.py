@@ -21,11 +21,12 @@ SPEC_S = np.array([0.0, 1.0, 0.6, 0.0])
 
 
 def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n=1.33, eff=1.0, L_wls=None,
-               wls_yield=0.7, wls_tau=0.0, wls_profile="delta"):
+               wls_yield=0.7, wls_tau=0.0, wls_profile="delta", ce=None):
     """Return a water sphere (radius a) inside a sensor shell: the crossing out of the water is ``surface``
     (SPECULAR_METAL: R mirror; PAINTED: R Lambertian) detecting the absorbed share with ``eff``.  With ``L_wls``
     the water holds a wavelength shifter absorbing above 3 eV with mean free path L_wls (m) and emitting on
-    2.2-2.8 eV with yield ``wls_yield``."""  # noqa: D205, D209
+    2.2-2.8 eV with yield ``wls_yield``.  ``ce``: a Table of the shell's collection efficiency over the cathode
+    position angle (rad, the polar angle of the crossing point about +z)."""  # noqa: D205, D209
     kinds = [solids.orb(a + 0.05, name="shell"), solids.orb(a, name="water")]
     vols = [navarrays.Volume("shell", 0, np.eye(3), np.zeros(3), -1, 1),
             navarrays.Volume("shell/water", 1, np.eye(3), np.zeros(3), 0, 0)]
@@ -57,9 +58,13 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
     row_of[0, 1] = 0
     row_of[0, 2] = 1
     sensor_of_vol = np.array([0, -1, -1], np.int32)
+    ce_key = None
+    if ce is not None:
+        values["sensors/pmt/ce"] = ce
+        ce_key = "sensors/pmt/ce"
     return Detector("sphere_toy", nav, [Medium("Water", "materials/Water/RINDEX", absorption, rayleigh, **wls)],
                     np.array([0, -1, -1], np.int32), ["Water", "Shell", "OUT"], rows, row_of, sensor_of_vol,
-                    ["shell"], [SensorGroup("pmt", np.arange(1))], Params(values))
+                    ["shell"], [SensorGroup("pmt", np.arange(1), ce=ce_key)], Params(values))
 
 
 MIE_CASES = {"A": (0.6, 0.3, 0.7), "B": (0.0, 0.8, 0.5), "water": (0.95, 0.3, 1.0)}  # (g_f, g_b, r)
