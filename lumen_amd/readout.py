@@ -9,6 +9,9 @@ efficiency are plain torch autograd: they act after tracking.
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+from typing import Any
+
 import numpy as np
 
 from .params import Params, Table
@@ -71,11 +74,13 @@ def time_masses(light, params: Params, det=None, group: int = 0):
     return H * scale[..., None], late * scale[..., None], early * scale[..., None]
 
 
-def pe_density(light, params: Params, gate, bin_width: float, tts_sigma: float, det=None, group: int = 0):
+def pe_density(light, params: Params, gate, bin_width: float, tts_sigma: float, det=None, group: int = 0,
+               dark_rate=0.0):
     """Return the expected photoelectrons per time bin (S, n_bins) over ``gate`` = (g0, g1) ns, bins of
     ``bin_width``: the node masses (``time_masses``), each spread by a Gaussian transit-time spread of width
     ``tts_sigma`` ns and integrated over the bins (its CDF differences); the overflow masses at their mean times
-    (``readout/detection.py`` ``density`` with a Gaussian ``spread``).  ``light.provenance`` carries the grid."""  # noqa: D205, D209
+    (``readout/detection.py`` ``density`` with a Gaussian ``spread``); plus the dark counts, ``dark_rate`` Hz (per
+    sensor or one for all) times the bin width.  ``light.provenance`` carries the grid."""  # noqa: D205, D209
     import math
 
     import torch
@@ -102,4 +107,99 @@ def pe_density(light, params: Params, gate, bin_width: float, tts_sigma: float, 
         q, qt = ov[..., 0], ov[..., 1]
         tm = torch.where(q != 0.0, qt / torch.where(q != 0.0, q, torch.ones_like(q)), torch.zeros_like(q))
         out = out + q[..., None] * spread(tm[..., None])[..., 0, :]
-    return out
+    dark = torch.as_tensor(np.asarray(dark_rate, np.float64), device=dev, dtype=dtype)
+    if dark.ndim > 0:
+        dark = dark[..., None]
+    return out + dark * (bin_width * 1e-9)
+
+
+def bigauss(t, t_peak: float, sigma_rise: float, sigma_fall: float):
+    """The single-photoelectron pulse of two half-Gaussians about ``t_peak`` (ns), unit peak (the reference's
+    ``sensors/components.py`` ``BiGaussPulse``)."""  # noqa: D205, D209
+    import torch
+
+    d = t - t_peak
+    s = torch.where(d < 0.0, torch.full_like(d, sigma_rise), torch.full_like(d, sigma_fall))
+    return torch.exp(-0.5 * (d / s) ** 2)
+
+
+@dataclass(frozen=True)
+class Waveform:
+    """The expected waveform: ``mean`` and ``variance`` (S, n), sample k at t0 + k dt (ns)."""
+
+    mean: Any
+    variance: Any
+    t0: float
+    dt: float
+
+
+PULSE_SUB = 4  # points per bin averaging the pulse
+PULSE_SPREAD = 6.0  # the pulse's support: t_peak - 6 sigma_rise to t_peak + 6 sigma_fall
+
+
+def expected_waveform(light, params: Params, gate, sampling: float, tts_sigma: float, pulse: dict, *, gain=1.0,
+                      charge_moments=(1.0, 1.0), baseline: float = 0.0, noise: float = 0.0, dark_rate=0.0,
+                      steps_per_sample: int = 4, det=None, group: int = 0) -> Waveform:
+    """Return the expected waveform per sensor, sampled every ``sampling`` ns over ``gate`` (the reference's
+    ``readout/waveform.py`` ``expected_waveform``):
+
+        E[V_s(t)]   = baseline + gain_s m1 (rho_s * <h>)(t)
+        Var[V_s(t)] = gain_s^2 m2 (rho_s * <h^2>)(t) + noise^2      (Campbell's theorem plus white noise)
+
+    rho the photoelectron density per bin (``pe_density`` with the dark counts) on bins of sampling /
+    ``steps_per_sample`` ns, <h> the pulse (``pulse`` = dict(t_peak, sigma_rise, sigma_fall), ``bigauss``) averaged
+    over each bin, (m1, m2) the single-photoelectron charge's mean and mean square (``charge_moments``).  Every step
+    is linear after the time histogram, so torch differentiates it exactly.  Fold n is the expectation at
+    g0 + n dt (a photoelectron uniform in bin m is lag (n - m - 1, n - m] bins back: the pulse's bin average); sample
+    k is fold k * steps_per_sample, at g0 + k sampling."""  # noqa: D205, D209
+    import math
+
+    import torch
+
+    if steps_per_sample < 1 or not sampling > 0:
+        raise ValueError("expected_waveform: sampling > 0 and steps_per_sample >= 1")
+    dt = sampling / steps_per_sample
+    rho = pe_density(light, params, gate, dt, tts_sigma, det, group, dark_rate)  # (S, nb)
+    dev, dtype = rho.device, rho.dtype
+    lo = pulse["t_peak"] - PULSE_SPREAD * pulse["sigma_rise"]
+    hi = pulse["t_peak"] + PULSE_SPREAD * pulse["sigma_fall"]
+    j0, j1 = int(math.floor(lo / dt)), int(math.ceil(hi / dt)) + 1
+    j = torch.arange(j0, j1 + 1, device=dev, dtype=dtype)
+    sub = (torch.arange(PULSE_SUB, device=dev, dtype=dtype) + 0.5) / PULSE_SUB
+    t = (j[:, None] - 1.0 + sub[None, :]) * dt  # bin j of the lag: [(j - 1) dt, j dt]
+    h = bigauss(t, pulse["t_peak"], pulse["sigma_rise"], pulse["sigma_fall"])
+    k1, k2 = h.mean(-1), (h * h).mean(-1)
+
+    def fold(k):  # W_n = sum_j k_j rho_{n - j}, lags j0..j1 (rho zero outside the gate)
+        nb = rho.shape[-1]
+        out = torch.zeros_like(rho)
+        for i, lag in enumerate(range(j0, j1 + 1)):
+            if lag >= nb or lag <= -nb:
+                continue
+            if lag >= 0:
+                out[..., lag:] += k[i] * rho[..., :nb - lag]
+            else:
+                out[..., :nb + lag] += k[i] * rho[..., -lag:]
+        return out
+
+    m1, m2 = charge_moments
+    g = torch.as_tensor(np.asarray(gain, np.float64), device=dev, dtype=dtype)
+    if g.ndim > 0:
+        g = g[..., None]
+    idx = torch.arange(0, rho.shape[-1], steps_per_sample, device=dev)
+    mean = baseline + g * m1 * fold(k1)[..., idx]
+    var = g * g * m2 * fold(k2)[..., idx] + noise * noise
+    return Waveform(mean, var, float(gate[0]), float(sampling))
+
+
+def waveform_nll(w: Waveform, data):
+    """Return the Gaussian negative log-likelihood of measured waveforms under the expected one (the reference's
+    ``readout/waveform.py`` ``waveform_nll``; an approximation at low occupancy)."""  # noqa: D205, D209
+    import math
+
+    import torch
+
+    d = torch.as_tensor(data, dtype=w.mean.dtype, device=w.mean.device)
+    if d.shape != w.mean.shape:
+        raise ValueError(f"waveform_nll: data of shape {tuple(d.shape)}, the expected waveform is {tuple(w.mean.shape)}")
+    return 0.5 * torch.sum((d - w.mean) ** 2 / w.variance + torch.log(2.0 * math.pi * w.variance))

@@ -287,3 +287,25 @@ def test_theta_axis_adjoint(gpu):
     h = 1e-3
     fd = (tot(1 + h) - tot(1 - h)) / (2 * h)
     assert abs(g - fd) <= 0.01 * abs(fd) + 1e-5 and abs(fd) > 1e-2, (g, fd)
+
+
+def test_expected_waveform_of_a_simulated_light(gpu):
+    """The expected waveform of a traced light (the sensor sphere, times on a grid): its integral over the gate is
+    gain m1 pe times the pulse's area sqrt(pi / 2)(sigma_rise + sigma_fall), and it peaks t_peak after the light."""
+    from lumen_amd.readout import expected_waveform, pe
+
+    a, n_idx = 1.0, 1.33
+    det = sphere_toy(a=a, R=0.0, L_abs=None, n=n_idx).timed(0.0, 0.25, 200)
+    light = L.simulate(det, L.sources.Point((0, 0, 0), lam=400.0, n_photons=1000.0), samples=1 << 16, seed=3)
+    pulse = dict(t_peak=8.0, sigma_rise=1.5, sigma_fall=4.0)
+    w = expected_waveform(light, det.params(), (0.0, 60.0), 0.5, 1.0, pulse, gain=2.0, charge_moments=(1.2, 1.6),
+                          baseline=10.0, det=det)
+    area = float((w.mean[0] - 10.0).sum()) * w.dt
+    want = 2.0 * 1.2 * float(pe(light, det.params(), det)[0]) * math.sqrt(math.pi / 2) * (1.5 + 4.0)
+    assert area == pytest.approx(want, rel=2e-3)
+    t_peak = w.t0 + w.dt * int(torch_argmax(w.mean[0]))
+    assert abs(t_peak - (a * n_idx / units.C_VACUUM + 8.0)) < 1.0
+
+
+def torch_argmax(x):
+    return int(x.argmax())
