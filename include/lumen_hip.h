@@ -113,13 +113,21 @@ typedef struct {
   int32_t blocks_per_cu; /* resident blocks of 256 per CU of the persistent grid (0: as many as fit) */
   int32_t tally_global;  /* 1: accumulate the tallies with global atomics even where they fit in LDS; else 0 */
   int32_t nodes_global;  /* 1: read the BVH nodes from global memory even where they fit in LDS; else 0 */
-  int32_t reserved;      /* 0 */
+  int32_t jump;          /* 1: the diffusion jump where the model holds a clearance grid (lh_model_jump); else 0 */
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.
  * Replaces the reference's Model + Operands build (src/lumen/transport/step.py:75-143). */
 int lh_model_create(const lh_geometry *geometry, const int32_t *layout, int32_t layout_words, lh_model *out);
 int lh_model_destroy(lh_model model);
+
+/* The diffusion jump's clearance grid of medium `medium` (lumen_amd/detector/clearance.py): nodes lo + h i for
+ * 0 <= i < n on each axis, `clear` (host, n[0] n[1] n[2] float32, index (i0 n[1] + i1) n[2] + i2) a proven lower
+ * bound of the distance from each node to every surface.  A medium whose only channel is Rayleigh: at a Rayleigh
+ * event in a launch whose controls set `jump`, the photon walks its own scatterings inside the ball of the clearance
+ * at its position with no geometry query (the reference's src/lumen/transport/fastpath.py `diffuse`, with the grids
+ * of its `prepare`, src/lumen/geometry/clearance.py).  Replaces an earlier grid of the medium; allocates. */
+int lh_model_jump(lh_model model, int32_t medium, const float lo[3], float h, const int32_t n[3], const float *clear);
 
 /* Query the model: number of sensors, of charge cells per sensor (the QE wavelength knots times the CE theta knots,
  * the layout's words 5 and 14) and tally doubles per event. */

@@ -31,7 +31,9 @@ class Controls(NamedTuple):
     ``roulette`` (epsilon > 0), ``max_vertices`` (the guard), ``chunk`` (photons per work-queue claim; 0 the
     engine's default), ``blocks_per_cu`` (the persistent grid's resident blocks per CU; 0 as many as fit),
     ``tally_global`` (accumulate the tallies with global atomics even where a block's copy fits in LDS),
-    ``nodes_global`` (read the BVH nodes from global memory even where they fit in the block's LDS)."""  # noqa: D205, D209
+    ``nodes_global`` (read the BVH nodes from global memory even where they fit in the block's LDS), ``jump`` (the
+    diffusion jump in Rayleigh-only media, ``transport/fastpath.py``: the reference's ``StepConfig(jump=True)``; its
+    clearance grids are built on first use, ``lumen_amd.detector.clearance``)."""  # noqa: D205, D209
 
     max_distance: Any = None
     roulette: float = ROULETTE
@@ -40,13 +42,14 @@ class Controls(NamedTuple):
     blocks_per_cu: int = 0
     tally_global: bool = False
     nodes_global: bool = False
+    jump: bool = False
 
     def struct(self) -> _lib.LhControls:
         if not (float(self.roulette) > 0.0):
             raise ValueError(f"Controls: the roulette is a number > 0, got {self.roulette!r}")
         return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
                                int(self.max_vertices), int(self.chunk), int(self.blocks_per_cu),
-                               int(bool(self.tally_global)), int(bool(self.nodes_global)), 0)
+                               int(bool(self.tally_global)), int(bool(self.nodes_global)), int(bool(self.jump)))
 
 
 class Health(NamedTuple):
@@ -141,6 +144,22 @@ class Engine:
         if self.stride != 3 * self.S + self.S * self.K + (self.S * (self.nt + 4) if self.nt else 0):
             raise _lib.EngineError(f"the engine's tally layout ({self.stride} cells per event) is not the detector's")
         weakref.finalize(self, _lib.lib().lh_model_destroy, h)
+        self.jump_grids = None  # the diffusion jump's clearance grids, built on first use
+
+    def ensure_jump(self):
+        """Build and upload the diffusion jump's clearance grids (once per engine); return them."""
+        if self.jump_grids is None:
+            from .detector.clearance import jump_grids
+
+            self.jump_grids = jump_grids(self.det)
+            for g in self.jump_grids:
+                lo = (_lib.C.c_float * 3)(*[float(x) for x in g.lo])
+                n = (_lib.C.c_int32 * 3)(*[int(x) for x in g.n])
+                clear = np.ascontiguousarray(g.clear, np.float32)
+                _lib.check(_lib.lib().lh_model_jump(self.handle, int(g.medium), lo, float(g.h), n,
+                                                    clear.ctypes.data_as(_lib.C.POINTER(_lib.C.c_float))),
+                           "lh_model_jump")
+        return self.jump_grids
 
     def last_launch(self) -> dict:
         """Return the shape of the last launch on this model (grid, residency, LDS use, kernel features)."""
@@ -259,6 +278,8 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     health = torch.zeros(_lib.HEALTH_N, dtype=torch.float64, device=eng.device)
     st = torch.cuda.current_stream(eng.device) if stream is None else stream
     ctl = controls.struct()
+    if controls.jump:
+        eng.ensure_jump()
     _lib.check(_lib.lib().lh_trace(eng.handle, _lib.C.c_void_p(values.data_ptr()),
                                    None if ref_values is None else _lib.C.c_void_p(ref_values.data_ptr()),
                                    int(values.numel()), rows, len(rows),

@@ -127,6 +127,8 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
         _PL = _function()
     eng = engine(det, device)
     controls = Controls() if controls is None else controls
+    if controls.jump:
+        eng.ensure_jump()
     p = det.params() if params is None else params
     keys = list(params_t)
     for k in keys:

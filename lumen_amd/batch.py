@@ -55,6 +55,8 @@ class BatchRun:
         self.graph = graph
         self.controls = controls if controls is not None else (
             tuned_controls(self, "forward") if tune else Controls())
+        if self.controls.jump:
+            self.eng.ensure_jump()
         self._g_fwd = self._g_both = None
         self._loss = None
 

@@ -94,6 +94,9 @@ struct Args {
   unsigned long long *__restrict__ queue;
   const float *__restrict__ ut;  // the unified facet tables (unified_tables.cpp)
   int lds_tally;                 // the block's tallies live in LDS (flushed once at exit), else global atomics
+  // the diffusion jump (null: off): per medium (lo, h) and (n, offset into jclear, -1 no grid) as float4 pairs
+  const float4 *__restrict__ jrec;
+  const float *__restrict__ jclear;
 };
 
 struct Photon {
@@ -439,6 +442,62 @@ __device__ inline float deposit_cotangent(const Args &a, const AdjArgs *b, const
 
 __device__ __forceinline__ float ratio(float live, float ref) { return ref != 0.0f ? live / ref : 0.0f; }
 
+// ------------------------------------------------------------------------------------------- the diffusion jump
+// At a Rayleigh event in a medium with a clearance grid, with a linear polarisation (ei = 0): the photon walks its
+// own scatterings inside the ball B(x, c) of the proven clearance c at x (the grid's nearest node less the distance
+// to it), with no geometry query: free paths Exp(mu_ref), the polarised Rayleigh draw and projection, until a flight
+// leaves the ball (the photon stops at the crossing) or the walk has made ceil((c mu_ref)^2) scatterings
+// (fastpath.py:166-224 `diffuse`).  Its uniforms are the photon's own, vertex word with the top bit set, block the
+// walk's step: no other draw uses such a counter.
+struct Jump {
+  float3 y, d, e;  // the displacement, the direction and the polarisation after the walk
+  float L;         // the path length (0: no walk)
+  int n;           // the scatterings
+};
+
+__device__ inline Jump diffuse(const Args &a, const Photon &p, const lh_rng &rng, int medium, float mu_r) {
+  Jump J{mk3(0.f, 0.f, 0.f), p.d, p.er, 0.0f, 0};
+  if (medium < 0 || !(mu_r > 0.0f) || p.ei.x != 0.0f || p.ei.y != 0.0f || p.ei.z != 0.0f) return J;
+  const float4 G0 = a.jrec[2 * medium], G1 = a.jrec[2 * medium + 1];
+  const int off = __float_as_int(G1.w);
+  if (off < 0) return J;
+  const int n0 = __float_as_int(G1.x), n1 = __float_as_int(G1.y), n2 = __float_as_int(G1.z);
+  const float fx = (p.x.x - G0.x) / G0.w, fy = (p.x.y - G0.y) / G0.w, fz = (p.x.z - G0.z) / G0.w;
+  if (!(fx > -0.5f && fx < (float)n0 - 0.5f && fy > -0.5f && fy < (float)n1 - 0.5f && fz > -0.5f &&
+        fz < (float)n2 - 0.5f))
+    return J;
+  const int i0 = min(max((int)floorf(fx + 0.5f), 0), n0 - 1), i1 = min(max((int)floorf(fy + 0.5f), 0), n1 - 1),
+            i2 = min(max((int)floorf(fz + 0.5f), 0), n2 - 1);
+  const float3 node = mk3(fmaf(G0.w, (float)i0, G0.x), fmaf(G0.w, (float)i1, G0.y), fmaf(G0.w, (float)i2, G0.z));
+  const float3 dx = sub3(p.x, node);
+  const float c = a.jclear[off + ((long long)i0 * n1 + i1) * n2 + i2] - sqrtf(dot3(dx, dx));
+  if (!(c > 0.0f)) return J;
+  const float cm = c * mu_r;
+  const int budget = (int)ceilf(fminf(cm * cm, 1073741824.0f));
+  const float3 zero = mk3(0.f, 0.f, 0.f);
+  for (int j = 0; j < budget; ++j) {
+    float u[4];
+    rng.draw4((uint32_t)p.vertex | 0x80000000u, (uint32_t)j, u);
+    const float s = -__logf(u[0]) / mu_r;
+    const float by = dot3(J.y, J.d);
+    const float t = -by + sqrtf(fmaxf(by * by - (dot3(J.y, J.y) - c * c), 0.0f));  // the flight's exit of the ball
+    if (s >= t) {
+      J.y = fma3(t, J.d, J.y);
+      J.L += t;
+      break;
+    }
+    float3 e2 = J.e, z = zero;
+    const float3 k = rayleigh_sample(J.e, zero, u[1], u[2], u[3]);
+    pol_project(e2, z, k);
+    J.y = fma3(s, J.d, J.y);
+    J.L += s;
+    J.d = unit3(k);
+    J.e = e2;
+    J.n += 1;
+  }
+  return J;
+}
+
 // the derivative terms of the rates over one segment: k_c d(mu_c)/d(value) v per rate (absorption, Rayleigh, Mie,
 // WLS), through the photon's cached table derivatives
 template <bool CHAN>
@@ -621,6 +680,24 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       p.d = unit3(dn);
       p.er = er0;
       p.ei = ei0;
+      if (ch == 0 && a.jrec) {  // the diffusion jump (fastpath.py `diffuse`), in a medium with a clearance grid
+        const Jump J = diffuse(a, p, rng, class_medium(a, p.cls), mr.mu_ray);
+        if (J.L > 0.0f) {
+          // the walk's likelihood ratio and absorption: exp(-mu_abs L) (mu / mu_ref)^n exp(-(mu - mu_ref) L)
+          const float wj = __expf(-ml.mu_abs * J.L) *
+                           (REF ? __expf((float)J.n * __logf(ml.mu_ray / mr.mu_ray) - (ml.mu_ray - mr.mu_ray) * J.L)
+                                : 1.0f);
+          acc.bulk += w_new * (1.0f - wj) * p.w0;  // the walk's factor is its path's: absorbed
+          w_new *= wj;
+          wr_new *= REF ? __expf(-mr.mu_abs * J.L) : __expf(-ml.mu_abs * J.L);
+          if (adj_event) rates_add<CHAN>(sg, p, -J.L, (float)J.n / ml.mu_ray - J.L, 0.0f, 0.0f, ahead);
+          p.x = add3(p.x, J.y);
+          p.d = J.d;
+          p.er = J.e;
+          p.t += J.L / v_g;
+          p.srf = -1;
+        }
+      }
     } else {  // a wavelength shifter (G4OpWLS): re-emitted isotropic, unpolarised, after the record's delay
       float ur[4];
       rng.draw4(p.vertex, 2, ur);  // (wavelength, delay, -, -)
@@ -941,6 +1018,12 @@ struct lh_model_s {
   int *layout = nullptr;
   int layout_words = 0;
   int S = 0, K = 1, Kl = 1, Kt = 1, n_media = 0;  // K = Kl wavelength knots x Kt theta knots
+  // the diffusion jump's grids: per medium two float4 records, the bounds of every grid in one buffer
+  std::vector<float4> jrec_h;
+  std::vector<float> jclear_h;
+  std::vector<int32_t> layout_host;
+  float4 *jrec = nullptr;
+  float *jclear = nullptr;
   long long tally_stride = 0;
   int NT = 0;  // the time histogram's nodes t0 + j dt (ns), j < NT (0: none)
   float t0 = 0.0f, dt = 1.0f;
@@ -1268,6 +1351,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
       if (layout[layout[6] + MEDIUM_WORDS * mi + 8] & (MED_WLS | MED_MIE)) m->has_chan = true;
   }
   m->n_media = layout[1];
+  m->layout_host.assign(layout, layout + layout_words);
   m->S = layout[4];
   m->Kl = layout[5];
   m->Kt = layout[14];
@@ -1307,9 +1391,52 @@ LH_API int lh_model_destroy(lh_model m) {
   (void)hipFree(m->src);
   (void)hipFree(m->src_end);
   (void)hipFree(m->ut);
+  (void)hipFree(m->jrec);
+  (void)hipFree(m->jclear);
   if (m->ev0) (void)hipEventDestroy(m->ev0);
   if (m->ev1) (void)hipEventDestroy(m->ev1);
   delete m;
+  return LH_OK;
+}
+
+LH_API int lh_model_jump(lh_model m, int32_t medium, const float lo[3], float h, const int32_t n[3],
+                         const float *clear) {
+  if (!m || !lo || !n || !clear) return fail(LH_ERR_ARG, "lh_model_jump: null argument");
+  if (medium < 0 || medium >= m->n_media) return fail(LH_ERR_ARG, "lh_model_jump: medium out of range");
+  const int32_t *M = m->layout_host.data() + m->layout_host[L_MEDIA_AT] + MEDIUM_WORDS * medium;
+  if ((M[8] & (MED_OPAQUE | MED_MIE | MED_WLS)) || M[7] < 1)
+    return fail(LH_ERR_ARG, "lh_model_jump: the medium's only channel must be Rayleigh");
+  if (!(h > 0.0f) || !std::isfinite(h) || !std::isfinite(lo[0]) || !std::isfinite(lo[1]) || !std::isfinite(lo[2]))
+    return fail(LH_ERR_ARG, "lh_model_jump: a grid needs a spacing h > 0 and a finite origin");
+  const long long nn = (long long)n[0] * n[1] * n[2];
+  if (n[0] < 1 || n[1] < 1 || n[2] < 1 || nn > (1LL << 26)) return fail(LH_ERR_ARG, "lh_model_jump: grid size out of range");
+  for (long long i = 0; i < nn; ++i)
+    if (!(clear[i] >= 0.0f) || !std::isfinite(clear[i])) return fail(LH_ERR_ARG, "lh_model_jump: a bound is negative or not finite");
+  auto bits = [](int i) {
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+  };
+  if (m->jrec_h.empty())  // every medium without a grid: offset -1
+    for (int k = 0; k < m->n_media; ++k) {
+      m->jrec_h.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+      m->jrec_h.push_back(make_float4(0.f, 0.f, 0.f, bits(-1)));
+    }
+  if ((long long)m->jclear_h.size() + nn > (1LL << 28)) return fail(LH_ERR_ARG, "lh_model_jump: the grids exceed 1 GiB");
+  const float ob = bits((int)m->jclear_h.size());
+  float nb[3];
+  memcpy(nb, n, 12);
+  m->jrec_h[2 * medium] = make_float4(lo[0], lo[1], lo[2], h);
+  m->jrec_h[2 * medium + 1] = make_float4(nb[0], nb[1], nb[2], ob);
+  m->jclear_h.insert(m->jclear_h.end(), clear, clear + nn);
+  (void)hipFree(m->jrec);
+  (void)hipFree(m->jclear);
+  m->jrec = nullptr;
+  m->jclear = nullptr;
+  HIP_TRY(hipMalloc((void **)&m->jrec, m->jrec_h.size() * sizeof(float4)));
+  HIP_TRY(hipMemcpy(m->jrec, m->jrec_h.data(), m->jrec_h.size() * sizeof(float4), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc((void **)&m->jclear, m->jclear_h.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(m->jclear, m->jclear_h.data(), m->jclear_h.size() * sizeof(float), hipMemcpyHostToDevice));
   return LH_OK;
 }
 
@@ -1403,6 +1530,8 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.lds_tally = 0;
   a.src = m->src;
   a.src_end = m->src_end;
+  a.jrec = c.jump && m->jrec ? m->jrec : nullptr;
+  a.jclear = m->jclear;
   return LH_OK;
 }
 

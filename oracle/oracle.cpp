@@ -777,9 +777,35 @@ struct Run {
   int S, K, Kl, Kt;  // sensors; charge cells per sensor = Kl wavelength (QE) knots x Kt theta (CE) knots
   long long stride;
   int nt;  // the time histogram's nodes t0 + j dt (0: none), its cells from time_at of each event's tally
+  bool jump;  // the diffusion jump where a medium has a grid
   double t0, dt;
   long long time_at;
 };
+
+// the diffusion jump's clearance grids by medium (oracle_set_jump): nodes lo + h i, bounds clear (float32)
+struct JumpGridO {
+  double lo[3], h;
+  int n[3];
+  std::vector<float> clear;
+};
+static std::vector<JumpGridO> g_jump;
+inline double jump_clearance(int medium, V3<double> x) {  // the nearest node's bound less the distance to it
+  if (medium < 0 || medium >= (int)g_jump.size() || g_jump[medium].clear.empty()) return 0.0;
+  const JumpGridO &G = g_jump[medium];
+  const double f[3] = {(x.x - G.lo[0]) / G.h, (x.y - G.lo[1]) / G.h, (x.z - G.lo[2]) / G.h};
+  int i[3];
+  double d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    if (!(f[a] > -0.5 && f[a] < G.n[a] - 0.5)) return 0.0;
+    i[a] = std::min(std::max((int)std::floor(f[a] + 0.5), 0), G.n[a] - 1);
+    const double dd = (a == 0 ? x.x : a == 1 ? x.y : x.z) - (G.lo[a] + G.h * i[a]);
+    d2 += dd * dd;
+  }
+  return (double)G.clear[((size_t)i[0] * G.n[1] + i[1]) * G.n[2] + i[2]] - std::sqrt(d2);
+}
+template <class R> inline double jump_clearance(int medium, V3<R> x) {
+  return jump_clearance(medium, v3(val(x.x), val(x.y), val(x.z)));
+}
 
 struct Tally {
   std::vector<double> t;   // n_events * stride
@@ -1108,6 +1134,47 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
       p.d = dn;
       p.er = er0;
       p.ei = ei0;
+      if (ch == 0 && r.jump) {  // the diffusion jump (transport/fastpath.py:166-224 `diffuse`)
+        const double c = jump_clearance(m, p.x);
+        const double mur = val(mr.mu_ray);
+        if (c > 0.0 && mur > 0.0 && val(dot(p.ei, p.ei)) == 0.0) {
+          const double budget = std::ceil(std::min(c * c * mur * mur, 1073741824.0));
+          V3<R> y = v3<R>(R(0.0), R(0.0), R(0.0)), d = p.d, ee = p.er;
+          const V3<R> zero = v3<R>(R(0.0), R(0.0), R(0.0));
+          R L = R(0.0);
+          int nsc = 0;
+          for (double j = 0; j < budget; j += 1.0) {
+            double u4[4];
+            p.rng.draw4((uint32_t)p.vertex | 0x80000000u, (uint32_t)j, u4);
+            const R sj = R(-std::log(u4[0]) / mur);
+            const R by = dot(y, d);
+            const R te = -by + SQRT(MAX<R>(by * by - (dot(y, y) - R(c * c)), R(0.0)));  // leaves the ball
+            if (!(sj < te)) {
+              y = y + te * d;
+              L = L + te;
+              break;
+            }
+            const double ul3[4] = {u4[1], u4[2], u4[3], 0.0};
+            const V3<R> k = rayleigh_dir(ee, zero, ul3);
+            V3<R> e2 = ee - dot(k, ee) * k, z2 = zero;
+            pol_normalise(e2, z2);
+            y = y + sj * d;
+            L = L + sj;
+            d = k;
+            ee = e2;
+            ++nsc;
+          }
+          const R wj = EXP(-ml.mu_abs * L) * EXP(R((double)nsc) * LOG(ml.mu_ray / mr.mu_ray) - (ml.mu_ray - mr.mu_ray) * L);
+          T.h[LH_H_ABSORBED_BULK] += val(w_new * (R(1.0) - wj)) * p.w0;
+          w_new = w_new * wj;
+          wr_new = wr_new * EXP(-mr.mu_abs * L);
+          p.x = p.x + y;
+          p.d = d;
+          p.er = ee;
+          p.t = p.t + L / v_g;
+          if (val(L) > 0.0) p.srf = -1;
+        }
+      }
     } else {  // G4OpWLS: isotropic, unpolarised, at the record's wavelength after its delay
       double ur[4];
       p.rng.draw4(p.vertex, 2, ur);
@@ -1244,6 +1311,7 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
   r.roulette = ctl ? ctl->roulette : 0.01;
   r.max_dist = (ctl && ctl->max_distance > 0) ? ctl->max_distance : s.reach;
   r.max_vert = (ctl && ctl->max_vertices > 0) ? ctl->max_vertices : (1 << 22);
+  r.jump = ctl && ctl->jump;
   r.S = layout[4];
   r.Kl = layout[5];
   r.Kt = layout[14];
@@ -1373,6 +1441,19 @@ ORC_API void oracle_lobes(const double *er, const double *ei, long long n, uint6
     ray[3 * i] = r.x, ray[3 * i + 1] = r.y, ray[3 * i + 2] = r.z;
     lam[3 * i] = l.x, lam[3 * i + 1] = l.y, lam[3 * i + 2] = l.z;
   }
+}
+
+// the diffusion jump's clearance grid of medium m (clear: n[0] n[1] n[2] float32, x-major); m < 0 clears them all
+ORC_API void oracle_set_jump(int m, const double *lo, double h, const int *n, const float *clear) {
+  if (m < 0) {
+    g_jump.clear();
+    return;
+  }
+  if ((int)g_jump.size() <= m) g_jump.resize(m + 1);
+  JumpGridO &G = g_jump[m];
+  for (int a = 0; a < 3; ++a) G.lo[a] = lo[a], G.n[a] = n[a];
+  G.h = h;
+  G.clear.assign(clear, clear + (size_t)n[0] * n[1] * n[2]);
 }
 
 // n Mie directions about wi drawn at the constants (gf, gb, r) under the cover of r, and the weight of the live

@@ -52,6 +52,8 @@ def lib():
         L.oracle_wls_emit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_longlong,
                                       C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_mie.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.c_longlong, C.c_uint64, C.c_void_p]
+        L.oracle_set_jump.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        L.oracle_set_jump.restype = None
         L.oracle_mie.restype = None
         L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
         L.oracle_mie_pdf.restype = None
@@ -87,7 +89,19 @@ class OracleRun:
         stride = 3 * self.S + self.S * int(pk.layout[5]) * int(pk.layout[14]) + (self.S * (nt + 4) if nt else 0)
         tally = np.zeros((n_ev, stride))
         health = np.zeros(16)
-        ctl = (controls or api.Controls()).struct()
+        controls = controls or api.Controls()
+        ctl = controls.struct()
+        lib().oracle_set_jump(-1, None, 0.0, None, None)
+        if controls.jump:  # the same clearance grids the engine reads
+            from lumen_amd.detector.clearance import jump_grids
+
+            if getattr(self, "_jump", None) is None:
+                self._jump = jump_grids(self.det)
+            for g in self._jump:
+                lo = np.ascontiguousarray(g.lo, np.float64)
+                n = np.ascontiguousarray(g.n, np.int32)
+                c = np.ascontiguousarray(g.clear, np.float32)
+                lib().oracle_set_jump(int(g.medium), _p(lo), float(g.h), _p(n), _p(c))
         vals = np.ascontiguousarray(pk.values, np.float32)
         lay = np.ascontiguousarray(pk.layout, np.int32)
         refv = None if ref is None else np.ascontiguousarray(ref, np.float32)

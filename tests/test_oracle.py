@@ -404,3 +404,41 @@ def test_charge_cells_on_theta_knots_closed_form():
     # a wrong axis (the azimuth instead of the polar angle) fails the same check
     mz, _ = _hat_means(CE_KNOTS * 2.0)
     assert np.any(np.abs((cells - n * mz) / np.sqrt(n * (m2 + 1e-3))) > 10.0)
+
+
+def test_diffusion_jump_is_the_plain_walk():
+    """The diffusion jump (transport/fastpath.py) in a Rayleigh-dominated sphere (radius 2 m, Rayleigh 0.1 m,
+    absorption 40 m): arrivals and mean arrival time with the jump agree with the plain walk's within their errors
+    (the jump is exact), with far fewer geometry steps; the clearance grid never exceeds the distance to the
+    spheres' surfaces."""
+    from lumen_amd.detector.clearance import jump_grids
+    from toys import sphere_toy
+
+    det = sphere_toy(a=2.0, R=0.0, L_abs=40.0, L_ray=0.1)
+    (g,) = jump_grids(det)
+    ax = [g.lo[a] + g.h * np.arange(g.n[a]) for a in range(3)]
+    X = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+    r = np.linalg.norm(X, axis=1)
+    exact = np.minimum(np.abs(r - 2.0), np.abs(r - 2.05))
+    assert np.all(g.clear.reshape(-1) <= exact) and g.clear.max() > 1.5
+    n, seeds = 1 << 12, 8
+    src = L.sources.Point((0.1, -0.2, 0.05), lam=400.0, n_photons=1.0)
+    o = O.OracleRun(det)
+    res = {}
+    for mode in (False, True):
+        arr, tm, hs = [], [], []
+        for k in range(seeds):  # independent replicas: their spread gives each mean's error
+            tal, h = o.trace(src, n, seed=31 + k, controls=L.api.Controls(jump=mode))
+            li = O.light_of(tal, h, 1, 1, n)
+            arr.append(li["arrivals"][0])
+            tm.append(li["mean_arrival_time"][0])
+            hs.append(h)
+        res[mode] = (np.array(arr), np.array(tm), np.sum(hs, 0))
+    (ap, tp, hp), (aj, tj, hj) = res[False], res[True]
+    for a, b in ((aj, ap), (tj, tp)):
+        z = (a.mean() - b.mean()) / np.hypot(a.std(ddof=1), b.std(ddof=1)) * np.sqrt(seeds)
+        assert abs(z) < 4.5, (a.mean(), b.mean(), z)
+    assert 0.3 < ap.mean() < 0.9  # a diffusing, absorbing sphere: neither all nor none
+    assert hj[12] < 0.2 * hp[12]  # vertices: the walk inside the balls makes no geometry step
+    for h in (hp, hj):  # every photon's start weight ends in a fate
+        assert abs(h[14] - sum(h[k] for k in (0, 1, 2, 3, 4, 5, 6, 7, 8))) < 1e-6 * h[14]
