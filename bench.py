@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--photons", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-lar", action="store_true", help="skip the liquid-argon voxel batch")
+    ap.add_argument("--lar-side", type=int, default=16, help="voxels per side (16: the 4,096 sources)")
+    ap.add_argument("--lar-photons", type=int, default=1 << 16, help="photons per voxel source")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--steps-adjoint", type=int, default=3)
     ap.add_argument("--batch-repeats", type=int, default=3)
@@ -173,13 +176,19 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "fp_ops_per_photon": fp_per_photon, "kernel": "lh::trace_kernel"},
     }
+    log(f"forward: {k_ms:.2f} ms per launch")
     if not args.no_adjoint:
         out["fwd_adjoint"] = fwd_adjoint_leg(det, src, n, args.steps_adjoint)
+        log("forward + adjoint done")
         out["value_fwd_adjoint"] = out["fwd_adjoint"]["value"]
     if not args.no_batch:
         out["batched"] = batched_leg(det, args.batch_repeats)
+        log("batched done")
+    if not args.no_lar:
+        out["lar_voxels"] = lar_leg(args.lar_side, args.lar_photons)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(det, src)
+        log("cpu baseline done")
     print(json.dumps(out), flush=True)
 
 
@@ -241,23 +250,50 @@ def fwd_adjoint_leg(det, src, n, steps):
             "loss": "Poisson NLL of pe vs a target at N(0, 5%)-perturbed parameters"}
 
 
+def log(msg):
+    """A progress line on stderr (the JSON result is stdout's one line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _poisson_cotangent(S, qe, target):
+    """Return loss(tally) -> dLoss/dtally of the per-source Poisson NLL sum_s pe - target log pe, pe = qe charge."""
+    import torch
+
+    def loss(tally):
+        pe = tally[:, 3 * S:4 * S] * qe
+        cot = torch.zeros_like(tally)
+        cot[:, 3 * S:4 * S] = qe * (1.0 - target / torch.clamp(pe, min=1e-30))
+        return cot
+
+    return loss
+
+
 def batched_leg(det, repeats):
     """256 point sources with uneven photon counts (2^12..2^20): one graph-captured persistent launch with the
-    autotuned work queue against one launch per source."""  # noqa: D205, D209
-    import numpy as np
+    autotuned work queue, forward and forward plus adjoint (the per-source Poisson NLL of pe against a target 5 %
+    above the forward's, gradient over the absorption and Rayleigh knots, the liner's reflectivity and roughness),
+    against one plain launch per source."""  # noqa: D205, D209
     import torch
 
     from lumen_amd.batch import BatchRun, c4_batch
 
     batch = c4_batch(det)
-    run = BatchRun(det, batch, seed=11)
-    run.forward()
+    run = BatchRun(det, batch, seed=11, grad_keys=C3_KEYS)
+    S = run.eng.S
+    target = 1.05 * run.forward()[:, 3 * S:4 * S].clone() * 0.25
+    loss = _poisson_cotangent(S, 0.25, target)
+    run.forward_adjoint(loss)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(repeats):
         run.forward()
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / repeats
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        run.forward_adjoint(loss)
+    torch.cuda.synchronize()
+    t_both = (time.perf_counter() - t0) / repeats
     run.sequential()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -266,9 +302,52 @@ def batched_leg(det, repeats):
     torch.cuda.synchronize()
     t_seq = (time.perf_counter() - t0) / repeats
     return {"sources": len(batch.sources), "photons": int(run.total), "batched_ms": 1e3 * t_batch,
-            "sequential_ms": 1e3 * t_seq, "speedup": t_seq / t_batch,
-            "photons_per_s_batched": run.total / t_batch,
+            "batched_fwd_adjoint_ms": 1e3 * t_both, "sequential_ms": 1e3 * t_seq, "speedup": t_seq / t_batch,
+            "photons_per_s_batched": run.total / t_batch, "photons_per_s_batched_fwd_adjoint": run.total / t_both,
             "autotuned": {"chunk": run.controls.chunk, "blocks_per_cu": run.controls.blocks_per_cu}}
+
+
+LAR_KEYS = ("materials/LAr/RAYLEIGH", "surfaces/PTFE/REFLECTIVITY")
+
+
+def lar_leg(n_side, photons):
+    """The liquid-argon box: n_side^3 voxel point sources of ``photons`` photons each at 128 nm in one graph-captured
+    batched launch, forward and forward plus adjoint over the Rayleigh length and the foils' reflectivity (the 120
+    QEs act after tracking) for the per-source Poisson NLL of pe against a target 5 % above the forward's."""  # noqa: D205, D209
+    import torch
+
+    from lumen_amd.batch import BatchRun
+    from lumen_amd.detector import lar_box, voxel_points
+    from lumen_amd.sources import Batch
+
+    from lumen_amd.api import Controls
+
+    det = lar_box()
+    pts = voxel_points(n=n_side)
+    batch = Batch(tuple(pts), tuple([int(photons)] * len(pts)))
+    # one untimed forward (the target's); no graph (its capture would run the adjoint once more untimed)
+    run = BatchRun(det, batch, seed=13, grad_keys=LAR_KEYS, graph=False, tune=False, controls=Controls())
+    S = run.eng.S
+    target = 1.05 * run.forward()[:, 3 * S:4 * S].clone() * 0.25
+    loss = _poisson_cotangent(S, 0.25, target)
+    torch.cuda.synchronize()
+    log(f"lar: {len(pts)} sources, {run.total} photons: warm forward done")
+    t0 = time.perf_counter()
+    run.forward()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    log(f"lar: forward {t_fwd:.2f} s")
+    t0 = time.perf_counter()
+    run.forward_adjoint(loss)
+    torch.cuda.synchronize()
+    t_both = time.perf_counter() - t0
+    log(f"lar: forward + adjoint {t_both:.2f} s")
+    h = run.health.cpu().numpy()
+    return {"workload": f"lar_box_120pmt_{n_side}^3_voxels_2^{int(photons).bit_length() - 1}_photons",
+            "sources": len(pts), "photons": int(run.total), "forward_ms": 1e3 * t_fwd,
+            "fwd_adjoint_ms": 1e3 * t_both, "photons_per_s": run.total / t_fwd,
+            "photons_per_s_fwd_adjoint": run.total / t_both, "vertices_per_photon": float(h[12] / max(h[13], 1.0)),
+            "parameters": int(run.n_grad) + S}
 
 
 if __name__ == "__main__":
