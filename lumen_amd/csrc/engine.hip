@@ -938,8 +938,16 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
+// The plain adjoint (no separate reference values, no Mie or WLS channel) runs at 3 wavefronts per SIMD (168
+// VGPRs, 27 spilled): 26.4 ms against 31.8 at its natural 2 (217 VGPRs) and 32.3 at 4 (123 spilled) on the
+// 1,000-PMT tank's 2^24 photons; the others keep their natural allocation (at 3 they would spill 110 or more).
+#ifndef LH_ADJ_WAVES_PER_EU
+#define LH_ADJ_WAVES_PER_EU 3
+#endif
 template <unsigned F>
-__global__ void __launch_bounds__(BLOCK) adjoint_kernel(Args a, AdjArgs b) {
+__global__ void __launch_bounds__(BLOCK)
+    __attribute__((amdgpu_waves_per_eu(((F) & (F_REF | F_CHAN)) ? 1 : LH_ADJ_WAVES_PER_EU)))
+    adjoint_kernel(Args a, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
