@@ -129,6 +129,14 @@ int lh_model_destroy(lh_model model);
  * of its `prepare`, src/lumen/geometry/clearance.py).  Replaces an earlier grid of the medium; allocates. */
 int lh_model_jump(lh_model model, int32_t medium, const float lo[3], float h, const int32_t n[3], const float *clear);
 
+/* Build one BVH over n boxes (lo, hi: n*3 doubles, host): binned SAH with `bins` bins per axis, leaves of at most
+ * `leaf_size` boxes, stored depth first with skip links (the navigator arrays' `lo`, `hi`, `skip`, `start`, `count`,
+ * `items`; node_lo/node_hi hold 2n*3 doubles and skip/start/count 2n ints, items n).  Host only, no device call.
+ * Replaces the reference's src/lumen/geometry/bvh.py:179-210 `build_many` (one range). */
+int lh_bvh_build(const double *lo, const double *hi, int32_t n, int32_t leaf_size, int32_t bins, double *node_lo,
+                 double *node_hi, int32_t *skip, int32_t *start, int32_t *count, int32_t *items, int32_t *n_nodes,
+                 int32_t *depth);
+
 /* Query the model: number of sensors, of charge cells per sensor (the QE wavelength knots times the CE theta knots,
  * the layout's words 5 and 14) and tally doubles per event. */
 int lh_model_info(lh_model model, int32_t *n_sensors, int32_t *n_qe, int64_t *tally_per_event);

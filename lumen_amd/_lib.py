@@ -59,7 +59,7 @@ _LIB = None
 
 SYMBOLS = ("lh_model_create", "lh_model_destroy", "lh_model_info", "lh_trace", "lh_adjoint", "lh_query",
            "lh_batch_create", "lh_batch_destroy", "lh_batch_trace", "lh_batch_adjoint", "lh_last_kernel_ms",
-           "lh_last_launch", "lh_model_jump", "lh_last_error")
+           "lh_last_launch", "lh_model_jump", "lh_bvh_build", "lh_last_error")
 
 
 def lib():
@@ -89,6 +89,8 @@ def lib():
         L.lh_last_kernel_ms.argtypes = [C.c_void_p, _f]
         L.lh_last_launch.argtypes = [C.c_void_p, _i]
         L.lh_model_jump.argtypes = [C.c_void_p, C.c_int32, _f, C.c_float, _i, _f]
+        L.lh_bvh_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 6 + [
+            _i, _i]
         L.lh_last_error.restype = C.c_char_p
         for name in SYMBOLS:
             getattr(L, name).restype = C.c_int if name != "lh_last_error" else C.c_char_p

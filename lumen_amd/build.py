@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "liblumen_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "engine.hip"), os.path.join(HERE, "csrc", "unified_tables.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "unified_tables.cpp", "bvh_build.cpp")]
 DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("nav.h", "optics.h", "philox.h", "adjoint.h", "unified.h")] + [
     os.path.join(ROOT, "include", "lumen_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
