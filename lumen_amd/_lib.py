@@ -68,6 +68,10 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise EngineNotBuilt(f"{LIB_PATH} is missing: build it with `python -m lumen_amd.build` (hipcc, gfx950)")
+        # torch's HIP runtime first: the library binds to the runtime already in the process, and a process whose
+        # first HIP runtime was the library's own found no device on the GPU machine
+        import torch  # noqa: F401
+
         L = C.CDLL(LIB_PATH)
         L.lh_model_create.argtypes = [C.POINTER(LhGeometry), _i, C.c_int32, C.POINTER(C.c_void_p)]
         L.lh_model_destroy.argtypes = [C.c_void_p]

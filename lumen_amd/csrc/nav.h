@@ -357,7 +357,10 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
         }
       }
     if (nl_ < 0 || nxt >= HUGEF) break;
-    if (member(nxt) != start) return Change{nxt, nl_, nf, leaf_normal(g, K.x + nl_, nf, o, d, nxt)};
+    // a change only where the value stays changed a band further on: two leaves meeting on a shared face (a
+    // polycone's sections, a union of touching pieces) may leave a gap of a few ulps between their intervals
+    if (member(nxt) != start && member(nxt + band_at(maxabs3(o) + fabsf(nxt))) != start)
+      return Change{nxt, nl_, nf, leaf_normal(g, K.x + nl_, nf, o, d, nxt)};
     cur = nxt;
   }
   if (exit_query) return Change{0.0f, 0, 0, leaf_normal(g, K.x, 0, o, d, 0.0f)};

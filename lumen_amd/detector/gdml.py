@@ -1,8 +1,9 @@
 """A GDML-subset reader and the detector builder behind ``lumen_amd.load`` (written for this engine).
 
 It reads what the BASELINE detectors use: ``define`` (``matrix``, ``constant``, ``position``, ``rotation``),
-``materials`` (material properties by matrix reference), ``solids`` (``box``, ``tube`` without a phi cut, ``orb``,
-``sphere`` as a full ball or a polar cap, the booleans, ``opticalsurface``), ``structure`` (``volume``, ``physvol``,
+``materials`` (material properties by matrix reference: RINDEX, ABSLENGTH, RAYLEIGH, the Mie MIEHG constants, the
+wavelength shifter's WLS properties), ``solids`` (``box``, ``tube`` (a phi cut too), ``cone``, ``polycone`` without a
+phi cut, ``orb``, ``sphere`` with polar cuts, ``ellipsoid``, ``trd``, the booleans, ``opticalsurface``), ``structure`` (``volume``, ``physvol``,
 ``bordersurface``, ``skinsurface``) and ``setup``.  A recipe names the detector volume (light leaving it is absorbed)
 and the sensor groups (counted across a border into each placement of a logical volume), as the reference's
 ``lumen.Recipe`` does (``examples/example_tank.py``).
@@ -116,12 +117,41 @@ def _parse(path: str) -> _Doc:
             u = _lunit(el)
             d.solids[name] = solids.box(*[0.5 * _num(el.get(a), d.consts) * u for a in "xyz"], name=name)
         elif el.tag == "tube":
+            u, au = _lunit(el), _aunit(el)
+            dphi = _num(el.get("deltaphi"), d.consts) * au if el.get("deltaphi") else 2 * math.pi
+            rmax, hz = _num(el.get("rmax"), d.consts) * u, 0.5 * _num(el.get("z"), d.consts) * u
+            rmin = _num(el.get("rmin", "0"), d.consts) * u
+            if abs(dphi - 2 * math.pi) > 1e-9:  # a phi cut: the cone of equal radii with its wedge
+                d.solids[name] = solids.cons(rmin, rmax, rmin, rmax, hz, _num(el.get("startphi", "0"), d.consts) * au,
+                                             dphi, name=name)
+            else:
+                d.solids[name] = solids.tube(rmax, hz, rmin, name=name)
+        elif el.tag == "cone":
+            u, au = _lunit(el), _aunit(el)
+            dphi = _num(el.get("deltaphi"), d.consts) * au if el.get("deltaphi") else 2 * math.pi
+            d.solids[name] = solids.cons(*[_num(el.get(k, "0"), d.consts) * u for k in ("rmin1", "rmax1", "rmin2",
+                                                                                      "rmax2")],
+                                         0.5 * _num(el.get("z"), d.consts) * u,
+                                         _num(el.get("startphi", "0"), d.consts) * au, dphi, name=name)
+        elif el.tag == "polycone":
+            u, au = _lunit(el), _aunit(el)
+            dphi = _num(el.get("deltaphi"), d.consts) * au if el.get("deltaphi") else 2 * math.pi
+            if abs(dphi - 2 * math.pi) > 1e-9:
+                raise ValueError(f"gdml: polycone {name!r} with a phi cut is not supported")
+            zp = el.findall("zplane")
+            d.solids[name] = solids.polycone([_num(z.get("z"), d.consts) * u for z in zp],
+                                             [_num(z.get("rmin", "0"), d.consts) * u for z in zp],
+                                             [_num(z.get("rmax"), d.consts) * u for z in zp], name=name)
+        elif el.tag == "ellipsoid":
             u = _lunit(el)
-            dphi = _num(el.get("deltaphi", "360"), d.consts) * _aunit(el, "deg") if el.get("deltaphi") else 2 * math.pi
-            if abs(dphi - 2 * math.pi) > 1e-12:
-                raise ValueError(f"gdml: tube {name!r} with a phi cut is not supported")
-            d.solids[name] = solids.tube(_num(el.get("rmax"), d.consts) * u, 0.5 * _num(el.get("z"), d.consts) * u,
-                                         _num(el.get("rmin", "0"), d.consts) * u, name=name)
+            d.solids[name] = solids.ellipsoid(
+                *[_num(el.get(k), d.consts) * u for k in ("ax", "by", "cz")],
+                _num(el.get("zcut1"), d.consts) * u if el.get("zcut1") else -math.inf,
+                _num(el.get("zcut2"), d.consts) * u if el.get("zcut2") else math.inf, name=name)
+        elif el.tag == "trd":
+            u = _lunit(el)
+            d.solids[name] = solids.trd(*[0.5 * _num(el.get(k), d.consts) * u for k in ("x1", "x2", "y1", "y2", "z")],
+                                        name=name)
         elif el.tag == "orb":
             d.solids[name] = solids.orb(_num(el.get("r"), d.consts) * _lunit(el), name=name)
         elif el.tag == "sphere":
@@ -237,14 +267,25 @@ def load(path: str, recipe: Recipe) -> Detector:
     for m in mats:
         props = d.materials.get(m, {})
         keys = {}
-        for p in ("RINDEX", "ABSLENGTH", "RAYLEIGH"):
+        for p in ("RINDEX", "ABSLENGTH", "RAYLEIGH", "MIEHG", "WLSABSLENGTH", "WLSCOMPONENT"):
             if p in props:
                 values[f"materials/{m}/{p}"] = _table(d, props[p])
+                keys[p] = f"materials/{m}/{p}"
+        for p in ("MIEHG_FORWARD", "MIEHG_BACKWARD", "MIEHG_FORWARD_RATIO", "WLSMEANNUMBERPHOTONS", "WLSTIMECONSTANT"):
+            if p in props:  # constant properties: one value
+                values[f"materials/{m}/{p}"] = np.float64(d.matrices[props[p]].reshape(-1)[-1])
                 keys[p] = f"materials/{m}/{p}"
         ray = keys.get("RAYLEIGH")
         if ray is None and m.split("0x")[0] == "Water" and "RINDEX" in keys:
             ray = "water_rule"
-        media.append(Medium(m, keys.get("RINDEX"), keys.get("ABSLENGTH"), ray))
+        extra = {}
+        if "MIEHG" in keys:
+            extra.update(mie_length=keys["MIEHG"], mie_forward=keys.get("MIEHG_FORWARD", 0.0),
+                         mie_backward=keys.get("MIEHG_BACKWARD", 0.0), mie_ratio=keys.get("MIEHG_FORWARD_RATIO", 1.0))
+        if "WLSABSLENGTH" in keys:
+            extra.update(wls_length=keys["WLSABSLENGTH"], wls_spectrum=keys.get("WLSCOMPONENT"),
+                         wls_yield=keys.get("WLSMEANNUMBERPHOTONS", 1.0), wls_tau=keys.get("WLSTIMECONSTANT", 0.0))
+        media.append(Medium(m, keys.get("RINDEX"), keys.get("ABSLENGTH"), ray, **extra))
     # surfaces: border (pv1 -> pv2) and skin (either way); classes by (material, surfaces entering it)
     pv_index = {p[0]: i for i, p in enumerate(placed)}
     srows, srow_of = [], {}

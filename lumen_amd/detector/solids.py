@@ -126,6 +126,115 @@ def sphere_cap(r: float, height: float, name="cap") -> Solid:
     return Solid([lf], [(0, 0)], np.array([-rc, -rc, z0]), np.array([rc, rc, r]), name, dict(r=r, height=height))
 
 
+def _cone_leaf(r1: float, r2: float, hz: float) -> Leaf:
+    """The full cone (frustum) x^2 + y^2 <= rho(z)^2, rho linear from r1 at z = -hz to r2 at z = +hz, |z| <= hz.
+
+    rho = a + b z with a = (r1 + r2) / 2, b = (r2 - r1) / (2 hz): x^2 + y^2 - b^2 z^2 - 2 a b z - a^2 <= 0; rho >= 0
+    on the slab (r1, r2 >= 0), so the quadric's other nappe lies outside it."""  # noqa: D205, D209
+    a, b = 0.5 * (r1 + r2), (r2 - r1) / (2.0 * hz)
+    return Leaf(1, _z3(), _z3(), np.diag([1.0, 1.0, -b * b]), np.array([0.0, 0.0, -a * b]), -a * a,
+                np.array([[0, 0, 1.0], [0, 0, -1.0]]), np.array([hz, hz]))
+
+
+def _phi_wedge(phi0: float, dphi: float, r: float, hz: float):
+    """Return the solid of the points whose azimuth lies in [phi0, phi0 + dphi] (0 < dphi < 2 pi), boxed by
+    (+-r, +-r, +-hz): one leaf of two half-planes through the z axis when dphi <= pi, else the union of two."""  # noqa: D205, D209
+    h = np.array([r, r, hz])
+
+    def half(phi):  # the points on the left of the direction phi: -sin x + cos y >= 0
+        return np.array([np.sin(phi), -np.cos(phi), 0.0])
+
+    def right(phi):  # the points on the right of the direction phi
+        return -half(phi)
+
+    if dphi <= np.pi:
+        lf = _planes_leaf(np.stack([half(phi0), right(phi0 + dphi)]), [0.0, 0.0])
+        return Solid([lf], [(0, 0)], -h, h, "wedge")
+    a = _planes_leaf(half(phi0)[None], [0.0])
+    b = _planes_leaf(right(phi0 + dphi)[None], [0.0])
+    return Solid([a, b], [(0, 0), (0, 1), (OPS["union"], 0)], -h, h, "wedge")
+
+
+def cons(rmin1: float, rmax1: float, rmin2: float, rmax2: float, hz: float, phi0: float = 0.0,
+         dphi: float = 2 * np.pi, name="cons") -> Solid:
+    """Return G4Cons: radii [rmin1, rmax1] at z = -hz and [rmin2, rmax2] at z = +hz, linear between, azimuths
+    [phi0, phi0 + dphi] (a full turn: no cut)."""  # noqa: D205, D209
+    if not (0.0 <= rmin1 <= rmax1 and 0.0 <= rmin2 <= rmax2 and rmax1 + rmax2 > 0.0 and hz > 0.0):
+        raise ValueError(f"cons {name!r}: 0 <= rmin <= rmax at both ends and hz > 0")
+    rmax = max(rmax1, rmax2)
+    h = np.array([rmax, rmax, hz])
+    s = Solid([_cone_leaf(rmax1, rmax2, hz)], [(0, 0)], -h, h, name)
+    if rmin1 > 0.0 or rmin2 > 0.0:
+        inner = Solid([_cone_leaf(rmin1, rmin2, hz)], [(0, 0)], -h, h)
+        s = boolean("sub", s, inner, name=name)
+    if dphi < 2 * np.pi - 1e-12:
+        s = boolean("inter", s, _phi_wedge(phi0, dphi, rmax, hz), name=name)
+    s.lo, s.hi = -h, h
+    s.params = dict(rmin1=rmin1, rmax1=rmax1, rmin2=rmin2, rmax2=rmax2, hz=hz, phi0=phi0, dphi=dphi)
+    return s
+
+
+def polycone(z, rmin, rmax, name="polycone") -> Solid:
+    """Return G4Polycone with a full turn: the union of the frusta between consecutive z planes (radii [rmin_k,
+    rmax_k] at z_k, linear between), each a cone leaf (less its inner cone) on its own slab."""  # noqa: D205, D209
+    z, rmin, rmax = (np.asarray(v, np.float64) for v in (z, rmin, rmax))
+    if len(z) < 2 or np.any(np.diff(z) < 0) or len(rmin) != len(z) or len(rmax) != len(z):
+        raise ValueError(f"polycone {name!r}: two or more z planes in order, one rmin and rmax each")
+    parts = []
+    for k in range(len(z) - 1):
+        hz = 0.5 * (z[k + 1] - z[k])
+        if hz <= 0.0:
+            continue
+        c = cons(rmin[k], rmax[k], rmin[k + 1], rmax[k + 1], hz, name=name)
+        parts.append((c, 0.5 * (z[k] + z[k + 1])))
+    s = None
+    for c, zc in parts:
+        s = _moved(c, zc, name) if s is None else boolean("union", s, c, np.eye(3), np.array([0.0, 0.0, zc]),
+                                                           name=name)
+    r = float(np.max(rmax))
+    s.lo, s.hi = np.array([-r, -r, z[0]]), np.array([r, r, z[-1]])
+    s.params = dict(z=z.tolist(), rmin=rmin.tolist(), rmax=rmax.tolist())
+    return s
+
+
+def _moved(s: Solid, zc: float, name: str) -> Solid:
+    """Return the solid translated by zc along z."""
+    t = np.array([0.0, 0.0, zc])
+    return Solid([lf.moved(np.eye(3), t) for lf in s.leaves], list(s.program), s.lo + t, s.hi + t, name)
+
+
+def ellipsoid(ax: float, by: float, cz: float, zcut1: float = -np.inf, zcut2: float = np.inf,
+              name="ellipsoid") -> Solid:
+    """Return G4Ellipsoid: (x/ax)^2 + (y/by)^2 + (z/cz)^2 <= 1 cut to zcut1 <= z <= zcut2 (one leaf)."""
+    if not (ax > 0 and by > 0 and cz > 0):
+        raise ValueError(f"ellipsoid {name!r}: positive semi-axes")
+    z1, z2 = max(-cz, zcut1), min(cz, zcut2)
+    N, H = [], []
+    if z2 < cz:
+        N.append([0, 0, 1.0]), H.append(z2)
+    if z1 > -cz:
+        N.append([0, 0, -1.0]), H.append(-z1)
+    lf = Leaf(1, _z3(), _z3(), np.diag([1 / ax**2, 1 / by**2, 1 / cz**2]), _z3(), -1.0,
+              np.asarray(N, np.float64).reshape(-1, 3), np.asarray(H, np.float64))
+    return Solid([lf], [(0, 0)], np.array([-ax, -by, z1]), np.array([ax, by, z2]), name,
+                 dict(ax=ax, by=by, cz=cz, zcut1=z1, zcut2=z2))
+
+
+def trd(x1: float, x2: float, y1: float, y2: float, hz: float, name="trd") -> Solid:
+    """Return G4Trd: half-lengths (x1, y1) at z = -hz and (x2, y2) at z = +hz, linear between (six planes)."""
+    if not (x1 >= 0 and x2 >= 0 and y1 >= 0 and y2 >= 0 and hz > 0 and x1 + x2 > 0 and y1 + y2 > 0):
+        raise ValueError(f"trd {name!r}: non-negative half-lengths, hz > 0")
+    # the side x <= xm + sx z (xm the mid half-length, sx its slope): (1, 0, -sx).(x, y, z) <= xm
+    xm, sx = 0.5 * (x1 + x2), (x2 - x1) / (2 * hz)
+    ym, sy = 0.5 * (y1 + y2), (y2 - y1) / (2 * hz)
+    N = np.array([[1, 0, -sx], [-1, 0, -sx], [0, 1, -sy], [0, -1, -sy], [0, 0, 1.0], [0, 0, -1.0]], np.float64)
+    H = np.array([xm, xm, ym, ym, hz, hz], np.float64)
+    nn = np.linalg.norm(N, axis=1)
+    lf = Leaf(0, _z3(), _z3(), np.zeros((3, 3)), _z3(), 0.0, N / nn[:, None], H / nn)
+    m = np.array([max(x1, x2), max(y1, y2), hz])
+    return Solid([lf], [(0, 0)], -m, m, name, dict(x1=x1, x2=x2, y1=y1, y2=y2, hz=hz))
+
+
 def polar_cone(theta: float, r: float, name="cone") -> Solid:
     """Return the points whose polar angle (from +z, at the origin) is at most ``theta`` (0 < theta < pi), boxed by
     +-r.  Below pi/2 one leaf: the cone (x^2 + y^2) cos^2 - z^2 sin^2 <= 0 cut by z >= 0; at pi/2 the half-space

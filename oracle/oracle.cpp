@@ -376,7 +376,8 @@ Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_quer
         if (iv.hi > cur && iv.hi < nxt && iv.hi < R(1e299)) { nxt = iv.hi; bl = l; bf = iv.fhi; }
       }
     if (bl < 0) break;
-    if (member(nxt) != start) return {nxt, bl, bf};
+    // a change only where the value stays changed a little further on (touching leaves' rounding gaps; FP64)
+    if (member(nxt) != start && member(nxt + R(1e-12) * (R(1.0) + FABS(nxt))) != start) return {nxt, bl, bf};
     cur = nxt;
   }
   if (exit_query) return {R(0.0), 0, 0};

@@ -240,3 +240,77 @@ def test_native_bvh_builder_builds_the_rules_trees():
     t0 = time.perf_counter()
     big = bvh.build(c - 0.05, c + 0.05)
     assert time.perf_counter() - t0 < 5.0 and len(big["skip"]) == 2 * 100_000 - 1
+
+
+def test_more_solids_membership():
+    """G4Cons (with an inner cone and an azimuth cut), G4Polycone, G4Ellipsoid (z cuts) and G4Trd as CSG leaves:
+    membership of random points against their analytic definitions, and every member inside the solid's box."""
+    rng = np.random.default_rng(3)
+
+    def check(s, inside_fn, lo, hi, n=60000):
+        x = rng.uniform(lo, hi, (n, 3))
+        want, got = inside_fn(x), s.inside(x)
+        assert np.array_equal(want, got), (s.name, int((want != got).sum()))
+        assert want.sum() > 100
+        assert np.all(x[got] >= s.lo - 1e-12) and np.all(x[got] <= s.hi + 1e-12)
+
+    def frustum(x, r1, r2, hz, zc=0.0):
+        z = x[:, 2] - zc
+        return (np.abs(z) <= hz) & (np.hypot(x[:, 0], x[:, 1]) <= r1 + (r2 - r1) * (z + hz) / (2 * hz))
+
+    s = solids.cons(0.1, 0.5, 0.15, 0.3, 0.4, phi0=0.3, dphi=4.0)
+    check(s, lambda x: frustum(x, 0.5, 0.3, 0.4) & ~frustum(x, 0.1, 0.15, 0.4)
+          & (np.mod(np.arctan2(x[:, 1], x[:, 0]) - 0.3, 2 * np.pi) <= 4.0), -0.6, 0.6)
+    s = solids.cons(0.0, 0.5, 0.0, 0.5, 0.4, phi0=1.0, dphi=1.2)
+    check(s, lambda x: frustum(x, 0.5, 0.5, 0.4) & (np.mod(np.arctan2(x[:, 1], x[:, 0]) - 1.0, 2 * np.pi) <= 1.2),
+          -0.6, 0.6)
+    zp, rmin, rmax = [-0.5, 0.0, 0.2, 0.6], [0.0, 0.1, 0.1, 0.0], [0.3, 0.5, 0.5, 0.2]
+
+    def poly(x):
+        out = np.zeros(len(x), bool)
+        for k in range(3):
+            hz, zc = 0.5 * (zp[k + 1] - zp[k]), 0.5 * (zp[k] + zp[k + 1])
+            out |= frustum(x, rmax[k], rmax[k + 1], hz, zc) & ~frustum(x, rmin[k], rmin[k + 1], hz, zc)
+        return out
+
+    check(solids.polycone(zp, rmin, rmax), poly, -0.7, 0.7)
+    check(solids.ellipsoid(0.3, 0.5, 0.7, -0.4, 0.6),
+          lambda x: ((x[:, 0] / 0.3) ** 2 + (x[:, 1] / 0.5) ** 2 + (x[:, 2] / 0.7) ** 2 <= 1) & (x[:, 2] >= -0.4)
+          & (x[:, 2] <= 0.6), -0.8, 0.8)
+
+    def trd_in(x, x1=0.2, x2=0.5, y1=0.4, y2=0.1, hz=0.3):
+        f = (x[:, 2] + hz) / (2 * hz)
+        return (np.abs(x[:, 2]) <= hz) & (np.abs(x[:, 0]) <= x1 + (x2 - x1) * f) & (np.abs(x[:, 1]) <= y1 + (y2 - y1) * f)
+
+    check(solids.trd(0.2, 0.5, 0.4, 0.1, 0.3), trd_in, -0.6, 0.6)
+
+
+
+
+def test_gdml_reads_the_added_solids_and_mie(tmp_path):
+    """The GDML reader's cone, tube with a phi cut, polycone, ellipsoid and trd (full lengths, mm) equal the solids
+    built directly (membership of random points in each placement's frame), and a material's MIEHG constants
+    become the Mie medium's keys."""
+    from lumen_amd.detector.gdml import Recipe, load
+    from toys import GDML_SOLIDS
+
+    f = tmp_path / "solids.gdml"
+    f.write_text(GDML_SOLIDS)
+    det = load(str(f), Recipe("solids", "World"))
+    names = [n.rsplit("/", 1)[-1] for n in det.nav.names]
+    m = det.media[0]
+    assert m.mie_length == "materials/Liq/MIEHG" and det.params()[m.mie_forward] == pytest.approx(0.6)
+    assert det.params()[m.mie_ratio] == pytest.approx(0.7) and det.params()["materials/Liq/MIEHG"].value[0] == 20.0
+    want = {"pc": solids.cons(0.1, 0.5, 0.15, 0.3, 0.4), "pt": solids.cons(0, 0.3, 0, 0.3, 0.2, np.radians(30),
+                                                                          np.radians(120)),
+            "pp": solids.polycone([-0.3, 0.0, 0.2], [0, 0.05, 0], [0.2, 0.3, 0.1]),
+            "pe": solids.ellipsoid(0.2, 0.3, 0.4, -0.3, 0.35), "pd": solids.trd(0.2, 0.1, 0.15, 0.25, 0.3)}
+    rng = np.random.default_rng(5)
+    h = det.nav.host
+    for pv, s in want.items():
+        v = names.index(pv)
+        x = rng.uniform(-0.6, 0.6, (20000, 3))
+        got = _inside_kind(det.nav, int(h["kind"][v]), x)
+        assert np.array_equal(got, s.inside(x)), pv
+        assert np.allclose(h["t"][v], {"pc": (-1, -1, 0), "pt": (1, -1, 0), "pp": (-1, 1, 0), "pe": (1, 1, 0),
+                                       "pd": (0, 0, 1.2)}[pv])

@@ -130,3 +130,47 @@ def lambert_closed_form(a, R, L):
 def group_velocity(n):
     """The group velocity of a flat index table (m/ns): c / n."""
     return units.C_VACUUM / n
+
+
+# a GDML of the added solids (cone, tube with a phi cut, polycone, ellipsoid, trd) in a Mie liquid: the reader's test
+GDML_SOLIDS = """<?xml version="1.0" encoding="UTF-8"?>
+<gdml>
+<define>
+  <matrix name="RIDX" coldim="2" values="1.0*eV 1.33 12.0*eV 1.33"/>
+  <matrix name="MIEL" coldim="2" values="1.0*eV 20*m 12.0*eV 20*m"/>
+  <matrix name="MIEF" coldim="1" values="0.6"/>
+  <matrix name="MIEB" coldim="1" values="0.3"/>
+  <matrix name="MIER" coldim="1" values="0.7"/>
+</define>
+<materials>
+  <material name="Liq"><property name="RINDEX" ref="RIDX"/><property name="MIEHG" ref="MIEL"/>
+    <property name="MIEHG_FORWARD" ref="MIEF"/><property name="MIEHG_BACKWARD" ref="MIEB"/>
+    <property name="MIEHG_FORWARD_RATIO" ref="MIER"/></material>
+</materials>
+<solids>
+  <box name="WorldBox" x="4" y="4" z="4" lunit="m"/>
+  <cone name="C" rmin1="100" rmax1="500" rmin2="150" rmax2="300" z="800" startphi="0" deltaphi="360" aunit="deg" lunit="mm"/>
+  <tube name="T" rmin="0" rmax="300" z="400" startphi="30" deltaphi="120" aunit="deg" lunit="mm"/>
+  <polycone name="P" startphi="0" deltaphi="360" aunit="deg" lunit="mm">
+    <zplane z="-300" rmin="0" rmax="200"/><zplane z="0" rmin="50" rmax="300"/><zplane z="200" rmin="0" rmax="100"/>
+  </polycone>
+  <ellipsoid name="E" ax="200" by="300" cz="400" zcut1="-300" zcut2="350" lunit="mm"/>
+  <trd name="D" x1="400" x2="200" y1="300" y2="500" z="600" lunit="mm"/>
+</solids>
+<structure>
+  <volume name="cV"><materialref ref="Liq"/><solidref ref="C"/></volume>
+  <volume name="tV"><materialref ref="Liq"/><solidref ref="T"/></volume>
+  <volume name="pV"><materialref ref="Liq"/><solidref ref="P"/></volume>
+  <volume name="eV"><materialref ref="Liq"/><solidref ref="E"/></volume>
+  <volume name="dV"><materialref ref="Liq"/><solidref ref="D"/></volume>
+  <volume name="World"><materialref ref="Liq"/><solidref ref="WorldBox"/>
+    <physvol name="pc"><volumeref ref="cV"/><position name="p1" x="-1" y="-1" z="0" unit="m"/></physvol>
+    <physvol name="pt"><volumeref ref="tV"/><position name="p2" x="1" y="-1" z="0" unit="m"/></physvol>
+    <physvol name="pp"><volumeref ref="pV"/><position name="p3" x="-1" y="1" z="0" unit="m"/></physvol>
+    <physvol name="pe"><volumeref ref="eV"/><position name="p4" x="1" y="1" z="0" unit="m"/></physvol>
+    <physvol name="pd"><volumeref ref="dV"/><position name="p5" x="0" y="0" z="1.2" unit="m"/></physvol>
+  </volume>
+</structure>
+<setup name="Default" version="1.0"><world ref="World"/></setup>
+</gdml>
+"""
