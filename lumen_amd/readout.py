@@ -61,9 +61,81 @@ def pe_error(light, params: Params, det=None, group: int = 0):
     return torch.where(a > 0, p / torch.where(a > 0, a, torch.ones_like(a)), torch.zeros_like(a)) * light.arrivals_error
 
 
-def time_masses(light, params: Params, det=None, group: int = 0):
+# The time histogram's cubic Lagrange split of a unit mass at u in [0, 1) past node j onto nodes j - 1 .. j + 2
+# (the kernel's deposit weights), as polynomial coefficients in u, lowest power first, and their antiderivatives.
+_SPLIT = {-1: np.array([0.0, -2.0, 3.0, -1.0]) / 6.0, 0: np.array([2.0, -1.0, -2.0, 1.0]) / 2.0,
+          1: np.array([0.0, 2.0, 1.0, -1.0]) / 2.0, 2: np.array([0.0, -1.0, 0.0, 1.0]) / 6.0}  # fmt: skip
+_SPLIT_INT = {o: np.concatenate([[0.0], c / np.arange(1.0, 5.0)]) for o, c in _SPLIT.items()}
+
+
+def delay_taps(edges, dt: float):
+    """Return (off0, M) for a delay table with bin ``edges`` (ns, non-decreasing, >= 0) on nodes of spacing ``dt``:
+    M (taps, bins) float64, M[k, b] the mass a delay uniform on bin b moves a unit node mass onto the node off0 + k
+    further on (the reference's ``sensors/components.py:231-258`` ``delay_taps``).  Each entry is the exact integral
+    over the bin of the histogram's own cubic split weight of that node, over the bin's width; a zero-width bin is a
+    point mass (the split weights at its point), so the delay is as exact as the node grid."""  # noqa: D205, D209
+    e = np.asarray(edges, np.float64) / float(dt)
+    if e.ndim != 1 or len(e) < 2 or np.any(np.diff(e) < 0) or not np.all(np.isfinite(e)):
+        raise ValueError("delay_taps: edges must be a finite, non-decreasing list of at least two values")
+    lo, hi = int(np.floor(e.min())) - 1, int(np.floor(e.max())) + 2
+    M = np.zeros((hi - lo + 1, len(e) - 1))
+    for b in range(len(e) - 1):
+        xa, xb = e[b], e[b + 1]
+        if xb == xa:
+            j = int(np.floor(xa))
+            for o, c in _SPLIT.items():
+                M[j + o - lo, b] += np.polynomial.polynomial.polyval(xa - j, c)
+            continue
+        for j in range(int(np.floor(xa)), int(np.ceil(xb))):
+            g0, g1 = max(xa, j) - j, min(xb, j + 1) - j
+            if g1 <= g0:
+                continue
+            for o, c in _SPLIT_INT.items():
+                pv = np.polynomial.polynomial.polyval
+                M[j + o - lo, b] += (pv(g1, c) - pv(g0, c)) / (xb - xa)
+    return lo, M
+
+
+def delay_apply(nodes, edges, pdf, dt: float):
+    """Return the node masses (..., nt) after a delay table (bin ``edges`` ns, probabilities ``pdf`` per bin,
+    normalised here; a torch tensor keeps its autograd): each node's mass spread onto the nodes off0 + k further on
+    by the taps M pdf; mass moved past the last node (or before the first) is dropped (the reference's
+    ``DelayTable.apply``).  A direct sum over the few taps, not an FFT: the taps span the delay's range only."""  # noqa: D205, D209
+    import torch
+
+    off0, M = delay_taps(edges, dt)
+    w = torch.as_tensor(pdf, dtype=nodes.dtype, device=nodes.device)
+    if w.shape[-1] != M.shape[1] or w.ndim != 1:
+        raise ValueError(f"delay_apply: pdf needs {M.shape[1]} values, one per bin")
+    taps = torch.as_tensor(M, dtype=nodes.dtype, device=nodes.device) @ (w / w.sum())
+    nt = nodes.shape[-1]
+    out = torch.zeros_like(nodes)
+    for k in range(M.shape[0]):
+        s = off0 + k  # out[n] += taps[k] nodes[n - s]
+        if s >= nt or s <= -nt:
+            continue
+        if s >= 0:
+            out[..., s:] = out[..., s:] + taps[k] * nodes[..., :nt - s]
+        else:
+            out[..., :nt + s] = out[..., :nt + s] + taps[k] * nodes[..., -s:]
+    return out
+
+
+def delay_mean(edges, pdf):
+    """Return the mean delay (ns) of a delay table (bins uniform: their midpoints, weighted by ``pdf``)."""
+    import torch
+
+    w = torch.as_tensor(pdf)
+    e = torch.as_tensor(np.asarray(edges, np.float64), dtype=w.dtype if w.is_floating_point() else torch.float64)
+    w = w.to(e.dtype)
+    return (w * 0.5 * (e[1:] + e[:-1])).sum() / w.sum()
+
+
+def time_masses(light, params: Params, det=None, group: int = 0, delay=None):
     """Return the time histogram as photoelectrons (node masses (S, nt), late (S, 2), early (S, 2); (q, q t) in each
-    overflow), scaled per sensor so nodes and overflow sum to ``pe`` (``readout/detection.py`` ``block_masses``)."""  # noqa: D205, D209
+    overflow), scaled per sensor so nodes and overflow sum to ``pe`` (``readout/detection.py:229-254``
+    ``block_masses``).  ``delay`` = dict(edges=..., pdf=...): the node masses go through that delay table
+    (``delay_apply``) and the overflow cells move by its mean delay."""  # noqa: D205, D209
     import torch
 
     if light.time is None:
@@ -71,23 +143,32 @@ def time_masses(light, params: Params, det=None, group: int = 0):
     H, late, early = light.time, light.late, light.early
     total = H.sum(-1) + late[..., 0] + early[..., 0]
     scale = pe(light, params, det, group) / torch.where(total != 0.0, total, torch.ones_like(total))
+    if delay is not None:
+        tg = light.provenance.get("time_grid") if light.provenance else None
+        if tg is None:
+            raise ValueError("time_masses: the Light does not carry its time grid")
+        H = delay_apply(H, delay["edges"], delay["pdf"], tg[1])
+        shift = delay_mean(delay["edges"], delay["pdf"]).to(device=H.device, dtype=H.dtype)
+        late = torch.stack([late[..., 0], late[..., 1] + late[..., 0] * shift], -1)
+        early = torch.stack([early[..., 0], early[..., 1] + early[..., 0] * shift], -1)
     return H * scale[..., None], late * scale[..., None], early * scale[..., None]
 
 
 def pe_density(light, params: Params, gate, bin_width: float, tts_sigma: float, det=None, group: int = 0,
-               dark_rate=0.0):
+               dark_rate=0.0, delay=None):
     """Return the expected photoelectrons per time bin (S, n_bins) over ``gate`` = (g0, g1) ns, bins of
     ``bin_width``: the node masses (``time_masses``), each spread by a Gaussian transit-time spread of width
     ``tts_sigma`` ns and integrated over the bins (its CDF differences); the overflow masses at their mean times
     (``readout/detection.py`` ``density`` with a Gaussian ``spread``); plus the dark counts, ``dark_rate`` Hz (per
-    sensor or one for all) times the bin width.  ``light.provenance`` carries the grid."""  # noqa: D205, D209
+    sensor or one for all) times the bin width.  ``light.provenance`` carries the grid; ``delay``: a delay table
+    (``time_masses``) before the spread."""  # noqa: D205, D209
     import math
 
     import torch
 
     if not tts_sigma > 0:
         raise ValueError("pe_density: tts_sigma > 0")
-    H, late, early = time_masses(light, params, det, group)
+    H, late, early = time_masses(light, params, det, group, delay)
     tg = light.provenance.get("time_grid") if light.provenance else None
     if tg is None:
         raise ValueError("pe_density: the Light does not carry its time grid")
@@ -139,7 +220,7 @@ PULSE_SPREAD = 6.0  # the pulse's support: t_peak - 6 sigma_rise to t_peak + 6 s
 
 def expected_waveform(light, params: Params, gate, sampling: float, tts_sigma: float, pulse: dict, *, gain=1.0,
                       charge_moments=(1.0, 1.0), baseline: float = 0.0, noise: float = 0.0, dark_rate=0.0,
-                      steps_per_sample: int = 4, det=None, group: int = 0) -> Waveform:
+                      steps_per_sample: int = 4, det=None, group: int = 0, delay=None) -> Waveform:
     """Return the expected waveform per sensor, sampled every ``sampling`` ns over ``gate`` (the reference's
     ``readout/waveform.py`` ``expected_waveform``):
 
@@ -148,10 +229,10 @@ def expected_waveform(light, params: Params, gate, sampling: float, tts_sigma: f
 
     rho the photoelectron density per bin (``pe_density`` with the dark counts) on bins of sampling /
     ``steps_per_sample`` ns, <h> the pulse (``pulse`` = dict(t_peak, sigma_rise, sigma_fall), ``bigauss``) averaged
-    over each bin, (m1, m2) the single-photoelectron charge's mean and mean square (``charge_moments``).  Every step
-    is linear after the time histogram, so torch differentiates it exactly.  Fold n is the expectation at
-    g0 + n dt (a photoelectron uniform in bin m is lag (n - m - 1, n - m] bins back: the pulse's bin average); sample
-    k is fold k * steps_per_sample, at g0 + k sampling."""  # noqa: D205, D209
+    over each bin, (m1, m2) the single-photoelectron charge's mean and mean square (``charge_moments``), ``delay`` a
+    delay table (``time_masses``).  Every step is linear after the time histogram, so torch differentiates it
+    exactly.  Fold n is the expectation at g0 + n dt (a photoelectron uniform in bin m is lag (n - m - 1, n - m] bins
+    back: the pulse's bin average); sample k is fold k * steps_per_sample, at g0 + k sampling."""  # noqa: D205, D209
     import math
 
     import torch
@@ -159,7 +240,7 @@ def expected_waveform(light, params: Params, gate, sampling: float, tts_sigma: f
     if steps_per_sample < 1 or not sampling > 0:
         raise ValueError("expected_waveform: sampling > 0 and steps_per_sample >= 1")
     dt = sampling / steps_per_sample
-    rho = pe_density(light, params, gate, dt, tts_sigma, det, group, dark_rate)  # (S, nb)
+    rho = pe_density(light, params, gate, dt, tts_sigma, det, group, dark_rate, delay)  # (S, nb)
     dev, dtype = rho.device, rho.dtype
     lo = pulse["t_peak"] - PULSE_SPREAD * pulse["sigma_rise"]
     hi = pulse["t_peak"] + PULSE_SPREAD * pulse["sigma_fall"]

@@ -82,3 +82,96 @@ def test_waveform_is_linear_and_differentiable_in_the_histogram():
     unit.charge = torch.ones(1, 1, dtype=torch.float64)
     wj = expected_waveform(unit, Params({}), (0.0, 100.0), 1.0, 2.0, PULSE, gain=2.0, baseline=0.0)
     assert float(H.grad[0, j]) == pytest.approx(float(wj.mean.sum()), rel=1e-9)
+
+
+def _lagrange(o, u):
+    """The cubic Lagrange basis weight of node offset o in {-1, 0, 1, 2} at u (a product over the other nodes)."""
+    w = 1.0
+    for m in (-1, 0, 1, 2):
+        if m != o:
+            w = w * (u - m) / (o - m)
+    return w
+
+
+@pytest.mark.parametrize("edges,pdf", [((0.0, 3.7), (1.0,)), ((1.25, 1.25, 2.0, 6.1), (0.3, 0.5, 0.2)),
+                                       ((0.0, 0.2, 0.2, 0.9, 4.0), (2.0, 1.0, 0.5, 3.0))])
+def test_delay_taps_reproduce_the_delays_moments(edges, pdf):
+    """The cubic split reproduces polynomials up to degree 3, so the taps' first three moments are the delay's own:
+    sum_k taps_k ((off0 + k) dt)^r = E[delay^r], r <= 3, for bins uniform on (a, b) and point masses."""
+    from lumen_amd.readout import delay_taps
+
+    dt = 0.5
+    off0, M = delay_taps(edges, dt)
+    w = np.asarray(pdf) / np.sum(pdf)
+    taps = M @ w
+    at = (off0 + np.arange(len(taps))) * dt
+    e = np.asarray(edges)
+    for r in range(4):
+        a, b = e[:-1], e[1:]
+        exact = np.where(b > a, (b ** (r + 1) - a ** (r + 1)) / ((r + 1) * np.where(b > a, b - a, 1.0)), a ** r)
+        assert float((taps * at ** r).sum()) == pytest.approx(float((w * exact).sum()), rel=1e-12, abs=1e-12)
+
+
+def test_delay_taps_against_a_quadrature_of_the_split_weights():
+    """Each tap is the bin's average of its node's cubic split weight (midpoint rule on 200,000 points per bin)."""
+    from lumen_amd.readout import delay_taps
+
+    edges, dt = (0.3, 1.9, 2.0, 5.55), 0.7
+    off0, M = delay_taps(edges, dt)
+    for b in range(3):
+        a, c = edges[b] / dt, edges[b + 1] / dt
+        x = a + (np.arange(200000) + 0.5) / 200000 * (c - a)
+        j = np.floor(x)
+        u = x - j
+        ref = np.zeros(M.shape[0])
+        for o in (-1, 0, 1, 2):
+            np.add.at(ref, (j + o - off0).astype(int), _lagrange(o, u) / len(x))
+        np.testing.assert_allclose(M[:, b], ref, atol=2e-10)
+
+
+def test_delay_apply_shifts_point_masses_and_keeps_the_total():
+    from lumen_amd.readout import delay_apply
+
+    nodes = torch.zeros(2, 50, dtype=torch.float64)
+    nodes[0, 10], nodes[1, 20] = 1.0, 3.0
+    out = delay_apply(nodes, (1.5, 1.5), (1.0,), 0.5)  # a point mass at 3 nodes on
+    assert torch.equal(torch.nonzero(out.abs() > 1e-15), torch.tensor([[0, 13], [1, 23]]))
+    assert torch.allclose(out[:, [13, 23]].diagonal(), torch.tensor([1.0, 3.0], dtype=torch.float64))
+    spread = delay_apply(nodes, (0.0, 0.4, 2.2), (1.0, 2.0), 0.5)
+    assert torch.allclose(spread.sum(-1), nodes.sum(-1), rtol=1e-12)
+    tail = delay_apply(nodes, (0.0, 20.0), (1.0,), 0.5)  # 40 nodes: part of it moved past the last node
+    assert float(tail[1].sum()) < 3.0 * (30 / 40) + 0.05
+
+
+def test_delay_moves_the_density_and_the_overflow_by_its_mean():
+    from lumen_amd.readout import time_masses
+
+    light = _light()
+    light.late[0, 0], light.late[0, 1] = 5.0, 5.0 * 120.0  # 5 pe at 120 ns, beyond the last node
+    rho0 = pe_density(light, Params({}), (0.0, 100.0), 0.5, 1.3)
+    delay = dict(edges=(2.0, 5.0), pdf=torch.tensor([1.0], dtype=torch.float64))
+    rho = pe_density(light, Params({}), (0.0, 100.0), 0.5, 1.3, delay=delay)
+    t = (torch.arange(200, dtype=torch.float64) + 0.5) * 0.5
+    c0, c = float((rho0[0] * t).sum() / rho0[0].sum()), float((rho[0] * t).sum() / rho[0].sum())
+    assert c - c0 == pytest.approx(3.5, abs=1e-6)
+    _, late, _ = time_masses(light, Params({}), delay=delay)
+    assert float(late[0, 1] / late[0, 0]) == pytest.approx(123.5, rel=1e-12)
+
+
+def test_delay_is_differentiable_in_its_table():
+    """The density is linear in the normalised table: its gradient matches central differences."""
+    light = _light()
+    pdf = torch.tensor([0.2, 0.5, 0.3], dtype=torch.float64, requires_grad=True)
+    edges = (0.0, 1.0, 3.0, 4.5)
+    probe = torch.linspace(0.0, 1.0, 200, dtype=torch.float64)
+
+    def f(p):
+        return (pe_density(light, Params({}), (0.0, 100.0), 0.5, 1.3, delay=dict(edges=edges, pdf=p))[0]
+                * probe).sum()
+
+    f(pdf).backward()
+    for i in range(3):
+        e = torch.zeros(3, dtype=torch.float64)
+        e[i] = 1e-6
+        fd = (f(pdf.detach() + e) - f(pdf.detach() - e)) / 2e-6
+        assert float(pdf.grad[i]) == pytest.approx(float(fd), rel=1e-6, abs=1e-9)
