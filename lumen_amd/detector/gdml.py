@@ -346,7 +346,7 @@ def load(path: str, recipe: Recipe) -> Detector:
                 row_of[vol_cls[i], vol_cls[p]] = r
     vols = [navarrays.Volume(p[0], kind_of[d.volumes[p[1]]["solid"]], p[2], p[3], p[4], vol_cls[i])
             for i, p in enumerate(placed)]
-    nav = navarrays.build(vols, kinds, out_cls)
+    nav, _ = navarrays.cached(vols, kinds, out_cls)
     sensor_of_vol = np.full(len(vols) + 1, -1, np.int32)
     names, groups = [], []
     for g in recipe.groups:

@@ -18,8 +18,12 @@ skips every BVH box over the part of its path inside the core.  ``skin_core`` sa
 
 from __future__ import annotations
 
+import getpass
+import hashlib
 import json
 import os
+import shutil
+import tempfile
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -131,8 +135,7 @@ def csg_tables(kinds: list[Solid], dtype=np.float32) -> dict:
     K = {k: np.asarray([x[j] for x in krows], np.int32) for j, k in enumerate(("l0", "nl", "p0", "pl"))}
     prog = prog or [(0, 0)]
     G = dict(op=np.asarray([g[0] for g in prog], np.int32), arg=np.asarray([g[1] for g in prog], np.int32))
-    return dict(T=T, K=K, G=G, empty_leaf=empty_leaf,
-                kind_box=[(0.5 * (k.lo + k.hi), 0.5 * (k.hi - k.lo)) for k in kinds])
+    return dict(T=T, K=K, G=G, empty_leaf=empty_leaf)
 
 
 # ---------------------------------------------------------------------------------------------- the skin core
@@ -295,3 +298,80 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
     stats = dict(volumes=n, kinds=len(kinds), groups=G, bvh_depth=depth,
                  max_daughters=max(len(k) for k in kids), skin_cores=len(cores))
     return NavArrays(host, csg, n, tuple(v.name for v in volumes), lo_w, hi_w, 0, precision, stats)
+
+
+# ---------------------------------------------------------------------------------------------- the build cache
+KIND = "nav"  # an entry's directory: <cache root>/build/nav-<key>
+FORMAT = "lumen-navarrays/1"
+_BUILDER_CODE = ("navarrays.py", "bvh.py", os.path.join("..", "csrc", "bvh_build.cpp"))  # what builds the arrays
+
+
+def cache_root() -> str:
+    """Return the per-user cache root (the reference's ``_env.cache_dir``): ``LUMEN_CACHE``, else ``lumen-<user>`` in
+    the system's temporary directory."""  # noqa: D205, D209
+    value = os.environ.get("LUMEN_CACHE")
+    return value if value else os.path.join(tempfile.gettempdir(), f"lumen-{getpass.getuser()}")
+
+
+def _digest_inputs(h, volumes, kinds, outside_cls, options):
+    """Feed a canonical encoding of the builder's inputs into hash h (arrays by dtype, shape and bytes)."""
+    def arr(x, dt=np.float64):
+        a = np.ascontiguousarray(np.asarray(x, dt))
+        h.update(f"{a.dtype.str}{a.shape}".encode())
+        h.update(a.tobytes())
+
+    h.update(f"{FORMAT}|{len(volumes)}|{len(kinds)}|{int(outside_cls)}|".encode())
+    h.update(json.dumps(options, sort_keys=True).encode())
+    for v in volumes:
+        h.update(f"|v|{v.name}|{int(v.kind)}|{int(v.parent)}|{int(v.cls)}|".encode())
+        arr(v.R)
+        arr(v.t)
+    for kd in kinds:
+        h.update(f"|k|{len(kd.leaves)}|".encode())
+        arr(np.asarray(kd.program, np.int64).reshape(-1, 2), np.int64)
+        arr(kd.lo)
+        arr(kd.hi)
+        for lf in kd.leaves:
+            h.update(f"|l|{int(lf.typ)}|".encode())
+            for x in (lf.origin, lf.qo, lf.A, lf.b, [lf.c], np.asarray(lf.N).reshape(-1, 3), lf.H):
+                arr(x)
+
+
+def key(volumes: list[Volume], kinds: list[Solid], outside_cls: int, **options) -> str:
+    """Return the cache key of the arrays ``build`` makes of these inputs (the reference's ``geometry/cache.py:175-183``
+    ``key``): sha256 of the inputs, the options and the builder's own source files."""  # noqa: D205, D209
+    h = hashlib.sha256()
+    _digest_inputs(h, volumes, kinds, outside_cls, options)
+    here = os.path.dirname(os.path.abspath(__file__))
+    for f in _BUILDER_CODE:
+        with open(os.path.join(here, f), "rb") as src:
+            h.update(hashlib.sha256(src.read()).digest())
+    h.update(np.__version__.encode())
+    return h.hexdigest()
+
+
+def cached(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision="float32",
+           leaf_size: int = bvh.LEAF_SIZE, root: str | None = None) -> tuple[NavArrays, str]:
+    """Return (the NavArrays of ``build``, their cache key): read from ``<root>/build/nav-<key>`` (``save``'s
+    ``arrays.npz`` + ``meta.json``), else built and stored there (the reference's ``geometry/cache.py:238-266``
+    ``nav_arrays``).  An entry is written into a temporary directory beside it and renamed into place, so a reader
+    never sees a partial one and concurrent builders of one key leave one entry."""  # noqa: D205, D209
+    options = dict(precision=precision, leaf_size=int(leaf_size))
+    k = key(volumes, kinds, outside_cls, **options)
+    base = os.path.join(root or cache_root(), "build")
+    path = os.path.join(base, f"{KIND}-{k}")
+    if os.path.isfile(os.path.join(path, "meta.json")):
+        return NavArrays.load(path), k
+    nav = build(volumes, kinds, outside_cls, precision, leaf_size)
+    os.makedirs(base, exist_ok=True)
+    tmp = tempfile.mkdtemp(prefix=f".{KIND}-", dir=base)
+    try:
+        nav.save(tmp)
+        try:
+            os.rename(tmp, path)
+        except OSError:  # another builder stored the same key first
+            pass
+    finally:
+        if os.path.isdir(tmp):
+            shutil.rmtree(tmp, ignore_errors=True)
+    return nav, k

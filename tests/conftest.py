@@ -12,6 +12,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP) GPU")
+    if "LUMEN_CACHE" not in os.environ:  # the navigator build cache of this session only
+        import tempfile
+
+        os.environ["LUMEN_CACHE"] = tempfile.mkdtemp(prefix="lumen-cache-")
 
 
 def gpu_available() -> bool:

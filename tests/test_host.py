@@ -314,3 +314,25 @@ def test_gdml_reads_the_added_solids_and_mie(tmp_path):
         assert np.array_equal(got, s.inside(x)), pv
         assert np.allclose(h["t"][v], {"pc": (-1, -1, 0), "pt": (1, -1, 0), "pp": (-1, 1, 0), "pe": (1, 1, 0),
                                        "pd": (0, 0, 1.2)}[pv])
+
+
+def test_navigator_build_cache(tmp_path):
+    """``navarrays.cached`` stores the arrays under build/nav-<sha256> and reads them back unchanged; the key follows
+    the inputs (a moved volume, another precision) and the same inputs give the same key."""
+    vols = [navarrays.Volume("box", 0, np.eye(3), np.zeros(3), -1, 0),
+            navarrays.Volume("ball", 1, np.eye(3), np.array([0.1, 0.0, 0.0]), 0, 1)]
+    kinds = [solids.box(1.0, 1.0, 1.0), solids.orb(0.2)]
+    nav, k = navarrays.cached(vols, kinds, 2, root=str(tmp_path))
+    assert re.fullmatch(r"[0-9a-f]{64}", k) and (tmp_path / "build" / f"nav-{k}" / "meta.json").is_file()
+    assert sorted(p.name for p in (tmp_path / "build").iterdir()) == [f"nav-{k}"]  # no temporary left behind
+    again, k2 = navarrays.cached(vols, kinds, 2, root=str(tmp_path))
+    assert k2 == k and again is not nav
+    for name, a in nav.host.items():
+        assert a.dtype == again.host[name].dtype and np.array_equal(a, again.host[name]), name
+    for tab in ("T", "K", "G"):
+        for name, a in nav.csg[tab].items():
+            assert np.array_equal(a, again.csg[tab][name]), (tab, name)
+    assert again.names == nav.names and again.stats == nav.stats and again.n == nav.n
+    moved = [vols[0], navarrays.Volume("ball", 1, np.eye(3), np.array([0.1, 0.0, 1e-7]), 0, 1)]
+    assert navarrays.key(moved, kinds, 2, precision="float32", leaf_size=bvh.LEAF_SIZE) != k
+    assert navarrays.cached(vols, kinds, 2, precision="float64", root=str(tmp_path))[1] != k
