@@ -6,7 +6,8 @@ per-group, per-navigator-id and per-volume arrays, and ``csg`` = (T, K, G) leaf,
 reference's column names.  ``save``/``load`` write and read ``arrays.npz`` plus ``meta.json`` in a directory.
 
 The builder here covers the solids this engine navigates (quadric-and-plane leaves; no meshes, no tori), volumes that
-lie inside their mother (no overlaps, so no ownership links) and world-frame daughter groups (``gR`` the identity).
+lie inside their mother (no overlaps, so no ownership links), and daughter groups in the world frame or, for daughters
+that share a rotation their world boxes fit badly, in that rotation's frame (``frame_groups``, ``gR``).
 Navigator ids: volumes 0..n-1, the outside n.  ``R``, ``t`` map a volume's frame to the world: x_w = R x + t.
 
 One array is this build's own, an acceleration the reference's format does not hold (absent, nothing changes):
@@ -218,9 +219,56 @@ def _world_box(kd: Solid, R, t):
     return cw - hw - pad, cw + hw + pad
 
 
+MIN_GROUP = 8  # a rotation shared by at least this many of a mother's daughters (or by all) may be a frame group
+GROUP_GAIN = 4.0  # ... when their world boxes are at least this times their own (summed surface terms)
+
+
+def _surface_terms(h):
+    return h[:, 0] * h[:, 1] + h[:, 1] * h[:, 2] + h[:, 2] * h[:, 0]
+
+
+def frame_groups(kids: list[int], volumes: list[Volume], kinds: list[Solid]) -> list:
+    """Return a mother's daughters as frame groups [(Rg, [daughters])], Rg the group frame's rotation (x_w = Rg x_g),
+    None for the world frame (the reference's ``geometry/bvh.py:217-255`` ``frame_groups_all``): a rotation (equal
+    entries) shared by at least MIN_GROUP daughters, or by all, that is not the identity and whose daughters' world
+    boxes are at least GROUP_GAIN times their own boxes (summed surface terms: what a BVH pays for) is a group of
+    its own, in the rotations' lexicographic order; the rest one world-frame group, last.  Daughters keep their
+    order within a group."""  # noqa: D205, D209
+    rows = np.stack([np.asarray(volumes[k].R, np.float64).reshape(9) for k in kids])
+    uniq, inv, cnt = np.unique(rows, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.ravel()
+    h = np.stack([0.5 * (kinds[volumes[k].kind].hi - kinds[volumes[k].kind].lo) for k in kids])
+    hw = np.einsum("nij,nj->ni", np.abs(rows.reshape(-1, 3, 3)), h)
+    tw, tb = _surface_terms(hw), _surface_terms(h)
+    out, rest = [], np.ones(len(kids), bool)
+    for g in range(len(uniq)):
+        m = np.flatnonzero(inv == g)
+        Rg = uniq[g].reshape(3, 3)
+        if not (cnt[g] >= MIN_GROUP or cnt[g] == len(kids)) or np.allclose(Rg, np.eye(3), atol=1e-9):
+            continue
+        if np.sum(tw[m]) < GROUP_GAIN * np.sum(tb[m]):
+            continue
+        out.append((Rg, [kids[i] for i in m]))
+        rest[m] = False
+    if rest.any():
+        out.append((None, [kids[i] for i in np.flatnonzero(rest)]))
+    return out
+
+
+def _group_box(kd: Solid, Rg, R, t):
+    """Return a placed solid's box (lo, hi) in a group frame (x_g = Rg^T x_w), padded as the world boxes."""
+    c, h = 0.5 * (kd.lo + kd.hi), 0.5 * (kd.hi - kd.lo)
+    M = Rg.T @ R
+    cg, hg = M @ c + Rg.T @ t, np.abs(M) @ h
+    pad = BOX_PAD_REL * (np.abs(cg) + hg) + BOX_PAD_ABS
+    return cg - hg - pad, cg + hg + pad
+
+
 def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision="float32",
-          leaf_size: int = bvh.LEAF_SIZE) -> NavArrays:
-    """Return the NavArrays of placed ``volumes`` (in depth-first order: a mother before its daughters)."""
+          leaf_size: int = bvh.LEAF_SIZE, framed: bool = True) -> NavArrays:
+    """Return the NavArrays of placed ``volumes`` (in depth-first order: a mother before its daughters).
+    ``framed``: rotated daughters in frame groups of their own (``frame_groups``), else every group in the world
+    frame (the same navigator, looser boxes)."""  # noqa: D205, D209
     fdt = np.float32 if precision == "float32" else np.float64
     n = len(volumes)
     if n == 0:
@@ -252,14 +300,22 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
     csg = csg_tables(list(kinds) + cores, fdt)
     nodes = dict(lo=[], hi=[], skip=[], start=[], count=[])
     items, g0, g1, vg0, vg1 = [], [], [], np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
+    gR = []
     depth = 0
     for u in range(n + 1):
         vg0[u] = len(g0)
-        if kids[u]:
-            ks = np.asarray(kids[u], np.int64)
-            tr = bvh.build(lo_w[ks], hi_w[ks], leaf_size)
+        groups = (frame_groups(kids[u], volumes, kinds) if framed else [(None, kids[u])]) if kids[u] else []
+        for Rg, members in groups:
+            ks = np.asarray(members, np.int64)
+            if Rg is None:
+                blo, bhi = lo_w[ks], hi_w[ks]
+            else:
+                bb = [_group_box(kinds[volumes[k].kind], Rg, volumes[k].R, volumes[k].t) for k in ks]
+                blo, bhi = np.stack([b_[0] for b_ in bb]), np.stack([b_[1] for b_ in bb])
+            tr = bvh.build(blo, bhi, leaf_size)
             base_n, base_i = len(nodes["lo"]), len(items)
             g0.append(base_n)
+            gR.append(np.eye(3) if Rg is None else Rg)
             nodes["lo"] += list(tr["lo"])
             nodes["hi"] += list(tr["hi"])
             nodes["skip"] += list(tr["skip"] + base_n)
@@ -282,7 +338,7 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
         hi=np.asarray(nodes["hi"] or np.zeros((1, 3))).astype(fdt),
         skip=np.asarray(nodes["skip"] or [0], np.int32), start=np.asarray(nodes["start"] or [0], np.int32),
         count=np.asarray(nodes["count"] or [0], np.int32), items=np.asarray(items or [0], np.int32),
-        gR=np.tile(np.eye(3), (max(G, 1), 1, 1)).astype(fdt),
+        gR=(np.stack(gR) if gR else np.eye(3)[None]).astype(fdt),
         g0=np.asarray(g0 or [0], np.int32), g1=np.asarray(g1 or [0], np.int32),
         vg0=vg0, vg1=vg1, vg1u=vg1.copy(),
         parent=np.concatenate([parent, [n]]).astype(np.int32),
@@ -295,7 +351,8 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
         ub=np.concatenate([[0], np.cumsum(nl)]).astype(np.int32),
         skin=np.where(core_of >= 0, csg["K"]["l0"][np.maximum(core_of, 0)], -1).astype(np.int32),
     )
-    stats = dict(volumes=n, kinds=len(kinds), groups=G, bvh_depth=depth,
+    framed_groups = int(sum(not np.array_equal(r, np.eye(3)) for r in gR))
+    stats = dict(volumes=n, kinds=len(kinds), groups=G, framed=framed_groups, bvh_depth=depth,
                  max_daughters=max(len(k) for k in kids), skin_cores=len(cores))
     return NavArrays(host, csg, n, tuple(v.name for v in volumes), lo_w, hi_w, 0, precision, stats)
 
@@ -356,7 +413,7 @@ def cached(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precisio
     ``arrays.npz`` + ``meta.json``), else built and stored there (the reference's ``geometry/cache.py:238-266``
     ``nav_arrays``).  An entry is written into a temporary directory beside it and renamed into place, so a reader
     never sees a partial one and concurrent builders of one key leave one entry."""  # noqa: D205, D209
-    options = dict(precision=precision, leaf_size=int(leaf_size))
+    options = dict(precision=precision, leaf_size=int(leaf_size), framed=True)
     k = key(volumes, kinds, outside_cls, **options)
     base = os.path.join(root or cache_root(), "build")
     path = os.path.join(base, f"{KIND}-{k}")

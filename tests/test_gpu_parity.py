@@ -341,3 +341,39 @@ def test_navigator_on_cones_ellipsoids_and_trds(gpu, tmp_path):
     err = np.abs(t32[same] - t64[same])
     good = err <= 1e-5 * t64[same] + 2e-6
     assert good.mean() >= 0.999 and err.max() < 1e-4, (err.max(), good.mean())
+
+
+def test_navigator_and_tallies_on_frame_groups(gpu, tmp_path):
+    """Plates sharing a rotation walked in their group's frame on the device: distances along random rays against
+    the FP64 referee, and the tallies of the same photons against the oracle's."""
+    from test_oracle import _framed_detector
+
+    det = _framed_detector(tmp_path)
+    assert det.nav.stats["framed"] == 1
+    rng = np.random.default_rng(23)
+    n = 8192
+    x = rng.uniform(-1.6, 1.6, (n, 3)) * np.array([1.0, 1.0, 0.2])  # the plates' layer, z in [-0.32, 0.32]
+    grid = np.array([(a, b, 0.0) for a in (-1.2, -0.4, 0.4, 1.2) for b in (-1.2, -0.4, 0.4, 1.2)])
+    x[:1024] = grid[rng.integers(0, 16, 1024)] + rng.uniform(-0.005, 0.005, (1024, 3))  # starts inside a plate
+    d = rng.normal(size=(n, 3))
+    aim = grid[rng.integers(0, 16, 3072)] + rng.uniform(-0.18, 0.18, (3072, 1)) * np.array([0.7071, 0.7071, 0.0])
+    d[1024:4096] = aim - x[1024:4096]  # aimed along a plate's long axis
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x32, d32 = x.astype(np.float32).astype(np.float64), d.astype(np.float32).astype(np.float64)
+    orc = O.OracleRun(det)
+    vol = orc.locate(x32)
+    t64, va64, _, _ = orc.query(x32, d32, vol)
+    t32, va32, _, _, loc = engine(det).query(x32, d32, vol)
+    assert np.mean(loc == vol) > 0.999 and np.all((vol[:1024] >= 1) & (vol[:1024] <= 16))
+    hit_plate = (va64 >= 1) & (va64 <= 16)
+    assert hit_plate.sum() > 1000
+    same = va32 == va64
+    assert same.mean() >= 0.999
+    err = np.abs(t32[same] - t64[same])
+    assert np.mean(err <= 1e-5 * t64[same] + 2e-6) >= 0.999 and err.max() < 1e-4, err.max()
+    src = L.sources.Point((0.05, 0.1, 0.6), lam=420.0, n_photons=1.0)
+    m = 1 << 16
+    g = L.simulate(det, src, samples=m, seed=6)
+    tal, h = orc.trace(src, m, seed=6, bvh=True)
+    o = O.light_of(tal, h, det.n_sensors, 1, m)
+    np.testing.assert_allclose(g.arrivals.cpu().numpy(), o["arrivals"], rtol=5e-3, atol=1e-9)

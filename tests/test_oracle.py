@@ -323,6 +323,33 @@ def test_bvh_walk_equals_brute_force():
     assert ha[12] == hb[12]
 
 
+def _framed_detector(tmp_path):
+    from lumen_amd.detector import DATA
+    from lumen_amd.detector.gdml import Group, Recipe, load
+    from toys import gdml_framed
+
+    f = tmp_path / "framed.gdml"
+    f.write_text(gdml_framed())
+    return load(str(f), Recipe("framed", "World", (Group("ball", "oV", values=str(DATA / "toy_pmt.json")),)))
+
+
+def test_frame_group_walk_equals_brute_force(tmp_path):
+    """Daughters sharing a rotation get a BVH in that rotation's frame (tighter boxes); the walk through it finds
+    the brute-force referee's crossings: the same photons give the same tallies."""
+    det = _framed_detector(tmp_path)
+    h = det.nav.host
+    assert det.nav.stats["framed"] == 1 and h["vg1"][0] - h["vg0"][0] == 2  # the plates' group, then the world's
+    Rg = np.asarray(h["gR"][h["vg0"][0]], np.float64)
+    assert abs(Rg[0, 0] - math.sqrt(0.5)) < 1e-6 and abs(Rg[2, 2] - 1.0) < 1e-6
+    o = O.OracleRun(det)
+    src = L.sources.Point((0.05, 0.1, 0.6), lam=420.0, n_photons=1.0)
+    a, ha = o.trace(src, 1 << 12, seed=5, bvh=True)
+    b, hb = o.trace(src, 1 << 12, seed=5, bvh=False)
+    assert a[0, :4].sum() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+    assert ha[12] == hb[12]
+
+
 def _timed_sphere(t0, dt, nt, a=1.5, n_idx=1.40):
     return sphere_toy(a=a, R=0.0, L_abs=None, n=n_idx).timed(t0, dt, nt)
 

@@ -174,3 +174,53 @@ GDML_SOLIDS = """<?xml version="1.0" encoding="UTF-8"?>
 <setup name="Default" version="1.0"><world ref="World"/></setup>
 </gdml>
 """
+
+
+def gdml_framed() -> str:
+    """A world of liquid holding 16 thin plates sharing one rotation (45 degrees about z: their world boxes are 6x
+    their own, a frame group), 3 plates tilted 30 degrees about x (too few for a group) and 4 orb sensors."""
+    pv = []
+    for i, (x, y) in enumerate((x, y) for x in (-1.2, -0.4, 0.4, 1.2) for y in (-1.2, -0.4, 0.4, 1.2)):
+        pv.append(f'<physvol name="plate{i}"><volumeref ref="plV"/><position name="pp{i}" x="{x}" y="{y}" z="0" '
+                  f'unit="m"/><rotationref ref="r45"/></physvol>')
+    for i, x in enumerate((-1.0, 0.0, 1.0)):
+        pv.append(f'<physvol name="tilt{i}"><volumeref ref="plV"/><position name="pt{i}" x="{x}" y="0.3" z="1.2" '
+                  f'unit="m"/><rotationref ref="rx30"/></physvol>')
+    for i, (x, y) in enumerate(((-0.8, -0.8), (0.8, -0.8), (-0.8, 0.8), (0.8, 0.8))):
+        pv.append(f'<physvol name="ball{i}"><volumeref ref="oV"/><position name="po{i}" x="{x}" y="{y}" z="-1.2" '
+                  f'unit="m"/></physvol>')
+    body = "\n    ".join(pv)
+    return f"""<?xml version="1.0" encoding="UTF-8"?>
+<gdml>
+<define>
+  <matrix name="RIDX" coldim="2" values="1.0*eV 1.33 12.0*eV 1.33"/>
+  <matrix name="RIDX2" coldim="2" values="1.0*eV 1.50 12.0*eV 1.50"/>
+  <matrix name="ABSL" coldim="2" values="1.0*eV 5*m 12.0*eV 5*m"/>
+  <matrix name="CREF" coldim="2" values="1.0*eV 0.0 12.0*eV 0.0"/>
+  <matrix name="CEFF" coldim="2" values="1.0*eV 1.0 12.0*eV 1.0"/>
+  <rotation name="r45" z="45" unit="deg"/>
+  <rotation name="rx30" x="30" unit="deg"/>
+</define>
+<materials>
+  <material name="Liq"><property name="RINDEX" ref="RIDX"/><property name="ABSLENGTH" ref="ABSL"/></material>
+  <material name="Glass"><property name="RINDEX" ref="RIDX2"/><property name="ABSLENGTH" ref="ABSL"/></material>
+</materials>
+<solids>
+  <box name="WorldBox" x="4" y="4" z="4" lunit="m"/>
+  <box name="Plate" x="400" y="20" z="20" lunit="mm"/>
+  <orb name="O" r="100" lunit="mm"/>
+  <opticalsurface name="Cathode" model="unified" finish="polished" type="dielectric_metal" value="0">
+    <property name="REFLECTIVITY" ref="CREF"/><property name="EFFICIENCY" ref="CEFF"/>
+  </opticalsurface>
+</solids>
+<structure>
+  <volume name="plV"><materialref ref="Glass"/><solidref ref="Plate"/></volume>
+  <volume name="oV"><materialref ref="Glass"/><solidref ref="O"/></volume>
+  <volume name="World"><materialref ref="Liq"/><solidref ref="WorldBox"/>
+    {body}
+  </volume>
+  <skinsurface name="cathodes" surfaceproperty="Cathode"><volumeref ref="oV"/></skinsurface>
+</structure>
+<setup name="Default" version="1.0"><world ref="World"/></setup>
+</gdml>
+"""
