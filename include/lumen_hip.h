@@ -148,7 +148,10 @@ int lh_model_info(lh_model model, int32_t *n_sensors, int32_t *n_qe, int64_t *ta
  * FP64).  Adds: zero
  * them first for a fresh run.  `values_dev`: the packed values (float32, device, n_values of them, each table
  * within them); `ref_values_dev`: the reference values every random choice is made at (NULL: values_dev); every
- * source's event is below n_events.  Replaces run.trace + run.rebuild (src/lumen/run.py:93-145). */
+ * source's event is below n_events.  The call only enqueues work on `stream` (the sources go through the model's
+ * pinned staging buffers): it returns before the kernel ends, and the model's calls are ordered by their stream,
+ * so calls on one model use one stream (or the caller orders them).  Replaces run.trace + run.rebuild
+ * (src/lumen/run.py:93-145). */
 int lh_trace(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
              const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
              double *tally_dev, int32_t n_events, double *health_dev, void *stream);
@@ -193,8 +196,16 @@ int lh_query(lh_model model, int32_t n, const float *x_dev, const float *d_dev, 
              const int32_t *last_dev, float tmax, float *t_dev, int32_t *vol_after_dev, float *normal_dev,
              int32_t *srf_dev, int32_t *located_dev, void *stream);
 
-/* The time of the last lh_trace / lh_adjoint's photon-loop kernel on its stream, in ms (HIP events). */
+/* The time of the last lh_trace / lh_adjoint's photon-loop kernel on its stream, in ms (HIP events); waits for that
+ * kernel to finish. */
 int lh_last_kernel_ms(lh_model model, float *ms);
+
+/* Launch bookkeeping, so a caller times launches without waiting after each: lh_launches gives the number of
+ * lh_trace / lh_adjoint launches made on the model so far (the next one is launch number *count), lh_kernel_ms_of
+ * the kernel time of launch number `launch`, one of the last 256 (it waits for that kernel to finish).
+ * Diagnostics: no counterpart in the reference (its runs are timed from the host). */
+int lh_launches(lh_model model, int64_t *count);
+int lh_kernel_ms_of(lh_model model, int64_t launch, float *ms);
 
 /* The shape of the model's last launch (8 int32): grid blocks, resident blocks per CU, dynamic LDS bytes per
  * block, tallies in LDS (0/1), BVH nodes in LDS (0/1), the LDS budget per block, the CU's LDS bytes, the kernel's

@@ -59,7 +59,7 @@ _LIB = None
 
 SYMBOLS = ("lh_model_create", "lh_model_destroy", "lh_model_info", "lh_trace", "lh_adjoint", "lh_query",
            "lh_batch_create", "lh_batch_destroy", "lh_batch_trace", "lh_batch_adjoint", "lh_last_kernel_ms",
-           "lh_last_launch", "lh_model_jump", "lh_bvh_build", "lh_last_error")
+           "lh_last_launch", "lh_model_jump", "lh_bvh_build", "lh_launches", "lh_kernel_ms_of", "lh_last_error")
 
 
 def lib():
@@ -91,6 +91,8 @@ def lib():
                                        C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                        C.c_void_p, C.c_void_p]
         L.lh_last_kernel_ms.argtypes = [C.c_void_p, _f]
+        L.lh_launches.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.lh_kernel_ms_of.argtypes = [C.c_void_p, C.c_int64, _f]
         L.lh_last_launch.argtypes = [C.c_void_p, _i]
         L.lh_model_jump.argtypes = [C.c_void_p, C.c_int32, _f, C.c_float, _i, _f]
         L.lh_bvh_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 6 + [

@@ -111,6 +111,29 @@ class Light:
         return torch.utils.dlpack.to_dlpack(getattr(self, name).contiguous())
 
 
+def _lazy(name, convert):
+    """A property over ``_<name>``: values that need the run to have finished (``simulate`` returns before it does)
+    are converted on first read, by ``convert``, and kept."""  # noqa: D205, D209
+    key = "_" + name
+
+    def get(self):
+        v = self.__dict__.get(key)
+        if callable(v) and not isinstance(v, Health):
+            v = convert(v())
+            self.__dict__[key] = v
+        return v
+
+    def put(self, v):
+        self.__dict__[key] = v
+
+    return property(get, put)
+
+
+# the run's health and kernel time: given as a callable by ``simulate`` (read after the kernel ends), else a value
+Light.health = _lazy("health", lambda h: h if isinstance(h, Health) else Health.of(h))
+Light.kernel_ms = _lazy("kernel_ms", float)
+
+
 def moment_error(q2, s, n: int):
     """Return sqrt(n / (n - 1) (q2 - s^2 / n)) clipped at 0: the error of a sum of n i.i.d. photons from its per-photon
     second moment (``readout/grid.py:48-60``)."""  # noqa: D205, D209
@@ -145,6 +168,7 @@ class Engine:
             raise _lib.EngineError(f"the engine's tally layout ({self.stride} cells per event) is not the detector's")
         weakref.finalize(self, _lib.lib().lh_model_destroy, h)
         self.jump_grids = None  # the diffusion jump's clearance grids, built on first use
+        self._uploads = {}  # packed values by size: (the host array, its device copy), the last of each size
 
     def ensure_jump(self):
         """Build and upload the diffusion jump's clearance grids (once per engine); return them."""
@@ -194,10 +218,32 @@ class Engine:
         torch.cuda.synchronize(dev)
         return t.cpu().numpy(), va.cpu().numpy(), nrm.cpu().numpy(), srf.cpu().numpy(), loc.cpu().numpy()
 
-    def kernel_ms(self) -> float:
+    def kernel_ms(self, launch: int | None = None) -> float:
+        """Return the kernel time (ms) of launch number ``launch`` on this model (None: the last); waits for it."""
         ms = _lib.C.c_float()
-        _lib.check(_lib.lib().lh_last_kernel_ms(self.handle, _lib.C.byref(ms)), "lh_last_kernel_ms")
+        if launch is None:
+            _lib.check(_lib.lib().lh_last_kernel_ms(self.handle, _lib.C.byref(ms)), "lh_last_kernel_ms")
+        else:
+            _lib.check(_lib.lib().lh_kernel_ms_of(self.handle, int(launch), _lib.C.byref(ms)), "lh_kernel_ms_of")
         return ms.value
+
+    def launches(self) -> int:
+        """Return the number of lh_trace / lh_adjoint launches made on this model."""
+        n = _lib.C.c_int64()
+        _lib.check(_lib.lib().lh_launches(self.handle, _lib.C.byref(n)), "lh_launches")
+        return n.value
+
+    def upload(self, values: np.ndarray):
+        """Return the packed values on the device, the previous upload again when they are the same (a copy from
+        pageable memory waits for the stream; unchanged values need none)."""  # noqa: D205, D209
+        import torch
+
+        last = self._uploads.get(values.nbytes)
+        if last is not None and np.array_equal(last[0], values):
+            return last[1]
+        t = torch.as_tensor(values, device=self.device)
+        self._uploads[values.nbytes] = (values.copy(), t)
+        return t
 
 
 _ENGINES: dict = {}
@@ -266,7 +312,7 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     pk = packing.pack(det, p, edge_rows=edge_rows)
     if not np.array_equal(pk.layout, eng.layout):
         raise ValueError("simulate: these values change the detector's structure (a table's knot count)")
-    values = torch.as_tensor(pk.values, device=eng.device)
+    values = eng.upload(pk.values)
     ref_values = None
     if reference is not None:
         pr = packing.pack(det, reference, edge_rows=edge_rows)
@@ -286,8 +332,13 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
                                    _lib.C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), _lib.C.byref(ctl),
                                    _lib.C.c_void_p(tally.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
                                    _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
-    light = light_of(eng, tally, health.cpu().numpy(), samples if not isinstance(source, Batch) else None, source)
-    light.kernel_ms = eng.kernel_ms()
+    launch = eng.launches() - 1
+    for t in (values, ref_values):  # read by the kernel after this returns: kept from reuse on other streams
+        if t is not None and stream is not None:
+            t.record_stream(st)
+    light = light_of(eng, tally, lambda: health.cpu().numpy(), samples if not isinstance(source, Batch) else None,
+                     source)
+    light.kernel_ms = lambda: eng.kernel_ms(launch)  # the call returns before the kernel ends: read on first use
     tg = getattr(det, "time_grid", None)
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, controls=controls._asdict(),
                             engine="liblumen_hip (gfx950)", reference=reference is not None,
@@ -319,5 +370,6 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
         arr, err, mean_t, charge, arr2 = arr[0], err[0], mean_t[0], charge[0], arr2[0]
         if time is not None:
             time, late, early = time[0], late[0], early[0]
-    return Light(arr, err, mean_t, charge, arr2, Health.of(health), samples or 0, time=time, late=late, early=early,
+    health = health if callable(health) else Health.of(health)
+    return Light(arr, err, mean_t, charge, arr2, health, samples or 0, time=time, late=late, early=early,
                  charge_axes=(eng.Kl, eng.Kt))

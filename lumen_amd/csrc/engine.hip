@@ -1020,6 +1020,8 @@ struct SrcBuf {
   int cap = 0;
 };
 
+constexpr int TIME_RING = 256;  // launches whose kernel times a model keeps
+
 struct lh_model_s {
   Geo g;
   void *geo_buf = nullptr;
@@ -1041,8 +1043,15 @@ struct lh_model_s {
   int src_cap = 0;
 
   long long values_needed = 0;  // the packed values a call must pass (the largest table's end)
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int timed = 0;
+  // the photon-loop kernel's start and end of the last TIME_RING launches (launch k in slot k % TIME_RING)
+  hipEvent_t ev0[TIME_RING] = {}, ev1[TIME_RING] = {};
+  long long launches = 0;
+  // pinned host staging for the sources, two buffers used in turn: a call copies from one while the copy from the
+  // other may still be queued, so no call waits for the stream (the `done` event marks a buffer's copy finished)
+  lh_source *pin_src[2] = {};
+  long long *pin_end[2] = {};
+  hipEvent_t pin_done[2] = {};
+  int pin_cap = 0, pin_next = 0;
   float *ut = nullptr;  // the unified facet tables on the device
   int cus = 0, lds_per_cu = 0, lds_per_block = 0;
   bool has_chan = false;  // a medium has a Mie or WLS channel
@@ -1385,8 +1394,11 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   HIP_TRY(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   m->lds_per_block = lds_blk > 0 ? lds_blk : 64 * 1024;
   m->cus = cus;
-  HIP_TRY(hipEventCreate(&m->ev0));
-  HIP_TRY(hipEventCreate(&m->ev1));
+  for (int k = 0; k < TIME_RING; ++k) {
+    HIP_TRY(hipEventCreate(&m->ev0[k]));
+    HIP_TRY(hipEventCreate(&m->ev1[k]));
+  }
+  for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreateWithFlags(&m->pin_done[k], hipEventDisableTiming));
   *out = m;
   return LH_OK;
 }
@@ -1401,8 +1413,16 @@ LH_API int lh_model_destroy(lh_model m) {
   (void)hipFree(m->ut);
   (void)hipFree(m->jrec);
   (void)hipFree(m->jclear);
-  if (m->ev0) (void)hipEventDestroy(m->ev0);
-  if (m->ev1) (void)hipEventDestroy(m->ev1);
+  for (int k = 0; k < 2; ++k) {
+    if (m->pin_done[k]) (void)hipEventSynchronize(m->pin_done[k]);
+    (void)hipHostFree(m->pin_src[k]);
+    (void)hipHostFree(m->pin_end[k]);
+    if (m->pin_done[k]) (void)hipEventDestroy(m->pin_done[k]);
+  }
+  for (int k = 0; k < TIME_RING; ++k) {
+    if (m->ev0[k]) (void)hipEventDestroy(m->ev0[k]);
+    if (m->ev1[k]) (void)hipEventDestroy(m->ev1[k]);
+  }
   delete m;
   return LH_OK;
 }
@@ -1456,13 +1476,20 @@ LH_API int lh_model_info(lh_model m, int32_t *n_sensors, int32_t *n_qe, int64_t 
   return LH_OK;
 }
 
-// copy the sources and their photon prefix sums into `buf` (grown as needed) and check them
+// the host side of a staging: the sources and their photon prefix sums, to be copied from (stage_into)
+struct Staging {
+  const lh_source *src;
+  const long long *end;
+};
+
+// copy the sources and their photon prefix sums into `buf` (grown as needed) and check them; with `m`, through the
+// model's pinned buffers without waiting for the stream, else from pageable memory, waiting for the copy
 static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_sources, long long *total,
-                      hipStream_t st);
+                      hipStream_t st, lh_model m = nullptr);
 
 static int stage_sources(lh_model m, const lh_source *sources, int n_sources, long long *total, hipStream_t st) {
   SrcBuf b{m->src, m->src_end, m->src_cap};
-  const int rc = stage_into(b, m->n_media, sources, n_sources, total, st);
+  const int rc = stage_into(b, m->n_media, sources, n_sources, total, st, m);
   m->src = b.src;
   m->src_end = b.src_end;
   m->src_cap = b.cap;
@@ -1470,7 +1497,7 @@ static int stage_sources(lh_model m, const lh_source *sources, int n_sources, lo
 }
 
 static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_sources, long long *total,
-                      hipStream_t st) {
+                      hipStream_t st, lh_model m) {
   if (!sources || n_sources <= 0) return fail(LH_ERR_ARG, "no sources");
   std::vector<long long> ends(n_sources);
   long long acc = 0;
@@ -1495,10 +1522,37 @@ static int stage_into(SrcBuf &buf, int n_media, const lh_source *sources, int n_
     HIP_TRY(hipMalloc((void **)&buf.src_end, sizeof(long long) * n_sources));
     buf.cap = n_sources;
   }
-  HIP_TRY(hipMemcpyAsync(buf.src, sources, sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(buf.src_end, ends.data(), sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipStreamSynchronize(st));  // the host staging vector goes out of scope
   *total = acc;
+  if (!m) {
+    HIP_TRY(hipMemcpyAsync(buf.src, sources, sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(buf.src_end, ends.data(), sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));  // the host staging vector goes out of scope
+    return LH_OK;
+  }
+  if (n_sources > m->pin_cap) {  // both buffers regrown, once their queued copies have finished
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(hipEventSynchronize(m->pin_done[k]));
+      (void)hipHostFree(m->pin_src[k]);
+      (void)hipHostFree(m->pin_end[k]);
+      m->pin_src[k] = nullptr;
+      m->pin_end[k] = nullptr;
+    }
+    m->pin_cap = 0;
+    const int cap = std::max(n_sources, 64);
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(hipHostMalloc((void **)&m->pin_src[k], sizeof(lh_source) * cap, hipHostMallocDefault));
+      HIP_TRY(hipHostMalloc((void **)&m->pin_end[k], sizeof(long long) * cap, hipHostMallocDefault));
+    }
+    m->pin_cap = cap;
+  }
+  const int k = m->pin_next;
+  m->pin_next ^= 1;
+  HIP_TRY(hipEventSynchronize(m->pin_done[k]));  // this buffer's previous copy has left it
+  memcpy(m->pin_src[k], sources, sizeof(lh_source) * n_sources);
+  memcpy(m->pin_end[k], ends.data(), sizeof(long long) * n_sources);
+  HIP_TRY(hipMemcpyAsync(buf.src, m->pin_src[k], sizeof(lh_source) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(buf.src_end, m->pin_end[k], sizeof(long long) * n_sources, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(m->pin_done[k], st));
   return LH_OK;
 }
 
@@ -1679,11 +1733,12 @@ LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n
   if (rc) return rc;
   a.nsrc = n_sources;
   a.total = total;
-  HIP_TRY(hipEventRecord(m->ev0, st));
+  const int slot = (int)(m->launches % TIME_RING);
+  HIP_TRY(hipEventRecord(m->ev0[slot], st));
   launch_trace(m, a, n_events, ctl, st);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m->ev1, st));
-  m->timed = 1;
+  HIP_TRY(hipEventRecord(m->ev1[slot], st));
+  ++m->launches;
   return LH_OK;
 }
 
@@ -1716,13 +1771,14 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   AdjArgs b{cot, slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, m->queue);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m->ev0, st));
+  const int slot = (int)(m->launches % TIME_RING);
+  HIP_TRY(hipEventRecord(m->ev0[slot], st));
   const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), BLOCK,
                                  (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(m->ev1, st));
-  m->timed = 1;
+  HIP_TRY(hipEventRecord(m->ev1[slot], st));
+  ++m->launches;
   return LH_OK;
 }
 
@@ -1856,8 +1912,22 @@ LH_API int lh_last_launch(lh_model m, int32_t *info) {
 
 LH_API int lh_last_kernel_ms(lh_model m, float *ms) {
   if (!m || !ms) return fail(LH_ERR_ARG, "lh_last_kernel_ms: null argument");
-  if (!m->timed) return fail(LH_ERR_ARG, "lh_last_kernel_ms: nothing launched yet");
-  HIP_TRY(hipEventSynchronize(m->ev1));
-  HIP_TRY(hipEventElapsedTime(ms, m->ev0, m->ev1));
+  if (m->launches == 0) return fail(LH_ERR_ARG, "lh_last_kernel_ms: nothing launched yet");
+  return lh_kernel_ms_of(m, m->launches - 1, ms);
+}
+
+LH_API int lh_launches(lh_model m, int64_t *count) {
+  if (!m || !count) return fail(LH_ERR_ARG, "lh_launches: null argument");
+  *count = m->launches;
+  return LH_OK;
+}
+
+LH_API int lh_kernel_ms_of(lh_model m, int64_t launch, float *ms) {
+  if (!m || !ms) return fail(LH_ERR_ARG, "lh_kernel_ms_of: null argument");
+  if (launch < 0 || launch >= m->launches || launch < m->launches - TIME_RING)
+    return fail(LH_ERR_ARG, "lh_kernel_ms_of: the launch is not among the last 256 of this model");
+  const int slot = (int)(launch % TIME_RING);
+  HIP_TRY(hipEventSynchronize(m->ev1[slot]));
+  HIP_TRY(hipEventElapsedTime(ms, m->ev0[slot], m->ev1[slot]));
   return LH_OK;
 }
