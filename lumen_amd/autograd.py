@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib, pack as packing
-from .api import Controls, Light, engine, light_of, source_rows
+from .api import Controls, Light, _lazy, engine, light_of, source_rows
 from .params import Params, Table
 
 
@@ -42,6 +42,11 @@ class _Run:
     adjoint_ms: float = float("nan")
 
 
+# the launches' kernel times: read on first use (the launches only enqueue)
+_Run.kernel_ms = _lazy("kernel_ms", float)
+_Run.adjoint_ms = _lazy("adjoint_ms", float)
+
+
 def _trace(run: _Run, values):
     import torch
 
@@ -57,7 +62,8 @@ def _trace(run: _Run, values):
                                    _lib.C.c_void_p(tally.data_ptr()), run.n_ev, _lib.C.c_void_p(health.data_ptr()),
                                    _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
     run.health = health
-    run.kernel_ms = eng.kernel_ms()
+    launch = eng.launches() - 1
+    run.kernel_ms = lambda: eng.kernel_ms(launch)
     return tally
 
 
@@ -80,7 +86,8 @@ def adjoint(run: _Run, values, cot):
                                      _lib.C.c_void_p(health.data_ptr()), _lib.C.c_void_p(st.cuda_stream)),
                "lh_adjoint")
     run.adj_health = health
-    run.adjoint_ms = eng.kernel_ms()
+    launch = eng.launches() - 1
+    run.adjoint_ms = lambda: eng.kernel_ms(launch)
     return grad[: run.n_grad]
 
 
@@ -139,11 +146,8 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
             raise ValueError(f"simulate_diff: {k}: the adjoint differentiates the path's rates and surface shares; "
                              "a re-emission's spectrum, yield and delay are calibrated by value (finite differences)")
     pk = packing.pack(det, p, grad_keys=keys, edge_rows=edge_rows)
-    values = torch.as_tensor(pk.values, device=eng.device)
-    slots = torch.as_tensor(pk.slots, device=eng.device)
-    idx = np.flatnonzero(pk.slots >= 0)
-    value_index = torch.as_tensor(idx, device=eng.device, dtype=torch.long)
-    slot_of_value = torch.as_tensor(pk.slots[idx], device=eng.device, dtype=torch.long)
+    values = eng.upload(pk.values)
+    slots, value_index, slot_of_value = _slot_tensors(eng, pk.slots)
     # scatter the tensors into the values, differentiably: each key's slots in order are its table's values
     starts = {k: (s0, n) for k, s0, n in pk.grad_keys}
     parts = []
@@ -168,12 +172,28 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
     run = _Run(eng, rows, n_ev, int(seed) & 0xFFFFFFFFFFFFFFFF, controls, ref_values, slots, pk.n_grad,
                value_index, slot_of_value)
     tally = _PL.apply(vals, run)
-    light = light_of(eng, tally, run.health.detach().cpu().numpy() if run.health is not None else np.zeros(16),
+    health = run.health
+    light = light_of(eng, tally, lambda: health.detach().cpu().numpy(),
                      samples if not hasattr(source, "sources") else None, source)
-    light.kernel_ms = run.kernel_ms
+    light.kernel_ms = lambda: run.kernel_ms
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, derivative="adjoint (replay)",
                             keys=keys, run=run)
     return light
+
+
+def _slot_tensors(eng, slots: np.ndarray):
+    """Return (slots int32, the packed values' indices that carry a slot, their slots) on the device, the previous
+    ones again when the slots are the same (uploads from pageable memory wait for the stream)."""  # noqa: D205, D209
+    import torch
+
+    last = getattr(eng, "_slot_upload", None)
+    if last is not None and np.array_equal(last[0], slots):
+        return last[1]
+    idx = np.flatnonzero(slots >= 0)
+    out = (torch.as_tensor(slots, device=eng.device), torch.as_tensor(idx, device=eng.device, dtype=torch.long),
+           torch.as_tensor(slots[idx], device=eng.device, dtype=torch.long))
+    eng._slot_upload = (slots.copy(), out)
+    return out
 
 
 def table_tensor(params: Params, key, requires_grad=True, device="cuda"):
