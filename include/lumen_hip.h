@@ -53,8 +53,10 @@ typedef struct lh_model_s *lh_model;
 
 /* The navigator arrays, by the reference's names (src/lumen/geometry/navigator_build.py:371-397 `host`,
  * src/lumen/geometry/csg_build.py:140-165 `T`, `K`, `G`).  Navigator ids: volumes 0..n_vol-1, the outside n_vol.
- * R, t: x_world = R x_local + t (R row-major 3x3).  A leaf is {(x-q)'A(x-q) + 2 b.(x-q) + c <= 0 (typ 1)} and
- * {n_k.(x-o) <= h_k}; a kind's program is postfix (op 0 push leaf arg, 1 union, 2 inter, 3 sub). */
+ * R, t: x_world = R x_local + t (R row-major 3x3).  A leaf is {(x-q)'A(x-q) + 2 b.(x-q) + c <= 0 (typ 1)}, or the
+ * solid torus about the unit axis b through q with swept radius A[0] and tube radius A[1] (typ 2: the reference's
+ * `R`, `r` columns), and {n_k.(x-o) <= h_k}; a kind's program is postfix (op 0 push leaf arg, 1 union, 2 inter,
+ * 3 sub). */
 typedef struct {
   int32_t n_vol, n_nodes, n_items, n_groups, n_leaves, n_planes, n_prog, n_kinds;
   const float *R;          /* n_vol*9 */
@@ -209,7 +211,8 @@ int lh_kernel_ms_of(lh_model model, int64_t launch, float *ms);
 
 /* The shape of the model's last launch (8 int32): grid blocks, resident blocks per CU, dynamic LDS bytes per
  * block, tallies in LDS (0/1), BVH nodes in LDS (0/1), the LDS budget per block, the CU's LDS bytes, the kernel's
- * feature bits (1 separate reference values, 2 a wavelength shifter, 4 a time histogram; 0x100 the adjoint).
+ * feature bits (1 separate reference values, 2 a wavelength shifter, 4 a time histogram, 8 BVH nodes read from
+ * memory, 16 torus leaves; 0x100 the adjoint).
  * Diagnostics: no counterpart in the reference. */
 int lh_last_launch(lh_model model, int32_t *info);
 

@@ -320,7 +320,9 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
             raise ValueError("simulate: the reference's structure differs from the detector's")
         ref_values = torch.as_tensor(pr.values, device=eng.device)
     rows, n_ev = source_rows(det, source, samples, first_id)
-    tally = torch.zeros((n_ev, eng.stride), dtype=torch.float64, device=eng.device)
+    # at least one cell: a detector without sensors has no tallies, and the C ABI takes no null buffer
+    buf = torch.zeros((n_ev, max(eng.stride, 1)), dtype=torch.float64, device=eng.device)
+    tally = buf[:, :eng.stride]
     health = torch.zeros(_lib.HEALTH_N, dtype=torch.float64, device=eng.device)
     st = torch.cuda.current_stream(eng.device) if stream is None else stream
     ctl = controls.struct()
@@ -330,7 +332,7 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
                                    None if ref_values is None else _lib.C.c_void_p(ref_values.data_ptr()),
                                    int(values.numel()), rows, len(rows),
                                    _lib.C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), _lib.C.byref(ctl),
-                                   _lib.C.c_void_p(tally.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
+                                   _lib.C.c_void_p(buf.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
                                    _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
     launch = eng.launches() - 1
     for t in (values, ref_values):  # read by the kernel after this returns: kept from reuse on other streams
@@ -352,12 +354,12 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
 
     S, K = eng.S, eng.K
     arr, arr_t, arr2 = tally[:, :S], tally[:, S:2 * S], tally[:, 2 * S:3 * S]
-    charge = tally[:, 3 * S:3 * S + S * K].reshape(-1, S, K)
+    charge = tally[:, 3 * S:3 * S + S * K].reshape(tally.shape[0], S, K)
     time = late = early = None
     if eng.nt > 0:
         t0 = 3 * S + S * K
-        time = tally[:, t0:t0 + S * eng.nt].reshape(-1, S, eng.nt)
-        ov = tally[:, t0 + S * eng.nt:t0 + S * (eng.nt + 4)].reshape(-1, S, 2, 2)
+        time = tally[:, t0:t0 + S * eng.nt].reshape(tally.shape[0], S, eng.nt)
+        ov = tally[:, t0 + S * eng.nt:t0 + S * (eng.nt + 4)].reshape(tally.shape[0], S, 2, 2)
         late, early = ov[:, :, 0], ov[:, :, 1]
     if isinstance(source, Batch):
         n = torch.as_tensor([max(int(x), 2) for x in source.samples], dtype=torch.float64, device=tally.device)

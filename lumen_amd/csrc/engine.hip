@@ -58,8 +58,8 @@ constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many ce
 constexpr int OCC_CACHE = 8;
 // a kernel's features (template parameter): separate reference values, bulk channels beyond Rayleigh (Mie,
 // WLS), the time histogram, BVH nodes read from global memory
-constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8;
-constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES;  // the features an adjoint kernel is specialised on
+constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8, F_TORUS = 16;
+constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES | F_TORUS;  // the features an adjoint kernel is specialised on
 
 // The forward kernel's register budget: 3 wavefronts per SIMD (168 VGPRs) hold the step without spills; at 4
 // (128 VGPRs) about 77 VGPRs spill to scratch, several GB of scratch traffic per launch, and the 1,000-PMT tank
@@ -287,6 +287,7 @@ __device__ __forceinline__ int source_of(const Args &a, long long gi) {
 }
 
 // emission: a source row's photon (sources/point.py:73-86, sources/cherenkov.py:33-66), located (item.py:24-36)
+template <bool TOR>
 __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   const int j = source_of(a, gi);
   const lh_source &S = a.src[j];
@@ -338,7 +339,7 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   }
   p.w = 1.0f;
   p.wr = 1.0f;
-  p.vol = locate(a.g, p.x);
+  p.vol = locate<TOR>(a.g, p.x);
   p.cls = a.g.vinfo[p.vol].z;
   p.srf = -1;
   p.nz = 0;
@@ -526,7 +527,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   if (p.nz >= ZERO_STEPS_BEFORE_PUSH) {  // G4Navigator's stuck-track push (step.py:269-282 `_recover`)
     if (p.nz == ZERO_STEPS_BEFORE_PUSH) rare_add(RF_PUSHED, p.w * p.w0);
     p.x = fma3(PUSH_BANDS * b0, p.d, p.x);
-    p.vol = locate(g, p.x);
+    p.vol = locate<(F & F_TORUS) != 0>(g, p.x);
     p.cls = g.vinfo[p.vol].z;
     p.srf = -1;
     b0 = band_at(maxabs3(p.x));
@@ -544,7 +545,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const float mu_r = CHAN ? (mr.mu_ray + mr.mu_mie) + mr.mu_wls : mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
-  const Cross cr = query_t<!(F & F_GNODES)>(g, p.x, p.d, p.vol, p.srf, max_dist);
+  const Cross cr = query_t<!(F & F_GNODES), (F & F_TORUS) != 0>(g, p.x, p.d, p.vol, p.srf, max_dist);
   const bool hit = cr.vol_after >= 0;
   const float dist = cr.t;
   const bool zero = dist <= ZERO_STEP_BANDS * b0;
@@ -868,9 +869,9 @@ __global__ void query_kernel(Geo g, int n, const float *__restrict__ x, const fl
   if (i >= n) return;
   const float3 xi = mk3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
   const float3 di = unit3(mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
-  const int v = vol[i] >= 0 ? vol[i] : locate(g, xi);
-  located[i] = locate(g, xi);
-  const Cross c = query(g, xi, di, v, last[i], tmax);
+  const int v = vol[i] >= 0 ? vol[i] : locate<true>(g, xi);
+  located[i] = locate<true>(g, xi);
+  const Cross c = query<true>(g, xi, di, v, last[i], tmax);
   t_out[i] = c.t;
   vol_out[i] = c.vol_after;
   n_out[3 * i] = c.n.x;
@@ -918,7 +919,7 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
       }
       const int rank = lanes_below(mask);
       const bool take = need && rank < avail;
-      if (take) alive = emit(a, cursor + rank, p, acc);
+      if (take) alive = emit<(F & F_TORUS) != 0>(a, cursor + rank, p, acc);
       cursor += min((long long)__popcll(mask), avail);
       need = need && !take;
       mask = __ballot(need);
@@ -980,7 +981,7 @@ __global__ void __launch_bounds__(BLOCK)
       const bool take = need && rank < avail;
       if (take) {
         p.gi = cursor + rank;
-        alive = emit(a, p.gi, p, acc);
+        alive = emit<(F & F_TORUS) != 0>(a, p.gi, p, acc);
         p.pass = 0;
         p.T = p.Dsum = 0.0f;
         p.hasdep = false;
@@ -995,7 +996,7 @@ __global__ void __launch_bounds__(BLOCK)
       if (!alive && p.pass == 0 && p.hasdep) {  // replay the photon from its counter, now adding the terms
         const float T = p.Dsum;
         Acc scratch{};
-        alive = emit(a, p.gi, p, scratch);
+        alive = emit<(F & F_TORUS) != 0>(a, p.gi, p, scratch);
         p.pass = 1;
         p.T = T;
         p.Dsum = 0.0f;
@@ -1054,7 +1055,8 @@ struct lh_model_s {
   int pin_cap = 0, pin_next = 0;
   float *ut = nullptr;  // the unified facet tables on the device
   int cus = 0, lds_per_cu = 0, lds_per_block = 0;
-  bool has_chan = false;  // a medium has a Mie or WLS channel
+  bool has_chan = false;   // a medium has a Mie or WLS channel
+  bool has_torus = false;  // a CSG leaf is a torus (the kernels with the torus root isolation)
   // the last launch's shape: grid blocks, blocks per CU, LDS bytes per block, tallies in LDS, nodes in LDS, the
   // LDS budget per block, the CU's LDS, the kernel's feature bits
   int32_t last_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1187,6 +1189,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   }
 
   // the device layout: float4 / int4 arrays in one allocation
+  bool has_torus = false;
   std::vector<float4> xf(3 * (size_t)n), gxf(3 * (size_t)std::max(G->n_groups, 1)), node(2 * (size_t)std::max(G->n_nodes, 1)),
       leaf(4 * (size_t)G->n_leaves), plane((size_t)std::max(G->n_planes, 1));
   std::vector<int4> vinfo((size_t)n + 1), kinds((size_t)G->n_kinds), vkind((size_t)n + 1);
@@ -1225,6 +1228,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   for (int l = 0; l < G->n_leaves; ++l) {
     const float *q = G->leaf_q + 3 * l, *o = G->leaf_o + 3 * l, *A = G->leaf_A + 9 * l, *b = G->leaf_b + 3 * l;
     int typ = G->leaf_typ[l], p0 = G->leaf_p0[l], np = G->leaf_np[l];
+    if (typ < 0 || typ > 2) return fail(LH_ERR_ARG, "lh_model_create: a CSG leaf type other than 0, 1, 2");
+    if (typ == 2 && !(A[0] > 0.0f && A[1] > 0.0f))
+      return fail(LH_ERR_ARG, "lh_model_create: a torus leaf needs radii R, r > 0 (its A00, A01)");
+    has_torus |= typ == 2;
     float4 L3;
     L3.x = b[2];
     memcpy(&L3.y, &typ, 4);
@@ -1301,6 +1308,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
                o_items = place(items.size() * 4), o_prog = place(prog.size() * 4),
                o_vcore = place(vcore.size() * 16);
   lh_model m = new lh_model_s();
+  m->has_torus = has_torus;
   char *buf = nullptr;
   hipError_t e = hipMalloc((void **)&buf, off);
   if (e != hipSuccess) {
@@ -1605,10 +1613,13 @@ static const void *kernel_f(bool adj) {
 }
 
 static unsigned features_of(lh_model m, const Args &a, bool adj) {
-  return (a.has_ref ? F_REF : 0u) | (m->has_chan ? F_CHAN : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u);
+  return (a.has_ref ? F_REF : 0u) | (m->has_chan ? F_CHAN : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u) |
+         (m->has_torus ? F_TORUS : 0u);
 }
 
-#define LH_FEATURES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+#define LH_FEATURES(X)                                                                                         \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) \
+      X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
 
 // the workgroup size of a launch's kernel
 static int block_for(bool adj, unsigned f) { return adj || (f & F_REF) ? BLOCK : LH_FWD_BLOCK; }

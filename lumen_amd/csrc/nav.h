@@ -131,7 +131,107 @@ struct LeafIv {
   int flo[2], fhi[2];
 };
 
-// a leaf's (at most two) intervals [lo, hi) along o + t d in its kind's frame; face 0 the quadric, k + 1 plane k
+// A torus leaf (type 2; swept radius R, tube radius r, unit axis w through its centre): along y + t d, |d| = 1, with
+// s = t - tc measured from the point nearest the centre, g(s) = s^2 + |p0|^2 + R^2 - r^2 - 2 R rho(s), rho(s)^2 =
+// A s^2 + 2 B s + C the squared distance from the axis; g <= 0 inside.  g'' = 2 - 2 R (A C - B^2) / rho^3 changes
+// sign at most twice (where rho = cbrt(R (A C - B^2)): closed form), so the bounding sphere's chord splits into at
+// most three convex or concave pieces, each with one extremum at most (bisection on g'), giving at most six
+// monotone pieces with one root each (bisection on the membership): up to four roots, two intervals, the chord's
+// start outside.  The reference's root isolation (geometry/leaf_kernels.py:212-248 torus_set), in FP32.
+constexpr int TORUS_BISECT = 30;
+
+struct TorusLine {
+  float R, r2, A, B, C, k;  // k = |p0|^2 + R^2 - r^2
+
+  __device__ __forceinline__ float rho(float s) const { return sqrtf(fmaxf(fmaf(fmaf(A, s, 2.0f * B), s, C), 0.0f)); }
+  __device__ __forceinline__ float g(float s) const { return fmaf(s, s, k) - 2.0f * R * rho(s); }
+  __device__ __forceinline__ float dg(float s) const {
+    const float rh = fmaxf(rho(s), 1e-30f);
+    return 2.0f * s - 2.0f * R * fmaf(A, s, B) / rh;
+  }
+};
+
+// the roots of g on [xl, xr] where g is monotone (at most one: where the membership changes)
+__device__ __forceinline__ bool torus_root(const TorusLine &L, float xl, float xr, float &root) {
+  const bool il = L.g(xl) <= 0.0f, ir = L.g(xr) <= 0.0f;
+  if (il == ir || !(xr > xl)) return false;
+  float lo = xl, hi = xr;
+  for (int k = 0; k < TORUS_BISECT; ++k) {
+    const float mid = 0.5f * (lo + hi);
+    if ((L.g(mid) <= 0.0f) == il) lo = mid; else hi = mid;
+  }
+  root = 0.5f * (lo + hi);
+  return true;
+}
+
+// the extremum of g on [xl, xr] where g' is monotone, or -HUGEF when g' keeps its sign there; g' read just inside
+// the ends (at a knot on the axis, rho = 0, g' jumps and the piece's own one-sided value is wanted)
+__device__ __forceinline__ float torus_extremum(const TorusLine &L, float xl, float xr) {
+  const float in = 1e-6f * (xr - xl);
+  const float gl = L.dg(xl + in), gr = L.dg(xr - in);
+  if (!(xr > xl) || (gl < 0.0f) == (gr < 0.0f)) return -HUGEF;
+  float lo = xl, hi = xr;
+  const bool nl = gl < 0.0f;
+  for (int k = 0; k < TORUS_BISECT; ++k) {
+    const float mid = 0.5f * (lo + hi);
+    if ((L.dg(mid) < 0.0f) == nl) lo = mid; else hi = mid;
+  }
+  return 0.5f * (lo + hi);
+}
+
+// the torus's intervals [a0, b0], [a1, b1] along y + t d (empty: a > b)
+__device__ inline void torus_intervals(float R, float r, float3 w, float3 y, float3 d, float &a0, float &b0,
+                                       float &a1, float &b1) {
+  a0 = a1 = HUGEF;
+  b0 = b1 = -HUGEF;
+  const float tc = -dot3(y, d);
+  const float3 p0 = fma3(tc, d, y);
+  const float pp = dot3(p0, p0), rb = R + r;
+  if (pp >= rb * rb) return;
+  const float sb = sqrtf(rb * rb - pp) * (1.0f + 1e-6f) + 1e-6f * rb;  // ends strictly outside the torus
+  const float pz = dot3(p0, w), dz = dot3(d, w);
+  const float3 pq = fma3(-pz, w, p0), dq = fma3(-dz, w, d);
+  TorusLine L{R, r * r, dot3(dq, dq), dot3(pq, dq), dot3(pq, pq), pp + R * R - r * r};
+  // the inflection points, clamped into the chord [-sb, sb]
+  float k1 = -sb, k2 = -sb;
+  const float det = fmaxf(L.A * L.C - L.B * L.B, 0.0f);
+  if (L.A > 1e-12f) {
+    const float ri = cbrtf(R * det);
+    const float disc = L.B * L.B - L.A * (L.C - ri * ri);
+    if (disc >= 0.0f) {
+      const float sq = sqrtf(disc);
+      k1 = fminf(fmaxf((-L.B - sq) / L.A, -sb), sb);
+      k2 = fminf(fmaxf((-L.B + sq) / L.A, -sb), sb);
+    }
+  }
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;  // the roots in order (registers: no indexed array)
+  int nr = 0;
+  auto push = [&](float v) {
+    r0 = nr == 0 ? v : r0;
+    r1 = nr == 1 ? v : r1;
+    r2 = nr == 2 ? v : r2;
+    r3 = nr == 3 ? v : r3;
+    nr = min(nr + 1, 4);
+  };
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float xl = i == 0 ? -sb : i == 1 ? k1 : k2, xr = i == 0 ? k1 : i == 1 ? k2 : sb;
+    const float e = torus_extremum(L, xl, xr);
+    float rt;
+    if (e > -HUGEF) {
+      if (torus_root(L, xl, e, rt)) push(rt);
+      if (torus_root(L, e, xr, rt)) push(rt);
+    } else if (torus_root(L, xl, xr, rt)) {
+      push(rt);
+    }
+  }
+  if (nr >= 2) { a0 = tc + r0; b0 = tc + r1; }
+  if (nr >= 4) { a1 = tc + r2; b1 = tc + r3; }
+}
+
+// a leaf's (at most two) intervals [lo, hi) along o + t d in its kind's frame; face 0 the quadric, k + 1 plane k.
+// TOR: the scene has torus leaves (a kernel feature: the root isolation costs registers every other kernel keeps)
+template <bool TOR>
 __device__ inline void leaf_intervals(const Geo &g, int li, float3 o, float3 d, LeafIv &iv) {
   const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
   const int typ = __float_as_int(L3.y), p0 = __float_as_int(L3.z), np = __float_as_int(L3.w);
@@ -156,8 +256,11 @@ __device__ inline void leaf_intervals(const Geo &g, int li, float3 o, float3 d, 
   }
   // the quadric's pieces [a0, b0] and [a1, b1] (a piece with a > b is empty); qa/qb: the end is a root
   float a0 = -HUGEF, b0 = HUGEF, a1 = HUGEF, b1 = -HUGEF;
-  bool qa0 = false, qb0 = false, qa1 = false;
-  if (typ == 1) {
+  bool qa0 = false, qb0 = false, qa1 = false, qb1 = false;
+  if (TOR && typ == 2) {
+    torus_intervals(L1.x, L1.y, mk3(L2.z, L2.w, L3.x), y, d, a0, b0, a1, b1);
+    qa0 = qb0 = qa1 = qb1 = true;
+  } else if (typ == 1) {
     const float A00 = L1.x, A01 = L1.y, A02 = L1.z, A11 = L1.w, A12 = L2.x, A22 = L2.y;
     const float3 b = mk3(L2.z, L2.w, L3.x);
     const float c = L0.w;
@@ -224,11 +327,12 @@ __device__ inline void leaf_intervals(const Geo &g, int li, float3 o, float3 d, 
     iv.lo[1] = ok ? lo : HUGEF;
     iv.hi[1] = ok ? hi : HUGEF;
     iv.flo[1] = (a1 > tp0 && qa1) ? 0 : fp0;
-    iv.fhi[1] = fp1;
+    iv.fhi[1] = (b1 < tp1 && qb1) ? 0 : fp1;
   }
 }
 
 // a leaf's membership of a point (its kind's frame)
+template <bool TOR>
 __device__ inline bool leaf_inside(const Geo &g, int li, float3 x) {
   const float4 L0 = g.leaf[4 * li], L3 = g.leaf[4 * li + 3];
   const int typ = __float_as_int(L3.y), p0 = __float_as_int(L3.z), np = __float_as_int(L3.w);
@@ -243,6 +347,11 @@ __device__ inline bool leaf_inside(const Geo &g, int li, float3 x) {
                           L1.z * y.x + L2.x * y.y + L2.y * y.z);
     const float q = dot3(y, Ay) + 2.0f * (L2.z * y.x + L2.w * y.y + L3.x * y.z) + L0.w;
     if (q > 0.0f) return false;
+  } else if (TOR && typ == 2) {  // |y|^2 - 2 R rho + R^2 - r^2 <= 0
+    const float4 L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2];
+    const float yy = dot3(y, y), z = dot3(y, mk3(L2.z, L2.w, L3.x));
+    const float rho = sqrtf(fmaxf(yy - z * z, 0.0f));
+    if (yy - 2.0f * L1.x * rho + L1.x * L1.x - L1.y * L1.y > 0.0f) return false;
   }
   return true;
 }
@@ -267,10 +376,11 @@ __device__ __forceinline__ bool run_program(const Geo &g, int p0, int pl, uint32
   return (st & 1u) != 0u;
 }
 
+template <bool TOR>
 __device__ inline bool solid_inside(const Geo &g, int4 K, float3 x) {
-  if (K.y == 1) return leaf_inside(g, K.x, x);
+  if (K.y == 1) return leaf_inside<TOR>(g, K.x, x);
   uint32_t bits = 0;
-  for (int k = 0; k < K.y && k < 32; ++k) bits |= (uint32_t)leaf_inside(g, K.x + k, x) << k;
+  for (int k = 0; k < K.y && k < 32; ++k) bits |= (uint32_t)leaf_inside<TOR>(g, K.x + k, x) << k;
   return run_program(g, K.z, K.w, bits);
 }
 
@@ -282,10 +392,17 @@ struct Change {
 
 // the normal of leaf li's face `face` (0 the quadric: A y + b at y = o + t d - q; k >= 1 plane k) in its frame; the
 // leaf's data was just read by its interval test, so these reads hit the cache
+template <bool TOR>
 __device__ __forceinline__ float3 leaf_normal(const Geo &g, int li, int face, float3 o, float3 d, float t) {
   const float4 L0 = g.leaf[4 * li], L1 = g.leaf[4 * li + 1], L2 = g.leaf[4 * li + 2], L3 = g.leaf[4 * li + 3];
   if (face == 0) {
     const float3 y = sub3(fma3(t, d, o), mk3(L0.x, L0.y, L0.z));
+    if (TOR && __float_as_int(L3.y) == 2) {  // a torus: y - R (the point's direction across the axis)
+      const float3 w = mk3(L2.z, L2.w, L3.x);
+      const float3 q = fma3(-dot3(y, w), w, y);
+      const float rho = fmaxf(sqrtf(dot3(q, q)), 1e-30f);
+      return fma3(-L1.x / rho, q, y);
+    }
     return mk3(L1.x * y.x + L1.y * y.y + L1.z * y.z + L2.z, L1.y * y.x + L1.w * y.y + L2.x * y.z + L2.w,
                L1.z * y.x + L2.x * y.y + L2.y * y.z + L3.x);
   }
@@ -297,12 +414,13 @@ __device__ __forceinline__ float3 leaf_normal(const Geo &g, int li, int face, fl
 // the first change of the solid's membership along the ray after t = 0, from the side `want_from` (true: the
 // point starts inside and the change is an exit; false: an entry).  Endpoints on the face `skip_code` within
 // `skip_t` of the start count as passed.  `vol` names the volume in the surface codes.
+template <bool TOR>
 __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, float3 d, bool exit_query,
                                       int skip_code, float skip_t) {
   Change ch{HUGEF, -1, -1, mk3(0.f, 0.f, 1.f)};
   if (K.y == 1) {  // one leaf: its intervals are the solid's
     LeafIv iv;
-    leaf_intervals(g, K.x, o, d, iv);
+    leaf_intervals<TOR>(g, K.x, o, d, iv);
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       float lo = iv.lo[p], hi = iv.hi[p];
@@ -320,14 +438,14 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
       ch.leaf = 0;
       ch.face = 0;
     }
-    if (ch.leaf >= 0) ch.n = leaf_normal(g, K.x, ch.face, o, d, ch.t);
+    if (ch.leaf >= 0) ch.n = leaf_normal<TOR>(g, K.x, ch.face, o, d, ch.t);
     return ch;
   }
   // several leaves: scan the endpoints in order for the first change of the program's value
   const int nl = K.y < MAX_CSG_LEAVES ? K.y : MAX_CSG_LEAVES;
   LeafIv ivs[MAX_CSG_LEAVES];
   for (int k = 0; k < nl; ++k) {
-    leaf_intervals(g, K.x + k, o, d, ivs[k]);
+    leaf_intervals<TOR>(g, K.x + k, o, d, ivs[k]);
     for (int p = 0; p < 2; ++p) {
       if (ivs[k].lo[p] >= HUGEF) continue;
       if (srf_code(vol, k, ivs[k].flo[p]) == skip_code && fabsf(ivs[k].lo[p]) <= skip_t) ivs[k].lo[p] = -BIGF;
@@ -343,7 +461,7 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
     return run_program(g, K.z, K.w, bits);
   };
   const bool start = member(0.0f);
-  if (exit_query && !start) return Change{0.0f, 0, 0, leaf_normal(g, K.x, 0, o, d, 0.0f)};
+  if (exit_query && !start) return Change{0.0f, 0, 0, leaf_normal<TOR>(g, K.x, 0, o, d, 0.0f)};
   if (!exit_query && start) return ch;  // inside a daughter already (rounding): no entry ahead
   float cur = 0.0f;
   for (int it = 0; it < 4 * nl + 5; ++it) {
@@ -360,10 +478,10 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
     // a change only where the value stays changed a band further on: two leaves meeting on a shared face (a
     // polycone's sections, a union of touching pieces) may leave a gap of a few ulps between their intervals
     if (member(nxt) != start && member(nxt + band_at(maxabs3(o) + fabsf(nxt))) != start)
-      return Change{nxt, nl_, nf, leaf_normal(g, K.x + nl_, nf, o, d, nxt)};
+      return Change{nxt, nl_, nf, leaf_normal<TOR>(g, K.x + nl_, nf, o, d, nxt)};
     cur = nxt;
   }
-  if (exit_query) return Change{0.0f, 0, 0, leaf_normal(g, K.x, 0, o, d, 0.0f)};
+  if (exit_query) return Change{0.0f, 0, 0, leaf_normal<TOR>(g, K.x, 0, o, d, 0.0f)};
   return ch;
 }
 
@@ -388,7 +506,7 @@ __device__ inline void core_interval(const Geo &g, int vol, float3 xl, float3 dl
   const float4 C0 = g.vcore[3 * vol], C2 = g.vcore[3 * vol + 2];
   if (code == 2) {
     LeafIv iv;
-    leaf_intervals(g, __float_as_int(C2.w), xl, dl, iv);
+    leaf_intervals<false>(g, __float_as_int(C2.w), xl, dl, iv);
     lo = iv.lo[0];
     hi = iv.hi[0];
     return;
@@ -421,7 +539,7 @@ __device__ inline void core_interval(const Geo &g, int vol, float3 xl, float3 dl
   }
 }
 
-template <bool LDS>
+template <bool LDS, bool TOR>
 __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
   const float skip_t = K_ON * band_at(maxabs3(x));
   Cross best{tmax, -1, -1, 0, mk3(0.f, 0.f, 1.f)};
@@ -433,7 +551,7 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
     float3 xl, dl;
     to_local(g, vol, x, d, xl, dl);
     core_interval(g, vol, xl, dl, core_lo, core_hi);
-    const Change ch = first_change(g, g.vkind[vol], vol, xl, dl, true, last, skip_t);
+    const Change ch = first_change<TOR>(g, g.vkind[vol], vol, xl, dl, true, last, skip_t);
     if (ch.leaf >= 0 && ch.t <= best.t) {
       best.t = ch.t;
       best.vol_after = g.vinfo[vol].y;
@@ -489,7 +607,7 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
           const int u = g.items[it];
           float3 xl, dl;
           rows_to_local(g.ixf + 3 * it, x, d, xl, dl);
-          const Change ch = first_change(g, g.ikind[it], u, xl, dl, false, last, skip_t);
+          const Change ch = first_change<TOR>(g, g.ikind[it], u, xl, dl, false, last, skip_t);
           if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && best.own))) {
             best.t = ch.t;
             best.vol_after = u;
@@ -511,11 +629,13 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
   return best;
 }
 
+template <bool TOR>
 __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
-  return query_t<false>(g, x, d, vol, last, tmax);
+  return query_t<false, TOR>(g, x, d, vol, last, tmax);
 }
 
 // the deepest navigator id holding x (the outside when no volume does)
+template <bool TOR>
 __device__ inline int locate(const Geo &g, float3 x) {
   int cur = g.nvol;
   for (int depth = 0; depth < 64; ++depth) {
@@ -542,7 +662,7 @@ __device__ inline int locate(const Geo &g, float3 x) {
             const int u = g.items[(sc >> 3) + k];
             float3 xl, dl;
             to_local(g, u, x, mk3(0.f, 0.f, 1.f), xl, dl);
-            if (solid_inside(g, g.vkind[u], xl)) { found = u; break; }
+            if (solid_inside<TOR>(g, g.vkind[u], xl)) { found = u; break; }
           }
           ni = skip;
         } else {

@@ -8,6 +8,7 @@ Restates the rules of the reference's ``geometry/clearance.py`` (and the grid ch
     plane n.(x - o) <= h:            |n.(x - o) - h| / |n|                                      (exact)
     quadric f = y'Ay + 2b.y + c:     the root r >= 0 of ||A|| r^2 + |grad f| r = |f|              (|f(y + e) - f(y)|
                                      <= |grad f| |e| + ||A|| |e|^2, so f keeps its sign within r)
+    torus (R, r about axis b):       | sqrt((rho - R)^2 + z^2) - r |                               (exact)
 - the least over every volume, at most ``cap``, less the navigator's band at the grid's coordinate scale, rounded
   down to float32.
 
@@ -69,6 +70,11 @@ def leaf_bound(lf, xl):
         else:
             r = np.where(gr > 0, af / np.maximum(gr, 1e-300), np.where(af > 0, np.inf, 0.0))
         out = np.minimum(out, r)
+    elif lf["typ"] == 2:  # a torus: the distance to its surface, | sqrt((rho - R)^2 + z^2) - r |, exactly
+        y = xl - lf["q"]
+        z = y @ lf["b"]
+        rho = np.sqrt(np.maximum((y * y).sum(1) - z * z, 0.0))
+        out = np.minimum(out, np.abs(np.hypot(rho - lf["A"][0, 0], z) - lf["A"][0, 1]))
     return out
 
 

@@ -3,7 +3,8 @@
 It reads what the BASELINE detectors use: ``define`` (``matrix``, ``constant``, ``position``, ``rotation``),
 ``materials`` (material properties by matrix reference: RINDEX, ABSLENGTH, RAYLEIGH, the Mie MIEHG constants, the
 wavelength shifter's WLS properties), ``solids`` (``box``, ``tube`` (a phi cut too), ``cone``, ``polycone`` without a
-phi cut, ``orb``, ``sphere`` with polar cuts, ``ellipsoid``, ``trd``, the booleans, ``opticalsurface``), ``structure`` (``volume``, ``physvol``,
+phi cut, ``orb``, ``sphere`` with polar cuts, ``ellipsoid``, ``trd``, ``torus``, the booleans, ``opticalsurface``),
+``structure`` (``volume``, ``physvol``,
 ``bordersurface``, ``skinsurface``) and ``setup``.  A recipe names the detector volume (light leaving it is absorbed)
 and the sensor groups (counted across a border into each placement of a logical volume), as the reference's
 ``lumen.Recipe`` does (``examples/example_tank.py``).
@@ -133,6 +134,11 @@ def _parse(path: str) -> _Doc:
                                                                                       "rmax2")],
                                          0.5 * _num(el.get("z"), d.consts) * u,
                                          _num(el.get("startphi", "0"), d.consts) * au, dphi, name=name)
+        elif el.tag == "torus":
+            u, au = _lunit(el), _aunit(el)
+            dphi = _num(el.get("deltaphi"), d.consts) * au if el.get("deltaphi") else 2 * math.pi
+            d.solids[name] = solids.torus(*[_num(el.get(k, "0"), d.consts) * u for k in ("rmin", "rmax", "rtor")],
+                                          _num(el.get("startphi", "0"), d.consts) * au, dphi, name=name)
         elif el.tag == "polycone":
             u, au = _lunit(el), _aunit(el)
             dphi = _num(el.get("deltaphi"), d.consts) * au if el.get("deltaphi") else 2 * math.pi

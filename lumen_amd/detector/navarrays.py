@@ -119,7 +119,8 @@ def csg_tables(kinds: list[Solid], dtype=np.float32) -> dict:
     f = lambda v: np.asarray(v, np.float64).astype(dtype)  # noqa: E731
     T = dict(typ=np.asarray([r["typ"] for r in rows], np.int32), p0=np.asarray([r["p0"] for r in rows], np.int32),
              np_=np.asarray([r["np_"] for r in rows], np.int32), sa=f([r["sa"] for r in rows]),
-             c=f([r["c"] for r in rows]), R=f(np.zeros(len(rows))), r=f(np.zeros(len(rows))))
+             c=f([r["c"] for r in rows]), R=f([r["A"][0, 0] if r["typ"] == 2 else 0.0 for r in rows]),
+             r=f([r["A"][0, 1] if r["typ"] == 2 else 0.0 for r in rows]))
     for a in range(3):
         T[f"o{a}"] = f([r["o"][a] for r in rows])
         T[f"b{a}"] = f([r["b"][a] for r in rows])
@@ -179,6 +180,8 @@ def skin_core(kd: Solid, boxes) -> Solid | None:
     if len(kd.leaves) != 1 or kd.program != [(0, 0)] or len(boxes) == 0:
         return None
     lf = kd.leaves[0]
+    if lf.typ not in (0, 1):
+        return None
     half = 0.5 * float(np.min(kd.hi - kd.lo))
     axes = None
     if lf.typ == 1:

@@ -34,7 +34,8 @@ class Leaf:
 
     def moved(self, R: np.ndarray, t: np.ndarray) -> "Leaf":
         """Return the leaf placed by x' = R x + t (R a rotation)."""
-        return Leaf(self.typ, R @ self.origin + t, R @ self.qo, R @ self.A @ R.T, R @ self.b, self.c,
+        A = self.A.copy() if self.typ == 2 else R @ self.A @ R.T  # a torus keeps its radii, its axis b turns
+        return Leaf(self.typ, R @ self.origin + t, R @ self.qo, A, R @ self.b, self.c,
                     self.N @ R.T if len(self.N) else self.N.copy(), self.H.copy())
 
     def inside(self, x: np.ndarray) -> np.ndarray:
@@ -45,6 +46,8 @@ class Leaf:
             y = x - (self.origin + self.qo)
             q = np.einsum("...i,ij,...j->...", y, self.A, y) + 2.0 * y @ self.b + self.c
             ok &= q <= 0.0
+        elif self.typ == 2:
+            ok &= torus_g(x - (self.origin + self.qo), self.b, self.A[0, 0], self.A[0, 1]) <= 0.0
         if len(self.N):
             ok &= np.all((x - self.origin) @ self.N.T - self.H <= 0.0, -1)
         return ok
@@ -124,6 +127,39 @@ def sphere_cap(r: float, height: float, name="cap") -> Solid:
     lf = Leaf(1, _z3(), _z3(), np.eye(3), _z3(), -r * r, np.array([[0, 0, -1.0]]), np.array([-z0]))
     rc = np.sqrt(max(r * r - z0 * z0, 0.0)) if z0 > 0 else r
     return Solid([lf], [(0, 0)], np.array([-rc, -rc, z0]), np.array([rc, rc, r]), name, dict(r=r, height=height))
+
+
+def torus_g(y, axis, R: float, r: float):
+    """Return |y|^2 - 2 R rho + R^2 - r^2 (rho the distance from the axis): <= 0 inside the torus of swept radius
+    R and tube radius r about ``axis`` through the origin; it has the sign of (rho - R)^2 + z^2 - r^2."""  # noqa: D205, D209
+    y = np.asarray(y, np.float64)
+    z = y @ np.asarray(axis, np.float64)
+    rho = np.sqrt(np.maximum((y * y).sum(-1) - z * z, 0.0))
+    return (y * y).sum(-1) - 2.0 * R * rho + R * R - r * r
+
+
+def torus_leaf(R: float, r: float) -> Leaf:
+    """A solid torus about z (swept radius R, tube radius r): a leaf of type 2, its radii in A[0, 0] and A[0, 1]
+    (the tables' ``A00``, ``A01``, and their ``R``, ``r`` columns) and its axis in b."""  # noqa: D205, D209
+    A = np.zeros((3, 3))
+    A[0, 0], A[0, 1], A[1, 0] = R, r, r
+    return Leaf(2, _z3(), _z3(), A, np.array([0.0, 0.0, 1.0]), 0.0, np.zeros((0, 3)), np.zeros(0))
+
+
+def torus(rmin: float, rmax: float, rtor: float, phi0: float = 0.0, dphi: float = 2 * np.pi,
+          name="torus") -> Solid:
+    """Return G4Torus: the tube of radii [rmin, rmax] swept at radius rtor about z, azimuths [phi0, phi0 + dphi]."""
+    if not (0.0 <= rmin < rmax and rtor > 0.0):
+        raise ValueError(f"torus {name!r}: 0 <= rmin < rmax and rtor > 0")
+    h = np.array([rtor + rmax, rtor + rmax, rmax])
+    s = Solid([torus_leaf(rtor, rmax)], [(0, 0)], -h, h, name)
+    if rmin > 0.0:
+        s = boolean("sub", s, Solid([torus_leaf(rtor, rmin)], [(0, 0)], -h, h), name=name)
+    if dphi < 2 * np.pi - 1e-12:
+        s = boolean("inter", s, _phi_wedge(phi0, dphi, rtor + rmax, rmax), name=name)
+    s.lo, s.hi = -h, h
+    s.params = dict(rmin=rmin, rmax=rmax, rtor=rtor, phi0=phi0, dphi=dphi)
+    return s
 
 
 def _cone_leaf(r1: float, r2: float, hz: float) -> Leaf:

@@ -87,6 +87,7 @@ inline double val(Counted x) { return x.v; }
   inline double name(double x) { return f(x); }           \
   inline Counted name(Counted x) { ++g_ops; return Counted(f(x.v)); }
 ORC_F1(SQRT, std::sqrt)
+ORC_F1(CBRT, std::cbrt)
 ORC_F1(EXP, std::exp)
 ORC_F1(LOG, std::log)
 ORC_F1(SIN, std::sin)
@@ -249,6 +250,70 @@ template <class R> struct Iv {
   R lo, hi;
   int flo, fhi;
 };
+// A solid torus about the unit axis w through the leaf's q (swept radius Rt = A[0], tube radius rt = A[1]): along
+// y + t d (|d| = 1), from the point nearest the centre (s = t - tc), g(s) = s^2 + |p0|^2 + Rt^2 - rt^2 - 2 Rt rho(s)
+// with rho(s)^2 = a s^2 + 2 b s + c; g <= 0 inside.  geometry/leaf_kernels.py:212-248 `torus_set`: the chord of the
+// bounding sphere split at g's inflection points (rho^3 = Rt (a c - b^2), closed form), each piece's extremum by
+// bisection on g', each monotone piece's root by bisection on the membership; the bracket's start is outside, so the
+// roots alternate entry and exit.
+constexpr int TORUS_BISECT_F64 = 80;
+template <class R> struct TorusG {
+  R Rt, a, b, c, k;
+  R rho(R s) const { return SQRT(MAX<R>(a * s * s + R(2.0) * b * s + c, R(0.0))); }
+  R g(R s) const { return s * s + k - R(2.0) * Rt * rho(s); }
+  R dg(R s) const { return R(2.0) * s - R(2.0) * Rt * (a * s + b) / MAX<R>(rho(s), R(1e-300)); }
+};
+template <class R> int torus_roots(R Rt, R rt, V3<R> w, V3<R> y, V3<R> d, R roots[4]) {
+  const R tc = -dot(y, d);
+  const V3<R> p0 = y + tc * d;
+  const R pp = dot(p0, p0), rb = Rt + rt;
+  if (pp >= rb * rb) return 0;
+  // the chord a little past the bounding sphere, so its ends lie strictly outside (an in-plane ray meets the
+  // sphere on the torus's outer equator)
+  const R sb = SQRT(rb * rb - pp) * R(1.0 + 1e-9) + R(1e-12) * rb;
+  const V3<R> pq = p0 - dot(p0, w) * w, dq = d - dot(d, w) * w;
+  const TorusG<R> G{Rt, dot(dq, dq), dot(pq, dq), dot(pq, pq), pp + Rt * Rt - rt * rt};
+  std::vector<R> knots = {-sb};
+  const R det = MAX<R>(G.a * G.c - G.b * G.b, R(0.0));
+  if (G.a > R(1e-24)) {
+    const R ri = CBRT(Rt * det), disc = G.b * G.b - G.a * (G.c - ri * ri);
+    if (disc >= R(0.0)) {
+      for (R sgn : {R(-1.0), R(1.0)}) {
+        const R s = (-G.b + sgn * SQRT(disc)) / G.a;
+        if (s > knots.back() && s < sb) knots.push_back(s);
+      }
+    }
+  }
+  knots.push_back(sb);
+  std::vector<R> mono = {knots[0]};  // the monotone pieces' ends: each piece's extremum added
+  for (size_t i = 0; i + 1 < knots.size(); ++i) {
+    R lo = knots[i], hi = knots[i + 1];
+    // g' just inside the piece: at a knot on the axis (rho = 0) g' jumps, and its one-sided values are the piece's
+    const R in = R(1e-12) * (hi - lo);
+    const bool nl = G.dg(lo + in) < R(0.0);
+    if (nl != (G.dg(hi - in) < R(0.0))) {
+      for (int it = 0; it < TORUS_BISECT_F64; ++it) {
+        const R mid = R(0.5) * (lo + hi);
+        if ((G.dg(mid) < R(0.0)) == nl) lo = mid; else hi = mid;
+      }
+      mono.push_back(R(0.5) * (lo + hi));
+    }
+    mono.push_back(knots[i + 1]);
+  }
+  int n = 0;
+  for (size_t i = 0; i + 1 < mono.size() && n < 4; ++i) {
+    R lo = mono[i], hi = mono[i + 1];
+    const bool il = G.g(lo) <= R(0.0);
+    if (il == (G.g(hi) <= R(0.0)) || !(hi > lo)) continue;
+    for (int it = 0; it < TORUS_BISECT_F64; ++it) {
+      const R mid = R(0.5) * (lo + hi);
+      if ((G.g(mid) <= R(0.0)) == il) lo = mid; else hi = mid;
+    }
+    roots[n++] = tc + R(0.5) * (lo + hi);
+  }
+  return n;
+}
+
 template <class R>
 int leaf_iv(const Leaf &L, V3<R> o, V3<R> d, Iv<R> out[2]) {
   const R INF = R(1e300);
@@ -271,7 +336,13 @@ int leaf_iv(const Leaf &L, V3<R> o, V3<R> d, Iv<R> out[2]) {
   }
   R qa[2] = {-INF, INF}, qb[2] = {INF, -INF};
   bool ra[2] = {false, false}, rb[2] = {false, false};
-  if (L.typ == 1) {
+  if (L.typ == 2) {
+    R rts[4];
+    const int n = torus_roots<R>(R(L.A[0]), R(L.A[1]), v3<R>(R(L.b[0]), R(L.b[1]), R(L.b[2])), y, d, rts);
+    qa[0] = INF, qb[0] = -INF;
+    if (n >= 2) { qa[0] = rts[0]; qb[0] = rts[1]; ra[0] = rb[0] = true; }
+    if (n >= 4) { qa[1] = rts[2]; qb[1] = rts[3]; ra[1] = rb[1] = true; }
+  } else if (L.typ == 1) {
     auto Av = [&](V3<R> v) {
       return v3<R>(R(L.A[0]) * v.x + R(L.A[1]) * v.y + R(L.A[2]) * v.z, R(L.A[3]) * v.x + R(L.A[4]) * v.y + R(L.A[5]) * v.z,
                    R(L.A[6]) * v.x + R(L.A[7]) * v.y + R(L.A[8]) * v.z);
@@ -396,7 +467,12 @@ template <class R> V3<R> normal_at(const Scene &s, int v, int leaf, int face, V3
   to_local(s, v, x, d, xl, dl);
   const Leaf &L = s.kinds[s.kind[v]].leaves[leaf];
   V3<R> nl;
-  if (face == 0) {
+  if (face == 0 && L.typ == 2) {  // a torus: y less the swept circle's point nearest it
+    const V3<R> y = (xl + t * dl) - v3<R>(R(L.q[0]), R(L.q[1]), R(L.q[2]));
+    const V3<R> w = v3<R>(R(L.b[0]), R(L.b[1]), R(L.b[2]));
+    const V3<R> q = y - dot(y, w) * w;
+    nl = y - (R(L.A[0]) / MAX<R>(SQRT(dot(q, q)), R(1e-300))) * q;
+  } else if (face == 0) {
     const V3<R> y = (xl + t * dl) - v3<R>(R(L.q[0]), R(L.q[1]), R(L.q[2]));
     nl = v3<R>(R(L.A[0]) * y.x + R(L.A[1]) * y.y + R(L.A[2]) * y.z + R(L.b[0]),
                R(L.A[3]) * y.x + R(L.A[4]) * y.y + R(L.A[5]) * y.z + R(L.b[1]),
@@ -486,6 +562,9 @@ inline bool inside(const Scene &s, int v, V3<double> x) {
         for (int j = 0; j < 3; ++j) q += y[i] * L.A[3 * i + j] * y[j];
       }
       b = b && q <= 0.0;
+    } else if (L.typ == 2) {
+      const double yy = y[0] * y[0] + y[1] * y[1] + y[2] * y[2], z = y[0] * L.b[0] + y[1] * L.b[1] + y[2] * L.b[2];
+      b = b && yy - 2.0 * L.A[0] * std::sqrt(std::max(yy - z * z, 0.0)) + L.A[0] * L.A[0] - L.A[1] * L.A[1] <= 0.0;
     }
     in[l] = b;
   }

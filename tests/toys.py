@@ -224,3 +224,37 @@ def gdml_framed() -> str:
 <setup name="Default" version="1.0"><world ref="World"/></setup>
 </gdml>
 """
+
+
+GDML_TORI = """<?xml version="1.0" encoding="UTF-8"?>
+<gdml>
+<define>
+  <matrix name="RIDX" coldim="2" values="1.0*eV 1.33 12.0*eV 1.33"/>
+  <matrix name="RIDX2" coldim="2" values="1.0*eV 1.50 12.0*eV 1.50"/>
+  <matrix name="ABSL" coldim="2" values="1.0*eV 3*m 12.0*eV 3*m"/>
+  <rotation name="rx" x="60" y="20" unit="deg"/>
+</define>
+<materials>
+  <material name="Liq"><property name="RINDEX" ref="RIDX"/></material>
+  <material name="Glass"><property name="RINDEX" ref="RIDX2"/><property name="ABSLENGTH" ref="ABSL"/></material>
+</materials>
+<solids>
+  <box name="WorldBox" x="4" y="4" z="4" lunit="m"/>
+  <torus name="T1" rmin="0" rmax="200" rtor="500" startphi="0" deltaphi="360" aunit="deg" lunit="mm"/>
+  <torus name="T2" rmin="60" rmax="150" rtor="400" startphi="30" deltaphi="240" aunit="deg" lunit="mm"/>
+  <torus name="T3" rmin="0" rmax="300" rtor="400" startphi="0" deltaphi="360" aunit="deg" lunit="mm"/>
+</solids>
+<structure>
+  <volume name="t1V"><materialref ref="Liq"/><solidref ref="T1"/></volume>
+  <volume name="t2V"><materialref ref="Glass"/><solidref ref="T2"/></volume>
+  <volume name="t3V"><materialref ref="Glass"/><solidref ref="T3"/></volume>
+  <volume name="World"><materialref ref="Liq"/><solidref ref="WorldBox"/>
+    <physvol name="p1"><volumeref ref="t1V"/><position name="q1" x="-0.9" y="-0.9" z="0" unit="m"/></physvol>
+    <physvol name="p2"><volumeref ref="t2V"/><position name="q2" x="0.9" y="0.9" z="0" unit="m"/>
+      <rotationref ref="rx"/></physvol>
+    <physvol name="p3"><volumeref ref="t3V"/><position name="q3" x="0.9" y="-0.9" z="0.8" unit="m"/></physvol>
+  </volume>
+</structure>
+<setup name="Default" version="1.0"><world ref="World"/></setup>
+</gdml>
+"""
