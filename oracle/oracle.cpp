@@ -151,6 +151,7 @@ constexpr double FLOOR32 = 1e-30, TINY_NORMAL = 1.1754943508222875e-38, DEFENSIV
 constexpr double FINE_STRUCTURE = 1.0 / 137.035999084;
 constexpr double ZERO_STEP = 1e-9;  // m: a crossing this close to the start is a zero-length step (FP64)
 constexpr double ON_FACE = 1e-9;    // m: the face a photon sits on is not met again this close
+constexpr double ON_FACE_TORUS = 1e-7;  // m: the same for a torus face (its grazing roots: sqrt(eps) conditioned)
 constexpr int ZERO_STEPS_BEFORE_PUSH = 10, ZERO_STEPS_BEFORE_KILL = 25;
 constexpr double PUSH = 1e-7;       // m
 enum { STRAIGHT = 0, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE, NOUT };
@@ -417,11 +418,14 @@ Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_quer
   std::vector<int> cnt(nl);
   for (int l = 0; l < nl; ++l) {
     cnt[l] = leaf_iv(K.leaves[l], o, d, ivs[l].data());
+    // a torus root met at a grazing angle is conditioned to sqrt(eps) of the tube, not eps: its face is passed
+    // within ON_FACE_TORUS
+    const R on_face = R(K.leaves[l].typ == 2 ? ON_FACE_TORUS : ON_FACE);
     for (int p = 0; p < cnt[l]; ++p) {
       Iv<R> &iv = ivs[l][p];
       const long long clo = ((long long)vol << 10) | (l << 5) | iv.flo, chi = ((long long)vol << 10) | (l << 5) | iv.fhi;
-      if (clo == skip && FABS(iv.lo) <= R(ON_FACE)) iv.lo = R(-1e299);
-      if (chi == skip && FABS(iv.hi) <= R(ON_FACE)) iv.hi = R(-1e299);
+      if (clo == skip && FABS(iv.lo) <= on_face) iv.lo = R(-1e299);
+      if (chi == skip && FABS(iv.hi) <= on_face) iv.hi = R(-1e299);
     }
   }
   auto member = [&](R t) {

@@ -434,9 +434,8 @@ def test_navigator_on_tori(gpu, tmp_path):
 
 def test_trace_with_tori_against_oracle(gpu, tmp_path):
     """A point source among the tori (two of glass, one hollow and cut in phi), traced with the torus kernel: the
-    photons' fates against the oracle's for the same photons.  Vertices are not compared: the FP64 oracle takes runs
-    of zero-length steps (then a push) on about 0.1 % of these photons at torus surfaces, which the device does not,
-    so its vertex count is a few percent higher (parity of torus traces is partial)."""
+    photons' fates and vertices against the oracle's for the same photons (the oracle passes the torus face a photon
+    sits on within 1e-7 m: a grazing torus root is conditioned to sqrt(eps))."""
     from test_oracle import _tori
 
     det = _tori(tmp_path)
@@ -447,5 +446,5 @@ def test_trace_with_tori_against_oracle(gpu, tmp_path):
     _, h = O.OracleRun(det).trace(src, n, seed=3, bvh=True)
     gh = np.asarray(g.health)
     np.testing.assert_allclose(gh[:4], h[:4], rtol=0, atol=2e-3)  # absorbed in bulk, at surfaces, lost, escaped
-    assert gh[9] == 0.0 and gh[5] == 0.0  # the device pushes and abandons none
-    assert 0.9 * h[12] < gh[12] <= h[12]
+    assert gh[9] <= h[9] + 1e-4 and gh[5] <= h[5] + 1e-4  # stuck-photon pushes, abandoned photons: rare in both
+    np.testing.assert_allclose(gh[12], h[12], rtol=1e-2)  # vertices
