@@ -403,9 +403,13 @@ def key(volumes: list[Volume], kinds: list[Solid], outside_cls: int, **options) 
     h = hashlib.sha256()
     _digest_inputs(h, volumes, kinds, outside_cls, options)
     here = os.path.dirname(os.path.abspath(__file__))
-    for f in _BUILDER_CODE:
-        with open(os.path.join(here, f), "rb") as src:
-            h.update(hashlib.sha256(src.read()).digest())
+    for f in _BUILDER_CODE:  # a copy without the sources (an installed package) keys by the file names alone
+        path = os.path.join(here, f)
+        if os.path.isfile(path):
+            with open(path, "rb") as src:
+                h.update(hashlib.sha256(src.read()).digest())
+        else:
+            h.update(f"missing:{f}".encode())
     h.update(np.__version__.encode())
     return h.hexdigest()
 
