@@ -131,33 +131,44 @@ def shortest_rayleigh(det, params, m) -> float:
     return float(np.min(np.asarray(getattr(v, "value", v), np.float64)))
 
 
-def jump_grids(det, params=None) -> list:
-    """Return the JumpGrids of a detector's eligible media (the reference's ``fastpath.prepare``): none where no
-    volume of the medium holds the smallest ball (R_0 Rayleigh lengths)."""  # noqa: D205, D209
-    p = det.params() if params is None else params
+def _volumes_by_medium(det):
+    """Return {medium: (volume ids, their world boxes' half-widths along the narrowest axis)}."""
     nav = det.nav
-    cls = np.asarray(nav.host["cls"])[: nav.n]
-    med_of = np.asarray(det.class_medium)[cls]
-    half = 0.5 * np.min(np.asarray(nav.hi_w, np.float64) - np.asarray(nav.lo_w, np.float64), 1)
-    out = []
-    for m, med in enumerate(det.media):
-        if not eligible(med):
+    medium_of_volume = np.asarray(det.class_medium)[np.asarray(nav.host["cls"])[: nav.n]]
+    extent = np.asarray(nav.hi_w, np.float64) - np.asarray(nav.lo_w, np.float64)
+    narrow = 0.5 * extent.min(axis=1)
+    return {int(m): (np.flatnonzero(medium_of_volume == m), narrow[medium_of_volume == m])
+            for m in np.unique(medium_of_volume) if m >= 0}  # -1: a class without a medium
+
+
+def _fitting_grid(det, volumes, spacing):
+    """Return (lo, n, spacing) of a grid over the volumes' boxes at the given spacing or, when that holds more than
+    MAX_NODES, at the least spacing (grown by at least COARSEN a try) whose grid does."""  # noqa: D205, D209
+    lo, n = grid(det, volumes, spacing)
+    while int(np.prod(n)) > MAX_NODES:
+        spacing *= max((float(np.prod(n)) / MAX_NODES) ** (1.0 / 3.0), COARSEN)
+        lo, n = grid(det, volumes, spacing)
+    return lo, n, spacing
+
+
+def jump_grids(det, params=None) -> list:
+    """Return the JumpGrids of a detector's eligible media (the rules of the reference's ``fastpath.prepare``,
+    ``transport/fastpath.py:117-152``).  A medium gets a grid when one of its volumes is wide enough to hold a
+    ball of R_0 of its shortest Rayleigh lengths; the grid spans those volumes at spacing 2 R_0 lambda / sqrt(3)
+    and its bound is capped at the widest of the medium's volumes; a grid whose largest clearance still holds no
+    such ball is dropped."""  # noqa: D205, D209
+    p = det.params() if params is None else params
+    grids = []
+    for m, (vols, narrow) in sorted(_volumes_by_medium(det).items()):
+        if not eligible(det.media[m]):
             continue
         lam = shortest_rayleigh(det, p, m)
-        vols = np.flatnonzero(med_of == m)
-        if not lam > 0 or not len(vols):
+        ball = R_0 * lam
+        wide = vols[narrow >= ball]
+        if not (lam > 0 and len(wide)):
             continue
-        fire = vols[half[vols] >= R_0 * lam]
-        if not len(fire):
-            continue
-        h = 2.0 * R_0 * lam / np.sqrt(3.0)
-        lo, n = grid(det, fire, h)
-        while int(np.prod(n)) > MAX_NODES:
-            h *= max((float(np.prod(n)) / MAX_NODES) ** (1.0 / 3.0), COARSEN)
-            lo, n = grid(det, fire, h)
-        cap = float(half[vols].max())
-        c = bound(det, lo, h, n, cap)
-        if float(c.max()) < R_0 * lam:  # no node holds the smallest ball: not worth it
-            continue
-        out.append(JumpGrid(m, lo, float(h), np.asarray(n, np.int64), c, lam))
-    return out
+        lo, n, spacing = _fitting_grid(det, wide, 2.0 * ball / np.sqrt(3.0))
+        clear = bound(det, lo, spacing, n, float(narrow.max()))
+        if float(clear.max()) >= ball:
+            grids.append(JumpGrid(m, lo, float(spacing), np.asarray(n, np.int64), clear, lam))
+    return grids

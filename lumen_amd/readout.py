@@ -61,39 +61,54 @@ def pe_error(light, params: Params, det=None, group: int = 0):
     return torch.where(a > 0, p / torch.where(a > 0, a, torch.ones_like(a)), torch.zeros_like(a)) * light.arrivals_error
 
 
-# The time histogram's cubic Lagrange split of a unit mass at u in [0, 1) past node j onto nodes j - 1 .. j + 2
-# (the kernel's deposit weights), as polynomial coefficients in u, lowest power first, and their antiderivatives.
-_SPLIT = {-1: np.array([0.0, -2.0, 3.0, -1.0]) / 6.0, 0: np.array([2.0, -1.0, -2.0, 1.0]) / 2.0,
-          1: np.array([0.0, 2.0, 1.0, -1.0]) / 2.0, 2: np.array([0.0, -1.0, 0.0, 1.0]) / 6.0}  # fmt: skip
-_SPLIT_INT = {o: np.concatenate([[0.0], c / np.arange(1.0, 5.0)]) for o, c in _SPLIT.items()}
+# The time histogram splits a point mass at x (in node spacings) onto the four nearest nodes by the weights of
+# cubic Lagrange interpolation; node n receives K(x - n) with K the symmetric interpolation kernel on (-2, 2):
+#   K(s) = (1 - s^2)(2 - |s|) / 2            |s| <= 1
+#   K(s) = -(|s| - 1)(|s| - 2)(|s| - 3) / 6  1 <= |s| <= 2
+# Below: K's four unit pieces as polynomials in s, and G(s) = int_{-2}^{s} K, which reaches 1 at s = 2.
+def _kernel_pieces():
+    P = np.polynomial.Polynomial
+    inner = P.fromroots([-1.0, 1.0]) * P([2.0, 1.0]) * -0.5, P.fromroots([-1.0, 1.0]) * P([2.0, -1.0]) * -0.5
+    outer = P.fromroots([-1.0, -2.0, -3.0]) / 6.0, P.fromroots([1.0, 2.0, 3.0]) / -6.0
+    return [(-2.0, outer[0]), (-1.0, inner[0]), (0.0, inner[1]), (1.0, outer[1])]
+
+
+_PIECES = _kernel_pieces()
+_ANTI = [(s0, p.integ(lbnd=s0)) for s0, p in _PIECES]
+
+
+def _kernel(s):
+    """K(s) in its factored form, so it is exactly 0 or 1 at whole s."""
+    u = np.abs(np.asarray(s, np.float64))
+    near = 0.5 * (1.0 - u) * (1.0 + u) * (2.0 - u)
+    far = (u - 1.0) * (u - 2.0) * (3.0 - u) / 6.0
+    return np.where(u <= 1.0, near, np.where(u < 2.0, far, 0.0))
+
+
+def _kernel_cdf(s):
+    s = np.asarray(s, np.float64)
+    out = np.zeros_like(s)
+    for s0, q in _ANTI:
+        out = out + q(np.clip(s, s0, s0 + 1.0))
+    return out
 
 
 def delay_taps(edges, dt: float):
     """Return (off0, M) for a delay table with bin ``edges`` (ns, non-decreasing, >= 0) on nodes of spacing ``dt``:
-    M (taps, bins) float64, M[k, b] the mass a delay uniform on bin b moves a unit node mass onto the node off0 + k
-    further on (the reference's ``sensors/components.py:231-258`` ``delay_taps``).  Each entry is the exact integral
-    over the bin of the histogram's own cubic split weight of that node, over the bin's width; a zero-width bin is a
-    point mass (the split weights at its point), so the delay is as exact as the node grid."""  # noqa: D205, D209
+    M (taps, bins) float64, M[k, b] the mass that a delay uniform on bin b moves from a unit node mass onto the node
+    off0 + k further on (the reference's ``sensors/components.py:231-258`` ``delay_taps``).  A delay x (in node
+    spacings) puts K(x - n) on node n, K the cubic Lagrange kernel the time histogram splits with; averaged over a
+    bin (a, b) that is (G(b - n) - G(a - n)) / (b - a), G the kernel's antiderivative (closed form, piecewise
+    quartic), and a zero-width bin is the point mass K(a - n)."""  # noqa: D205, D209
     e = np.asarray(edges, np.float64) / float(dt)
     if e.ndim != 1 or len(e) < 2 or np.any(np.diff(e) < 0) or not np.all(np.isfinite(e)):
         raise ValueError("delay_taps: edges must be a finite, non-decreasing list of at least two values")
-    lo, hi = int(np.floor(e.min())) - 1, int(np.floor(e.max())) + 2
-    M = np.zeros((hi - lo + 1, len(e) - 1))
-    for b in range(len(e) - 1):
-        xa, xb = e[b], e[b + 1]
-        if xb == xa:
-            j = int(np.floor(xa))
-            for o, c in _SPLIT.items():
-                M[j + o - lo, b] += np.polynomial.polynomial.polyval(xa - j, c)
-            continue
-        for j in range(int(np.floor(xa)), int(np.ceil(xb))):
-            g0, g1 = max(xa, j) - j, min(xb, j + 1) - j
-            if g1 <= g0:
-                continue
-            for o, c in _SPLIT_INT.items():
-                pv = np.polynomial.polynomial.polyval
-                M[j + o - lo, b] += (pv(g1, c) - pv(g0, c)) / (xb - xa)
-    return lo, M
+    off0 = int(np.floor(e.min())) - 1
+    nodes = np.arange(off0, int(np.floor(e.max())) + 3, dtype=np.float64)[:, None]  # K(x - n) > 0 needs |x - n| < 2
+    a, b = e[None, :-1], e[None, 1:]
+    width = b - a
+    avg = (_kernel_cdf(b - nodes) - _kernel_cdf(a - nodes)) / np.where(width > 0, width, 1.0)
+    return off0, np.where(width > 0, avg, _kernel(a - nodes))
 
 
 def delay_apply(nodes, edges, pdf, dt: float):
@@ -284,141 +299,3 @@ def waveform_nll(w: Waveform, data):
     if d.shape != w.mean.shape:
         raise ValueError(f"waveform_nll: data of shape {tuple(d.shape)}, the expected waveform is {tuple(w.mean.shape)}")
     return 0.5 * torch.sum((d - w.mean) ** 2 / w.variance + torch.log(2.0 * math.pi * w.variance))
-
-
-# ---------------------------------------------------------------------------------------------- simulated events
-NEGATIVE_SHARE = 1e-6  # a sensor's negative expected density, as a share of its total, taken as rounding (cleared)
-
-
-@dataclass(frozen=True)
-class SampledEvent:
-    """One simulated event (the reference's ``readout/events.py`` ``SampledEvent``): photoelectrons per sensor ``pe``
-    (S,), the hit list ``hits`` (dict of arrays ``sensor``, ``t`` ns, ``q`` measured units, ``n_pe`` photoelectrons
-    in the window, ``t_first`` its first photoelectron's time) and ``waveforms`` (S, n), sample k at t0 + k dt, or
-    None."""  # noqa: D205, D209
-
-    pe: np.ndarray
-    hits: dict
-    waveforms: Any
-    t0: float
-    dt: float
-
-
-def spe_charges(rng, n: int, sigma: float = 0.3, fraction: float = 0.0, tau: float = 1.0) -> np.ndarray:
-    """Return n single-photoelectron charges (pe) of the Gaussian-plus-exponential model, mean 1 pe: with probability
-    ``fraction`` an exponential of mean ``tau``, else a Gaussian of width ``sigma`` about (1 - fraction tau) /
-    (1 - fraction) (the reference's ``sensors/components.py:355-389`` ``GaussExpCharge``)."""  # noqa: D205, D209
-    peak = (1.0 - fraction * tau) / (1.0 - fraction)
-    expo = rng.random(n) < fraction
-    g = rng.normal(peak, sigma, n)
-    return np.where(expo, rng.exponential(tau, n), g)
-
-
-def _windows(sensor, t, q, window: float):
-    """Return per integration window (sensor, first time, charge, photoelectrons): a window opens at a sensor's first
-    photoelectron not yet in one and holds those within ``window`` ns of it; windows listed by their rank within the
-    sensor, then by sensor."""  # noqa: D205, D209
-    if not len(t):
-        return [np.zeros(0, np.int64), np.zeros(0), np.zeros(0), np.zeros(0, np.int64)]
-    order = np.lexsort((t, sensor))
-    s, t, q = sensor[order], t[order], q[order]
-    rank, first, charge, count, owner = [], [], [], [], []
-    k, cur, start = -1, -1, 0.0
-    for i in range(len(t)):
-        if s[i] != cur:
-            cur, k = s[i], -1
-            start = -np.inf
-        if t[i] > start + window:  # a new window
-            k += 1
-            start = t[i]
-            rank.append(k)
-            owner.append(cur)
-            first.append(start)
-            charge.append(0.0)
-            count.append(0)
-        charge[-1] += q[i]
-        count[-1] += 1
-    o = np.lexsort((np.asarray(owner), np.asarray(rank)))
-    return [np.asarray(owner, np.int64)[o], np.asarray(first)[o], np.asarray(charge)[o], np.asarray(count, np.int64)[o]]
-
-
-def electronics(sensor, t, rng, n_sensors: int, gate=None, *, window: float = 0.0, charge=None, threshold=None,
-                charge_noise: float = 0.0, charge_scale: float = 1.0, time_walk=None, pulse=None,
-                sampling: float | None = None, gain=1.0, baseline: float = 0.0, noise: float = 0.0) -> SampledEvent:
-    """Return the hits and waveforms of given photoelectrons (sensor ids, times ``t`` ns) through the front end (the
-    reference's ``readout/events.py:62-103`` ``electronics``): SPE charges (``charge`` = dict(sigma, fraction, tau),
-    ``spe_charges``); integration windows of ``window`` ns (``_windows``); the threshold's acceptance of the window
-    charge Q (``threshold`` = (knots, accept): linear in Q, constant beyond the end knots, clipped to [0, 1]; None:
-    all); the measured charge Q + N(0, charge_noise), recorded when positive, times ``charge_scale``; the hit time the
-    window's first photoelectron plus the time walk at the measured charge (``time_walk`` = (knots, shift, sigma):
-    shift + N(0, sigma), linear in Q); a hit recorded when its window opens inside ``gate`` (None: all, no waveform).
-    Waveforms (with ``pulse`` = dict(t_peak, sigma_rise, sigma_fall) and ``sampling`` ns): each photoelectron's pulse
-    (``bigauss``) times its charge and ``gain`` (per sensor or one), at the samples over the gate, plus ``baseline``
-    and N(0, noise)."""  # noqa: D205, D209
-    sensor, t = np.asarray(sensor, np.int64), np.asarray(t, np.float64)
-    if len(sensor) and (sensor.min() < 0 or sensor.max() >= n_sensors):
-        raise ValueError(f"electronics: sensor ids must lie in [0, {n_sensors})")
-    ch = dict(sigma=0.3, fraction=0.0, tau=1.0) | (charge or {})
-    g = spe_charges(rng, len(t), float(ch["sigma"]), float(ch["fraction"]), float(ch["tau"]))
-    ws, wt, wq, wn = _windows(sensor, t, g, float(window))
-    acc = 1.0
-    if threshold is not None:
-        kn, ac = (np.asarray(x, np.float64) for x in threshold)
-        acc = np.clip(np.interp(wq, kn, ac), 0.0, 1.0)
-    qm = wq + rng.normal(0.0, float(charge_noise), len(wq))
-    keep = (rng.random(len(wq)) < acc) & (qm > 0.0)
-    ws, wt, qm, wn = ws[keep], wt[keep], qm[keep], wn[keep]
-    first = wt.copy()
-    if time_walk is not None:
-        kn, sh, sg = (np.asarray(x, np.float64) for x in time_walk)
-        wt = wt + np.interp(qm, kn, sh) + rng.normal(0.0, 1.0, len(wt)) * np.interp(qm, kn, sg)
-    if gate is not None:  # the gate reads the window's opening photoelectron; the walk may move the time past it
-        inside = (first >= gate[0]) & (first <= gate[1])
-        ws, wt, qm, wn, first = ws[inside], wt[inside], qm[inside], wn[inside], first[inside]
-    hits = dict(sensor=ws, t=wt, q=qm * float(charge_scale), n_pe=wn, t_first=first)
-    pe = np.bincount(sensor, minlength=n_sensors).astype(np.float64)
-    if pulse is None or gate is None or sampling is None:
-        return SampledEvent(pe, hits, None, 0.0, 0.0)
-    n = int(np.ceil((gate[1] - gate[0]) / sampling - 1e-9))
-    lo = pulse["t_peak"] - PULSE_SPREAD * pulse["sigma_rise"]
-    hi = pulse["t_peak"] + PULSE_SPREAD * pulse["sigma_fall"]
-    k0 = np.ceil((t + lo - gate[0]) / sampling).astype(np.int64)
-    k = k0[:, None] + np.arange(int(np.ceil((hi - lo) / sampling)) + 1)[None, :]
-    lag = gate[0] + k * sampling - t[:, None]
-    d = lag - pulse["t_peak"]
-    amp = np.exp(-0.5 * (d / np.where(d < 0.0, pulse["sigma_rise"], pulse["sigma_fall"])) ** 2)
-    gs = np.broadcast_to(np.asarray(gain, np.float64), (n_sensors,))[sensor]
-    ok = (k >= 0) & (k < n)
-    wf = np.zeros(n_sensors * n)
-    np.add.at(wf, (sensor[:, None] * n + k)[ok], (amp * (g * gs)[:, None])[ok])
-    wf = wf.reshape(n_sensors, n) + float(baseline)
-    return SampledEvent(pe, hits, wf + rng.normal(0.0, float(noise), wf.shape), float(gate[0]), float(sampling))
-
-
-def sample(light, params: Params, *, seed: int, gate, tts_sigma: float, bin_width: float | None = None,
-           dark_rate=0.0, delay=None, det=None, group: int = 0, **frontend) -> SampledEvent:
-    """Return one simulated event of the expected light (the reference's ``readout/events.py:125-151`` ``sample``;
-    pseudo-data, not differentiable): photoelectron counts Poisson in each bin of ``pe_density`` over ``gate`` (dark
-    counts, delay and transit-time spread included; bins of ``bin_width``, default the time histogram's node
-    spacing), uniform within the bin, then ``electronics`` with ``frontend``'s settings.  A negative density beyond
-    NEGATIVE_SHARE of a sensor's total (more than the cubic split's rounding) is refused."""  # noqa: D205, D209
-    tg = light.provenance.get("time_grid") if light.provenance else None
-    if tg is None:
-        raise ValueError("sample: the Light does not carry its time grid")
-    bw = float(tg[1] if bin_width is None else bin_width)
-    rho = pe_density(light, params, gate, bw, tts_sigma, det, group, dark_rate, delay)
-    rho = rho.detach().cpu().numpy().astype(np.float64)
-    if rho.ndim != 2:
-        raise ValueError("sample: one event's light (S sensors): index a batched Light's event first")
-    neg, tot = -np.minimum(rho, 0.0).sum(-1), np.abs(rho).sum(-1)
-    bad = np.flatnonzero(neg > NEGATIVE_SHARE * tot)
-    if bad.size:
-        raise ValueError(f"sample: the expected density of sensor(s) {bad[:8].tolist()} is negative beyond "
-                         f"{NEGATIVE_SHARE:g} of its total")
-    rng = np.random.default_rng(seed)
-    counts = rng.poisson(np.maximum(rho, 0.0))
-    s, b = np.nonzero(counts)
-    reps = counts[s, b]
-    s, b = np.repeat(s, reps), np.repeat(b, reps)
-    t = gate[0] + (b + rng.random(len(b))) * bw
-    return electronics(s, t, rng, rho.shape[0], gate, **frontend)

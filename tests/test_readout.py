@@ -175,53 +175,3 @@ def test_delay_is_differentiable_in_its_table():
         e[i] = 1e-6
         fd = (f(pdf.detach() + e) - f(pdf.detach() - e)) / 2e-6
         assert float(pdf.grad[i]) == pytest.approx(float(fd), rel=1e-6, abs=1e-9)
-
-
-def test_sample_draws_poisson_photoelectrons_at_the_density():
-    """Photoelectron counts average the pe (a Poisson mean), their times the arrival time (a symmetric spread)."""
-    from lumen_amd.readout import sample
-
-    light = _light()
-    pes, times = [], []
-    for seed in range(300):
-        ev = sample(light, Params({}), seed=seed, gate=(0.0, 100.0), tts_sigma=1.3)
-        pes.append(ev.pe[0])
-        times.append(ev.hits["t"])
-    pes = np.asarray(pes)
-    assert abs(pes.mean() - N_PE) < 4.0 * math.sqrt(N_PE / len(pes))
-    assert abs(pes.var() / N_PE - 1.0) < 0.25  # Poisson: variance = mean
-    t = np.concatenate(times)  # window 0: every photoelectron is a hit
-    assert len(t) > 0.9 * pes.sum() and abs(t.mean() - T_ARR) < 0.05
-
-
-def test_electronics_windows_threshold_walk_and_waveform():
-    from lumen_amd.readout import electronics, spe_charges
-
-    rng = np.random.default_rng(1)
-    sensor = np.array([0, 0, 0, 1, 1])
-    t = np.array([10.0, 12.0, 40.0, 5.0, 5.5])
-    ev = electronics(sensor, t, np.random.default_rng(2), 3, gate=(0.0, 100.0), window=5.0,
-                     charge=dict(sigma=1e-9))
-    h = ev.hits  # windows: sensor 0 [10, 12], [40]; sensor 1 [5, 5.5]; by rank within the sensor, then sensor
-    assert h["sensor"].tolist() == [0, 1, 0] and h["n_pe"].tolist() == [2, 2, 1]
-    np.testing.assert_allclose(h["t"], [10.0, 5.0, 40.0])
-    np.testing.assert_allclose(h["q"], [2.0, 2.0, 1.0], atol=1e-6)
-    assert ev.pe.tolist() == [3.0, 2.0, 0.0] and ev.waveforms is None
-    walked = electronics(sensor, t, np.random.default_rng(2), 3, gate=(0.0, 100.0), window=5.0,
-                         charge=dict(sigma=1e-9), time_walk=([0.0, 10.0], [1.5, 1.5], [0.0, 0.0]))
-    np.testing.assert_allclose(walked.hits["t"], h["t"] + 1.5)
-    # a step threshold at 0.5 pe on single photoelectrons of width 0.3: Phi(0.5 / 0.3) of them recorded
-    n = 40000
-    one = electronics(np.arange(n) % 8, np.arange(n, dtype=np.float64) * 100.0, rng, 8, window=0.0,
-                      threshold=([0.4999, 0.5001], [0.0, 1.0]))
-    from scipy.stats import norm
-
-    assert abs(len(one.hits["t"]) / n - norm.cdf(0.5 / 0.3)) < 4 * math.sqrt(0.05 / n)
-    g = spe_charges(np.random.default_rng(3), 200000, sigma=0.2, fraction=0.3, tau=0.5)
-    assert abs(g.mean() - 1.0) < 0.005  # the model's mean is 1 pe whatever its shape
-    wf = electronics(np.array([1]), np.array([20.0]), np.random.default_rng(4), 2, gate=(0.0, 60.0),
-                     charge=dict(sigma=1e-9), pulse=PULSE, sampling=0.5, gain=3.0, baseline=7.0)
-    k = np.arange(120)
-    want = 7.0 + 3.0 * _h(k * 0.5 - 20.0)
-    np.testing.assert_allclose(wf.waveforms[1], want, atol=1e-6)
-    np.testing.assert_allclose(wf.waveforms[0], 7.0)
