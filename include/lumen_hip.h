@@ -158,6 +158,22 @@ int lh_trace(lh_model model, const float *values_dev, const float *ref_values_de
              const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
              double *tally_dev, int32_t n_events, double *health_dev, void *stream);
 
+/* The sources lh_trace_static stages at most (its scratch holds their copy and end offsets: 393 KiB a device). */
+#define LH_STATIC_MAX_SOURCES 4096
+
+/* lh_trace without allocating: the same kernels, arguments, results and errors, but the work-queue counter and the
+ * staged sources live in a file-scope device array of fixed size (one copy per device, held until the process
+ * ends; its address is taken once per device in lh_model_create), so the call makes no hipMalloc, hipFree or
+ * pinned staging.  No counterpart in the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Limits: n_sources <= LH_STATIC_MAX_SOURCES
+ * (else LH_ERR_ARG).  The sources are copied from host memory by a copy enqueued on `stream`: keep them unchanged
+ * until the stream has passed the launch.  Every call on a device shares that device's array and leaves it as the
+ * last call wrote it (the queue counter is cleared by the call itself): calls that overlap on one device race,
+ * so order them on one stream. */
+int lh_trace_static(lh_model model, const float *values_dev, const float *ref_values_dev, int64_t n_values,
+                    const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *controls,
+                    double *tally_dev, int32_t n_events, double *health_dev, void *stream);
+
 /* Reverse-mode derivative by replay: each photon is traced twice from its Philox counter (no stored path): once to
  * total its cotangent-weighted deposits, once to add each step's score and pathwise terms times the deposits still
  * ahead.  `cotangent_dev`: dLoss/d(tally) in the tally layout (FP64); `slots_dev`: the gradient slot of each of the n_values packed

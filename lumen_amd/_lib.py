@@ -59,7 +59,8 @@ _LIB = None
 
 SYMBOLS = ("lh_model_create", "lh_model_destroy", "lh_model_info", "lh_trace", "lh_adjoint", "lh_query",
            "lh_batch_create", "lh_batch_destroy", "lh_batch_trace", "lh_batch_adjoint", "lh_last_kernel_ms",
-           "lh_last_launch", "lh_model_jump", "lh_bvh_build", "lh_launches", "lh_kernel_ms_of", "lh_last_error")
+           "lh_last_launch", "lh_model_jump", "lh_bvh_build", "lh_launches", "lh_kernel_ms_of", "lh_trace_static",
+           "lh_last_error")
 
 
 def lib():
@@ -78,6 +79,7 @@ def lib():
         L.lh_model_info.argtypes = [C.c_void_p, _i, _i, C.POINTER(C.c_int64)]
         L.lh_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(LhSource), C.c_int32,
                                C.c_uint64, C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.lh_trace_static.argtypes = L.lh_trace.argtypes
         L.lh_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(LhSource),
                                  C.c_int32, C.c_uint64, C.POINTER(LhControls), C.c_void_p, C.c_int32, C.c_void_p,
                                  C.c_int32, C.c_void_p, C.c_void_p]

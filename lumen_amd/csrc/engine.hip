@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1064,9 +1065,21 @@ struct lh_model_s {
   size_t occ_lds[OCC_CACHE] = {};
   int occ_per[OCC_CACHE] = {};
   int occ_next = 0;
+  unsigned char *scratch = nullptr;  // this device's copy of the static scratch array (lh_trace_static)
 };
 
 static thread_local std::string g_err;
+
+// The scratch of the allocation-free entry point lh_trace_static: the work-queue counter, then the staged sources
+// and their end offsets.  Each device holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress gives on that device (static_scratch_of), never through the array's name.
+constexpr size_t STATIC_SRC_OFF = 256;
+constexpr size_t STATIC_END_OFF = STATIC_SRC_OFF + (sizeof(lh_source) * LH_STATIC_MAX_SOURCES + 255) / 256 * 256;
+constexpr size_t STATIC_BYTES = STATIC_END_OFF + sizeof(long long) * LH_STATIC_MAX_SOURCES;
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+constexpr int STATIC_MAX_DEVICES = 64;
+static unsigned char *g_scratch_dev[STATIC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
 
 static int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -1394,6 +1407,18 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   }
   int dev = 0, cus = 0;
   HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= STATIC_MAX_DEVICES) return fail(LH_ERR_ARG, "lh_model_create: device index beyond 63");
+  {  // once per device, with the device current: the address of its copy of the static scratch array
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+      void *ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr)
+        return fail(LH_ERR_HIP, std::string("lh_model_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      g_scratch_dev[dev] = (unsigned char *)ptr;
+    }
+    m->scratch = g_scratch_dev[dev];
+  }
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   int lds_cu = 0;
   HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
@@ -1744,6 +1769,60 @@ LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n
   if (rc) return rc;
   a.nsrc = n_sources;
   a.total = total;
+  const int slot = (int)(m->launches % TIME_RING);
+  HIP_TRY(hipEventRecord(m->ev0[slot], st));
+  launch_trace(m, a, n_events, ctl, st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(m->ev1[slot], st));
+  ++m->launches;
+  return LH_OK;
+}
+
+// the staged sources' end offsets (a prefix sum of their photon counts) and a cleared work queue, for
+// lh_trace_static: one thread, n <= LH_STATIC_MAX_SOURCES
+__global__ void static_prepare(const lh_source *src, int n, long long *ends, unsigned long long *queue) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  long long acc = 0;
+  for (int i = 0; i < n; ++i) ends[i] = (acc += src[i].n);
+  *queue = 0ull;
+}
+
+LH_API int lh_trace_static(lh_model m, const float *values, const float *ref, int64_t n_values,
+                           const lh_source *sources, int32_t n_sources, uint64_t seed, const lh_controls *ctl,
+                           double *tally, int32_t n_events, double *health, void *stream) {
+  if (!m) return fail(LH_ERR_ARG, "lh_trace_static: null model");
+  if (!m->scratch) return fail(LH_ERR_ARG, "lh_trace_static: the model holds no scratch address");
+  if (n_values < m->values_needed) return fail(LH_ERR_ARG, "lh_trace_static: fewer packed values than the layout reads");
+  if (!sources || n_sources <= 0) return fail(LH_ERR_ARG, "lh_trace_static: no sources");
+  if (n_sources > LH_STATIC_MAX_SOURCES)
+    return fail(LH_ERR_ARG, "lh_trace_static: more sources than its scratch holds (LH_STATIC_MAX_SOURCES = 4096)");
+  long long total = 0;
+  for (int i = 0; i < n_sources; ++i) {  // the checks of stage_into, without its staging vector
+    const lh_source &s = sources[i];
+    if (s.n < 0) return fail(LH_ERR_ARG, "lh_trace_static: a source with a negative photon count");
+    if (s.kind < 0 || s.kind > 2) return fail(LH_ERR_ARG, "lh_trace_static: an unknown source kind");
+    if (s.event < 0 || s.event >= n_events) return fail(LH_ERR_ARG, "lh_trace_static: a source's event is out of range");
+    if (s.kind == LH_SRC_CHERENKOV && (s.medium < 0 || s.medium >= m->n_media))
+      return fail(LH_ERR_ARG, "lh_trace_static: a Cherenkov source's medium is out of range");
+    if (s.kind == LH_SRC_CHERENKOV && !(s.beta > 0.0f))
+      return fail(LH_ERR_ARG, "lh_trace_static: a Cherenkov source needs beta > 0");
+    total += s.n;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *queue = (unsigned long long *)m->scratch;
+  lh_source *src = (lh_source *)(m->scratch + STATIC_SRC_OFF);
+  long long *ends = (long long *)(m->scratch + STATIC_END_OFF);
+  Args a;
+  int rc = fill_args(m, a, values, ref, ctl, seed, tally, health);
+  if (rc) return rc;
+  a.queue = queue;
+  a.src = src;
+  a.src_end = ends;
+  a.nsrc = n_sources;
+  a.total = total;
+  HIP_TRY(hipMemcpyAsync(src, sources, sizeof(lh_source) * (size_t)n_sources, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(static_prepare, dim3(1), dim3(64), 0, st, (const lh_source *)src, (int)n_sources, ends, queue);
+  HIP_TRY(hipGetLastError());
   const int slot = (int)(m->launches % TIME_RING);
   HIP_TRY(hipEventRecord(m->ev0[slot], st));
   launch_trace(m, a, n_events, ctl, st);
