@@ -32,7 +32,15 @@
 //   Rayleigh, Lambert pinned: their closed-form angular densities (chi2 on the cosine)
 //   Fresnel           pinned: textbook normal-incidence and Brewster-angle values, energy conservation
 //   whole loop        pinned: the tutorial's published arrivals (example_tank), the specular-sphere closed form
-//   Cherenkov weight  pinned: Frank-Tamm photon yield per metre in closed form
+//   Cherenkov         pinned: Frank-Tamm photon yield per metre in closed form; and (tests/test_oracle_emission_jones.py)
+//                     every photon on its cone cos = 1/(beta n(lambda)) under a dispersive index (1e-12), the weighted
+//                     spectrum against Frank-Tamm's dN/dlambda (chi2), the start uniform along the track (KS) at
+//                     t = s/(beta c), the polarisation transverse, in the track's plane, against the track; the cone
+//                     toy's ring shares and times against the geometry's closed form
+//   Jones steps       pinned (tests/test_oracle_emission_jones.py): reflection and refraction recombined from the
+//                     Fresnel amplitudes satisfy Maxwell's boundary conditions (tangential E, H; normal D) for complex
+//                     states; total internal reflection's s-p phase (Born and Wolf); normal incidence gives r E, t E;
+//                     the mirror is a perfect conductor's (E + E' along h, no loss, transverse)
 //   moment error      pinned: brute force on a small run
 //   WLS emission      pinned (tests/test_oracle_wls.py): the trapezoid sampler's interval masses (chi2), the truncation
 //                     at the absorbed energy with weight yield (1 - (1 - F)^100), the live/reference reweighting
@@ -1576,4 +1584,45 @@ ORC_API int oracle_wls_emit(const int32_t *layout, const float *values, const fl
     out[4 * i] = e.lam, out[4 * i + 1] = e.delay, out[4 * i + 2] = e.w, out[4 * i + 3] = e.wr;
   }
   return 0;
+}
+
+// The first n photons of the sources as emitted (before any step): out (n*16) = x (3), d (3), t, lambda, the start
+// weight w0, the polarisation er (3) and ei (3), the volume located at x
+ORC_API int oracle_emit(const lh_geometry *G, const int32_t *layout, const float *values, const lh_source *src,
+                        int nsrc, uint64_t seed, long long n, double *out) {
+  Scene s = scene_of(G);
+  Run r;
+  make_run(r, s, layout, values, nullptr, src, nsrc, seed, nullptr);
+  const long long total = std::min(n, r.ends.empty() ? 0LL : r.ends.back());
+  Tally T;
+  T.t.assign((size_t)64 * r.stride, 0.0);
+  for (long long gi = 0; gi < total; ++gi) {
+    Photon<double> p;
+    emit<double>(r, gi, p, T);
+    double *o = out + 16 * gi;
+    const double v[16] = {p.x.x, p.x.y, p.x.z, p.d.x, p.d.y, p.d.z, p.t, p.lam, p.w0,
+                          p.er.x, p.er.y, p.er.z, p.ei.x, p.ei.y, p.ei.z, (double)p.vol};
+    for (int k = 0; k < 16; ++k) o[k] = v[k];
+  }
+  return (int)total;
+}
+
+// The Jones step of a specular reflection or a refraction: the state e = (er, ei) of a photon along wi meeting a
+// surface of normal n, recombined on the s axis with the amplitudes (a_s, a_p) (amps = re, im of each) into the
+// direction k; out = the new (er, ei), normalised
+ORC_API void oracle_pol_recombine(const double wi[3], const double n[3], const double amps[4], const double k[3],
+                                  const double e[6], double out[6]) {
+  V3<double> er = v3(e[0], e[1], e[2]), ei = v3(e[3], e[4], e[5]);
+  const V3<double> w = v3(wi[0], wi[1], wi[2]), nn = v3(n[0], n[1], n[2]);
+  pol_recombine(er, ei, w, s_axis(w, nn), C2{amps[0], amps[1]}, C2{amps[2], amps[3]}, v3(k[0], k[1], k[2]));
+  const double v[6] = {er.x, er.y, er.z, ei.x, ei.y, ei.z};
+  for (int i = 0; i < 6; ++i) out[i] = v[i];
+}
+
+// The state after a mirror of normal h (the Lambertian and facet paths): out = the new (er, ei)
+ORC_API void oracle_pol_mirror(const double h[3], const double e[6], double out[6]) {
+  V3<double> er = v3(e[0], e[1], e[2]), ei = v3(e[3], e[4], e[5]);
+  pol_mirror(er, ei, v3(h[0], h[1], h[2]));
+  const double v[6] = {er.x, er.y, er.z, ei.x, ei.y, ei.z};
+  for (int i = 0; i < 6; ++i) out[i] = v[i];
 }

@@ -57,6 +57,12 @@ def lib():
         L.oracle_mie.restype = None
         L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
         L.oracle_mie_pdf.restype = None
+        L.oracle_emit.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.POINTER(E.LhSource), C.c_int,
+                                  C.c_uint64, C.c_longlong, C.c_void_p]
+        L.oracle_pol_recombine.argtypes = [C.c_void_p] * 6
+        L.oracle_pol_recombine.restype = None
+        L.oracle_pol_mirror.argtypes = [C.c_void_p] * 3
+        L.oracle_pol_mirror.restype = None
         _LIB = L
     return _LIB
 
@@ -125,6 +131,21 @@ class OracleRun:
         lib().oracle_count_ops(C.byref(self.geo), _p(lay), _p(vals), rows, len(rows), C.c_uint64(seed),
                                C.byref(ctl), int(n), _p(out))
         return out
+
+    def emit(self, source, samples, n, *, seed=0, params=None):
+        """Return the first n photons as emitted: dict of x (n, 3), d (n, 3), t, lam, w0, er (n, 3), ei (n, 3), vol."""
+        from lumen_amd import api, pack
+
+        pk = pack.pack(self.det, self.det.params() if params is None else params)
+        rows, _ = api.source_rows(self.det, source, samples)
+        out = np.zeros((int(n), 16))
+        vals = np.ascontiguousarray(pk.values, np.float32)
+        lay = np.ascontiguousarray(pk.layout, np.int32)
+        got = lib().oracle_emit(C.byref(self.geo), _p(lay), _p(vals), rows, len(rows), C.c_uint64(int(seed)), int(n),
+                                _p(out))
+        out = out[:got]
+        return dict(x=out[:, 0:3], d=out[:, 3:6], t=out[:, 6], lam=out[:, 7], w0=out[:, 8], er=out[:, 9:12],
+                    ei=out[:, 12:15], vol=out[:, 15].astype(np.int64))
 
     def query(self, x, d, vol, last=None, tmax=1e30):
         """Return (t, vol_after, normal, srf) of rays (FP64 referee)."""
@@ -225,3 +246,22 @@ def light_of(tally, health, S, K, samples):
         ov = rest[S * nt:].reshape(S, 2, 2)
         out["late"], out["early"] = ov[:, 0], ov[:, 1]
     return out
+
+
+def pol_recombine(wi, n, a_s, a_p, k, er, ei):
+    """Return the oracle's (er, ei) after a specular reflection or refraction: the state (er, ei) along wi at a
+    surface of normal n, recombined on the s axis with the complex amplitudes (a_s, a_p) into direction k."""
+    v = [np.ascontiguousarray(x, np.float64) for x in (wi, n, [a_s.real, a_s.imag, a_p.real, a_p.imag], k)]
+    e = np.ascontiguousarray(np.concatenate([er, ei]), np.float64)
+    out = np.zeros(6)
+    lib().oracle_pol_recombine(*(_p(x) for x in v), _p(e), _p(out))
+    return out[:3], out[3:]
+
+
+def pol_mirror(h, er, ei):
+    """Return the oracle's (er, ei) after a mirror of normal h (the Lambertian and facet paths)."""
+    hh = np.ascontiguousarray(h, np.float64)
+    e = np.ascontiguousarray(np.concatenate([er, ei]), np.float64)
+    out = np.zeros(6)
+    lib().oracle_pol_mirror(_p(hh), _p(e), _p(out))
+    return out[:3], out[3:]

@@ -102,6 +102,44 @@ def mie_toy(case="A", rings=1, length=MIE_L):
                     [SensorGroup("rings", np.arange(rings))], Params(values))
 
 
+def cone_toy(rings=90, n=MIE_N):
+    """Return the Cherenkov cone toy: plain water (RINDEX n flat over 1.5-5 eV, nothing absorbs or scatters) in a
+    world orb of 1.2 m inside a counting shell r in [1, 1.01] m cut into ``rings`` polar rings of equal angle, each
+    its own sensor absorbing and counting every photon that enters it."""  # noqa: D205, D209
+    d = np.pi / rings
+    kinds = [solids.orb(1.2, name="world")] + [solids.sphere(MIE_R, 1.01, i * d, d, name=f"ring{i}")
+                                              for i in range(rings)]
+    vols = [navarrays.Volume("world", 0, np.eye(3), np.zeros(3), -1, 0)]
+    vols += [navarrays.Volume(f"world/ring{i}", 1 + i, np.eye(3), np.zeros(3), 0, 1) for i in range(rings)]
+    nav = navarrays.build(vols, kinds, outside_cls=2)
+    values = {"materials/Water/RINDEX": Table(E_KNOTS, np.array([n, n])),
+              "materials/SensLike/RINDEX": Table(E_KNOTS, np.array([n, n]))}
+    media = [Medium("Water", "materials/Water/RINDEX"), Medium("SensLike", "materials/SensLike/RINDEX")]
+    rows = [SurfaceRow("ring", "SPECULAR_METAL", {"R": 0.0, "eff": 1.0}), SurfaceRow("absorber", "ABSORBER")]
+    row_of = np.full((3, 3), -1, np.int32)
+    row_of[0, 1] = 0
+    row_of[0, 2] = row_of[1, 0] = row_of[1, 2] = 1
+    sensor_of_vol = np.array([-1] + list(range(rings)) + [-1], np.int32)
+    return Detector(f"cone_{rings}", nav, media, np.array([0, 1, -1], np.int32), ["Water", "SensLike", "OUT"],
+                    rows, row_of, sensor_of_vol, [f"ring{i}" for i in range(rings)],
+                    [SensorGroup("rings", np.arange(rings))], Params(values))
+
+
+def cone_rings_closed_form(rings, z0, z1, cos_c, beta, n, R=MIE_R, m=200001):
+    """Return (share, mean time ns) per ring of the cone toy for a track along +z from z0 to z1 (m) started at t = 0:
+    a photon born at z (uniform on the track, at (z - z0) / (beta c)) along the cone's polar angle reaches the sphere of
+    radius R at the polar angle alpha(z) of z + t cos, t the root of t^2 + 2 z cos t + z^2 - R^2 = 0, after the
+    time t n / c (the group velocity c / n of a flat index); the azimuth plays no part.  Midpoint rule on m points."""  # noqa: D205, D209
+    z = z0 + (np.arange(m) + 0.5) / m * (z1 - z0)
+    t = -z * cos_c + np.sqrt(z * z * cos_c * cos_c - z * z + R * R)
+    alpha = np.arccos(np.clip((z + t * cos_c) / R, -1.0, 1.0))
+    ring = np.minimum((alpha / (np.pi / rings)).astype(np.int64), rings - 1)
+    arrive = (z - z0) / (beta * units.C_VACUUM) + t * n / units.C_VACUUM
+    share = np.bincount(ring, minlength=rings) / m
+    tsum = np.bincount(ring, weights=arrive, minlength=rings) / m
+    return share, np.where(share > 0, tsum / np.where(share > 0, share, 1.0), 0.0)
+
+
 def mie_hg_cdf_angle(theta, g):
     """P(scattering angle <= theta) of HG(g), float64 (the textbook CDF in 1 - cos theta = 2 sin^2(theta/2))."""
     s2 = np.sin(0.5 * np.asarray(theta, np.float64)) ** 2
