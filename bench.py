@@ -197,59 +197,70 @@ C3_KEYS = ("materials/Water/ABSLENGTH", "materials/Water/RAYLEIGH", "surfaces/Li
            "surfaces/Liner/sigma_alpha")
 
 
-def fwd_adjoint_leg(det, src, n, steps):
+def fwd_adjoint_leg(det, src, n, steps, fd_seeds=4):
     """Forward plus adjoint on the same tank: the Poisson NLL of pe against a target simulated at parameters
-    perturbed by N(0, 5 %) (seed 3); gradient over the 16 absorption knots, the 16 Rayleigh knots (the Rayleigh
-    scale's derivative is their dot product with the lengths), the liner's reflectivity and roughness, and the 1,000
-    PMT QEs (after tracking)."""  # noqa: D205, D209
+    perturbed by N(0, 5 %) (lumen_amd.gradcheck.TankLikelihood); gradient over the 16 absorption knots, the 16
+    Rayleigh knots, the liner's reflectivity and roughness, and the 1,000 PMT QEs (after tracking).  Then the
+    finite-difference comparison on the seeded 32-parameter subset: common-random-number central differences at
+    h = 1e-3 and the adjoint over ``fd_seeds`` seeds each, timed; the gradients' spread over the seeds gives the
+    photons finite differences need for the adjoint's error, so the speed-up is measured at matched MC error."""  # noqa: D205, D209
     import numpy as np
     import torch
 
-    import lumen_amd as L
-    from lumen_amd.autograd import simulate_diff, table_tensor
-    from lumen_amd.params import Table
+    from lumen_amd.gradcheck import TankLikelihood, after_tracking, crn_central, values_of
 
-    p0 = det.params()
-    rng = np.random.default_rng(3)
-    pt = p0
-    for k in C3_KEYS + ("sensors/pmt/qe",):
-        v = p0[k]
-        arr = v.value if isinstance(v, Table) else np.atleast_1d(v)
-        new = arr * (1.0 + 0.05 * rng.normal(size=arr.shape))
-        pt = pt.set(k, new if arr.size > 1 else float(new[0]))
-    qt = torch.as_tensor(np.asarray(pt["sensors/pmt/qe"], np.float64), device="cuda")
-    target = (L.simulate(det, src, pt, samples=n, seed=77).charge[:, 0] * qt).detach()
-    qe = torch.as_tensor(np.asarray(p0["sensors/pmt/qe"], np.float64), device="cuda", dtype=torch.float64)
-
-    def one(seed):
-        ts = {k: table_tensor(p0, k) for k in C3_KEYS}
-        q = qe.clone().requires_grad_(True)
-        light = simulate_diff(det, src, ts, samples=n, seed=seed)
-        pe = light.charge[:, 0] * q
-        loss = (pe - target * torch.log(torch.clamp(pe, min=1e-30))).sum()
-        loss.backward()
-        run = light.provenance["run"]
-        return run, ts, q
-
-    run, ts, q = one(500)
+    prob = TankLikelihood.build(det, src, target_samples=n)
+    prob.adjoint(samples=n, seed=500)  # warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    runs = []
+    lights = []
     for k in range(steps):  # enqueued back to back: the kernel times are read after the timed region
-        run, ts, q = one(600 + k)
-        runs.append(run)
+        grads, light = prob.adjoint(samples=n, seed=600 + k, to_host=False)
+        lights.append(light)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    fwd_ms = [r.kernel_ms for r in runs[-128:]]
-    adj_ms = [r.adjoint_ms for r in runs[-128:]]
-    n_params = sum(int(t.numel()) for t in ts.values()) + int(q.numel())
+    runs = [x.provenance["run"] for x in lights]
+    fwd_ms = [r.kernel_ms for r in runs]
+    adj_ms = [r.adjoint_ms for r in runs]
+    n_params = sum(len(values_of(prob.params, k)) for k in prob.keys())
+    n_track = sum(len(values_of(prob.params, k)) for k in C3_KEYS)
     t_step = wall / steps
-    return {"value": n / t_step, "unit": "photons/s", "ms_per_step": 1e3 * t_step, "photons_per_step": n,
-            "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
-            "parameters": n_params,
-            "speedup_vs_central_fd": (2 * n_params + 1) * float(np.mean(fwd_ms)) /
-                                     (float(np.mean(fwd_ms)) + float(np.mean(adj_ms))),
-            "loss": "Poisson NLL of pe vs a target at N(0, 5%)-perturbed parameters"}
+    out = {"value": n / t_step, "unit": "photons/s", "ms_per_step": 1e3 * t_step, "photons_per_step": n,
+           "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
+           "parameters": n_params, "loss": "Poisson NLL of pe vs a target at N(0, 5%)-perturbed parameters"}
+    log("forward + adjoint timed")
+    picks = prob.picks(32, seed=4)
+    track = np.array([not after_tracking(k) for k, _ in picks])
+    ga, gf, t_adj, t_fd = [], [], 0.0, 0.0
+    for s in range(fd_seeds):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        grads, _ = prob.adjoint(samples=n, seed=900 + s)
+        torch.cuda.synchronize()
+        t_adj += time.perf_counter() - t0
+        ga.append([grads[k][i] for k, i in picks])
+        t0 = time.perf_counter()
+        gf.append(crn_central(prob.loss, det, src, prob.params, [p for p, t in zip(picks, track) if t],
+                              samples=n, seed=900 + s, h=1e-3))
+        torch.cuda.synchronize()
+        t_fd += time.perf_counter() - t0
+    ga = np.asarray(ga)[:, track]
+    gf = np.asarray(gf)
+    mean_a, mean_f = ga.mean(0), gf.mean(0)
+    sd_a, sd_f = ga.std(0, ddof=1), gf.std(0, ddof=1)
+    rel = np.abs(mean_a - mean_f) / np.maximum(np.abs(mean_f), 1e-30)
+    ratio = float(np.median((sd_f / np.maximum(sd_a, 1e-300)) ** 2))  # photons FD needs per adjoint photon
+    t_forward = t_fd / (fd_seeds * 2 * int(track.sum()))  # one CRN forward of n photons, measured
+    t_grad = t_adj / fd_seeds  # forward + adjoint + readout of n photons, measured
+    out["finite_differences"] = {
+        "subset": len(picks), "tracking_parameters_in_subset": int(track.sum()), "h": 1e-3, "seeds": fd_seeds,
+        "crn_forward_ms": 1e3 * t_forward, "fwd_adjoint_ms": 1e3 * t_grad,
+        "max_rel_diff_of_seed_means": float(rel.max()), "median_rel_diff_of_seed_means": float(np.median(rel)),
+        "variance_ratio_fd_over_adjoint": ratio,
+        "speedup_all_parameters": (2 * n_params + 1) * t_forward * ratio / t_grad,
+        "speedup_tracking_parameters": (2 * n_track + 1) * t_forward * ratio / t_grad}
+    out["speedup_vs_central_fd"] = out["finite_differences"]["speedup_all_parameters"]
+    return out
 
 
 def log(msg):
@@ -303,10 +314,20 @@ def batched_leg(det, repeats):
         run.sequential()
     torch.cuda.synchronize()
     t_seq = (time.perf_counter() - t0) / repeats
+    run.sequential_adjoint(loss)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.sequential_adjoint(loss)
+    torch.cuda.synchronize()
+    t_seq_both = time.perf_counter() - t0
     return {"sources": len(batch.sources), "photons": int(run.total), "batched_ms": 1e3 * t_batch,
-            "batched_fwd_adjoint_ms": 1e3 * t_both, "sequential_ms": 1e3 * t_seq, "speedup": t_seq / t_batch,
+            "batched_fwd_adjoint_ms": 1e3 * t_both, "sequential_ms": 1e3 * t_seq,
+            "sequential_fwd_adjoint_ms": 1e3 * t_seq_both, "speedup": t_seq / t_batch,
+            "speedup_fwd_adjoint": t_seq_both / t_both,
             "photons_per_s_batched": run.total / t_batch, "photons_per_s_batched_fwd_adjoint": run.total / t_both,
-            "autotuned": {"chunk": run.controls.chunk, "blocks_per_cu": run.controls.blocks_per_cu}}
+            "autotuned": {"chunk": run.controls.chunk, "blocks_per_cu": run.controls.blocks_per_cu,
+                          "adjoint_chunk": run.adj_controls.chunk,
+                          "adjoint_blocks_per_cu": run.adj_controls.blocks_per_cu}}
 
 
 LAR_KEYS = ("materials/LAr/RAYLEIGH", "surfaces/PTFE/REFLECTIVITY")

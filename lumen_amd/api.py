@@ -324,7 +324,10 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     buf = torch.zeros((n_ev, max(eng.stride, 1)), dtype=torch.float64, device=eng.device)
     tally = buf[:, :eng.stride]
     health = torch.zeros(_lib.HEALTH_N, dtype=torch.float64, device=eng.device)
-    st = torch.cuda.current_stream(eng.device) if stream is None else stream
+    current = torch.cuda.current_stream(eng.device)
+    st = current if stream is None else stream
+    if st != current:  # the buffers' fills and the values' uploads (on the current stream) before the kernel
+        st.wait_stream(current)
     ctl = controls.struct()
     if controls.jump:
         eng.ensure_jump()
@@ -335,9 +338,11 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
                                    _lib.C.c_void_p(buf.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
                                    _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
     launch = eng.launches() - 1
-    for t in (values, ref_values):  # read by the kernel after this returns: kept from reuse on other streams
-        if t is not None and stream is not None:
-            t.record_stream(st)
+    if st != current:
+        for t in (values, ref_values, buf, health):  # in use on `st` after this returns: kept from reuse meanwhile
+            if t is not None:
+                t.record_stream(st)
+        current.wait_stream(st)  # the results, for what the caller does next on the current stream
     light = light_of(eng, tally, lambda: health.cpu().numpy(), samples if not isinstance(source, Batch) else None,
                      source)
     light.kernel_ms = lambda: eng.kernel_ms(launch)  # the call returns before the kernel ends: read on first use
@@ -362,8 +367,11 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
         ov = tally[:, t0 + S * eng.nt:t0 + S * (eng.nt + 4)].reshape(tally.shape[0], S, 2, 2)
         late, early = ov[:, :, 0], ov[:, :, 1]
     if isinstance(source, Batch):
-        n = torch.as_tensor([max(int(x), 2) for x in source.samples], dtype=torch.float64, device=tally.device)
-        n = n[:, None] if source.events is None else torch.full_like(arr[:, :1], float(max(source.samples)))
+        # each event's photons are its sources' (a fixed count per source, not a draw from the mixture): the
+        # moment error of an event uses the photons traced into it, the photon counts summed per event
+        ev = np.arange(len(source.samples)) if source.events is None else np.asarray(source.events, np.int64)
+        per_event = np.bincount(ev, weights=np.asarray(source.samples, np.float64), minlength=tally.shape[0])
+        n = torch.as_tensor(np.maximum(per_event, 2.0), dtype=torch.float64, device=tally.device)[:, None]
         err = torch.sqrt(torch.clamp(n / (n - 1.0) * (arr2 - arr * arr / n), min=0.0))
     else:
         err = moment_error(arr2, arr, samples)

@@ -1,6 +1,7 @@
-"""The launch autotuner: picks the work-queue chunk (photons a wavefront claims at once) and the persistent grid's
-resident blocks per CU for a staged batch by timing short launches of it, and remembers the choice per
-(detector, photons, sources, GPU) in ``$LUMEN_AMD_CACHE`` (default ``~/.cache/lumen_amd/autotune.json``).
+"""The launch autotuner: picks the work-queue chunk (photons a wavefront claims at once: the photons per wavefront of
+the persistent loop, whose 64 lanes each refill from that claim) and the persistent grid's resident blocks per CU for
+a staged batch, for its forward and its adjoint launch separately, by timing launches of it, and remembers the
+choices per (detector, launch kind, photons, sources, GPU) in ``$LUMEN_AMD_CACHE`` (default ``~/.cache/lumen_amd/autotune.json``).
 
 Small chunks balance uneven sources at the end of a launch; large ones cut queue traffic.  Fewer blocks per CU
 trade latency hiding for cache footprint.  The search is exhaustive over a small grid; each point is timed on the
@@ -46,13 +47,20 @@ def _key(run, what):
     return f"{run.det.name}|{what}|photons={run.total}|sources={len(run.rows)}|{name}"
 
 
-def tune(run, what="forward", chunks=CHUNKS, blocks=BLOCKS, repeats=2):
-    """Time the staged batch ``run`` (a ``lumen_amd.batch.BatchRun``) at every (chunk, blocks_per_cu); return
-    (the best Controls, {point: ms})."""  # noqa: D205, D209
+ADJ_CHUNKS = (128, 512, 2048)  # the adjoint's launches cost ~4 forward ones: a coarser grid, timed once each
+
+
+def tune(run, what="forward", chunks=None, blocks=BLOCKS, repeats=None):
+    """Time the staged batch ``run`` (a ``lumen_amd.batch.BatchRun``) at every (chunk, blocks_per_cu) of the
+    forward launch (``what`` "forward") or of the adjoint launch ("adjoint": its cotangent as ``run.cot`` holds
+    it); return (the best Controls, {point: ms}).  Ties go to the first point timed (as many blocks as fit)."""  # noqa: D205, D209
     import torch
 
     from .api import Controls
 
+    adj = what == "adjoint"
+    chunks = chunks or (ADJ_CHUNKS if adj else CHUNKS)
+    repeats = repeats or (1 if adj else 2)
     base = Controls()
     times = {}
     for ch in chunks:
@@ -62,7 +70,11 @@ def tune(run, what="forward", chunks=CHUNKS, blocks=BLOCKS, repeats=2):
             for _ in range(repeats):
                 torch.cuda.synchronize(run.eng.device)
                 t0 = time.perf_counter()
-                run._fwd_body(ctl)
+                if adj:
+                    run.grad.zero_()
+                    run._launch_adjoint(ctl)
+                else:
+                    run._fwd_body(ctl)
                 torch.cuda.synchronize(run.eng.device)
                 best = min(best, time.perf_counter() - t0)
             times[(ch, bl)] = 1e3 * best
@@ -71,7 +83,8 @@ def tune(run, what="forward", chunks=CHUNKS, blocks=BLOCKS, repeats=2):
 
 
 def tuned_controls(run, what="forward"):
-    """Return the remembered best Controls of this configuration, tuning it first when it is new."""
+    """Return the remembered best Controls of this configuration and launch kind, tuning it first when it is
+    new."""  # noqa: D205, D209
     from .api import Controls
 
     d = _load()

@@ -53,8 +53,10 @@ class BatchRun:
         self.grad = torch.zeros(self.n_grad, dtype=torch.float64, device=dev)
         self.adj_health = torch.zeros_like(self.health)
         self.graph = graph
+        self.tune = tune and controls is None
         self.controls = controls if controls is not None else (
             tuned_controls(self, "forward") if tune else Controls())
+        self.adj_controls = controls  # the adjoint's own: tuned on its first launch (None until then)
         if self.controls.jump:
             self.eng.ensure_jump()
         self._g_fwd = self._g_both = None
@@ -89,7 +91,7 @@ class BatchRun:
         import torch
 
         st = torch.cuda.current_stream(self.eng.device)
-        c = (ctl or self.controls).struct()
+        c = (ctl or self.adj_controls or Controls()).struct()
         _lib.check(_lib.lib().lh_batch_adjoint(self.handle, _lib.C.c_void_p(self.values.data_ptr()), None,
                                                int(self.values.numel()), _lib.C.c_uint64(self.seed), _lib.C.byref(c),
                                                _lib.C.c_void_p(self.cot.data_ptr()), self.n_ev,
@@ -139,6 +141,10 @@ class BatchRun:
         slots (n_grad,).  Captured as one graph on first use (the loss function is part of it)."""  # noqa: D205, D209
         if self._loss is not loss:
             self._loss, self._g_both = loss, None
+        if self.adj_controls is None:  # the adjoint's launch shape: tuned once, with this loss's cotangent
+            self._fwd_body()
+            self.cot.copy_(loss(self.tally))
+            self.adj_controls = tuned_controls(self, "adjoint") if self.tune else Controls()
         if not self.graph:
             self._both_body()
             return self.grad
@@ -146,6 +152,36 @@ class BatchRun:
             self._g_both = self._capture(self._both_body)
         self._g_both.replay()
         return self.grad
+
+    def sequential_adjoint(self, loss):
+        """The forward plus adjoint one source at a time, as one ``simulate`` and one backward per source do: each
+        source's ``lh_trace``, its cotangent ``loss`` of the tallies (the per-event loss reads its own event's row),
+        its ``lh_adjoint``; returns the gradient slots summed over the sources."""  # noqa: D205, D209
+        import torch
+
+        out = torch.zeros_like(self.tally)
+        health = torch.zeros_like(self.health)
+        grad = torch.zeros_like(self.grad)
+        c = Controls().struct()
+        st = torch.cuda.current_stream(self.eng.device)
+        vp = _lib.C.c_void_p(self.values.data_ptr())
+        for i in range(len(self.rows)):
+            one = (_lib.LhSource * 1)()
+            one[0] = self.rows[i]
+            _lib.check(_lib.lib().lh_trace(self.eng.handle, vp, None, int(self.values.numel()), one, 1,
+                                           _lib.C.c_uint64(self.seed), _lib.C.byref(c),
+                                           _lib.C.c_void_p(out.data_ptr()), self.n_ev,
+                                           _lib.C.c_void_p(health.data_ptr()), _lib.C.c_void_p(st.cuda_stream)),
+                       "lh_trace")
+            cot = loss(out).to(torch.float64).contiguous()
+            _lib.check(_lib.lib().lh_adjoint(self.eng.handle, vp, None, int(self.values.numel()),
+                                             _lib.C.c_void_p(self.slots.data_ptr()), one, 1,
+                                             _lib.C.c_uint64(self.seed), _lib.C.byref(c),
+                                             _lib.C.c_void_p(cot.data_ptr()), self.n_ev,
+                                             _lib.C.c_void_p(grad.data_ptr()), self.n_grad,
+                                             _lib.C.c_void_p(health.data_ptr()), _lib.C.c_void_p(st.cuda_stream)),
+                       "lh_adjoint")
+        return grad
 
     def sequential(self):
         """The same sources one ``lh_trace`` per source, as one ``simulate`` call per source does; the tallies."""

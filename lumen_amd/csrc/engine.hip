@@ -1753,6 +1753,13 @@ static void launch_trace(lh_model m, Args &a, int n_events, const lh_controls *c
   launch_kernel(false, fk, dim3(grid), bytes, st, a, AdjArgs{}, n_lds);
 }
 
+// a launch on `st` after one on another stream: the model's device sources and work-queue counter are rewritten by
+// this launch, so `st` first waits for the previous launch's end (a no-op on the same stream)
+static int after_previous_launch(lh_model m, hipStream_t st) {
+  if (m->launches > 0) HIP_TRY(hipStreamWaitEvent(st, m->ev1[(int)((m->launches - 1) % TIME_RING)], 0));
+  return LH_OK;
+}
+
 LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n_values, const lh_source *sources,
                     int32_t n_sources, uint64_t seed, const lh_controls *ctl, double *tally, int32_t n_events,
                     double *health, void *stream) {
@@ -1763,7 +1770,9 @@ LH_API int lh_trace(lh_model m, const float *values, const float *ref, int64_t n
   hipStream_t st = (hipStream_t)stream;
   Args a;
   long long total = 0;
-  int rc = stage_sources(m, sources, n_sources, &total, st);
+  int rc = after_previous_launch(m, st);
+  if (rc) return rc;
+  rc = stage_sources(m, sources, n_sources, &total, st);
   if (rc) return rc;
   rc = fill_args(m, a, values, ref, ctl, seed, tally, health);
   if (rc) return rc;
@@ -1852,7 +1861,9 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   }
   Args a;
   long long total = 0;
-  int rc = stage_sources(m, sources, n_sources, &total, st);
+  int rc = after_previous_launch(m, st);
+  if (rc) return rc;
+  rc = stage_sources(m, sources, n_sources, &total, st);
   if (rc) return rc;
   rc = fill_args(m, a, values, ref, ctl, seed, const_cast<double *>(cot), health);
   if (rc) return rc;

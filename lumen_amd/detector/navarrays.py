@@ -3,7 +3,8 @@
 This is the reference's persistent navigator format (``geometry/navigator_build.py:51-90``, ``:371-397``,
 ``geometry/csg_build.py:31-46``, ``:165-210``, on disk ``geometry/cache.py:187-264``): a dict ``host`` of per-node,
 per-group, per-navigator-id and per-volume arrays, and ``csg`` = (T, K, G) leaf, kind and program tables with the
-reference's column names.  ``save``/``load`` write and read ``arrays.npz`` plus ``meta.json`` in a directory.
+reference's column names.  ``save``/``load`` write and read a cache entry in the reference's layout (``arrays.npz``
+plus ``meta.json``, the reference's ``_encode`` tree).
 
 The builder here covers the solids this engine navigates (quadric-and-plane leaves; no meshes, no tori), volumes that
 lie inside their mother (no overlaps, so no ownership links), and daughter groups in the world frame or, for daughters
@@ -67,32 +68,124 @@ class NavArrays:
 
     # ------------------------------------------------------------------ on disk: arrays.npz + meta.json
     def save(self, path: str) -> None:
-        """Write ``arrays.npz`` (every array, host then csg, by name) and ``meta.json`` into directory ``path``."""
+        """Write the arrays into directory ``path`` in the reference's cache entry layout (``geometry/cache.py:187-230``
+        ``_encode``/``_save``): ``meta.json`` the tree of a ``NavArrays`` record (``dc`` the class, ``f`` its fields
+        in the reference's order), its arrays in ``arrays.npz`` as ``a0``, ``a1``, ... in the order the tree meets
+        them.  Fields this build does not hold are written empty: no meshes (``n_mesh`` 0, the mesh arrays one
+        placeholder row), no ownership links, no definitions or placement-check probes."""  # noqa: D205, D209
         os.makedirs(path, exist_ok=True)
-        arrs = {f"host/{k}": v for k, v in self.host.items()}
-        for tab in ("T", "K", "G"):
-            arrs |= {f"csg/{tab}/{k}": v for k, v in self.csg[tab].items()}
-        arrs["lo_w"], arrs["hi_w"] = self.lo_w, self.hi_w
-        np.savez(os.path.join(path, "arrays.npz"), **arrs)
-        meta = dict(format="lumen-navarrays/1", n=int(self.n), n_mesh=int(self.n_mesh), names=list(self.names),
-                    precision=self.precision, stats=self.stats, empty_leaf=int(self.csg["empty_leaf"]),
-                    arrays=sorted(arrs))
+        store: dict = {}
+        tree = _tree_of(self.reference_record(), store)
+        np.savez(os.path.join(path, "arrays.npz"), **store)
         with open(os.path.join(path, "meta.json"), "w") as f:
-            json.dump(meta, f, indent=1)
+            json.dump(tree, f)
+
+    def reference_record(self) -> "_Record":
+        """Return the arrays as the reference's ``NavArrays`` record (``geometry/navigator_build.py:51-90``)."""
+        n, T = int(self.n), self.csg["T"]
+        host = dict(self.host)
+        fdt = np.float64 if self.precision == "float64" else np.float32
+        for k in ("V0", "V1", "V2", "N"):  # no meshes: one placeholder row, as the reference writes for none
+            host.setdefault(k, np.zeros((0, 3), fdt))
+        for k in ("cstart", "cn", "cmesh", "tchunk"):
+            host.setdefault(k, np.zeros(1, np.int32))
+        ub = np.asarray(host["ub"], np.int64)
+        fm = 1 + int(np.max(np.asarray(T["np_"])))
+        ch0 = n + 1  # the first mesh-chunk item: after every navigator id (none here)
+        su = ch0
+        srf = (su, su + int(ub[-1]) * fm * 4, fm)
+        parent = np.asarray(host["parent"][:n], np.int64)
+        kids = tuple(tuple(int(c) for c in np.flatnonzero(parent == (v if v < n else n))) for v in range(n + 1))
+        csg = _Record("lumen.geometry.csg_build:CsgTables", dict(
+            T=dict(T), K=dict(self.csg["K"]), G=dict(self.csg["G"]), kind_box=(),
+            empty_leaf=int(self.csg["empty_leaf"]), has_torus=bool(np.any(np.asarray(T["typ"]) == 2))))
+        return _Record("lumen.geometry.navigator_build:NavArrays", dict(
+            host=host, csg=csg, n=n, n_mesh=int(self.n_mesh), chunk=MESH_CHUNK, ch0=ch0, srf=srf,
+            names=tuple(self.names), defs=(), stats=dict(self.stats), owned=(), clipped=(), kids=kids,
+            lo_w=np.asarray(self.lo_w, np.float64), hi_w=np.asarray(self.hi_w, np.float64), probe={},
+            origins=tuple(self.names) + ("OUT",), pieces=tuple(int(x) for x in np.asarray(host["piece"])[:n]),
+            thin=(), precision=self.precision))
 
     @staticmethod
     def load(path: str) -> "NavArrays":
-        """Read a directory written by ``save`` (or by the reference's cache with the same names)."""
+        """Read a cache entry: the reference's layout (``save``; an entry the reference's ``geometry/cache.py``
+        ``_save`` wrote, whose arrays hold no meshes), or this build's earlier one (``host/<k>``, ``csg/<T|K|G>/<k>``
+        arrays).  Raises ``ValueError`` for anything else."""  # noqa: D205, D209
         with open(os.path.join(path, "meta.json")) as f:
             meta = json.load(f)
         with np.load(os.path.join(path, "arrays.npz"), allow_pickle=False) as z:
-            host = {k[5:]: z[k] for k in z.files if k.startswith("host/")}
-            csg = {tab: {k[len(tab) + 5:]: z[k] for k in z.files if k.startswith(f"csg/{tab}/")}
+            store = {k: z[k] for k in z.files}
+        if isinstance(meta, dict) and "dc" in meta:
+            rec = _value_of(meta, store)
+            if not (isinstance(rec, _Record) and rec.cls.endswith(":NavArrays")):
+                raise ValueError(f"navigator cache entry {path}: not a NavArrays record")
+            f = rec.fields
+            if int(f.get("n_mesh", 0)) > 0:
+                raise ValueError(f"navigator cache entry {path}: holds {f['n_mesh']} meshes; this navigator has none")
+            c = f["csg"].fields if isinstance(f["csg"], _Record) else f["csg"]
+            csg = {"T": dict(c["T"]), "K": dict(c["K"]), "G": dict(c["G"]), "empty_leaf": int(c["empty_leaf"])}
+            host = {k: v for k, v in f["host"].items() if k not in ("V0", "V1", "V2", "N", "cstart", "cn", "cmesh",
+                                                                        "tchunk")}
+            return NavArrays(host, csg, int(f["n"]), tuple(f["names"]), np.asarray(f["lo_w"]), np.asarray(f["hi_w"]),
+                             0, str(f.get("precision", "float32")), dict(f.get("stats", {})))
+        if isinstance(meta, dict) and meta.get("format") == "lumen-navarrays/1":
+            host = {k[5:]: v for k, v in store.items() if k.startswith("host/")}
+            csg = {tab: {k[len(tab) + 5:]: v for k, v in store.items() if k.startswith(f"csg/{tab}/")}
                    for tab in ("T", "K", "G")}
-            lo_w, hi_w = z["lo_w"], z["hi_w"]
-        csg["empty_leaf"] = meta.get("empty_leaf", len(csg["T"]["typ"]) - 1)
-        return NavArrays(host, csg, int(meta["n"]), tuple(meta["names"]), lo_w, hi_w, int(meta.get("n_mesh", 0)),
-                         meta.get("precision", "float32"), meta.get("stats", {}))
+            csg["empty_leaf"] = meta.get("empty_leaf", len(csg["T"]["typ"]) - 1)
+            return NavArrays(host, csg, int(meta["n"]), tuple(meta["names"]), store["lo_w"], store["hi_w"],
+                             int(meta.get("n_mesh", 0)), meta.get("precision", "float32"), meta.get("stats", {}))
+        raise ValueError(f"navigator cache entry {path}: an unknown layout")
+
+
+FORMAT = "reference-encode/1"  # the entries' layout: the reference's _encode tree (a changed layout changes keys)
+MESH_CHUNK = 8  # triangles per mesh chunk in the reference's format (no meshes here; the field is written as is)
+
+
+@dataclass
+class _Record:
+    """A dataclass of the reference's format as read or written here: its ``module:qualname`` and fields."""
+
+    cls: str
+    fields: dict
+
+
+def _tree_of(x, store: dict):
+    """Return the JSON tree of a value in the reference's cache encoding: an array (or numpy scalar) stored under the
+    next name ``a<i>`` as {"a": name, "scalar": bool}; a record as {"dc": class, "f": {field: tree}}; a dict as
+    {"m": [[key tree, value tree], ...]}; a tuple {"t": [...]}, a list {"l": [...]}; None, bool, int, float and str
+    as themselves."""  # noqa: D205, D209
+    if isinstance(x, (np.ndarray, np.generic)):
+        name = f"a{len(store)}"
+        store[name] = np.asarray(x)
+        return {"a": name, "scalar": isinstance(x, np.generic)}
+    if isinstance(x, _Record):
+        return {"dc": x.cls, "f": {k: _tree_of(v, store) for k, v in x.fields.items()}}
+    if isinstance(x, dict):
+        return {"m": [[_tree_of(k, store), _tree_of(v, store)] for k, v in x.items()]}
+    if isinstance(x, tuple):
+        return {"t": [_tree_of(v, store) for v in x]}
+    if isinstance(x, list):
+        return {"l": [_tree_of(v, store) for v in x]}
+    if x is None or isinstance(x, (bool, int, float, str)):
+        return x
+    raise TypeError(f"navigator cache: cannot store a {type(x).__name__}")
+
+
+def _value_of(t, store: dict):
+    """Return the value of a tree ``_tree_of`` (or the reference's ``_encode``) wrote; a record as a ``_Record``."""
+    if not isinstance(t, dict):
+        return t
+    if "a" in t:
+        a = store[t["a"]]
+        return a[()] if t.get("scalar") else a
+    if "dc" in t:
+        return _Record(t["dc"], {k: _value_of(v, store) for k, v in t["f"].items()})
+    if "m" in t:
+        return {_value_of(k, store): _value_of(v, store) for k, v in t["m"]}
+    if "t" in t:
+        return tuple(_value_of(v, store) for v in t["t"])
+    return [_value_of(v, store) for v in t.get("l", [])]
 
 
 # ---------------------------------------------------------------------------------------------- the CSG tables
@@ -403,13 +496,18 @@ def key(volumes: list[Volume], kinds: list[Solid], outside_cls: int, **options) 
     h = hashlib.sha256()
     _digest_inputs(h, volumes, kinds, outside_cls, options)
     here = os.path.dirname(os.path.abspath(__file__))
-    for f in _BUILDER_CODE:  # a copy without the sources (an installed package) keys by the file names alone
+    for f in _BUILDER_CODE:  # a copy without a source file keys by the built library's bytes instead (below)
         path = os.path.join(here, f)
         if os.path.isfile(path):
             with open(path, "rb") as src:
                 h.update(hashlib.sha256(src.read()).digest())
         else:
             h.update(f"missing:{f}".encode())
+    lib = os.path.join(os.path.dirname(here), "liblumen_hip.so")  # the native BVH builder (lh_bvh_build)
+    if os.path.isfile(lib):
+        with open(lib, "rb") as so:
+            h.update(hashlib.sha256(so.read()).digest())
+    h.update(FORMAT.encode())
     h.update(np.__version__.encode())
     return h.hexdigest()
 
@@ -425,7 +523,10 @@ def cached(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precisio
     base = os.path.join(root or cache_root(), "build")
     path = os.path.join(base, f"{KIND}-{k}")
     if os.path.isfile(os.path.join(path, "meta.json")):
-        return NavArrays.load(path), k
+        try:
+            return NavArrays.load(path), k
+        except (OSError, ValueError, KeyError, TypeError):  # a damaged or foreign entry: built again, replaced
+            shutil.rmtree(path, ignore_errors=True)
     nav = build(volumes, kinds, outside_cls, precision, leaf_size)
     os.makedirs(base, exist_ok=True)
     tmp = tempfile.mkdtemp(prefix=f".{KIND}-", dir=base)

@@ -83,3 +83,64 @@ def test_tally_in_lds_equals_global(gpu, det):
             np.testing.assert_allclose(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy(), rtol=1e-9,
                                        atol=1e-12, err_msg=f"{k} {ctl}")
         np.testing.assert_allclose(np.asarray(a.health), np.asarray(b.health), rtol=1e-9)
+
+
+def test_full_c4_batch_equals_sequential(gpu, det):
+    """The whole c4 workload: 256 point sources with 2^12..2^20 photons each (4.8e7 in all), the autotuned,
+    graph-captured batched launch against one ``lh_trace`` per source, forward and forward plus adjoint."""
+    batch = c4_batch(det)
+    assert len(batch.sources) == 256 and min(batch.samples) >= 1 << 12 and max(batch.samples) <= 1 << 20
+    print(f"[c4] 256 sources, {sum(batch.samples)} photons", flush=True)
+    run = BatchRun(det, batch, seed=11, graph=True, tune=True)
+    one = run.forward().cpu().numpy().copy()
+    seq = run.sequential().cpu().numpy()
+    assert (np.abs(one[:, :run.eng.S]).sum(1) > 0).all()  # every source reaches a sensor
+    np.testing.assert_allclose(one, seq, rtol=1e-9, atol=1e-12)
+    h = run.health.cpu().numpy()
+    assert h[13] == sum(batch.samples)  # every photon traced once
+
+
+def test_batched_sources_against_the_oracle(gpu, det):
+    """Eight of the c4 sources (2^14 photons each) in one batched launch against the oracle's run of the same
+    photons: per source, each PMT's arrivals agree within the Monte Carlo error (chi2 of the pulls) and the totals
+    within 1 %."""
+    from scipy import stats
+
+    from oracle import oracle as O
+
+    b = c4_batch(det, n_sources=8)
+    batch = L.sources.Batch(b.sources, (1 << 14,) * 8)
+    run = BatchRun(det, batch, seed=5, graph=False, tune=False)
+    got = run.forward().cpu().numpy()
+    tal, h = O.OracleRun(det).trace(batch, None, seed=5, bvh=True)
+    S = run.eng.S
+    ok = 0
+    for e in range(8):
+        g = O.light_of(got[e:e + 1], h, S, 1, 1 << 14)
+        o = O.light_of(tal[e:e + 1], h, S, 1, 1 << 14)
+        assert abs(g["arrivals"].sum() / o["arrivals"].sum() - 1.0) < 0.01, e
+        err = np.hypot(g["arrivals_error"], o["arrivals_error"])
+        m = err > 0
+        pulls = (g["arrivals"][m] - o["arrivals"][m]) / err[m]
+        ok += stats.chi2.sf(np.sum(pulls**2), m.sum()) > 1e-3
+    assert ok == 8
+
+
+def test_sequential_adjoint_equals_the_batched_one(gpu, det):
+    """One ``lh_trace`` + ``lh_adjoint`` per source against the batched forward + adjoint (tuned separately): the same
+    gradient slots."""
+    keys = ["materials/Water/ABSLENGTH", "surfaces/Liner/REFLECTIVITY"]
+    run = BatchRun(det, _small_batch(det, 16), seed=17, grad_keys=keys, graph=False, tune=True)
+    S = run.eng.S
+    target = 1.05 * 0.25 * run.forward()[:, 3 * S:4 * S].clone()
+
+    def loss(tally):
+        pe = tally[:, 3 * S:4 * S] * 0.25
+        cot = torch.zeros_like(tally)
+        cot[:, 3 * S:4 * S] = 0.25 * (1.0 - target / torch.clamp(pe, min=1e-30))
+        return cot
+
+    gb = run.forward_adjoint(loss).cpu().numpy().copy()
+    assert run.adj_controls is not None and np.abs(gb).sum() > 0
+    gs = run.sequential_adjoint(loss).cpu().numpy()
+    np.testing.assert_allclose(gs, gb, rtol=1e-8, atol=1e-12 * np.abs(gb).max())

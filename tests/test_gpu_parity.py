@@ -80,7 +80,9 @@ def test_water_tank_pulls_against_the_oracle(gpu, tank2):
     pulls = []
     n = 1 << 20
     for k in range(4):
-        g = _np(L.simulate(tank2, src, samples=n, seed=1000 + k))
+        light = L.simulate(tank2, src, samples=n, seed=1000 + k)
+        assert light.health.moment_spills == 0  # no photon deposits at more sensors than its moment slots hold
+        g = _np(light)
         tal, h = orc.trace(src, n, seed=2000 + k, bvh=True)  # the BVH walk: brute force's result, 30x faster
         o = O.light_of(tal, h, 1000, 1, n)
         pulls.append(_pulls(g["arrivals"], g["arrivals_error"], o["arrivals"], o["arrivals_error"]))
@@ -403,6 +405,28 @@ def test_enqueued_runs_keep_their_own_inputs(gpu, tank2):
     assert float(runs[0].arrivals.sum()) > float(runs[3].arrivals.sum())  # the shorter absorption length
 
 
+def test_runs_on_two_streams_keep_their_own_inputs(gpu, tank2):
+    """``simulate(stream=...)`` on two side streams (default priority) back to back, with no wait between: the
+    launches share the model's staged sources and work queue, so each waits for the previous one's end; each run
+    equals the same run made alone on the current stream."""
+    import torch
+
+    src, p0 = _cherenkov(), tank2.params()
+    p1 = p0.scale("materials/Water/ABSLENGTH", 0.5)
+    n = 1 << 19
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    a = L.simulate(tank2, src, p0, samples=n, seed=31, stream=s1)
+    b = L.simulate(tank2, SRC1, p1, samples=n // 2, seed=32, stream=s2)
+    c = L.simulate(tank2, src, p1, samples=n, seed=33, stream=s1)
+    got = [x.arrivals.cpu().numpy() for x in (a, b, c)]  # read on the current stream, which waits for each
+    torch.cuda.synchronize()
+    alone = [L.simulate(tank2, src, p0, samples=n, seed=31), L.simulate(tank2, SRC1, p1, samples=n // 2, seed=32),
+             L.simulate(tank2, src, p1, samples=n, seed=33)]
+    for x, y, z in zip(got, alone, (a, b, c)):
+        np.testing.assert_allclose(x, y.arrivals.cpu().numpy(), rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(np.asarray(z.health), np.asarray(y.health), rtol=1e-6, atol=1e-6)
+
+
 def test_navigator_on_tori(gpu, tmp_path):
     """Torus leaves (a ring, a hollow rotated ring cut in phi, a fat ring) walked by the torus kernels: distances
     along random rays, some aimed through the rings' centres, against the FP64 referee within 1e-5 relative."""
@@ -421,15 +445,23 @@ def test_navigator_on_tori(gpu, tmp_path):
     orc = O.OracleRun(det)
     vol = orc.locate(x32)
     assert len(np.unique(vol)) == 4  # the three tori and the world
-    t64, va64, _, _ = orc.query(x32, d32, vol)
+    t64, va64, n64, _ = orc.query(x32, d32, vol)
     t32, va32, _, _, loc = engine(det).query(x32, d32, vol)
-    assert engine(det).last_launch()["features"] & 16 == 0 or True  # the query kernel always carries tori
     assert np.mean(loc == vol) > 0.999
+    # a torus root met at a grazing angle is conditioned to sqrt(eps) of the tube (the quartic's double root), so the
+    # 1e-5 bound is asserted on every ray meeting its face at |cos| >= 0.05, and the grazing rest is held to the
+    # float32 root's own bound, 4 sqrt(eps32 t r) for a tube of radius r <= 0.3 m
+    cos = np.abs((d32 * n64).sum(1))
+    steep = cos >= 0.05
+    print(f"[tori] {steep.mean():.4f} of the rays at |cos| >= 0.05", flush=True)
     same = va32 == va64
-    assert same.mean() >= 0.995, same.mean()
-    err = np.abs(t32[same] - t64[same])
-    good = err <= 1e-5 * t64[same] + 2e-6
-    assert good.mean() >= 0.995 and np.quantile(err, 0.999) < 1e-3, (err.max(), good.mean())
+    assert same[steep].all(), np.flatnonzero(~same & steep)[:8]
+    err = np.abs(t32 - t64)
+    bound = 1e-5 * t64 + 2e-6
+    assert (err[steep] <= bound[steep]).all(), (err[steep] / np.maximum(t64[steep], 1e-9)).max()
+    graze = ~steep & same
+    assert (err[graze] <= 4.0 * np.sqrt(1.19e-7 * t64[graze] * 0.3) + bound[graze]).all()
+    assert same[~steep].mean() >= 0.9
 
 
 def test_trace_with_tori_against_oracle(gpu, tmp_path):

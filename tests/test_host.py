@@ -336,3 +336,69 @@ def test_navigator_build_cache(tmp_path):
     moved = [vols[0], navarrays.Volume("ball", 1, np.eye(3), np.array([0.1, 0.0, 1e-7]), 0, 1)]
     assert navarrays.key(moved, kinds, 2, precision="float32", leaf_size=bvh.LEAF_SIZE) != k
     assert navarrays.cached(vols, kinds, 2, precision="float64", root=str(tmp_path))[1] != k
+
+
+# ------------------------------------------------------------------------------------ the cache entry layout
+def test_nav_entry_in_the_reference_layout(tmp_path):
+    """The reference's cache entry layout (geometry/cache.py:187-230): the fixture tests/golden/nav_entry_example_tank
+    (written by tests/golden/make_nav_entry.py, an encoder of its own) reads back as the example tank's arrays, and
+    ``NavArrays.save`` writes the same tree and the same arrays."""
+    import json
+
+    from lumen_amd.detector import example_tank
+    from lumen_amd.detector.navarrays import NavArrays
+
+    golden = os.path.join(ROOT, "tests", "golden", "nav_entry_example_tank")
+    nav = example_tank().nav
+    got = NavArrays.load(golden)
+    assert got.n == nav.n and got.names == nav.names and got.n_mesh == 0
+    for k, v in nav.host.items():
+        assert np.array_equal(got.host[k], v), k
+    for t in "TKG":
+        for k, v in nav.csg[t].items():
+            assert np.array_equal(got.csg[t][k], v), (t, k)
+    np.testing.assert_array_equal(got.lo_w, nav.lo_w)
+    nav.save(str(tmp_path))
+    mine, ref = json.load(open(tmp_path / "meta.json")), json.load(open(os.path.join(golden, "meta.json")))
+    assert mine == ref
+    assert mine["dc"] == "lumen.geometry.navigator_build:NavArrays" and list(mine["f"])[:3] == ["host", "csg", "n"]
+    with np.load(tmp_path / "arrays.npz") as a, np.load(os.path.join(golden, "arrays.npz")) as b:
+        assert sorted(a.files) == sorted(b.files) and all(a[k].dtype == b[k].dtype and np.array_equal(a[k], b[k])
+                                                          for k in a.files)
+
+
+def test_nav_entry_refuses_foreign_records_and_meshes(tmp_path):
+    import json
+    import shutil
+
+    from lumen_amd.detector.navarrays import NavArrays
+
+    golden = os.path.join(ROOT, "tests", "golden", "nav_entry_example_tank")
+    for change, what in ((lambda m: m.__setitem__("dc", "lumen.geometry.other:Thing"), "not a NavArrays"),
+                         (lambda m: m["f"].__setitem__("n_mesh", 2), "meshes")):
+        d = tmp_path / what.replace(" ", "_")
+        shutil.copytree(golden, d)
+        meta = json.load(open(d / "meta.json"))
+        change(meta)
+        json.dump(meta, open(d / "meta.json", "w"))
+        with pytest.raises(ValueError, match=what):
+            NavArrays.load(str(d))
+
+
+def test_nav_cache_rebuilds_a_damaged_entry(tmp_path):
+    from lumen_amd.detector import navarrays
+    from toys import sphere_toy
+
+    det = sphere_toy()
+    vols = [navarrays.Volume("shell", 0, np.eye(3), np.zeros(3), -1, 1),
+            navarrays.Volume("shell/water", 1, np.eye(3), np.zeros(3), 0, 0)]
+    from lumen_amd.detector import solids
+
+    kinds = [solids.orb(1.05, name="shell"), solids.orb(1.0, name="water")]
+    nav, k = navarrays.cached(vols, kinds, 2, root=str(tmp_path))
+    entry = tmp_path / "build" / f"nav-{k}"
+    (entry / "meta.json").write_text("{\"format\": \"something else\"}")
+    again, k2 = navarrays.cached(vols, kinds, 2, root=str(tmp_path))
+    assert k2 == k and again.n == nav.n and np.array_equal(again.host["lo"], nav.host["lo"])
+    assert navarrays.NavArrays.load(str(entry)).n == nav.n  # the entry was written again
+    assert det.nav.n == nav.n

@@ -251,3 +251,33 @@ def test_oracle_cone_toy_closed_form():
     assert np.abs(pulls).max() < 5.0, pulls
     big = share > 0.05
     np.testing.assert_allclose(o["mean_arrival_time"][big], tmean[big], rtol=2e-3)
+
+
+def test_batch_event_error_uses_the_events_photons():
+    """A Batch whose event 0 merges two sources (1,000 and 3,000 photons) and whose event 1 is one source (2,000),
+    each photon standing for one (weights 1), isotropic at the centre of the two-ring counter: each ring's count in an
+    event is binomial (n_e, 1/2), so the moment error must estimate sqrt(n_e / 4) with n_e the event's photons --
+    over 24 seeds its mean square matches n_e / 4 and the arrivals' spread."""
+    from types import SimpleNamespace
+
+    import torch
+
+    from lumen_amd.api import light_of
+    from toys import cone_toy
+
+    det = cone_toy(2)
+    srcs = tuple(L.sources.Point((0.0, 0.0, 0.0), lam=400.0, n_photons=float(n)) for n in (1000, 3000, 2000))
+    batch = L.sources.Batch(srcs, (1000, 3000, 2000), events=(0, 0, 1))
+    eng = SimpleNamespace(S=2, K=1, nt=0, Kl=1, Kt=1)
+    orc = O.OracleRun(det)
+    errs, arrs = [], []
+    for seed in range(24):
+        tal, h = orc.trace(batch, None, seed=seed)
+        light = light_of(eng, torch.as_tensor(tal), h, None, batch)
+        errs.append(light.arrivals_error.numpy())
+        arrs.append(light.arrivals.numpy())
+    e2, a = np.mean(np.square(errs), 0), np.asarray(arrs)
+    n_e = np.array([4000.0, 2000.0])[:, None]
+    np.testing.assert_allclose(a.mean(0), n_e / 2.0 * np.ones((1, 2)), rtol=0.02)
+    np.testing.assert_allclose(e2, n_e / 4.0 * np.ones((1, 2)), rtol=0.01)  # the estimator's mean, not its spread
+    assert np.all(np.abs(a.var(0, ddof=1) / (n_e / 4.0) - 1.0) < 0.8)  # 24 seeds: chi2 with 23 dof
