@@ -42,8 +42,9 @@ def work_model(det, src, n=1 << 14):
                 fp_ops_per_vertex=fp / max(vert, 1))
 
 
-def pmc_traffic():
-    """Return HBM bytes per launch of the photon-loop kernel from the committed rocprofv3 counter summary, or None."""
+def pmc_traffic(key="hbm_bytes_per_launch"):
+    """Return a figure of the committed rocprofv3 counter summary of the photon-loop kernel (HBM bytes per launch by
+    default; ``valu_issue_frac``, ``valu_busy``, ``valu_active_lanes``), or None."""
     if not os.path.exists(PMC_SUMMARY):
         return None
     vals = {}
@@ -55,35 +56,51 @@ def pmc_traffic():
                     vals[k.strip()] = float(v.split()[0])
                 except ValueError:
                     pass
-    if "hbm_bytes_per_launch" in vals:
-        return vals["hbm_bytes_per_launch"]
-    return None
+    return vals.get(key)
 
 
-def cpu_baseline(det, src, seconds=15.0):
-    """Time the oracle (FP64, OpenMP on the host cores this process may use) on a bounded sample of the workload:
-    a warm-up, a pilot that sizes the sample to about ``seconds``, then the timed sample."""
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's, from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(det, src, seconds=10.0):
+    """Time the oracle (OpenMP on the host cores this process may use) on a bounded sample of the workload, in FP64
+    and with the photon's arithmetic in FP32 (the reference's CPU path traces in FP32): per precision a warm-up, a
+    pilot that sizes the sample to about ``seconds``, then the timed sample.  ``value`` is the FP32 figure."""  # noqa: D205, D209
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
     run = O.OracleRun(det)
-    run.trace(src, 1 << 10, seed=98, threads=threads, bvh=True)
-    n = 1 << 14
-    while True:
+    out = {}
+    for fp32 in (True, False):
+        run.trace(src, 1 << 10, seed=98, threads=threads, bvh=True, fp32=fp32)
+        n = 1 << 14
+        while True:
+            t0 = time.perf_counter()
+            run.trace(src, n, seed=99, threads=threads, bvh=True, fp32=fp32)
+            dt = time.perf_counter() - t0
+            if dt > 1.0 or n >= 1 << 22:
+                break
+            n *= 4
+        n = int(min(1 << 24, max(n, n * seconds / max(dt, 1e-3))))
         t0 = time.perf_counter()
-        run.trace(src, n, seed=99, threads=threads, bvh=True)
+        run.trace(src, n, seed=100, threads=threads, bvh=True, fp32=fp32)
         dt = time.perf_counter() - t0
-        if dt > 1.0 or n >= 1 << 22:
-            break
-        n *= 4
-    n = int(min(1 << 24, max(n, n * seconds / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    run.trace(src, n, seed=100, threads=threads, bvh=True)
-    dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="photons/s", cores=threads, kind="port",
-                sample=f"{n} photons of the same workload (1,000-PMT tank, 1 m Cherenkov track), FP64 oracle with the "
-                       "reference's BVH walk, "
-                       f"{dt:.1f} s on {threads} threads")
+        out["fp32" if fp32 else "fp64"] = (n, dt)
+    (n, dt), (n64, dt64) = out["fp32"], out["fp64"]
+    return dict(value=n / dt, unit="photons/s", cores=threads, kind="port", cpu=cpu_model(),
+                value_fp64=n64 / dt64,
+                sample=f"{n} photons of the same workload (1,000-PMT tank, 1 m Cherenkov track) in {dt:.1f} s on "
+                       f"{threads} threads: the oracle's loop with the photon's arithmetic in FP32 and the reference's "
+                       f"BVH walk (FP64 tallies); FP64 beside it: {n64} photons in {dt64:.1f} s")
 
 
 def copy_calibration(nbytes=1 << 30, repeats=3):
@@ -175,7 +192,9 @@ def main():
         "kernel_ms": k_ms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "fp_ops_per_photon": fp_per_photon, "kernel": "lh::trace_kernel"},
+                     "fp_ops_per_photon": fp_per_photon, "kernel": "lh::trace_kernel",
+                     "valu_issue_frac": pmc_traffic("valu_issue_frac"), "valu_busy": pmc_traffic("valu_busy"),
+                     "valu_active_lanes": pmc_traffic("valu_active_lanes")},
     }
     log(f"forward: {k_ms:.2f} ms per launch")
     if not args.no_adjoint:

@@ -116,6 +116,10 @@ typedef struct {
   int32_t tally_global;  /* 1: accumulate the tallies with global atomics even where they fit in LDS; else 0 */
   int32_t nodes_global;  /* 1: read the BVH nodes from global memory even where they fit in LDS; else 0 */
   int32_t jump;          /* 1: the diffusion jump where the model holds a clearance grid (lh_model_jump); else 0 */
+  int32_t keep_zero_weight; /* 1: trace a photon on after its live weight became exactly 0 (an edge row's floor
+                             * drew an outcome the live values give no probability), as the reference does; 0: end
+                             * it there.  It deposits nothing more either way: only the records and vertices
+                             * diagnostics differ */
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.

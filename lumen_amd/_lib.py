@@ -52,7 +52,7 @@ class LhSource(C.Structure):
 class LhControls(C.Structure):
     _fields_ = [("roulette", C.c_float), ("max_distance", C.c_float), ("max_vertices", C.c_int32),
                 ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32), ("tally_global", C.c_int32),
-                ("nodes_global", C.c_int32), ("reserved", C.c_int32)]
+                ("nodes_global", C.c_int32), ("jump", C.c_int32), ("keep_zero_weight", C.c_int32)]
 
 
 _LIB = None

@@ -33,7 +33,10 @@ class Controls(NamedTuple):
     ``tally_global`` (accumulate the tallies with global atomics even where a block's copy fits in LDS),
     ``nodes_global`` (read the BVH nodes from global memory even where they fit in the block's LDS), ``jump`` (the
     diffusion jump in Rayleigh-only media, ``transport/fastpath.py``: the reference's ``StepConfig(jump=True)``; its
-    clearance grids are built on first use, ``lumen_amd.detector.clearance``)."""  # noqa: D205, D209
+    clearance grids are built on first use, ``lumen_amd.detector.clearance``), ``keep_zero_weight`` (trace a photon
+    on after its live weight became exactly 0, as the reference does -- an edge row's defensive floor drew an outcome
+    the live values give no probability; such a photon deposits nothing more, so only the ``records`` and
+    ``vertices`` diagnostics change; off: it ends there)."""  # noqa: D205, D209
 
     max_distance: Any = None
     roulette: float = ROULETTE
@@ -43,13 +46,15 @@ class Controls(NamedTuple):
     tally_global: bool = False
     nodes_global: bool = False
     jump: bool = False
+    keep_zero_weight: bool = False
 
     def struct(self) -> _lib.LhControls:
         if not (float(self.roulette) > 0.0):
             raise ValueError(f"Controls: the roulette is a number > 0, got {self.roulette!r}")
         return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
                                int(self.max_vertices), int(self.chunk), int(self.blocks_per_cu),
-                               int(bool(self.tally_global)), int(bool(self.nodes_global)), int(bool(self.jump)))
+                               int(bool(self.tally_global)), int(bool(self.nodes_global)), int(bool(self.jump)),
+                               int(bool(self.keep_zero_weight)))
 
 
 class Health(NamedTuple):

@@ -84,6 +84,7 @@ struct Args {
   uint32_t k0, k1;
   float roulette, max_dist, reach;
   int max_vert, chunk;
+  int keep_zero;  // 1: a photon whose live weight became 0 walks on (the reference's records); 0: it ends
   double *__restrict__ tally;
   long long tally_stride;  // doubles per event
   int S, K;  // sensors; charge cells per sensor, Kl wavelength (QE) knots x Kt theta (CE) knots
@@ -817,6 +818,9 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       }
     }
   }
+  // a photon left without live weight (an edge row's floor drew an outcome of live probability 0) deposits nothing
+  // more and adds 0 to every fate: it ends here unless the caller keeps it for the records and vertices counts
+  if (alive && w_new == 0.0f && !a.keep_zero) alive = false;
   p.nz = (hit && !scatter && dist <= ZERO_STEP_BANDS * b0) ? p.nz + 1 : 0;
   p.w = w_new;
   p.wr = wr_new;
@@ -1600,7 +1604,7 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.has_ref = (ref && ref != values) ? 1 : 0;
   a.k0 = (uint32_t)seed;
   a.k1 = (uint32_t)(seed >> 32);
-  lh_controls c{0.01f, 0.0f, 1 << 22, 0, 0, 0, 0, 0};
+  lh_controls c{0.01f, 0.0f, 1 << 22, 0, 0, 0, 0, 0, 0};
   if (ctl) c = *ctl;
   if (c.blocks_per_cu < 0 || c.blocks_per_cu > 64) return fail(LH_ERR_ARG, "blocks_per_cu out of range (0 .. 64)");
   if (!(c.roulette > 0.0f)) return fail(LH_ERR_ARG, "the roulette is a number > 0");
@@ -1609,6 +1613,7 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.max_dist = c.max_distance > 0.0f ? c.max_distance : m->g.reach;
   a.max_vert = c.max_vertices > 0 ? c.max_vertices : (1 << 22);
   a.chunk = c.chunk > 0 ? c.chunk : DEFAULT_CHUNK;
+  a.keep_zero = c.keep_zero_weight ? 1 : 0;
   a.tally = tally;
   a.tally_stride = m->tally_stride;
   a.nt = m->NT;

@@ -54,19 +54,31 @@ struct Geo {
   int lds_node_off;
 };
 
-// a BVH node's float4 i (2 per node): from LDS where the kernel staged them (LDS: a compile-time choice, so each
-// walk compiles to LDS reads or to global reads, never to a generic load that decides per access)
+// a BVH node's two float4s, (lo, skip) and (hi, start * 8 + count): from LDS where the kernel staged them (LDS: a
+// compile-time choice, so each walk compiles to LDS reads or to global reads, never to a generic load that decides
+// per access).  In LDS the two halves are two arrays (every lo, then every hi): a node's halves 16 bytes apart in
+// memory would put the lanes' reads of different nodes on 4 bank groups of the 32 banks, 16-byte strides on 8
 template <bool LDS>
-__device__ __forceinline__ float4 nav_node(const Geo &g, int i) {
-  if (LDS) return reinterpret_cast<const float4 *>(lds_cells)[g.lds_node_off + i];
-  return g.node[i];
+__device__ __forceinline__ void nav_node(const Geo &g, int ni, float4 &lo, float4 &hi) {
+  if (LDS) {
+    const float4 *s = reinterpret_cast<const float4 *>(lds_cells) + g.lds_node_off;
+    lo = s[ni];
+    hi = s[g.n_nodes + ni];
+  } else {
+    lo = g.node[2 * ni];
+    hi = g.node[2 * ni + 1];
+  }
 }
 
-// copy the BVH nodes into the block's LDS (every thread of the block calls it; a barrier follows)
+// copy the BVH nodes into the block's LDS, the lo halves then the hi halves (every thread of the block calls it; a
+// barrier follows)
 __device__ inline void stage_nodes(const Geo &g) {
   if (!g.lds_nodes) return;
   float4 *dst = reinterpret_cast<float4 *>(lds_cells) + g.lds_node_off;
-  for (int i = threadIdx.x; i < 2 * g.n_nodes; i += blockDim.x) dst[i] = g.node[i];
+  for (int i = threadIdx.x; i < 2 * g.n_nodes; i += blockDim.x) {
+    const int ni = i >> 1;
+    dst[(i & 1) ? g.n_nodes + ni : ni] = g.node[i];
+  }
 }
 
 __device__ __forceinline__ float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }
@@ -583,7 +595,8 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
     for (;;) {
       int npend = 0, pl0 = 0, pl1 = 0, pl2 = 0;  // the leaves found (start * 8 + count), the latest in pl0
       while (ni < nr.y && npend < WALK_PENDING) {
-        const float4 lo = nav_node<LDS>(g, 2 * ni), hi = nav_node<LDS>(g, 2 * ni + 1);
+        float4 lo, hi;
+        nav_node<LDS>(g, ni, lo, hi);
         const float tx0 = fmaf(lo.x, inv.x, -xi.x), tx1 = fmaf(hi.x, inv.x, -xi.x);
         const float ty0 = fmaf(lo.y, inv.y, -xi.y), ty1 = fmaf(hi.y, inv.y, -xi.y);
         const float tz0 = fmaf(lo.z, inv.z, -xi.z), tz1 = fmaf(hi.z, inv.z, -xi.z);
@@ -634,11 +647,29 @@ __device__ inline Cross query(const Geo &g, float3 x, float3 d, int vol, int las
   return query_t<false, TOR>(g, x, d, vol, last, tmax);
 }
 
+// whether a point (the volume's frame) lies in the volume's daughter-free core: then no daughter holds it.  The core
+// is the solid shrunk by its daughters' depth plus a margin, so a rounding at its boundary cannot hide a daughter
+__device__ inline bool core_holds(const Geo &g, int vol, float3 xl) {
+  const float4 C1 = g.vcore[3 * vol + 1];
+  const int code = __float_as_int(C1.w);
+  if (code == 0) return false;
+  const float4 C0 = g.vcore[3 * vol], C2 = g.vcore[3 * vol + 2];
+  if (code == 2) return leaf_inside<false>(g, __float_as_int(C2.w), xl);  // a core leaf is never a torus
+  const float3 y = sub3(xl, mk3(C0.x, C0.y, C0.z));
+  if (y.x < C1.x || y.x > C2.x || y.y < C1.y || y.y > C2.y || y.z < C1.z || y.z > C2.z) return false;
+  return !(C0.w < HUGEF && y.x * y.x + y.y * y.y > C0.w);
+}
+
 // the deepest navigator id holding x (the outside when no volume does)
 template <bool TOR>
 __device__ inline int locate(const Geo &g, float3 x) {
   int cur = g.nvol;
   for (int depth = 0; depth < 64; ++depth) {
+    if (cur < g.nvol) {  // inside the current volume's daughter-free core: no daughter to look for
+      float3 xl, dl;
+      to_local(g, cur, x, mk3(0.f, 0.f, 1.f), xl, dl);
+      if (core_holds(g, cur, xl)) return cur;
+    }
     int found = -1;
     const int2 gr = g.vgrp[cur];
     for (int gi = gr.x; gi < gr.y && found < 0; ++gi) {

@@ -90,9 +90,11 @@ inline bool operator==(Counted a, Counted b) { ++g_ops; return a.v == b.v; }
 inline bool operator!=(Counted a, Counted b) { ++g_ops; return a.v != b.v; }
 
 inline double val(double x) { return x; }
+inline double val(float x) { return x; }
 inline double val(Counted x) { return x.v; }
 #define ORC_F1(name, f)                                  \
   inline double name(double x) { return f(x); }           \
+  inline float name(float x) { return f(x); }             \
   inline Counted name(Counted x) { ++g_ops; return Counted(f(x.v)); }
 ORC_F1(SQRT, std::sqrt)
 ORC_F1(CBRT, std::cbrt)
@@ -103,6 +105,7 @@ ORC_F1(COS, std::cos)
 ORC_F1(ACOS, std::acos)
 ORC_F1(FABS, std::fabs)
 inline double ATAN2(double y, double x) { return std::atan2(y, x); }
+inline float ATAN2(float y, float x) { return std::atan2(y, x); }
 inline Counted ATAN2(Counted y, Counted x) { ++g_ops; return Counted(std::atan2(y.v, x.v)); }
 template <class R> inline R MIN(R a, R b) { return b < a ? b : a; }
 template <class R> inline R MAX(R a, R b) { return a < b ? b : a; }
@@ -870,6 +873,7 @@ struct Run {
   long long stride;
   int nt;  // the time histogram's nodes t0 + j dt (0: none), its cells from time_at of each event's tally
   bool jump;  // the diffusion jump where a medium has a grid
+  bool keep_zero;  // photons without live weight walk on
   double t0, dt;
   long long time_at;
 };
@@ -1354,6 +1358,9 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
       alive = false;
     }
   }
+  // a photon left without live weight deposits nothing more: ended unless the controls keep it (the engine's rule;
+  // the reference walks it on, which changes only the records and vertices counts)
+  if (alive && val(w_new) == 0.0 && !r.keep_zero) alive = false;
   p.nz = (hit && !scatter && zero) ? p.nz + 1 : 0;
   p.w = w_new;
   p.wr = wr_new;
@@ -1404,6 +1411,7 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
   r.max_dist = (ctl && ctl->max_distance > 0) ? ctl->max_distance : s.reach;
   r.max_vert = (ctl && ctl->max_vertices > 0) ? ctl->max_vertices : (1 << 22);
   r.jump = ctl && ctl->jump;
+  r.keep_zero = ctl && ctl->keep_zero_weight;
   r.S = layout[4];
   r.Kl = layout[5];
   r.Kt = layout[14];
@@ -1422,9 +1430,29 @@ static int make_run(Run &r, const Scene &s, const int32_t *layout, const float *
 // Trace every photon of the sources in FP64 on `threads` OpenMP threads; add into tally (n_events * stride) and
 // health (LH_HEALTH_N), both host arrays.  `use_bvh`: daughters by the BVH walk (the reference's navigator, the CPU
 // baseline's mode); 0: by brute force over every daughter (the independent referee).
+template <class R>
+static int trace_all(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
+                     const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally, int n_events,
+                     double *health, int threads, int use_bvh);
+
 ORC_API int oracle_trace(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
                          const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally,
                          int n_events, double *health, int threads, int use_bvh) {
+  return trace_all<double>(G, layout, values, ref, src, nsrc, seed, ctl, tally, n_events, health, threads, use_bvh);
+}
+
+// The same loop with the photon's state and arithmetic in float32 (the reference's CPU path traces in FP32; tallies
+// stay FP64): the CPU baseline's FP32 figure.  Not a checker: its navigator is the FP64 referee's, in float32.
+ORC_API int oracle_trace_f32(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
+                             const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally,
+                             int n_events, double *health, int threads, int use_bvh) {
+  return trace_all<float>(G, layout, values, ref, src, nsrc, seed, ctl, tally, n_events, health, threads, use_bvh);
+}
+
+template <class R>
+static int trace_all(const lh_geometry *G, const int32_t *layout, const float *values, const float *ref,
+                     const lh_source *src, int nsrc, uint64_t seed, const lh_controls *ctl, double *tally, int n_events,
+                     double *health, int threads, int use_bvh) {
   Scene s = scene_of(G);
   s.bvh = use_bvh != 0;
   Run r;
@@ -1439,7 +1467,7 @@ ORC_API int oracle_trace(const lh_geometry *G, const int32_t *layout, const floa
     Tally &T = parts[omp_get_thread_num()];
     T.t.assign((size_t)n_events * r.stride, 0.0);
 #pragma omp for schedule(dynamic, 1024)
-    for (long long gi = 0; gi < total; ++gi) trace_range<double>(r, gi, gi + 1, T);
+    for (long long gi = 0; gi < total; ++gi) trace_range<R>(r, gi, gi + 1, T);
   }
   for (auto &T : parts) {
     for (size_t i = 0; i < T.t.size(); ++i) tally[i] += T.t[i];

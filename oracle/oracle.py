@@ -57,6 +57,7 @@ def lib():
         L.oracle_mie.restype = None
         L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
         L.oracle_mie_pdf.restype = None
+        L.oracle_trace_f32.argtypes = L.oracle_trace.argtypes
         L.oracle_emit.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.POINTER(E.LhSource), C.c_int,
                                   C.c_uint64, C.c_longlong, C.c_void_p]
         L.oracle_pol_recombine.argtypes = [C.c_void_p] * 6
@@ -82,9 +83,10 @@ class OracleRun:
         self.S = det.n_sensors
 
     def trace(self, source, samples=None, *, seed=0, params=None, reference=None, controls=None, threads=0,
-              first_id=0, edge_rows="structure", bvh=False):
+              first_id=0, edge_rows="structure", bvh=False, fp32=False):
         """Return (tally (n_events, stride) float64, health (16,)) of the oracle's run.  ``bvh``: find daughters by
-        the navigator arrays' BVH (the reference's walk; the CPU baseline), else by brute force (the referee)."""
+        the navigator arrays' BVH (the reference's walk; the CPU baseline), else by brute force (the referee).
+        ``fp32``: the photon's state and arithmetic in float32 (the CPU baseline's FP32 figure; not a checker)."""
         from lumen_amd import api, pack
 
         p = self.det.params() if params is None else params
@@ -111,7 +113,8 @@ class OracleRun:
         vals = np.ascontiguousarray(pk.values, np.float32)
         lay = np.ascontiguousarray(pk.layout, np.int32)
         refv = None if ref is None else np.ascontiguousarray(ref, np.float32)
-        rc = lib().oracle_trace(C.byref(self.geo), _p(lay), _p(vals), None if refv is None else _p(refv), rows,
+        fn = lib().oracle_trace_f32 if fp32 else lib().oracle_trace
+        rc = fn(C.byref(self.geo), _p(lay), _p(vals), None if refv is None else _p(refv), rows,
                                 len(rows), C.c_uint64(int(seed)), C.byref(ctl), _p(tally), n_ev, _p(health),
                                 int(threads), int(bool(bvh)))
         if rc != 0:

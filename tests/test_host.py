@@ -30,7 +30,7 @@ def test_abi_exports_every_declared_symbol():
 def test_abi_structs_match_the_header():
     """ctypes' structs have the header's sizes (lh_source: 4 ints, 14 floats, 2 int64)."""
     assert ctypes.sizeof(_lib.LhSource) == 4 * 4 + 14 * 4 + 2 * 8
-    assert ctypes.sizeof(_lib.LhControls) == 8 * 4
+    assert ctypes.sizeof(_lib.LhControls) == 9 * 4
     assert _lib.LhGeometry.skin.offset == _lib.LhGeometry.reach.offset + 8  # the float, padding, the pointer
 
 
