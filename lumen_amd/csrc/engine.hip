@@ -1179,7 +1179,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
         return fail(LH_ERR_LAYOUT, "lh_model_create: a surface row index is out of range");
     for (int r = 0; r < nr; ++r) {
       const int32_t *R = layout + layout[9] + ROW_WORDS * r;
-      if (R[0] < 0 || R[0] > 1) return fail(LH_ERR_LAYOUT, "lh_model_create: an unknown surface model");
+      if (R[0] < 0 || R[0] > 2) return fail(LH_ERR_LAYOUT, "lh_model_create: an unknown surface model");
       for (int c = 0; c < N_COL; ++c)
         if (R[2 + 2 * c] < 0 || R[3 + 2 * c] < 1) return fail(LH_ERR_LAYOUT, "lh_model_create: a surface column is empty");
     }

@@ -23,7 +23,7 @@ constexpr float DEFENSIVE = 0.01f;
 constexpr float WLS_TRIES = 100.0f;  // G4OpWLS redraws an emission energy above the absorbed one up to 100 times
 constexpr int N_OUT = 6;
 enum { STRAIGHT = 0, REFRACT = 1, SPECULAR = 2, BACKSCATTER = 3, GLOSSY = 4, DIFFUSE = 5 };
-enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1 };
+enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1, MODEL_COMPLEX = 2 };
 enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4 };
 enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, N_COL };
 // the packed layout's header words (lumen_amd/pack.py)
@@ -290,6 +290,25 @@ __device__ inline Amps fresnel(float cos_i, float n1, float n2) {
   return a;
 }
 
+// Fresnel amplitudes from real n1 into a complex index nr + i nk (an absorbing medium, nk >= 0) at incidence cos_i:
+// q = sqrt(n2^2 - n1^2 sin^2) with Im q >= 0, r_s = (n1 c - q) / (n1 c + q), r_p = (n2^2 c - n1 q) / (n2^2 c + n1 q)
+// (fresnel.py `amplitudes` with a complex n2); t_s, t_p likewise (unused: the photon never enters the medium)
+__device__ inline Amps fresnel_complex(float cos_i, float n1, float nr, float nk) {
+  const float c = clip01(cos_i);
+  const cf n2sq{nr * nr - nk * nk, 2.0f * nr * nk};
+  const cf q = decaying_sqrt(cf{n2sq.re - n1 * n1 * (1.0f - c * c), n2sq.im});
+  const cf ds{n1 * c + q.re, q.im};
+  const cf n2sqc{n2sq.re * c, n2sq.im * c};
+  const cf dp{n2sqc.re + n1 * q.re, n2sqc.im + n1 * q.im};
+  Amps a;
+  const bool oks = sqrtf(cabs2(ds)) > 1e-12f, okp = sqrtf(cabs2(dp)) > 1e-12f;
+  a.rs = oks ? cdiv(cf{n1 * c - q.re, -q.im}, ds) : cf{-1.f, 0.f};
+  a.rp = okp ? cdiv(cf{n2sqc.re - n1 * q.re, n2sqc.im - n1 * q.im}, dp) : cf{1.f, 0.f};
+  a.ts = oks ? cdiv(cf{2.0f * n1 * c, 0.f}, ds) : cf{0.f, 0.f};
+  a.tp = okp ? cdiv(cmul(cf{2.0f * n1 * c, 0.f}, cf{nr, nk}), dp) : cf{0.f, 0.f};
+  return a;
+}
+
 // the s axis unit(wi x n); at normal incidence a fixed perpendicular of wi (polar.py `s_axis`)
 __device__ __forceinline__ float3 s_axis(float3 wi, float3 n) {
   const float3 c = cross3(wi, n);
@@ -333,8 +352,8 @@ __device__ inline SurfEval surface_eval(const int *__restrict__ L, const float *
   s.flags = flags;
   const float E = HC / lam;
   const float Rv = col_at(R, P, COL_R, E), Tv = col_at(R, P, COL_T, E), eff = col_at(R, P, COL_EFF, E);
-  const float c1 = clip01(Rv), c2 = clip01(Rv + Tv);
-  const float r = c1, t = c2 - c1, a = 1.0f - c2;
+  float c1 = clip01(Rv), c2 = clip01(Rv + Tv);
+  float r = c1, t = c2 - c1, a = 1.0f - c2;
   s.sigma = col_at(R, P, COL_SIGMA, E);
   for (int k = 0; k < N_OUT; ++k) s.p[k] = 0.0f;
   s.amps = Amps{cf{-1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}};  // the perfect mirror's
@@ -349,14 +368,25 @@ __device__ inline SurfEval surface_eval(const int *__restrict__ L, const float *
     s.p[DIFFUSE] = r * lm;
   } else {
     const float cos_i = clip01(-dot3(wi, n));
-    s.amps = fresnel(cos_i, n1, n2);
+    const bool cx = model == MODEL_COMPLEX;  // a complex index (nr, nk in the p_lobe, p_back slots)
+    s.amps = cx ? fresnel_complex(cos_i, n1, col_at(R, P, COL_LOBE, E), col_at(R, P, COL_BACK, E))
+                : fresnel(cos_i, n1, n2);
     const float3 sx = s_axis(wi, n);
     const float3 px = cross3(wi, sx);
     const float es2 = dot3(er, sx) * dot3(er, sx) + dot3(ei, sx) * dot3(ei, sx);
     const float ep2 = dot3(er, px) * dot3(er, px) + dot3(ei, px) * dot3(ei, px);
     const float F = clip01(cabs2(s.amps.rs) * es2 + cabs2(s.amps.rp) * ep2);
-    s.p[SPECULAR] = r * F;
-    s.p[REFRACT] = r * (1.0f - F);
+    if (cx) {  // the reflectance is the cut (Geant4's CalculateReflectivity): the mirror's share, then T, the rest
+      c1 = F;  // absorbed and detected with eff (types.py `_complex` in `_geant4_chain`)
+      c2 = clip01(F + Tv);
+      r = c1;
+      t = c2 - c1;
+      a = 1.0f - c2;
+      s.p[SPECULAR] = r;
+    } else {
+      s.p[SPECULAR] = r * F;
+      s.p[REFRACT] = r * (1.0f - F);
+    }
   }
   s.p[STRAIGHT] = t;
   if (flags & FLAG_FAR_OPAQUE) s.p[STRAIGHT] = s.p[REFRACT] = 0.0f;

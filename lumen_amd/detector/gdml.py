@@ -213,23 +213,40 @@ def _parse(path: str) -> _Doc:
     return d
 
 
-def _surface_row(name, s: dict) -> SurfaceRow:
-    """Return the engine's surface row of a GDML optical surface (G4OpBoundaryProcess's types and finishes)."""
-    cols = {G4_COLUMN[k]: f"surfaces/{name}/{k}" for k in s["props"] if k in G4_COLUMN}
-    typ, fin = s["type"], s["finish"]
+def _surface_row(name, s: dict):
+    """Return (the engine's surface row of a GDML optical surface, the defaults it adds as parameters) by
+    G4OpBoundaryProcess's types and finishes (the reference's ``rules/optics.py:28-50`` table): a dielectric_metal
+    reads REFLECTIVITY at Geant4's default 1 when the surface omits it (a parameter at that value), a complex index
+    (REALRINDEX, IMAGINARYRINDEX) makes a photocathode, TRANSMITTANCE a transmitting row."""  # noqa: D205, D209
+    props = s["props"]
+    cols = {G4_COLUMN[k]: f"surfaces/{name}/{k}" for k in props if k in G4_COLUMN}
+    defaults = {}
+    typ, fin, model = s["type"], s["finish"], s["model"]
+    trans = "T" in cols
     if typ == "dielectric_metal":
-        if fin == "polished":
-            return SurfaceRow(name, "SPECULAR_METAL", cols)
-        if fin == "ground" and s["model"] == "unified":
+        if "R" not in cols and not ("nr" in cols and "nk" in cols):
+            cols["R"] = f"surfaces/{name}/REFLECTIVITY"
+            defaults[cols["R"]] = np.float64(1.0)
+        if fin == "polished" and "nr" in cols and "nk" in cols:
+            cols.pop("R", None)
+            return SurfaceRow(name, "PHOTOCATHODE", cols, transmitting=trans), defaults
+        if fin == "polished" or (model == "glisur" and float(s["value"]) >= 1.0):
+            return SurfaceRow(name, "SPECULAR_METAL", cols, transmitting=trans), defaults
+        if fin == "ground" and model == "unified":
+            if float(s["value"]) == 0.0:  # sigma_alpha 0: GetFacetNormal returns the normal
+                return SurfaceRow(name, "UNIFIED_METAL_NORMAL", cols, transmitting=trans), defaults
             cols["sigma_alpha"] = f"surfaces/{name}/sigma_alpha"
-            return SurfaceRow(name, "UNIFIED_METAL", cols)
+            return SurfaceRow(name, "UNIFIED_METAL", cols, transmitting=trans), defaults
     if typ == "dielectric_dielectric":
         if fin in ("groundfrontpainted", "polishedfrontpainted"):
+            if "R" not in cols:
+                cols["R"] = f"surfaces/{name}/REFLECTIVITY"
+                defaults[cols["R"]] = np.float64(1.0)
             cols["p_spike"] = 1.0 if fin == "polishedfrontpainted" else 0.0
-            return SurfaceRow(name, "PAINTED", cols)
+            return SurfaceRow(name, "PAINTED_T" if trans else "PAINTED", cols), defaults
         if fin == "polished":
-            return SurfaceRow(name, "DIELECTRIC_RT" if "R" in cols or "T" in cols else "DIELECTRIC", cols)
-    raise ValueError(f"gdml: optical surface {name!r} ({s['model']}, {typ}, {fin}) is not supported")
+            return SurfaceRow(name, "DIELECTRIC_RT" if "R" in cols or "T" in cols else "DIELECTRIC", cols), defaults
+    raise ValueError(f"gdml: optical surface {name!r} ({model}, {typ}, {fin}) is not supported")
 
 
 def _table(d: _Doc, ref: str) -> Table:
@@ -297,7 +314,9 @@ def load(path: str, recipe: Recipe) -> Detector:
     srows, srow_of = [], {}
     for sname, s in d.osurfaces.items():
         srow_of[sname] = len(srows)
-        srows.append(_surface_row(sname, s))
+        row, defaults = _surface_row(sname, s)
+        srows.append(row)
+        values.update(defaults)
         for prop, ref in s["props"].items():
             if prop in G4_COLUMN:
                 values[f"surfaces/{sname}/{prop}"] = _table(d, ref)

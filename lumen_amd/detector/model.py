@@ -4,16 +4,25 @@ This restates the structure of the reference's built detector on the path (``bui
 table, ``transport/step.py:280-295`` rows by class pair, ``media/bulk.py:76-110`` the media, ``surfaces/types.py``
 the surface types) for the types the engine implements:
 
-==================  ==========  ==========================================================================
-type                 model      reads (columns)
-==================  ==========  ==========================================================================
-ABSORBER             reflect    nothing: every photon absorbed, nothing detected
-SPECULAR_METAL       reflect    R (mirror), eff (the absorbed share detected)
-UNIFIED_METAL        reflect    R split by p_spike / p_lobe (facet path, sigma_alpha) / p_back / Lambertian, eff
-PAINTED              reflect    R split by f_spec (p_spike column: mirror) and Lambertian, eff
-DIELECTRIC           fresnel    the Fresnel interface of the two media (R fixed at 1)
-DIELECTRIC_RT        fresnel    R (into the interface), T (straight on), eff
-==================  ==========  ==========================================================================
+====================  =======  ========================================================================
+type                   model    reads (columns)
+====================  =======  ========================================================================
+ABSORBER               reflect  nothing: every photon absorbed, nothing detected
+SPECULAR_METAL         reflect  R (mirror), eff (the absorbed share detected)
+UNIFIED_METAL          reflect  R split by p_spike / p_lobe (facet path, sigma_alpha) / p_back / Lambertian, eff
+UNIFIED_METAL_NORMAL   reflect  as UNIFIED_METAL at sigma_alpha 0: the lobe reflects about the global normal
+PAINTED                reflect  R split by f_spec (p_spike column: mirror) and Lambertian, eff
+PAINTED_T              reflect  PAINTED with T (straight on): Geant4's frontpainted finish with TRANSMITTANCE
+COUNTING               reflect  eff: every photon absorbed, the share eff detected (a counting-only sensor)
+DIELECTRIC             fresnel  the Fresnel interface of the two media (R fixed at 1)
+DIELECTRIC_RT          fresnel  R (into the interface), T (straight on), eff
+PHOTOCATHODE           complex  nr, nk (a complex index seen from the dielectric: its Fresnel reflectance is the
+                                mirror's share, the absorbed rest detected with eff); nr, nk in the p_lobe and
+                                p_back column slots, which no other part of the type reads
+====================  =======  ========================================================================
+
+A row with ``transmitting`` set (Geant4's dielectric_metal with TRANSMITTANCE, the reference's
+``surfaces/types.py:497-512`` ``transmitting``) also reads T: the share past R up to R + T goes straight on.
 
 ``row_of[c_from, c_to]`` is the surface row of a crossing from class ``c_from`` into ``c_to`` (-1: no surface
 joins them; the photon is lost).  A volume's class decides its medium (``class_medium``).
@@ -30,7 +39,7 @@ from .navarrays import NavArrays
 
 # surface models (the kernel's branches) and the six continuing outcomes, as the reference's
 # ``surfaces/types.py:288-296`` and ``surfaces/outcomes.py:36-53``
-MODEL_REFLECT, MODEL_FRESNEL = 0, 1
+MODEL_REFLECT, MODEL_FRESNEL, MODEL_COMPLEX = 0, 1, 2
 STRAIGHT, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE = range(6)
 COLUMNS = ("R", "T", "eff", "p_spike", "p_lobe", "p_back", "sigma_alpha", "polish")
 NEUTRAL = dict(R=0.0, T=0.0, eff=0.0, p_spike=0.0, p_lobe=0.0, p_back=0.0, sigma_alpha=0.1, polish=0.0)
@@ -38,7 +47,8 @@ FLAG_NORMAL, FLAG_FAR_OPAQUE, FLAG_EDGE = 1, 2, 4
 
 # Geant4 property names -> columns
 G4_COLUMN = {"REFLECTIVITY": "R", "TRANSMITTANCE": "T", "EFFICIENCY": "eff", "SPECULARSPIKECONSTANT": "p_spike",
-             "SPECULARLOBECONSTANT": "p_lobe", "BACKSCATTERCONSTANT": "p_back", "sigma_alpha": "sigma_alpha"}
+             "SPECULARLOBECONSTANT": "p_lobe", "BACKSCATTERCONSTANT": "p_back", "sigma_alpha": "sigma_alpha",
+             "REALRINDEX": "nr", "IMAGINARYRINDEX": "nk"}
 
 # the outcomes each type can give with its constants fixed (its support: what the defensive mixture covers,
 # ``surfaces/types.py:404-440`` support rules), as bits by outcome index
@@ -46,19 +56,29 @@ SUPPORT = {
     "ABSORBER": (),
     "SPECULAR_METAL": (SPECULAR,),
     "UNIFIED_METAL": (SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE),
+    "UNIFIED_METAL_NORMAL": (SPECULAR, BACKSCATTER, DIFFUSE),
     "PAINTED": (SPECULAR, DIFFUSE),
+    "PAINTED_T": (STRAIGHT, SPECULAR, DIFFUSE),
+    "COUNTING": (),
     "DIELECTRIC": (REFRACT, SPECULAR),
     "DIELECTRIC_RT": (REFRACT, SPECULAR, STRAIGHT),
+    "PHOTOCATHODE": (SPECULAR,),
 }
 
+# type: (model, the parameters it reads, its constants, its flags, {parameter: the column slot it is read from})
 TYPES = {
-    "ABSORBER": (MODEL_REFLECT, (), {}, 0),
-    "SPECULAR_METAL": (MODEL_REFLECT, ("R", "eff"), {"p_spike": 1.0}, FLAG_NORMAL),
-    "UNIFIED_METAL": (MODEL_REFLECT, ("R", "p_spike", "p_lobe", "p_back", "sigma_alpha", "eff"), {}, 0),
-    "PAINTED": (MODEL_REFLECT, ("R", "p_spike", "eff"), {}, 0),
-    "DIELECTRIC": (MODEL_FRESNEL, (), {"R": 1.0}, 0),
-    "DIELECTRIC_RT": (MODEL_FRESNEL, ("R", "T", "eff"), {}, 0),
+    "ABSORBER": (MODEL_REFLECT, (), {}, 0, {}),
+    "SPECULAR_METAL": (MODEL_REFLECT, ("R", "eff"), {"p_spike": 1.0}, FLAG_NORMAL, {}),
+    "UNIFIED_METAL": (MODEL_REFLECT, ("R", "p_spike", "p_lobe", "p_back", "sigma_alpha", "eff"), {}, 0, {}),
+    "UNIFIED_METAL_NORMAL": (MODEL_REFLECT, ("R", "p_spike", "p_lobe", "p_back", "eff"), {}, FLAG_NORMAL, {}),
+    "PAINTED": (MODEL_REFLECT, ("R", "p_spike", "eff"), {}, 0, {}),
+    "PAINTED_T": (MODEL_REFLECT, ("R", "T", "p_spike", "eff"), {}, 0, {}),
+    "COUNTING": (MODEL_REFLECT, ("eff",), {}, 0, {}),
+    "DIELECTRIC": (MODEL_FRESNEL, (), {"R": 1.0}, 0, {}),
+    "DIELECTRIC_RT": (MODEL_FRESNEL, ("R", "T", "eff"), {}, 0, {}),
+    "PHOTOCATHODE": (MODEL_COMPLEX, ("nr", "nk", "eff"), {}, 0, {"nr": "p_lobe", "nk": "p_back"}),
 }
+ZERO_ROLE = ("R", "T", "eff", "p_spike", "p_lobe", "p_back")  # parameters that can zero an outcome's probability
 
 
 WLS_PROFILES = ("delta", "exponential")  # G4OpWLS's time profiles (delta: Geant4's default)
@@ -109,19 +129,56 @@ class SurfaceRow:
 
     name: str
     type: str
-    columns: dict = field(default_factory=dict)  # column -> key (str) or number
+    columns: dict = field(default_factory=dict)  # parameter -> key (str) or number
     far_opaque: bool = False
+    transmitting: bool = False  # also reads T: Geant4's dielectric_metal with TRANSMITTANCE
+
+    def __post_init__(self):
+        if self.type not in TYPES:
+            raise ValueError(f"surface row {self.name!r}: unknown type {self.type!r} (one of {sorted(TYPES)})")
+        if self.transmitting and self.type not in ("SPECULAR_METAL", "UNIFIED_METAL", "UNIFIED_METAL_NORMAL",
+                                                   "PHOTOCATHODE"):
+            raise ValueError(f"surface row {self.name!r}: only a dielectric_metal type transmits (T)")
 
     @property
     def model(self) -> int:
         return TYPES[self.type][0]
 
+    def params(self) -> tuple:
+        """The parameters the row reads: its type's, and T when it transmits."""
+        p = TYPES[self.type][1]
+        return p + ("T",) if self.transmitting and "T" not in p else p
+
+    def _fixed(self, name):
+        """A parameter's value when the row holds it as a constant (not a key), else None."""
+        v = self.columns.get(name)
+        if isinstance(v, str):
+            return None
+        if v is None:  # no key: the column holds the type's constant, else the neutral value
+            return float(TYPES[self.type][2].get(name, NEUTRAL.get(name, 0.0)))
+        return float(v)
+
     def flags(self) -> int:
-        """Return the row's flag word: type flags, far-opaque, and its support's outcome bits at bit 8."""
+        """Return the row's flag word: type flags, far-opaque, and its support's outcome bits at bit 8: the
+        outcomes its fixed constants leave possible (the reference's support rules, ``surfaces/types.py:404-440``
+        and ``transmitting``)."""  # noqa: D205, D209
         sup = set(SUPPORT[self.type])
-        if self.type == "PAINTED" and not isinstance(self.columns.get("p_spike"), str):
-            fs = float(self.columns.get("p_spike", 0.0))  # a fixed f_spec of 0 (1) leaves no mirror (Lambertian)
+        if self.type in ("PAINTED", "PAINTED_T"):  # a fixed f_spec of 0 (1) leaves no mirror (Lambertian)
+            fs = self._fixed("p_spike")
             sup -= {SPECULAR} if fs == 0.0 else {DIFFUSE} if fs == 1.0 else set()
+        if self.type == "UNIFIED_METAL_NORMAL":
+            ps, pl, pb = self._fixed("p_spike"), self._fixed("p_lobe"), self._fixed("p_back")
+            if ps == 0.0 and pl == 0.0:
+                sup.discard(SPECULAR)
+            if pb == 0.0:
+                sup.discard(BACKSCATTER)
+            if None not in (ps, pl, pb) and ps + pl + pb >= 1.0:
+                sup.discard(DIFFUSE)
+        if self.transmitting or self.type == "PAINTED_T":
+            if self._fixed("T") == 0.0:
+                sup.discard(STRAIGHT)
+            elif self.transmitting:
+                sup.add(STRAIGHT)
         if self.far_opaque:
             sup -= {STRAIGHT, REFRACT}
         bits = sum(1 << k for k in sup)
@@ -131,14 +188,22 @@ class SurfaceRow:
         """Return whether the row is an edge row: a parameter it reads is keyed and can zero an outcome's
         probability (the cuts and the reflection split: the defensive role "zero", ``surfaces/types.py:322-331``;
         decided by structure, never by a value, as ``api.py:797-814`` ``_edge_rows``)."""  # noqa: D205, D209
-        return any(isinstance(self.columns.get(c), str) for c in ("R", "T", "eff", "p_spike", "p_lobe", "p_back")
-                   if c in TYPES[self.type][1])
+        return any(isinstance(self.columns.get(c), str) for c in ZERO_ROLE
+                   if c in self.params() and c not in TYPES[self.type][4])
 
     def column(self, c):
-        """Return the column's key, or its constant (the type's fixed value, else the neutral value)."""
-        if c in self.columns and c in TYPES[self.type][1]:
+        """Return what column slot ``c`` holds for this row: the key or constant of the parameter read from it (an
+        alias slot, e.g. a photocathode's nr in p_lobe), else the type's fixed value, else the neutral value."""  # noqa: D205, D209
+        for name, slot in TYPES[self.type][4].items():
+            if slot == c:
+                return self.columns[name]
+        if c in self.columns and c in self.params():
             return self.columns[c]
         return TYPES[self.type][2].get(c, NEUTRAL[c])
+
+    def alias_keys(self) -> tuple:
+        """The keys this row reads through an alias slot (no derivative: the slot's own rules do not apply)."""
+        return tuple(self.columns[n] for n in TYPES[self.type][4] if isinstance(self.columns.get(n), str))
 
 
 @dataclass(frozen=True)

@@ -126,6 +126,11 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
     p = det.params() if params is None else params
     if isinstance(edge_rows, str):
         edge_rows = {r for r, row in enumerate(det.rows) if row.edge()}
+    for row in det.rows:  # a complex index sits in another column's slot: no derivative reads it
+        bad = set(row.alias_keys()) & set(grad_keys)
+        if bad:
+            raise ValueError(f"pack: no derivative in {sorted(bad)} (surface {row.name!r}, {row.type}): the adjoint "
+                             "has no terms for a complex index")
     b = _Builder(grad_keys)
     nm, nc, nr = len(det.media), len(det.class_medium), len(det.rows)
     med = np.zeros((nm, MEDIUM_WORDS), np.int32)

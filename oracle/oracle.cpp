@@ -770,6 +770,30 @@ inline Fres fresnel_amps(double c, double n1, double n2) {
   return f;
 }
 
+// Fresnel amplitudes from real n1 into a complex index nr + i nk (src/lumen/surfaces/fresnel.py:25-45 `amplitudes`
+// with a complex n2; the photocathode model src/lumen/surfaces/types.py:262-274 `_complex`)
+inline C2 csqrt_decaying(C2 z) {
+  const double m = std::sqrt(z.re * z.re + z.im * z.im);
+  if (m < 1e-24) return {0.0, 0.0};
+  double re = std::sqrt(std::max(0.5 * (m + z.re), 0.0)), im = std::sqrt(std::max(0.5 * (m - z.re), 0.0));
+  if (z.im < 0.0) im = -im;
+  if (im < 0.0) re = -re, im = -im;
+  return {re, im};
+}
+inline Fres fresnel_amps_complex(double c, double n1, double nr, double nk) {
+  c = std::min(std::max(c, 0.0), 1.0);
+  const C2 n2{nr, nk}, n2sq = cm(n2, n2);
+  const C2 q = csqrt_decaying({n2sq.re - n1 * n1 * (1.0 - c * c), n2sq.im});
+  const C2 ds{n1 * c + q.re, q.im}, dp{n2sq.re * c + n1 * q.re, n2sq.im * c + n1 * q.im};
+  Fres f;
+  const bool oks = std::sqrt(ca2(ds)) > 1e-12, okp = std::sqrt(ca2(dp)) > 1e-12;
+  f.rs = oks ? cd({n1 * c - q.re, -q.im}, ds) : C2{-1, 0};
+  f.rp = okp ? cd({n2sq.re * c - n1 * q.re, n2sq.im * c - n1 * q.im}, dp) : C2{1, 0};
+  f.ts = oks ? cd({2 * n1 * c, 0}, ds) : C2{0, 0};
+  f.tp = okp ? cd(cm({2 * n1 * c, 0}, n2), dp) : C2{0, 0};
+  return f;
+}
+
 template <class R> struct Surf {
   R p[NOUT];
   R detect, sigma;
@@ -792,8 +816,8 @@ Surf<R> surface_eval(const Optics &o, int row, double lam, V3<R> wi, V3<R> n, do
   s.flags = Rw[1];
   const double E = HC / lam;
   auto col = [&](int c) { return R(o.tab(Rw[2 + 2 * c], Rw[3 + 2 * c], E)); };
-  const R c1 = CLIP01<R>(col(COL_R)), c2 = CLIP01<R>(col(COL_R) + col(COL_T));
-  const R r = c1, t = c2 - c1, a = R(1.0) - c2;
+  R c1 = CLIP01<R>(col(COL_R)), c2 = CLIP01<R>(col(COL_R) + col(COL_T));
+  R r = c1, t = c2 - c1, a = R(1.0) - c2;
   s.sigma = col(COL_SIGMA);
   for (int k = 0; k < NOUT; ++k) s.p[k] = R(0.0);
   s.amps = Fres{{-1, 0}, {1, 0}, {1, 0}, {1, 0}};
@@ -806,13 +830,24 @@ Surf<R> surface_eval(const Optics &o, int row, double lam, V3<R> wi, V3<R> n, do
     s.p[BACKSCATTER] = r * (k3 - k2);
     s.p[DIFFUSE] = r * (R(1.0) - k3);
   } else {
-    s.amps = fresnel_amps(val(-dot(wi, n)), n1, n2);
+    const bool cx = Rw[0] == 2;  // a complex index (nr, nk in the p_lobe, p_back slots): its reflectance is the cut
+    s.amps = cx ? fresnel_amps_complex(val(-dot(wi, n)), n1, val(col(COL_LOBE)), val(col(COL_BACK)))
+                : fresnel_amps(val(-dot(wi, n)), n1, n2);
     const V3<R> sx = s_axis(wi, n), px = cross(wi, sx);
     const R es2 = dot(er, sx) * dot(er, sx) + dot(ei, sx) * dot(ei, sx);
     const R ep2 = dot(er, px) * dot(er, px) + dot(ei, px) * dot(ei, px);
     const R F = CLIP01<R>(R(ca2(s.amps.rs)) * es2 + R(ca2(s.amps.rp)) * ep2);
-    s.p[SPECULAR] = r * F;
-    s.p[REFRACT] = r * (R(1.0) - F);
+    if (cx) {
+      c1 = F;
+      c2 = CLIP01<R>(F + col(COL_T));
+      r = c1;
+      t = c2 - c1;
+      a = R(1.0) - c2;
+      s.p[SPECULAR] = r;
+    } else {
+      s.p[SPECULAR] = r * F;
+      s.p[REFRACT] = r * (R(1.0) - F);
+    }
   }
   s.p[STRAIGHT] = t;
   if (Rw[1] & 2) s.p[STRAIGHT] = s.p[REFRACT] = R(0.0);
@@ -1653,4 +1688,11 @@ ORC_API void oracle_pol_mirror(const double h[3], const double e[6], double out[
   pol_mirror(er, ei, v3(h[0], h[1], h[2]));
   const double v[6] = {er.x, er.y, er.z, ei.x, ei.y, ei.z};
   for (int i = 0; i < 6; ++i) out[i] = v[i];
+}
+
+// Fresnel amplitudes (re, im of r_s, r_p, t_s, t_p) at incidence cosine c from n1 into the complex index nr + i nk
+ORC_API void oracle_fresnel_complex(double c, double n1, double nr, double nk, double out8[8]) {
+  const Fres f = fresnel_amps_complex(c, n1, nr, nk);
+  const C2 a[4] = {f.rs, f.rp, f.ts, f.tp};
+  for (int i = 0; i < 4; ++i) out8[2 * i] = a[i].re, out8[2 * i + 1] = a[i].im;
 }

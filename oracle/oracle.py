@@ -58,6 +58,8 @@ def lib():
         L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
         L.oracle_mie_pdf.restype = None
         L.oracle_trace_f32.argtypes = L.oracle_trace.argtypes
+        L.oracle_fresnel_complex.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.oracle_fresnel_complex.restype = None
         L.oracle_emit.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.POINTER(E.LhSource), C.c_int,
                                   C.c_uint64, C.c_longlong, C.c_void_p]
         L.oracle_pol_recombine.argtypes = [C.c_void_p] * 6
@@ -198,6 +200,13 @@ def fresnel(c, n1, n2):
     """Return complex (r_s, r_p, t_s, t_p) at incidence cosine c from n1 into n2."""
     out = np.zeros(8)
     lib().oracle_fresnel(float(c), float(n1), float(n2), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def fresnel_complex(c, n1, nr, nk):
+    """Return the oracle's complex amplitudes (r_s, r_p, t_s, t_p) from n1 into the complex index nr + i nk."""
+    out = np.zeros(8)
+    lib().oracle_fresnel_complex(float(c), float(n1), float(nr), float(nk), _p(out))
     return out[0::2] + 1j * out[1::2]
 
 
