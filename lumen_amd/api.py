@@ -298,19 +298,48 @@ def source_rows(det, source, samples, first_id=0):
     return rows, 1
 
 
+_GATED: dict = {}  # (id(detector), t0, dt, nt) -> (the detector, its timed copy): one model per gate
+
+
+def gated(det, gate, step: float | None = None):
+    """Return ``det`` with its time histogram over ``gate`` = (g0, g1) ns (the reference's ``simulate(gate=)``):
+    nodes g0 + j dt up to the first at or past g1, dt ``step`` (default: the detector's own node spacing, else
+    1 ns).  Light before or after the gate is in the early / late cells and in every total."""  # noqa: D205, D209
+    import math
+
+    g0, g1 = float(gate[0]), float(gate[1])
+    if not (math.isfinite(g0) and math.isfinite(g1) and g1 > g0):
+        raise ValueError(f"simulate: gate must be (g0, g1) with g1 > g0, got {gate!r}")
+    dt = float(step) if step is not None else (det.time_grid.dt if det.time_grid is not None else 1.0)
+    if not dt > 0:
+        raise ValueError("simulate: the gate's node spacing must be > 0")
+    nt = int(math.ceil((g1 - g0) / dt - 1e-9)) + 1
+    k = (id(det), g0, dt, nt)
+    hit = _GATED.get(k)
+    if hit is None or hit[0] is not det:
+        hit = _GATED[k] = (det, det.timed(g0, dt, nt))
+    return hit[1]
+
+
 def simulate(det, source, params: Params | None = None, *, samples: int | None = None, seed: int = 0,
-             reference: Params | None = None, controls: Controls | None = None, first_id: int = 0,
-             edge_rows="structure", device="cuda", stream=None) -> Light:
+             gate=None, events: int | None = None, reference: Params | None = None, controls: Controls | None = None,
+             first_id: int = 0, edge_rows="structure", device="cuda", stream=None, gate_step: float | None = None
+             ) -> Light:
     """Trace ``samples`` Monte Carlo photons of ``source`` through ``det`` and return the light at each sensor.
 
     ``params``: the values (default ``det.params()``); ``reference``: the values every random choice is made at
     (default: ``params``) -- held fixed while ``params`` moves, the light at a seed is smooth in ``params`` (common
-    random numbers).  A ``Batch`` source runs every source in one launch, one event each: the outputs gain a leading
-    event axis.  ``edge_rows``: the surface rows drawn with the defensive mixture ("structure": the reference's rule;
-    ``()``: none -- the same light in expectation, without the zero-weight photons an edge row sends on).
+    random numbers).  A ``Batch`` source runs every source in one launch, one event each (or its ``events``
+    labels): the outputs gain a leading event axis.  ``events`` (the reference's ``simulate(events=)``): E, the
+    outputs' leading event axis for any source, every source's event below E.  ``gate`` = (g0, g1) ns: the time
+    histogram's extent (``gated``; ``gate_step`` its node spacing).  ``edge_rows``: the surface rows drawn with the
+    defensive mixture ("structure": the reference's rule; ``()``: none -- the same light in expectation, without the
+    zero-weight photons an edge row sends on).
     """
     import torch
 
+    if gate is not None:
+        det = gated(det, gate, gate_step)
     eng = engine(det, device)
     controls = Controls() if controls is None else controls
     p = det.params() if params is None else params
@@ -325,6 +354,10 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
             raise ValueError("simulate: the reference's structure differs from the detector's")
         ref_values = torch.as_tensor(pr.values, device=eng.device)
     rows, n_ev = source_rows(det, source, samples, first_id)
+    if events is not None:
+        if int(events) < n_ev:
+            raise ValueError(f"simulate: events={events} but a source's event is {n_ev - 1}")
+        n_ev = int(events)
     # at least one cell: a detector without sensors has no tallies, and the C ABI takes no null buffer
     buf = torch.zeros((n_ev, max(eng.stride, 1)), dtype=torch.float64, device=eng.device)
     tally = buf[:, :eng.stride]
@@ -349,7 +382,7 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
                 t.record_stream(st)
         current.wait_stream(st)  # the results, for what the caller does next on the current stream
     light = light_of(eng, tally, lambda: health.cpu().numpy(), samples if not isinstance(source, Batch) else None,
-                     source)
+                     source, keep_events=events is not None)
     light.kernel_ms = lambda: eng.kernel_ms(launch)  # the call returns before the kernel ends: read on first use
     tg = getattr(det, "time_grid", None)
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, controls=controls._asdict(),
@@ -358,8 +391,9 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     return light
 
 
-def light_of(eng, tally, health, samples, source=None) -> Light:
-    """Return the Light of raw tallies (n_events, stride)."""
+def light_of(eng, tally, health, samples, source=None, keep_events=False) -> Light:
+    """Return the Light of raw tallies (n_events, stride); one event of one source drops the event axis unless
+    ``keep_events``."""  # noqa: D205, D209
     import torch
 
     S, K = eng.S, eng.K
@@ -376,12 +410,13 @@ def light_of(eng, tally, health, samples, source=None) -> Light:
         # moment error of an event uses the photons traced into it, the photon counts summed per event
         ev = np.arange(len(source.samples)) if source.events is None else np.asarray(source.events, np.int64)
         per_event = np.bincount(ev, weights=np.asarray(source.samples, np.float64), minlength=tally.shape[0])
+        per_event = per_event[: tally.shape[0]]
         n = torch.as_tensor(np.maximum(per_event, 2.0), dtype=torch.float64, device=tally.device)[:, None]
         err = torch.sqrt(torch.clamp(n / (n - 1.0) * (arr2 - arr * arr / n), min=0.0))
     else:
         err = moment_error(arr2, arr, samples)
     mean_t = torch.where(arr > 0, arr_t / torch.where(arr > 0, arr, torch.ones_like(arr)), torch.zeros_like(arr))
-    if tally.shape[0] == 1 and not isinstance(source, Batch):
+    if tally.shape[0] == 1 and not isinstance(source, Batch) and not keep_events:
         arr, err, mean_t, charge, arr2 = arr[0], err[0], mean_t[0], charge[0], arr2[0]
         if time is not None:
             time, late, early = time[0], late[0], early[0]

@@ -82,7 +82,7 @@ __device__ inline void surface_dlogs(const int *__restrict__ R, const float *__r
   const float iR = (Rv >= 0.0f && Rv <= 1.0f) ? 1.0f : 0.0f;
   const float iRT = (Rv + Tv >= 0.0f && Rv + Tv <= 1.0f) ? 1.0f : 0.0f;
   const float r = clip01(Rv), t = clip01(Rv + Tv) - r, a = 1.0f - clip01(Rv + Tv);
-  if (a > 0.0f) ddet[COL_R] = ddet[COL_T] = -iRT / a;
+  if (a > 0.0f && !(R[1] & FLAG_PER_NORMAL)) ddet[COL_R] = ddet[COL_T] = -iRT / a;  // per normal: a / A_n = 1
   if (eff != 0.0f) ddet[COL_EFF] = 1.0f / eff;
   if (o < 0) return;
   if (o == STRAIGHT) {

@@ -24,7 +24,7 @@ constexpr float WLS_TRIES = 100.0f;  // G4OpWLS redraws an emission energy above
 constexpr int N_OUT = 6;
 enum { STRAIGHT = 0, REFRACT = 1, SPECULAR = 2, BACKSCATTER = 3, GLOSSY = 4, DIFFUSE = 5 };
 enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1, MODEL_COMPLEX = 2 };
-enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4 };
+enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4, FLAG_PER_NORMAL = 8 };
 enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, N_COL };
 // the packed layout's header words (lumen_amd/pack.py)
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
@@ -391,6 +391,16 @@ __device__ inline SurfEval surface_eval(const int *__restrict__ L, const float *
   s.p[STRAIGHT] = t;
   if (flags & FLAG_FAR_OPAQUE) s.p[STRAIGHT] = s.p[REFRACT] = 0.0f;
   s.detect = a * eff;
+  if (flags & FLAG_PER_NORMAL) {  // read for a QE applied after tracking: the detected share over A_n, the type's
+    // absorbed share at normal incidence (types.py `per_normal`, `normal_absorptance`): the cuts' a for a reflector
+    // or an interface, 1 - clip(R_n + T) for a complex index
+    float an = a;
+    if (model == MODEL_COMPLEX) {
+      const Amps n0 = fresnel_complex(1.0f, n1, col_at(R, P, COL_LOBE, E), col_at(R, P, COL_BACK, E));
+      an = 1.0f - clip01(cabs2(n0.rs) + Tv);
+    }
+    s.detect = an > 0.0f ? s.detect / an : 0.0f;
+  }
   return s;
 }
 

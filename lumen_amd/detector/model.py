@@ -43,7 +43,7 @@ MODEL_REFLECT, MODEL_FRESNEL, MODEL_COMPLEX = 0, 1, 2
 STRAIGHT, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE = range(6)
 COLUMNS = ("R", "T", "eff", "p_spike", "p_lobe", "p_back", "sigma_alpha", "polish")
 NEUTRAL = dict(R=0.0, T=0.0, eff=0.0, p_spike=0.0, p_lobe=0.0, p_back=0.0, sigma_alpha=0.1, polish=0.0)
-FLAG_NORMAL, FLAG_FAR_OPAQUE, FLAG_EDGE = 1, 2, 4
+FLAG_NORMAL, FLAG_FAR_OPAQUE, FLAG_EDGE, FLAG_PER_NORMAL = 1, 2, 4, 8
 
 # Geant4 property names -> columns
 G4_COLUMN = {"REFLECTIVITY": "R", "TRANSMITTANCE": "T", "EFFICIENCY": "eff", "SPECULARSPIKECONSTANT": "p_spike",
@@ -132,6 +132,7 @@ class SurfaceRow:
     columns: dict = field(default_factory=dict)  # parameter -> key (str) or number
     far_opaque: bool = False
     transmitting: bool = False  # also reads T: Geant4's dielectric_metal with TRANSMITTANCE
+    per_normal: bool = False  # read for a QE applied after tracking: the detected share over A_n (types.py:576)
 
     def __post_init__(self):
         if self.type not in TYPES:
@@ -139,6 +140,8 @@ class SurfaceRow:
         if self.transmitting and self.type not in ("SPECULAR_METAL", "UNIFIED_METAL", "UNIFIED_METAL_NORMAL",
                                                    "PHOTOCATHODE"):
             raise ValueError(f"surface row {self.name!r}: only a dielectric_metal type transmits (T)")
+        if self.per_normal and "eff" not in TYPES[self.type][1]:
+            raise ValueError(f"surface row {self.name!r}: per_normal needs a detecting type (one reading eff)")
 
     @property
     def model(self) -> int:
@@ -182,7 +185,8 @@ class SurfaceRow:
         if self.far_opaque:
             sup -= {STRAIGHT, REFRACT}
         bits = sum(1 << k for k in sup)
-        return TYPES[self.type][3] | (FLAG_FAR_OPAQUE if self.far_opaque else 0) | (bits << 8)
+        return (TYPES[self.type][3] | (FLAG_FAR_OPAQUE if self.far_opaque else 0) |
+                (FLAG_PER_NORMAL if self.per_normal else 0) | (bits << 8))
 
     def edge(self) -> bool:
         """Return whether the row is an edge row: a parameter it reads is keyed and can zero an outcome's
@@ -203,7 +207,10 @@ class SurfaceRow:
 
     def alias_keys(self) -> tuple:
         """The keys this row reads through an alias slot (no derivative: the slot's own rules do not apply)."""
-        return tuple(self.columns[n] for n in TYPES[self.type][4] if isinstance(self.columns.get(n), str))
+        keys = [self.columns[n] for n in TYPES[self.type][4] if isinstance(self.columns.get(n), str)]
+        if self.per_normal and self.model == MODEL_COMPLEX and isinstance(self.columns.get("T"), str):
+            keys.append(self.columns["T"])  # a / A_n then depends on the crossing: no derivative term for T
+        return tuple(keys)
 
 
 @dataclass(frozen=True)

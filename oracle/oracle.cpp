@@ -852,6 +852,14 @@ Surf<R> surface_eval(const Optics &o, int row, double lam, V3<R> wi, V3<R> n, do
   s.p[STRAIGHT] = t;
   if (Rw[1] & 2) s.p[STRAIGHT] = s.p[REFRACT] = R(0.0);
   s.detect = a * col(COL_EFF);
+  if (Rw[1] & 8) {  // per normal (src/lumen/surfaces/types.py:576-592): over A_n, the absorbed share at normal incidence
+    R an = a;
+    if (Rw[0] == 2) {
+      const Fres f0 = fresnel_amps_complex(1.0, n1, val(col(COL_LOBE)), val(col(COL_BACK)));
+      an = R(1.0) - CLIP01<R>(R(ca2(f0.rs)) + col(COL_T));
+    }
+    s.detect = an > R(0.0) ? s.detect / an : R(0.0);
+  }
   return s;
 }
 

@@ -480,3 +480,22 @@ def test_trace_with_tori_against_oracle(gpu, tmp_path):
     np.testing.assert_allclose(gh[:4], h[:4], rtol=0, atol=2e-3)  # absorbed in bulk, at surfaces, lost, escaped
     assert gh[9] <= h[9] + 1e-4 and gh[5] <= h[5] + 1e-4  # stuck-photon pushes, abandoned photons: rare in both
     np.testing.assert_allclose(gh[12], h[12], rtol=1e-2)  # vertices
+
+
+def test_gate_and_events_arguments(gpu, tank1):
+    """``simulate(gate=, events=)`` (the reference's api.py:876-878): a gate is the time histogram over it, the same
+    light as the detector timed on those nodes; ``events`` gives the outputs a leading event axis of that length."""
+    n = 1 << 17
+    a = L.simulate(tank1, SRC1, samples=n, seed=3, gate=(0.0, 12.0), gate_step=0.5)
+    b = L.simulate(tank1.timed(0.0, 0.5, 25), SRC1, samples=n, seed=3)
+    np.testing.assert_allclose(a.time.cpu().numpy(), b.time.cpu().numpy(), rtol=1e-9, atol=1e-12)
+    assert a.time.shape[-1] == 25 and a.provenance["time_grid"] == (0.0, 0.5, 25)
+    c = L.simulate(tank1, SRC1, samples=n, seed=3, events=3)
+    assert tuple(c.arrivals.shape) == (3, 8)
+    np.testing.assert_allclose(c.arrivals[0].cpu().numpy(), b.arrivals.cpu().numpy(), rtol=1e-9, atol=1e-12)
+    assert float(c.arrivals[1:].abs().sum()) == 0.0
+    batch = L.sources.Batch((SRC1, SRC1), (n, n), events=(2, 0))
+    d = L.simulate(tank1, batch, seed=3, events=4)
+    assert tuple(d.arrivals.shape) == (4, 8) and float(d.arrivals[1].abs().sum()) == 0.0
+    with pytest.raises(ValueError, match="events"):
+        L.simulate(tank1, batch, seed=3, events=2)

@@ -23,15 +23,19 @@ CASES = {
                                         extra={"T": 0.2, "p_spike": 0.3, "p_lobe": 0.3, "p_back": 0.4},
                                         transmitting=True, far_opaque=True),
     "painted_t": dict(surface="PAINTED_T", R=0.6, eff=1.0, extra={"T": 0.15}, far_opaque=True),
+    "photocathode_per_normal": dict(surface="PHOTOCATHODE", R=0.0, eff=0.7, extra={"nr": 2.9, "nk": 1.6},
+                                    per_normal=True),
 }
 
 
 def _closed(case):
     c = CASES[case]
     e = math.exp(-A / L_ABS)
-    if case == "photocathode":
+    if case.startswith("photocathode"):
         nr, nk = c["extra"]["nr"], c["extra"]["nk"]
         rn = ((N_IDX - nr) ** 2 + nk**2) / ((N_IDX + nr) ** 2 + nk**2)
+        if c.get("per_normal"):
+            return c["eff"] * e / (1 - rn * math.exp(-2 * A / L_ABS))
         return c["eff"] * specular_closed_form(A, rn, L_ABS)
     if case == "counting":
         return c["eff"] * e

@@ -402,3 +402,15 @@ def test_nav_cache_rebuilds_a_damaged_entry(tmp_path):
     assert k2 == k and again.n == nav.n and np.array_equal(again.host["lo"], nav.host["lo"])
     assert navarrays.NavArrays.load(str(entry)).n == nav.n  # the entry was written again
     assert det.nav.n == nav.n
+
+
+def test_gate_grid():
+    from lumen_amd.api import gated
+
+    det = example_tank()
+    g = gated(det, (2.0, 12.0), 0.5)
+    assert g.time_grid.t0 == 2.0 and g.time_grid.dt == 0.5 and g.time_grid.nt == 21
+    assert gated(det, (2.0, 12.0), 0.5) is g  # one model per gate
+    assert gated(det, (0.0, 10.2)).time_grid.nt == 12  # 1 ns nodes by default, the last at or past the gate's end
+    with pytest.raises(ValueError):
+        gated(det, (5.0, 5.0))

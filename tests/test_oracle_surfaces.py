@@ -104,3 +104,16 @@ def test_support_rules_of_fixed_constants():
     assert support(SurfaceRow("pc", "PHOTOCATHODE", {"nr": "a", "nk": "b", "eff": "e"})) == {SPECULAR}
     with pytest.raises(ValueError):
         SurfaceRow("x", "PAINTED", {}, transmitting=True)
+
+
+@pytest.mark.parametrize("surface,extra", [("PHOTOCATHODE", {"nr": 2.9, "nk": 1.6}), ("SPECULAR_METAL", {})])
+def test_per_normal_detection_is_eff_at_normal_incidence(surface, extra):
+    """Read for a QE applied after tracking, a crossing's detected share is over A_n (the absorbed share at normal
+    incidence): at the sphere's normal incidence every crossing detects eff of what reaches it,
+    Q/N = eff e^(-a/L) / (1 - R_n e^(-2a/L))."""
+    eff, R = 0.7, 0.5
+    det = sphere_toy(A, surface, R=R, L_abs=L_ABS, n=N_IDX, eff=eff, extra=extra, per_normal=True)
+    q, err, _ = _q(det)
+    rn = _normal_reflectance(N_IDX, extra["nr"], extra["nk"]) if extra else R
+    want = eff * math.exp(-A / L_ABS) / (1 - rn * math.exp(-2 * A / L_ABS))
+    assert abs(q - want) < 1e-6 * want + 5 * err, (q, want, err)

@@ -22,7 +22,7 @@ SPEC_S = np.array([0.0, 1.0, 0.6, 0.0])
 
 def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n=1.33, eff=1.0, L_wls=None,
                wls_yield=0.7, wls_tau=0.0, wls_profile="delta", ce=None, extra=None, transmitting=False,
-               far_opaque=False):
+               far_opaque=False, per_normal=False):
     """Return a water sphere (radius a) inside a sensor shell: the crossing out of the water is ``surface``
     (SPECULAR_METAL: R mirror; PAINTED: R Lambertian) detecting the absorbed share with ``eff``.  With ``L_wls``
     the water holds a wavelength shifter absorbing above 3 eV with mean free path L_wls (m) and emitting on
@@ -58,7 +58,8 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
         key = f"surfaces/Wall/{c}"
         values[key] = v if isinstance(v, Table) else np.float64(v)
         cols[c] = key
-    rows = [SurfaceRow("Wall", surface, cols, far_opaque=far_opaque, transmitting=transmitting),
+    rows = [SurfaceRow("Wall", surface, cols, far_opaque=far_opaque, transmitting=transmitting,
+                       per_normal=per_normal),
             SurfaceRow("absorber", "ABSORBER")]
     row_of = np.full((3, 3), -1, np.int32)
     row_of[0, 1] = 0
