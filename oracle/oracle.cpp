@@ -998,6 +998,147 @@ template <class R> V3<R> facet_path(const Photon<R> &p, double sigma, V3<R> wi, 
   return w;
 }
 
+// ------------------------------------------------------------------------------------------ GLISUR facet paths
+// src/lumen/surfaces/facets.py:14-40, :226-300, :341-365, :431-530 (GetFacetNormal glisur; DielectricMetal and
+// DielectricDielectric's facet loops, G4OpBoundaryProcess 11.3.0).  Blocks: a facet's attempts FACET_BLOCK + (k << 12)
+// + t (the ball point), its own GLISUR_BLOCK + k (u0: the cover's shell choice, u1: the Fresnel choice)
+constexpr uint32_t GLISUR_BLOCK = 0x400000u;
+
+// M = z + r s with s uniform in the unit ball (on the unit sphere when `shell`), redrawn until V . M > 0; unit
+inline V3<double> glisur_facet(const Rng &g, uint32_t vertex, int k, double r, V3<double> V, bool shell) {
+  V3<double> M = v3(0.0, 0.0, 1.0);
+  for (int t = 0; t < 4096; ++t) {
+    double u[4];
+    g.draw4(vertex, FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
+    const double z = 1.0 - 2.0 * u[1], ph = TWO_PI * u[2], sxy = std::sqrt(std::max(1.0 - z * z, 0.0));
+    const double rad = r * (shell ? 1.0 : std::cbrt(u[0]));
+    const V3<double> c = v3(rad * sxy * std::cos(ph), rad * sxy * std::sin(ph), 1.0 + rad * z);
+    if (dot(V, c) > 0.0) { M = c; break; }
+  }
+  return unit(M);
+}
+
+// the log density per solid angle of the unit facet m given V (local frame): p_m = 2 q (3 c^2 + q^2) / (3 Vol), c =
+// m_z, q^2 = r^2 - sin^2, Vol the ball's volume on V's side (facets.py `_glisur_log_density`); -inf off the support
+inline double glisur_lp(double r, V3<double> V, V3<double> m) {
+  const double c = m.z, q2 = r * r - (m.x * m.x + m.y * m.y);
+  if (!(q2 > 0.0 && c > 0.0 && dot(V, m) > 0.0)) return -INFINITY;
+  const double d = std::max(r - V.z, 0.0);
+  const double vol = 4.0 / 3.0 * PI * r * r * r - PI * d * d * (2.0 * r + V.z) / 3.0;
+  return std::log(2.0 / 3.0) + 0.5 * std::log(q2) + std::log(3.0 * c * c + q2) - std::log(vol);
+}
+
+// the cover mixture's log density (1 - e) p_m + e p_shell, p_shell = 2 r (c^2 + q^2) / (q Area) (facets.py
+// `_glisur_log_proposal`)
+inline double glisur_lq(double r, V3<double> V, V3<double> m, double e) {
+  const double lp = glisur_lp(r, V, m);
+  if (!std::isfinite(lp)) return -INFINITY;
+  const double c = m.z, q2 = r * r - (m.x * m.x + m.y * m.y);
+  const double area = V.z >= r ? 4.0 * PI * r * r : TWO_PI * r * (r + std::min(V.z, r));
+  const double ls = std::log(2.0 * r) + std::log(c * c + q2) - 0.5 * std::log(q2) - std::log(area);
+  const double a = std::log1p(-e) + lp, b = std::log(e) + ls, mx = std::max(a, b);
+  return mx + std::log(std::exp(a - mx) + std::exp(b - mx));
+}
+
+struct GlisurPath {
+  V3<double> w;
+  bool far;
+  double logw;  // the sum of the facets' log(live density / drawn density)
+};
+
+// Geant4's DielectricMetal loop with GLISUR facets: a facet for the current direction, the mirror reflection about it,
+// repeated while the direction points into the surface; the state mirrored about each facet.  Drawn at width
+// r_draw (from the cover mixture when `cover`); logw accumulates log p_m(r_live) / drawn density per facet
+inline GlisurPath glisur_metal_path(const Rng &g, uint32_t vertex, double r_draw, double r_live, bool cover,
+                                    V3<double> wi, V3<double> n, V3<double> &er, V3<double> &ei) {
+  V3<double> t1, t2;
+  basis(n, t1, t2);
+  auto local = [&](V3<double> v) { return v3(dot(v, t1), dot(v, t2), dot(v, n)); };
+  V3<double> w = wi;
+  GlisurPath out{wi, false, 0.0};
+  for (int k = 0; k < 256; ++k) {
+    double u[4];
+    g.draw4(vertex, GLISUR_BLOCK + (uint32_t)k, u);
+    const V3<double> V = local(-1.0 * w);
+    const bool shell = cover && u[0] < DEFENSIVE;
+    const V3<double> m = glisur_facet(g, vertex, k, r_draw, V, shell);
+    if (cover || r_live != r_draw)
+      out.logw += glisur_lp(r_live, V, m) - (cover ? glisur_lq(r_draw, V, m, DEFENSIVE) : glisur_lp(r_draw, V, m));
+    const V3<double> mw = to_world(m, n);
+    const V3<double> wo = w - 2.0 * dot(w, mw) * mw;
+    pol_mirror(er, ei, mw);
+    w = wo;
+    if (dot(w, n) >= 0.0) break;
+  }
+  out.w = w;
+  return out;
+}
+
+// Geant4's DielectricDielectric loop with GLISUR facets: a facet about the current side's normal, reflected or
+// refracted by the Fresnel shares of the photon's state there (its own uniform), the state recombined; a refraction
+// that still points back at the surface swaps the side; ends when the direction leaves the surface on either side.
+// `g4conv`: Geant4's own conventions for the state between facets (the in-plane part as a magnitude; at total
+// reflection the real vector's mirror), only for the comparison with Geant4's histograms
+inline GlisurPath glisur_dielectric_path(const Rng &g, uint32_t vertex, double r_draw, double r_live, bool cover,
+                                         V3<double> wi, V3<double> n, double n1, double n2, V3<double> &er,
+                                         V3<double> &ei, bool g4conv) {
+  GlisurPath out{wi, false, 0.0};
+  V3<double> w = wi;
+  double side = 1.0;
+  for (int k = 0; k < 256; ++k) {
+    const V3<double> N = side * n;
+    const double na = side > 0 ? n1 : n2, nb = side > 0 ? n2 : n1;
+    V3<double> t1, t2;
+    basis(N, t1, t2);
+    const V3<double> V = v3(-dot(w, t1), -dot(w, t2), -dot(w, N));
+    double u[4];
+    g.draw4(vertex, GLISUR_BLOCK + (uint32_t)k, u);
+    const bool shell = cover && u[0] < DEFENSIVE;
+    const V3<double> ml = glisur_facet(g, vertex, k, r_draw, V, shell);
+    if (cover || r_live != r_draw)
+      out.logw += glisur_lp(r_live, V, ml) - (cover ? glisur_lq(r_draw, V, ml, DEFENSIVE) : glisur_lp(r_draw, V, ml));
+    const V3<double> m = to_world(ml, N);
+    const double c1 = std::min(std::max(-dot(w, m), 0.0), 1.0);
+    const Fres f = fresnel_amps(c1, na, nb);
+    const V3<double> s_ax = s_axis(w, m), p_in = cross(w, s_ax);
+    C2 es{dot(er, s_ax), dot(ei, s_ax)}, ep{dot(er, p_in), dot(ei, p_in)};
+    const bool tir = (na / nb) * (na / nb) * (1.0 - c1 * c1) >= 1.0;
+    if (g4conv && !tir) ep = {std::sqrt(ca2(ep)), 0.0};
+    const double R = std::min(std::max(ca2(f.rs) * ca2(es) + ca2(f.rp) * ca2(ep), 0.0), 1.0) /
+                     std::max(ca2(es) + ca2(ep), 1e-300);
+    const bool matched = std::fabs(na - nb) <= 1e-6 * std::max(na, nb);
+    const bool refr = u[1] >= R;
+    V3<double> w1;
+    if (refr) {
+      if (matched) {
+        w1 = w;
+      } else {
+        const double eta = na / nb, s2t = eta * eta * (1.0 - c1 * c1);
+        w1 = unit(eta * w + (eta * c1 - std::sqrt(std::max(1.0 - s2t, 0.0))) * m);
+      }
+    } else {
+      w1 = w - 2.0 * dot(w, m) * m;
+    }
+    if (!(refr && matched)) {  // recombine the state on the facet's s axis into the new direction
+      C2 as = refr ? f.ts : f.rs, ap = refr ? f.tp : f.rp;
+      if (g4conv && tir && !refr) as = {-1.0, 0.0}, ap = {1.0, 0.0};
+      const C2 cs = cm(as, es), cp = cm(ap, ep);
+      const V3<double> p_out = cross(w1, s_ax);
+      er = cs.re * s_ax + cp.re * p_out;
+      ei = cs.im * s_ax + cp.im * p_out;
+      pol_normalise(er, ei);
+    }
+    const double d = dot(w1, N);
+    const bool end = refr ? d <= 0.0 : d >= 0.0;
+    out.far = out.far != refr;
+    w = w1;
+    if (end) break;
+    if (refr) side = -side;
+  }
+  out.w = w;
+  return out;
+}
+
 // Rayleigh, polarised (phase.py `rayleigh_sample`, `dipole_cos`): the dipole axis er with probability |er|^2, else
 // ei; c = cos(k', axis) from 3/4 (1 - c^2) by its inverse CDF
 template <class R> V3<R> rayleigh_dir(V3<R> er, V3<R> ei, const double ul[4]) {
@@ -1703,4 +1844,32 @@ ORC_API void oracle_fresnel_complex(double c, double n1, double nr, double nk, d
   const Fres f = fresnel_amps_complex(c, n1, nr, nk);
   const C2 a[4] = {f.rs, f.rp, f.ts, f.tp};
   for (int i = 0; i < 4; ++i) out8[2 * i] = a[i].re, out8[2 * i + 1] = a[i].im;
+}
+
+// n GLISUR facet paths off a flat surface (normal +z) at incidence wi, width r = 1 - polish (`dielectric`: from n1 into
+// n2, else the metal's), the state er (n*3 per photon, ei = 0); out (n*4): the last direction and the far side
+ORC_API void oracle_glisur_paths(int dielectric, double r, const double *wi, double n1, double n2, const double *er,
+                                 long long n, uint64_t seed, int g4conv, double *out) {
+  for (long long i = 0; i < n; ++i) {
+    const Rng g{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)i, (uint32_t)(i >> 32)};
+    V3<double> e_r = v3(er[3 * i], er[3 * i + 1], er[3 * i + 2]), e_i = v3(0.0, 0.0, 0.0);
+    const V3<double> w = unit(v3(wi[0], wi[1], wi[2])), z = v3(0.0, 0.0, 1.0);
+    const GlisurPath P = dielectric ? glisur_dielectric_path(g, 0, r, r, false, w, z, n1, n2, e_r, e_i, g4conv != 0)
+                                    : glisur_metal_path(g, 0, r, r, false, w, z, e_r, e_i);
+    out[4 * i] = P.w.x, out[4 * i + 1] = P.w.y, out[4 * i + 2] = P.w.z, out[4 * i + 3] = P.far ? 1.0 : 0.0;
+  }
+}
+
+// n GLISUR facets (local frame) for V at width r, drawn from the cover mixture with share e when e > 0; out (n*4):
+// the facet and its log density log p_m (e = 0) or the mixture's
+ORC_API void oracle_glisur_facets(double r, const double *V, double e, long long n, uint64_t seed, double *out) {
+  const V3<double> v = unit(v3(V[0], V[1], V[2]));
+  for (long long i = 0; i < n; ++i) {
+    const Rng g{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)i, (uint32_t)(i >> 32)};
+    double u[4];
+    g.draw4(0, GLISUR_BLOCK, u);
+    const V3<double> m = glisur_facet(g, 0, 0, r, v, e > 0.0 && u[0] < e);
+    out[4 * i] = m.x, out[4 * i + 1] = m.y, out[4 * i + 2] = m.z;
+    out[4 * i + 3] = e > 0.0 ? glisur_lq(r, v, m, e) : glisur_lp(r, v, m);
+  }
 }

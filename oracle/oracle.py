@@ -58,6 +58,11 @@ def lib():
         L.oracle_mie_pdf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_longlong, C.c_void_p, C.c_void_p]
         L.oracle_mie_pdf.restype = None
         L.oracle_trace_f32.argtypes = L.oracle_trace.argtypes
+        L.oracle_glisur_paths.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                          C.c_longlong, C.c_uint64, C.c_int, C.c_void_p]
+        L.oracle_glisur_paths.restype = None
+        L.oracle_glisur_facets.argtypes = [C.c_double, C.c_void_p, C.c_double, C.c_longlong, C.c_uint64, C.c_void_p]
+        L.oracle_glisur_facets.restype = None
         L.oracle_fresnel_complex.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p]
         L.oracle_fresnel_complex.restype = None
         L.oracle_emit.argtypes = [C.POINTER(E.LhGeometry), C.c_void_p, C.c_void_p, C.POINTER(E.LhSource), C.c_int,
@@ -277,3 +282,22 @@ def pol_mirror(h, er, ei):
     out = np.zeros(6)
     lib().oracle_pol_mirror(_p(hh), _p(e), _p(out))
     return out[:3], out[3:]
+
+
+def glisur_paths(r, wi, n, seed=0, er=None, dielectric=False, n1=1.0, n2=1.0, g4conv=False):
+    """Return (last directions (n, 3), far side (n,)) of n GLISUR facet paths off a flat surface (normal +z) at
+    incidence wi, width r = 1 - polish: the dielectric's from n1 into n2 or the metal's; ``er`` the states (n, 3)."""
+    w = np.ascontiguousarray(wi, np.float64)
+    e = np.ascontiguousarray(np.tile([0.0, 1.0, 0.0], (int(n), 1)) if er is None else er, np.float64)
+    out = np.zeros((int(n), 4))
+    lib().oracle_glisur_paths(int(bool(dielectric)), float(r), _p(w), float(n1), float(n2), _p(e), int(n),
+                              C.c_uint64(int(seed)), int(bool(g4conv)), _p(out))
+    return out[:, :3], out[:, 3] > 0.5
+
+
+def glisur_facets(r, V, n, seed=0, e=0.0):
+    """Return (facets (n, 3), their log densities) of GLISUR facets drawn for V (local frame) at width r."""
+    v = np.ascontiguousarray(V, np.float64)
+    out = np.zeros((int(n), 4))
+    lib().oracle_glisur_facets(float(r), _p(v), float(e), int(n), C.c_uint64(int(seed)), _p(out))
+    return out[:, :3], out[:, 3]
