@@ -105,6 +105,8 @@ struct Photon {
   float3 x, d, er, ei;
   float t, lam, w, wr, w0;
   int vol, srf, nz, vertex, cls, event;
+  float xs;  // the coordinate scale |x0| + t of the ray whose crossing put the photon on face srf: the start's
+             // rounding off that face is within that ray's band, however near the origin the crossing lies
   uint32_t id_lo, id_hi;  // the photon's id: its Philox counter words (the key is the launch's seed)
   // the medium cache: the class it was evaluated for, live and reference values
   int mcls;
@@ -344,6 +346,7 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   p.vol = locate<TOR>(a.g, p.x);
   p.cls = a.g.vinfo[p.vol].z;
   p.srf = -1;
+  p.xs = 0.0f;
   p.nz = 0;
   p.vertex = 0;
   p.mcls = -2;
@@ -399,6 +402,126 @@ __device__ inline float3 facet_path(const Photon &p, const lh_rng &rng, float si
     w = wo;
     V = mul3(-1.0f, Lo);
     if (Lo.z >= 0.0f) break;
+  }
+  return w;
+}
+
+// GLISUR's facet (G4OpBoundaryProcess::GetFacetNormal, glisur; facets.py:14-40): M = z + r s with s uniform in the
+// unit ball (on the unit sphere when `shell`, the cover's share), redrawn while V . M <= 0; the unit M.  Local frame
+__device__ inline float3 glisur_facet(const lh_rng &rng, uint32_t vertex, int k, float r, float3 V, bool shell) {
+  float3 M = mk3(0.0f, 0.0f, 1.0f);
+  for (int t = 0; t < MAX_FACET_ATTEMPTS; ++t) {
+    float u[4];
+    rng.draw4(vertex, LH_FACET_BLOCK + ((uint32_t)k << 12) + (uint32_t)t, u);
+    const float z = 1.0f - 2.0f * u[1], sxy = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+    const float rad = shell ? r : r * cbrtf(u[0]);
+    float sp, cp;
+    __sincosf(TWO_PI * u[2], &sp, &cp);
+    const float3 c = mk3(rad * sxy * cp, rad * sxy * sp, fmaf(rad, z, 1.0f));
+    if (dot3(V, c) > 0.0f) {
+      M = c;
+      break;
+    }
+  }
+  return unit3(M);
+}
+
+// the facet's log density per solid angle log p_m, p_m = 2 q (3 c^2 + q^2) / (3 Vol) (c = m_z, q^2 = r^2 - sin^2, Vol
+// the ball's volume on V's side), and in dl its derivative in r: 3 r (c^2 + q^2) / (q^2 (3 c^2 + q^2)) - Area / Vol
+// (d Vol / dr is the ball's surface on V's side; facets.py `_glisur_log_density`).  -inf (dl 0) off the support.
+// With `cover`, lq receives the cover mixture's log density (1 - e) p_m + e p_shell, p_shell = 2 r (c^2 + q^2) /
+// (q Area) (facets.py `_glisur_log_proposal`)
+__device__ inline float glisur_lp(float r, float3 V, float3 m, float &dl, bool cover, float &lq) {
+  const float c = m.z, q2 = r * r - (m.x * m.x + m.y * m.y);
+  dl = 0.0f;
+  lq = -INFINITY;
+  if (!(q2 > 0.0f && c > 0.0f && dot3(V, m) > 0.0f)) return -INFINITY;
+  const float h = fmaxf(r - V.z, 0.0f);
+  const float vol = (4.0f / 3.0f) * PI_F * r * r * r - PI_F * h * h * (2.0f * r + V.z) * (1.0f / 3.0f);
+  const float area = V.z >= r ? 4.0f * PI_F * r * r : TWO_PI * r * (r + fminf(V.z, r));
+  const float b = 3.0f * c * c + q2;
+  dl = 3.0f * r * (c * c + q2) / (q2 * b) - area / vol;
+  const float lp = __logf(2.0f * q2 * b / (3.0f * vol)) - 0.5f * __logf(q2);  // log(2 q b / (3 Vol))
+  if (cover) {
+    const float ps = 2.0f * r * (c * c + q2) * rsqrtf(q2) / area;
+    lq = __logf((1.0f - DEFENSIVE) * __expf(lp) + DEFENSIVE * ps);
+  }
+  return lp;
+}
+
+// one facet's weight terms: the live log density over the density drawn from (the reference width's, or the cover
+// mixture's) into logw when `weigh`, the live log density's derivative in r into dlogw when `deriv`.  A facet that
+// rounds off the drawn density's support weighs 0 (surface_event.py:124-129)
+__device__ inline void glisur_weigh(float r, float r_live, bool cover, bool weigh, bool deriv, float3 V, float3 m,
+                                    float &logw, float &dlogw) {
+  if (!(weigh || deriv)) return;
+  float dl, lq, dr, lqr;
+  const float lp = glisur_lp(r_live, V, m, dl, false, lq);
+  if (weigh) {
+    const float ld = cover ? (glisur_lp(r, V, m, dr, true, lqr), lqr) : glisur_lp(r, V, m, dr, false, lqr);
+    logw = ld > -INFINITY ? logw + (lp - ld) : -INFINITY;
+  }
+  if (deriv) dlogw += dl;
+}
+
+// Geant4's DielectricMetal loop with GLISUR facets (facets.py:431-470): a facet for the current direction, the mirror
+// reflection about it, repeated while the direction points into the surface; the state mirrored about each facet.
+// Drawn at width r (from the cover mixture when `cover`), weighed at r_live
+__device__ inline float3 glisur_metal(const lh_rng &rng, uint32_t vertex, float r, float r_live, bool cover,
+                                      bool weigh, bool deriv, float3 wi, float3 n, float3 &er, float3 &ei,
+                                      float &logw, float &dlogw) {
+  float3 w = wi;
+  for (int k = 0; k < MAX_FACETS; ++k) {
+    float u[4];
+    rng.draw4(vertex, LH_GLISUR_BLOCK + (uint32_t)k, u);
+    const float3 V = to_local_frame(mul3(-1.0f, w), n);
+    const float3 m = glisur_facet(rng, vertex, k, r, V, cover && u[0] < DEFENSIVE);
+    glisur_weigh(r, r_live, cover, weigh, deriv, V, m, logw, dlogw);
+    const float3 mw = to_world(m, n);
+    w = fma3(-2.0f * dot3(w, mw), mw, w);
+    pol_mirror(er, ei, mw);
+    if (dot3(w, n) >= 0.0f) break;
+  }
+  return w;
+}
+
+// Geant4's DielectricDielectric loop with GLISUR facets (facets.py:471-546): a facet about the current side's normal,
+// reflected or refracted by the Fresnel shares of the photon's state there (its own uniform; total reflection always
+// reflects), the state recombined into the new direction; a refraction that still points back at the surface swaps
+// the side; ends when the direction leaves the surface on either side.  `far`: an odd number of refractions (the
+// photon ends beyond the surface).  n1 the index on n's side, n2 the far one; matched indices keep the direction
+__device__ inline float3 glisur_dielectric(const lh_rng &rng, uint32_t vertex, float r, float r_live, bool cover,
+                                           bool weigh, bool deriv, float3 wi, float3 n, float n1, float n2,
+                                           float3 &er, float3 &ei, float &logw, float &dlogw, bool &far) {
+  float3 w = wi;
+  float side = 1.0f;
+  far = false;
+  const bool matched = fabsf(n1 - n2) <= INDEX_MATCH * fmaxf(n1, n2);
+  for (int k = 0; k < MAX_FACETS; ++k) {
+    const float3 N = mul3(side, n);
+    const float na = side > 0.0f ? n1 : n2, nb = side > 0.0f ? n2 : n1;
+    float u[4];
+    rng.draw4(vertex, LH_GLISUR_BLOCK + (uint32_t)k, u);
+    const float3 V = to_local_frame(mul3(-1.0f, w), N);
+    const float3 ml = glisur_facet(rng, vertex, k, r, V, cover && u[0] < DEFENSIVE);
+    glisur_weigh(r, r_live, cover, weigh, deriv, V, ml, logw, dlogw);
+    const float3 m = to_world(ml, N);
+    const float c1 = clip01(-dot3(w, m));
+    const Amps f = fresnel(c1, na, nb);
+    const float3 sx = s_axis(w, m), px = cross3(w, sx);
+    const float es2 = dot3(er, sx) * dot3(er, sx) + dot3(ei, sx) * dot3(ei, sx);
+    const float ep2 = dot3(er, px) * dot3(er, px) + dot3(ei, px) * dot3(ei, px);
+    const float Rf = clip01(cabs2(f.rs) * es2 + cabs2(f.rp) * ep2) / fmaxf(es2 + ep2, FLOOR32);
+    const float eta = na / nb;
+    const bool tir = eta * eta * (1.0f - c1 * c1) >= 1.0f;
+    const bool refr = !tir && u[1] >= Rf;
+    const float3 w1 = refr ? (matched ? w : refract_dir(w, m, eta)) : fma3(-2.0f * dot3(w, m), m, w);
+    if (!(refr && matched)) pol_recombine(er, ei, w, sx, refr ? f.ts : f.rs, refr ? f.tp : f.rp, w1);
+    const float d = dot3(w1, N);
+    far = far != refr;
+    w = w1;
+    if (refr ? d <= 0.0f : d >= 0.0f) break;
+    if (refr) side = -side;
   }
   return w;
 }
@@ -547,7 +670,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const float mu_r = CHAN ? (mr.mu_ray + mr.mu_mie) + mr.mu_wls : mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
-  const Cross cr = query_t<!(F & F_GNODES), (F & F_TORUS) != 0>(g, p.x, p.d, p.vol, p.srf, max_dist);
+  const Cross cr = query_t<!(F & F_GNODES), (F & F_TORUS) != 0>(g, p.x, p.d, p.vol, p.srf, max_dist, p.xs);
   const bool hit = cr.vol_after >= 0;
   const float dist = cr.t;
   const bool zero = dist <= ZERO_STEP_BANDS * b0;
@@ -765,13 +888,30 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       } else if (o == GLOSSY) {
         float logw = 0.0f, dlogw = 0.0f;
         const bool want_ratio = REF && sl.sigma != sr.sigma;
-        dn = facet_path(p, rng, sr.sigma, sl.sigma, a.ut, want_ratio, adj_event, wi, n, er0, ei0, logw, dlogw);
-        if (want_ratio) w_lobe = __expf(logw);
-        if (adj_event) {
+        const bool glisur = sr.flags & FLAG_GLISUR;
+        if (glisur) {  // GLISUR's facet path at radius r = 1 - polish: a mirror metal's, or an interface's that
+          // refracts too and may end beyond the surface (at the live indices, as a polished interface)
+          const bool cover = sr.flags & FLAG_COVER, weigh = want_ratio || cover;
+          if (sr.flags & FLAG_REFRACTS) {
+            const float n2l = index_at(a.L, a.P, class_medium(a, ca), p.lam);
+            dn = glisur_dielectric(rng, p.vertex, sr.sigma, sl.sigma, cover, weigh, adj_event, wi, n, ml.n, n2l,
+                                   er0, ei0, logw, dlogw, far);
+          } else {
+            dn = glisur_metal(rng, p.vertex, sr.sigma, sl.sigma, cover, weigh, adj_event, wi, n, er0, ei0, logw,
+                              dlogw);
+          }
+          if (weigh) w_lobe = __expf(logw);
+        } else {
+          dn = facet_path(p, rng, sr.sigma, sl.sigma, a.ut, want_ratio, adj_event, wi, n, er0, ei0, logw, dlogw);
+          if (want_ratio) w_lobe = __expf(logw);
+        }
+        if (adj_event) {  // the live log density's derivative in the width: sigma_alpha, or -d/dr for polish
+          const int col = glisur ? COL_POLISH : COL_SIGMA;
+          const float sgn = glisur ? -1.0f : 1.0f;
           const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * row;
-          const TabD t = tab_d(a.P, b->slots, Rw[2 + 2 * COL_SIGMA], Rw[3 + 2 * COL_SIGMA], HC / p.lam);
-          grad_add(sg, t.s0, dlogw * t.w0 * ahead);
-          grad_add(sg, t.s1, dlogw * t.w1 * ahead);
+          const TabD t = tab_d(a.P, b->slots, Rw[2 + 2 * col], Rw[3 + 2 * col], HC / p.lam);
+          grad_add(sg, t.s0, sgn * dlogw * t.w0 * ahead);
+          grad_add(sg, t.s1, sgn * dlogw * t.w1 * ahead);
         }
       } else {  // DIFFUSE
         dn = lambert_sample(n, ul[0], ul[1]);
@@ -779,6 +919,7 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
       }
       w_new = w_if * w_lobe;
       acc.rew += (w_at * att * s_cont - w_new) * p.w0;
+      p.xs = maxabs3(p.x) + dist;
       p.x = fma3(dist, p.d, p.x);
       p.t += dist / v_g;
       p.d = unit3(dn);

@@ -11,7 +11,8 @@
 //
 // Surface identity: the face a photon sits on (its `srf` code: volume, leaf, face) is not met again within K_ON
 // bands of the start; the band is BAND_ULPS float32 ulps of the coordinates plus a floor
-// (src/lumen/rules/tolerances.py:76-83).
+// (src/lumen/rules/tolerances.py:76-83), the coordinates' scale the larger of the start's and that of the ray whose
+// crossing put the photon on the face (the crossing point's rounding is that ray's).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -551,9 +552,11 @@ __device__ inline void core_interval(const Geo &g, int vol, float3 xl, float3 dl
   }
 }
 
+// `scale`: the coordinate scale of the ray that put the start on face `last` (its crossing's rounding is within
+// that ray's band, not the start's own: a crossing near the origin reached from far away)
 template <bool LDS, bool TOR>
-__device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int last, float tmax) {
-  const float skip_t = K_ON * band_at(maxabs3(x));
+__device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int last, float tmax, float scale = 0.0f) {
+  const float skip_t = K_ON * band_at(fmaxf(maxabs3(x), scale));
   Cross best{tmax, -1, -1, 0, mk3(0.f, 0.f, 1.f)};
   int best_leaf = -1, best_face = -1, best_vol = -1;
   // the ray's stretch [core_lo, core_hi] inside the volume's daughter-free core: a box the ray crosses only there

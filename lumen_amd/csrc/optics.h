@@ -20,11 +20,16 @@ constexpr float HALF_PI = 1.5707963267948966f;
 constexpr float FLOOR32 = 1e-30f;
 constexpr float TINY_NORMAL = 1.17549435e-38f;
 constexpr float DEFENSIVE = 0.01f;
+constexpr float INDEX_MATCH = 128.0f * 1.1920929e-7f;  // |n_a - n_b| / max at or below: a facet refraction keeps w
 constexpr float WLS_TRIES = 100.0f;  // G4OpWLS redraws an emission energy above the absorbed one up to 100 times
 constexpr int N_OUT = 6;
 enum { STRAIGHT = 0, REFRACT = 1, SPECULAR = 2, BACKSCATTER = 3, GLOSSY = 4, DIFFUSE = 5 };
 enum { MODEL_REFLECT = 0, MODEL_FRESNEL = 1, MODEL_COMPLEX = 2 };
-enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4, FLAG_PER_NORMAL = 8 };
+// FLAG_GLISUR: GLOSSY is GLISUR's facet path (width 1 - polish); FLAG_REFRACTS: that path refracts at each facet
+// (GLISUR_DIELECTRIC, which may end beyond the surface); FLAG_COVER: its facets are drawn from the cover mixture
+// (polish is a parameter: it moves the support's edge)
+enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4, FLAG_PER_NORMAL = 8, FLAG_GLISUR = 16, FLAG_REFRACTS = 32,
+       FLAG_COVER = 64 };
 enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, N_COL };
 // the packed layout's header words (lumen_amd/pack.py)
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
@@ -333,7 +338,7 @@ __device__ inline void pol_recombine(float3 &er, float3 &ei, float3 wi, float3 s
 struct SurfEval {
   float p[N_OUT];  // the continuing shares
   float detect;    // the detected share
-  float sigma;     // the glossy lobe's width
+  float sigma;     // the glossy lobe's width (GLISUR: its radius 1 - polish)
   Amps amps;       // (r_s, r_p, t_s, t_p) of the deterministic outcomes
   int flags;
 };
@@ -354,7 +359,7 @@ __device__ inline SurfEval surface_eval(const int *__restrict__ L, const float *
   const float Rv = col_at(R, P, COL_R, E), Tv = col_at(R, P, COL_T, E), eff = col_at(R, P, COL_EFF, E);
   float c1 = clip01(Rv), c2 = clip01(Rv + Tv);
   float r = c1, t = c2 - c1, a = 1.0f - c2;
-  s.sigma = col_at(R, P, COL_SIGMA, E);
+  s.sigma = (flags & FLAG_GLISUR) ? 1.0f - col_at(R, P, COL_POLISH, E) : col_at(R, P, COL_SIGMA, E);
   for (int k = 0; k < N_OUT; ++k) s.p[k] = 0.0f;
   s.amps = Amps{cf{-1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}, cf{1.f, 0.f}};  // the perfect mirror's
   if (model == MODEL_REFLECT) {

@@ -47,7 +47,9 @@ struct lh_rng {
 };
 
 // block layout of one vertex: 0 = (path, channel, outcome, roulette), 1 = (lobe0, lobe1, lobe2, polarisation),
-// 2 = (reemit0, reemit1, -, -); a facet path's draws use blocks FACET_BLOCK + (facet << 12) + attempt; the source's
-// draws use vertex SOURCE_VERTEX (beyond every real vertex: the vertex guard is below 2^31)
+// 2 = (reemit0, reemit1, -, -); a facet path's draws use blocks FACET_BLOCK + (facet << 12) + attempt, and a GLISUR
+// facet's own choices (the cover's shell, the Fresnel choice) GLISUR_BLOCK + facet; the source's draws use vertex
+// SOURCE_VERTEX (beyond every real vertex: the vertex guard is below 2^31)
 #define LH_FACET_BLOCK 0x100u
+#define LH_GLISUR_BLOCK 0x400000u
 #define LH_SOURCE_VERTEX 0xFFFFFFFFu

@@ -227,10 +227,17 @@ def _surface_row(name, s: dict):
         if "R" not in cols and not ("nr" in cols and "nk" in cols):
             cols["R"] = f"surfaces/{name}/REFLECTIVITY"
             defaults[cols["R"]] = np.float64(1.0)
-        if fin == "polished" and "nr" in cols and "nk" in cols:
+        if "nr" in cols and "nk" in cols:
+            if not (fin == "polished" or (model == "glisur" and float(s["value"]) >= 1.0)):
+                raise ValueError(f"gdml: optical surface {name!r}: a complex index on a ground finish (the facet's "
+                                 "Fresnel) is not supported")
             cols.pop("R", None)
             return SurfaceRow(name, "PHOTOCATHODE", cols, transmitting=trans), defaults
-        if fin == "polished" or (model == "glisur" and float(s["value"]) >= 1.0):
+        if model == "glisur" and fin == "ground" and float(s["value"]) < 1.0:  # the facet path at 1 - polish
+            cols["polish"] = f"surfaces/{name}/polish"
+            defaults[cols["polish"]] = np.float64(s["value"])
+            return SurfaceRow(name, "GLISUR_METAL", cols, transmitting=trans), defaults
+        if fin == "polished" or model == "glisur":  # glisur: any other finish, or polish 1, is the mirror
             return SurfaceRow(name, "SPECULAR_METAL", cols, transmitting=trans), defaults
         if fin == "ground" and model == "unified":
             if float(s["value"]) == 0.0:  # sigma_alpha 0: GetFacetNormal returns the normal
@@ -244,8 +251,15 @@ def _surface_row(name, s: dict):
                 defaults[cols["R"]] = np.float64(1.0)
             cols["p_spike"] = 1.0 if fin == "polishedfrontpainted" else 0.0
             return SurfaceRow(name, "PAINTED_T" if trans else "PAINTED", cols), defaults
-        if fin == "polished":
+        if fin == "polished" or (model == "glisur" and float(s["value"]) >= 1.0):
             return SurfaceRow(name, "DIELECTRIC_RT" if "R" in cols or "T" in cols else "DIELECTRIC", cols), defaults
+        if model == "glisur" and fin == "ground":  # the facet path reflecting or refracting at each facet
+            if "R" not in cols:
+                cols["R"] = f"surfaces/{name}/REFLECTIVITY"
+                defaults[cols["R"]] = np.float64(1.0)
+            cols["polish"] = f"surfaces/{name}/polish"
+            defaults[cols["polish"]] = np.float64(s["value"])
+            return SurfaceRow(name, "GLISUR_DIELECTRIC", cols), defaults
     raise ValueError(f"gdml: optical surface {name!r} ({model}, {typ}, {fin}) is not supported")
 
 
@@ -320,7 +334,7 @@ def load(path: str, recipe: Recipe) -> Detector:
         for prop, ref in s["props"].items():
             if prop in G4_COLUMN:
                 values[f"surfaces/{sname}/{prop}"] = _table(d, ref)
-        if s["type"] == "dielectric_metal" and s["finish"] == "ground":
+        if s["type"] == "dielectric_metal" and s["finish"] == "ground" and s["model"] == "unified":
             values[f"surfaces/{sname}/sigma_alpha"] = np.float64(s["value"])
     border_in = {}
     for sname, a, b in d.borders:
@@ -359,16 +373,20 @@ def load(path: str, recipe: Recipe) -> Detector:
                     dielectric = len(srows)
                     srows.append(SurfaceRow("dielectric", "DIELECTRIC"))
                 row_of[a, b] = dielectric
+    def put(a, b, r):  # an interface into a medium without RINDEX: Geant4 kills the photon before the chain
+        interface = srows[r].type in ("DIELECTRIC", "DIELECTRIC_RT", "GLISUR_DIELECTRIC")
+        row_of[a, b] = absorber if interface and (b == out_cls or class_medium[b] < 0) else r
+
     for i, lst in border_in.items():
         for sname, j in lst:
-            row_of[vol_cls[j], vol_cls[i]] = srow_of[sname]
+            put(vol_cls[j], vol_cls[i], srow_of[sname])
     for i, (_, lv, _, _, _) in enumerate(placed):
         if lv in skin_lv:
             r = srow_of[skin_lv[lv]]
             p = placed[i][4]
             if p >= 0:
-                row_of[vol_cls[p], vol_cls[i]] = r
-                row_of[vol_cls[i], vol_cls[p]] = r
+                put(vol_cls[p], vol_cls[i], r)
+                put(vol_cls[i], vol_cls[p], r)
     vols = [navarrays.Volume(p[0], kind_of[d.volumes[p[1]]["solid"]], p[2], p[3], p[4], vol_cls[i])
             for i, p in enumerate(placed)]
     nav, _ = navarrays.cached(vols, kinds, out_cls)

@@ -19,10 +19,18 @@ DIELECTRIC_RT          fresnel  R (into the interface), T (straight on), eff
 PHOTOCATHODE           complex  nr, nk (a complex index seen from the dielectric: its Fresnel reflectance is the
                                 mirror's share, the absorbed rest detected with eff); nr, nk in the p_lobe and
                                 p_back column slots, which no other part of the type reads
+GLISUR_METAL           reflect  R into GLISUR's facet path (radius 1 - polish), eff: Geant4's dielectric_metal,
+                                glisur, ground, polish < 1
+GLISUR_DIELECTRIC      reflect  R into GLISUR's facet path that reflects or refracts at each facet by the Fresnel
+                                shares of the two media (it may end beyond the surface), T straight on, eff:
+                                Geant4's dielectric_dielectric, glisur, ground, polish < 1
 ====================  =======  ========================================================================
 
 A row with ``transmitting`` set (Geant4's dielectric_metal with TRANSMITTANCE, the reference's
 ``surfaces/types.py:497-512`` ``transmitting``) also reads T: the share past R up to R + T goes straight on.
+
+A GLISUR row whose polish is a parameter (a key) draws its facets from the cover mixture (1 - e) p_m + e p_shell
+(the reference's ``surfaces/facets.py:14-27, 557-583``): polish moves the facet support's edge.
 
 ``row_of[c_from, c_to]`` is the surface row of a crossing from class ``c_from`` into ``c_to`` (-1: no surface
 joins them; the photon is lost).  A volume's class decides its medium (``class_medium``).
@@ -44,6 +52,8 @@ STRAIGHT, REFRACT, SPECULAR, BACKSCATTER, GLOSSY, DIFFUSE = range(6)
 COLUMNS = ("R", "T", "eff", "p_spike", "p_lobe", "p_back", "sigma_alpha", "polish")
 NEUTRAL = dict(R=0.0, T=0.0, eff=0.0, p_spike=0.0, p_lobe=0.0, p_back=0.0, sigma_alpha=0.1, polish=0.0)
 FLAG_NORMAL, FLAG_FAR_OPAQUE, FLAG_EDGE, FLAG_PER_NORMAL = 1, 2, 4, 8
+FLAG_GLISUR, FLAG_REFRACTS, FLAG_COVER = 16, 32, 64  # GLOSSY is GLISUR's path; it refracts; drawn under the cover
+GLISUR_R_MIN = 1e-3  # the smallest 1 - polish of a GLISUR facet path (rules/choices.py:97-100)
 
 # Geant4 property names -> columns
 G4_COLUMN = {"REFLECTIVITY": "R", "TRANSMITTANCE": "T", "EFFICIENCY": "eff", "SPECULARSPIKECONSTANT": "p_spike",
@@ -63,6 +73,8 @@ SUPPORT = {
     "DIELECTRIC": (REFRACT, SPECULAR),
     "DIELECTRIC_RT": (REFRACT, SPECULAR, STRAIGHT),
     "PHOTOCATHODE": (SPECULAR,),
+    "GLISUR_METAL": (GLOSSY,),
+    "GLISUR_DIELECTRIC": (GLOSSY, STRAIGHT),
 }
 
 # type: (model, the parameters it reads, its constants, its flags, {parameter: the column slot it is read from})
@@ -77,6 +89,9 @@ TYPES = {
     "DIELECTRIC": (MODEL_FRESNEL, (), {"R": 1.0}, 0, {}),
     "DIELECTRIC_RT": (MODEL_FRESNEL, ("R", "T", "eff"), {}, 0, {}),
     "PHOTOCATHODE": (MODEL_COMPLEX, ("nr", "nk", "eff"), {}, 0, {"nr": "p_lobe", "nk": "p_back"}),
+    "GLISUR_METAL": (MODEL_REFLECT, ("R", "polish", "eff"), {"p_lobe": 1.0}, FLAG_GLISUR, {}),
+    "GLISUR_DIELECTRIC": (MODEL_REFLECT, ("R", "T", "polish", "eff"), {"p_lobe": 1.0}, FLAG_GLISUR | FLAG_REFRACTS,
+                          {}),
 }
 ZERO_ROLE = ("R", "T", "eff", "p_spike", "p_lobe", "p_back")  # parameters that can zero an outcome's probability
 
@@ -138,10 +153,23 @@ class SurfaceRow:
         if self.type not in TYPES:
             raise ValueError(f"surface row {self.name!r}: unknown type {self.type!r} (one of {sorted(TYPES)})")
         if self.transmitting and self.type not in ("SPECULAR_METAL", "UNIFIED_METAL", "UNIFIED_METAL_NORMAL",
-                                                   "PHOTOCATHODE"):
+                                                   "PHOTOCATHODE", "GLISUR_METAL"):
             raise ValueError(f"surface row {self.name!r}: only a dielectric_metal type transmits (T)")
         if self.per_normal and "eff" not in TYPES[self.type][1]:
             raise ValueError(f"surface row {self.name!r}: per_normal needs a detecting type (one reading eff)")
+        if self.far_opaque and TYPES[self.type][3] & FLAG_REFRACTS:
+            raise ValueError(f"surface row {self.name!r}: {self.type}'s facet path decides its own side (no "
+                             "far_opaque)")
+        if self.type.startswith("GLISUR"):
+            if "polish" not in self.columns:
+                raise ValueError(f"surface row {self.name!r}: {self.type} reads polish")
+            self.check_polish(self._fixed("polish"))
+
+    def check_polish(self, v):
+        """Refuse a GLISUR polish outside [0, 1 - GLISUR_R_MIN] (None: a key, checked with its value)."""
+        if v is not None and not (0.0 <= v <= 1.0 - GLISUR_R_MIN):
+            raise ValueError(f"surface row {self.name!r}: polish {v} outside [0, {1 - GLISUR_R_MIN}] (polish 1 is "
+                             "the polished reading: SPECULAR_METAL or DIELECTRIC_RT)")
 
     @property
     def model(self) -> int:
@@ -177,7 +205,7 @@ class SurfaceRow:
                 sup.discard(BACKSCATTER)
             if None not in (ps, pl, pb) and ps + pl + pb >= 1.0:
                 sup.discard(DIFFUSE)
-        if self.transmitting or self.type == "PAINTED_T":
+        if self.transmitting or self.type in ("PAINTED_T", "GLISUR_DIELECTRIC"):
             if self._fixed("T") == 0.0:
                 sup.discard(STRAIGHT)
             elif self.transmitting:
@@ -185,8 +213,9 @@ class SurfaceRow:
         if self.far_opaque:
             sup -= {STRAIGHT, REFRACT}
         bits = sum(1 << k for k in sup)
+        cover = FLAG_COVER if TYPES[self.type][3] & FLAG_GLISUR and isinstance(self.columns.get("polish"), str) else 0
         return (TYPES[self.type][3] | (FLAG_FAR_OPAQUE if self.far_opaque else 0) |
-                (FLAG_PER_NORMAL if self.per_normal else 0) | (bits << 8))
+                (FLAG_PER_NORMAL if self.per_normal else 0) | cover | (bits << 8))
 
     def edge(self) -> bool:
         """Return whether the row is an edge row: a parameter it reads is keyed and can zero an outcome's

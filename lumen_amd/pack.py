@@ -131,6 +131,16 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
         if bad:
             raise ValueError(f"pack: no derivative in {sorted(bad)} (surface {row.name!r}, {row.type}): the adjoint "
                              "has no terms for a complex index")
+    for r, row in enumerate(det.rows):  # a GLISUR interface's Fresnel choices at each facet read both indices
+        if row.type != "GLISUR_DIELECTRIC":
+            continue
+        for a_, b_ in zip(*np.nonzero(np.asarray(det.row_of) == r)):
+            for c in (a_, b_):
+                m = det.class_medium[c]
+                if m >= 0 and det.media[m].rindex in grad_keys:
+                    raise ValueError(f"pack: no derivative in {det.media[m].rindex!r}: surface {row.name!r} "
+                                     "(GLISUR_DIELECTRIC) chooses at each facet by the indices, and the adjoint has "
+                                     "no index terms for its facet path")
     b = _Builder(grad_keys)
     nm, nc, nr = len(det.media), len(det.class_medium), len(det.rows)
     med = np.zeros((nm, MEDIUM_WORDS), np.int32)
@@ -184,6 +194,10 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
         for j, c in enumerate(COLUMNS):
             v = row.column(c)
             rows[r, 2 + 2 * j: 4 + 2 * j] = b.value(p, v) if isinstance(v, str) else b.table([0.0], [float(v)])
+        if row.type.startswith("GLISUR") and isinstance(row.columns["polish"], str):
+            pv = p[row.columns["polish"]]
+            for x in np.atleast_1d(pv.value if isinstance(pv, Table) else pv):
+                row.check_polish(float(x))
     qe_off, n_qe = 0, 1
     q = p[det.groups[0].qe] if det.groups and det.groups[0].qe is not None else None
     if isinstance(q, Table):  # the charge histogram's wavelength knots (nm); QE itself is read after tracking

@@ -46,6 +46,7 @@ _INFO = {
     "SPECULARLOBECONSTANT": Info("-", (0.0, 1.0), "reweight"),
     "BACKSCATTERCONSTANT": Info("-", (0.0, 1.0), "reweight"),
     "sigma_alpha": Info("rad", (0.0, np.inf), "reweight"),
+    "polish": Info("-", (0.0, 1.0 - 1e-3), "reweight"),
     "efficiency": Info("-", (0.0, 1.0), "after_tracking"),
     "qe": Info("-", (0.0, 1.0), "after_tracking"),
 }

@@ -422,7 +422,7 @@ template <class R> struct Change {
 // value scanned between consecutive endpoints); `exit_query`: from inside, else from outside.  Endpoints of the face
 // the photon sits on (volume `vol`, code `skip`) within ON_FACE of the start count as passed.
 template <class R>
-Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_query, long long skip) {
+Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_query, long long skip, double on_face_f32) {
   const Kind &K = s.kinds[s.kind[vol]];
   const int nl = (int)K.leaves.size();
   std::vector<std::array<Iv<R>, 2>> ivs(nl);
@@ -431,7 +431,9 @@ Change<R> first_change(const Scene &s, int vol, V3<R> o, V3<R> d, bool exit_quer
     cnt[l] = leaf_iv(K.leaves[l], o, d, ivs[l].data());
     // a torus root met at a grazing angle is conditioned to sqrt(eps) of the tube, not eps: its face is passed
     // within ON_FACE_TORUS
-    const R on_face = R(K.leaves[l].typ == 2 ? ON_FACE_TORUS : ON_FACE);
+    // in float32 (the engine's arithmetic) the engine's band rule: K_ON bands at the crossing ray's scale
+    const R on_face = sizeof(R) == 4 ? R(std::max(on_face_f32, K.leaves[l].typ == 2 ? ON_FACE_TORUS : 0.0))
+                                     : R(K.leaves[l].typ == 2 ? ON_FACE_TORUS : ON_FACE);
     for (int p = 0; p < cnt[l]; ++p) {
       Iv<R> &iv = ivs[l][p];
       const long long clo = ((long long)vol << 10) | (l << 5) | iv.flo, chi = ((long long)vol << 10) | (l << 5) | iv.fhi;
@@ -501,14 +503,20 @@ template <class R> V3<R> normal_at(const Scene &s, int v, int leaf, int face, V3
 }
 
 // the next visible crossing: the own exit, then every daughter (a daughter wins a tie)
-template <class R> Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, long long last, R tmax) {
+// `scale`: the coordinate scale of the ray that put the start on face `last` (float32: its band bounds the start's
+// rounding off that face)
+template <class R>
+Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, long long last, R tmax, double scale = 0.0) {
+  const double xa = std::max(std::max(std::fabs((double)val(x.x)), std::fabs((double)val(x.y))),
+                             std::fabs((double)val(x.z)));
+  const double on_face_f32 = 4.0 * (8.0 * 1.1920928955078125e-7 * std::max(xa, scale) + 1e-9);
   Cross<R> best{tmax, -1, -1, v3<R>(R(0.0), R(0.0), R(1.0))};
   int bv = -1, bl = -1, bf = -1;
   bool own = false;
   if (vol < s.n) {
     V3<R> xl, dl;
     to_local(s, vol, x, d, xl, dl);
-    const Change<R> ch = first_change(s, vol, xl, dl, true, last);
+    const Change<R> ch = first_change(s, vol, xl, dl, true, last, on_face_f32);
     if (ch.leaf >= 0 && ch.t <= best.t) {
       best.t = ch.t; best.vol_after = s.parent[vol]; bv = vol; bl = ch.leaf; bf = ch.face; own = true;
     }
@@ -516,7 +524,7 @@ template <class R> Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, lon
   auto visit = [&](int u) {
     V3<R> xl, dl;
     to_local(s, u, x, d, xl, dl);
-    const Change<R> ch = first_change(s, u, xl, dl, false, last);
+    const Change<R> ch = first_change(s, u, xl, dl, false, last, on_face_f32);
     if (ch.leaf >= 0 && (ch.t < best.t || (ch.t == best.t && own))) {
       best.t = ch.t; best.vol_after = u; bv = u; bl = ch.leaf; bf = ch.face; own = false;
     }
@@ -818,7 +826,7 @@ Surf<R> surface_eval(const Optics &o, int row, double lam, V3<R> wi, V3<R> n, do
   auto col = [&](int c) { return R(o.tab(Rw[2 + 2 * c], Rw[3 + 2 * c], E)); };
   R c1 = CLIP01<R>(col(COL_R)), c2 = CLIP01<R>(col(COL_R) + col(COL_T));
   R r = c1, t = c2 - c1, a = R(1.0) - c2;
-  s.sigma = col(COL_SIGMA);
+  s.sigma = (Rw[1] & 16) ? R(1.0) - col(COL_POLISH) : col(COL_SIGMA);  // GLISUR: the facet radius 1 - polish
   for (int k = 0; k < NOUT; ++k) s.p[k] = R(0.0);
   s.amps = Fres{{-1, 0}, {1, 0}, {1, 0}, {1, 0}};
   if (Rw[0] == 0) {
@@ -957,6 +965,7 @@ template <class R> struct Photon {
   double lam, w0;
   int vol, cls, nz, vertex, event;
   long long srf;
+  double xs;  // the scale |x0| + t of the ray whose crossing put the photon on face srf
   Rng rng;
 };
 
@@ -1003,6 +1012,7 @@ template <class R> V3<R> facet_path(const Photon<R> &p, double sigma, V3<R> wi, 
 // DielectricDielectric's facet loops, G4OpBoundaryProcess 11.3.0).  Blocks: a facet's attempts FACET_BLOCK + (k << 12)
 // + t (the ball point), its own GLISUR_BLOCK + k (u0: the cover's shell choice, u1: the Fresnel choice)
 constexpr uint32_t GLISUR_BLOCK = 0x400000u;
+constexpr double INDEX_MATCH = 128.0 * 1.1920928955078125e-7;  // src/lumen/rules/choices.py:98
 
 // M = z + r s with s uniform in the unit ball (on the unit sphere when `shell`), redrawn until V . M > 0; unit
 inline V3<double> glisur_facet(const Rng &g, uint32_t vertex, int k, double r, V3<double> V, bool shell) {
@@ -1040,6 +1050,13 @@ inline double glisur_lq(double r, V3<double> V, V3<double> m, double e) {
   return mx + std::log(std::exp(a - mx) + std::exp(b - mx));
 }
 
+// one facet's log weight: the live density over the density drawn from; a facet that rounds off the drawn density's
+// support weighs 0 (src/lumen/transport/surface_event.py:124-129)
+inline void glisur_weigh(double &logw, double r_draw, double r_live, bool cover, V3<double> V, V3<double> m) {
+  const double ld = cover ? glisur_lq(r_draw, V, m, DEFENSIVE) : glisur_lp(r_draw, V, m);
+  logw = std::isfinite(ld) ? logw + glisur_lp(r_live, V, m) - ld : -INFINITY;
+}
+
 struct GlisurPath {
   V3<double> w;
   bool far;
@@ -1062,8 +1079,7 @@ inline GlisurPath glisur_metal_path(const Rng &g, uint32_t vertex, double r_draw
     const V3<double> V = local(-1.0 * w);
     const bool shell = cover && u[0] < DEFENSIVE;
     const V3<double> m = glisur_facet(g, vertex, k, r_draw, V, shell);
-    if (cover || r_live != r_draw)
-      out.logw += glisur_lp(r_live, V, m) - (cover ? glisur_lq(r_draw, V, m, DEFENSIVE) : glisur_lp(r_draw, V, m));
+    if (cover || r_live != r_draw) glisur_weigh(out.logw, r_draw, r_live, cover, V, m);
     const V3<double> mw = to_world(m, n);
     const V3<double> wo = w - 2.0 * dot(w, mw) * mw;
     pol_mirror(er, ei, mw);
@@ -1095,8 +1111,7 @@ inline GlisurPath glisur_dielectric_path(const Rng &g, uint32_t vertex, double r
     g.draw4(vertex, GLISUR_BLOCK + (uint32_t)k, u);
     const bool shell = cover && u[0] < DEFENSIVE;
     const V3<double> ml = glisur_facet(g, vertex, k, r_draw, V, shell);
-    if (cover || r_live != r_draw)
-      out.logw += glisur_lp(r_live, V, ml) - (cover ? glisur_lq(r_draw, V, ml, DEFENSIVE) : glisur_lp(r_draw, V, ml));
+    if (cover || r_live != r_draw) glisur_weigh(out.logw, r_draw, r_live, cover, V, ml);
     const V3<double> m = to_world(ml, N);
     const double c1 = std::min(std::max(-dot(w, m), 0.0), 1.0);
     const Fres f = fresnel_amps(c1, na, nb);
@@ -1106,8 +1121,8 @@ inline GlisurPath glisur_dielectric_path(const Rng &g, uint32_t vertex, double r
     if (g4conv && !tir) ep = {std::sqrt(ca2(ep)), 0.0};
     const double R = std::min(std::max(ca2(f.rs) * ca2(es) + ca2(f.rp) * ca2(ep), 0.0), 1.0) /
                      std::max(ca2(es) + ca2(ep), 1e-300);
-    const bool matched = std::fabs(na - nb) <= 1e-6 * std::max(na, nb);
-    const bool refr = u[1] >= R;
+    const bool matched = std::fabs(na - nb) <= INDEX_MATCH * std::max(na, nb);
+    const bool refr = !tir && u[1] >= R;
     V3<double> w1;
     if (refr) {
       if (matched) {
@@ -1244,6 +1259,7 @@ template <class R> bool emit(const Run &r, long long gi, Photon<R> &p, Tally &T)
   p.vol = locate(*r.s, v3<double>(val(p.x.x), val(p.x.y), val(p.x.z)));
   p.cls = r.s->cls[p.vol];
   p.srf = -1;
+  p.xs = 0.0;
   p.nz = 0;
   p.vertex = 0;
   T.h[LH_H_PHOTONS] += 1;
@@ -1282,7 +1298,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   const Med<R> mr = r.has_ref ? medium_at<R>(r.ref, m, p.lam) : ml;
   const R mu = (ml.mu_ray + ml.mu_mie) + ml.mu_wls, mu_r = (mr.mu_ray + mr.mu_mie) + mr.mu_wls;  // total rates
   const R s_free = -LOG(R(u[0])) / MAX<R>(mu_r, R(FLOOR32));
-  const Cross<R> cr = query<R>(s, p.x, p.d, p.vol, p.srf, R(r.max_dist));
+  const Cross<R> cr = query<R>(s, p.x, p.d, p.vol, p.srf, R(r.max_dist), p.xs);
   const bool hit = cr.vol_after >= 0;
   const R dist = cr.t;
   const bool zero = dist <= R(ZERO_STEP);
@@ -1485,6 +1501,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
       const V3<R> wi = p.d, n = cr.n;
       V3<R> dn = wi;
       bool far = false;
+      R w_lobe = R(1.0);  // a GLISUR facet path's live / drawn density
       if (o == STRAIGHT) {
         far = true;
       } else if (o == REFRACT) {
@@ -1501,14 +1518,32 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
         dn = R(-1.0) * wi;
         er0 = R(-1.0) * er0;
         ei0 = R(-1.0) * ei0;
+      } else if (o == GLOSSY && (sr.flags & 16)) {  // GLISUR's facet path at r = 1 - polish (types.py:460-467)
+        auto dbl = [](V3<R> v) { return v3(val(v.x), val(v.y), val(v.z)); };
+        auto back = [](V3<double> v) { return v3<R>(R(v.x), R(v.y), R(v.z)); };
+        V3<double> e_r = dbl(er0), e_i = dbl(ei0);
+        const bool cover = sr.flags & 64;
+        const double rd = val(sr.sigma), rl = val(sl.sigma);
+        const GlisurPath P =
+            (sr.flags & 32) ? glisur_dielectric_path(p.rng, (uint32_t)p.vertex, rd, rl, cover, dbl(wi), dbl(n),
+                                                     val(ml.n), index_at(r.live, r.live.cls_medium(ca), p.lam), e_r,
+                                                     e_i, false)
+                            : glisur_metal_path(p.rng, (uint32_t)p.vertex, rd, rl, cover, dbl(wi), dbl(n), e_r, e_i);
+        dn = back(P.w);
+        er0 = back(e_r);
+        ei0 = back(e_i);
+        far = P.far;
+        w_lobe = R(std::exp(P.logw));
       } else if (o == GLOSSY) {
         dn = facet_path(p, val(sr.sigma), wi, n, er0, ei0);
       } else {
         dn = lambert_dir(n, ul);
         pol_mirror(er0, ei0, unit(dn - wi));
       }
-      w_new = w_if;
+      w_new = w_if * w_lobe;
       T.h[LH_H_REWEIGHTED] += val(w_at * att * s_cont - w_new) * p.w0;
+      p.xs = std::max(std::max(std::fabs((double)val(p.x.x)), std::fabs((double)val(p.x.y))),
+                      std::fabs((double)val(p.x.z))) + (double)val(dist);
       p.x = p.x + dist * p.d;
       p.t = p.t + dist / v_g;
       p.d = unit(dn);

@@ -414,3 +414,35 @@ def test_gate_grid():
     assert gated(det, (0.0, 10.2)).time_grid.nt == 12  # 1 ns nodes by default, the last at or past the gate's end
     with pytest.raises(ValueError):
         gated(det, (5.0, 5.0))
+
+
+def test_gdml_glisur_surfaces_map_to_geant4s_paths():
+    """G4OpBoundaryProcess's readings of a glisur surface (the reference's rules/optics.py:28-50, 453-522): ground
+    with polish < 1 is the facet path (metal, or an interface that refracts), any other finish or polish 1 the
+    polished reading; polish is a parameter at the surface's value (so the facets are drawn under the cover), and a
+    complex index on a ground finish is refused."""
+    from lumen_amd.detector.gdml import _surface_row
+    from lumen_amd.detector.model import FLAG_COVER, FLAG_GLISUR, FLAG_REFRACTS, SurfaceRow
+
+    def row(typ, fin, model, value, props=()):
+        return _surface_row("S", dict(props={p: "m" for p in props}, type=typ, finish=fin, model=model, value=value))
+
+    r, dflt = row("dielectric_metal", "ground", "glisur", 0.3)
+    assert r.type == "GLISUR_METAL" and dflt == {"surfaces/S/REFLECTIVITY": 1.0, "surfaces/S/polish": 0.3}
+    assert r.flags() & 0xFF == FLAG_GLISUR | FLAG_COVER
+    assert row("dielectric_metal", "polished", "glisur", 0.3)[0].type == "SPECULAR_METAL"
+    assert row("dielectric_metal", "ground", "glisur", 1.0)[0].type == "SPECULAR_METAL"
+    r, dflt = row("dielectric_dielectric", "ground", "glisur", 0.5, ("TRANSMITTANCE",))
+    assert r.type == "GLISUR_DIELECTRIC" and r.columns["T"] == "surfaces/S/TRANSMITTANCE"
+    assert r.flags() & 0xFF == FLAG_GLISUR | FLAG_REFRACTS | FLAG_COVER
+    assert row("dielectric_dielectric", "ground", "glisur", 1.0)[0].type == "DIELECTRIC"
+    assert row("dielectric_metal", "ground", "unified", 0.1)[0].type == "UNIFIED_METAL"
+    with pytest.raises(ValueError, match="complex index on a ground"):
+        row("dielectric_metal", "ground", "unified", 0.1, ("REALRINDEX", "IMAGINARYRINDEX"))
+    with pytest.raises(ValueError, match="polish"):
+        SurfaceRow("S", "GLISUR_METAL", {"R": 0.9, "polish": 1.0, "eff": 0.0})
+    with pytest.raises(ValueError, match="decides its own side"):
+        SurfaceRow("S", "GLISUR_DIELECTRIC", {"R": 1.0, "polish": 0.5}, far_opaque=True)
+    # a fixed polish draws from p_m itself; STRAIGHT is in the support only with a T that is not fixed at 0
+    r = SurfaceRow("S", "GLISUR_DIELECTRIC", {"R": 1.0, "polish": 0.5})
+    assert r.flags() & 0xFF == FLAG_GLISUR | FLAG_REFRACTS and (r.flags() >> 8) == 1 << 4

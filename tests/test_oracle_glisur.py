@@ -106,3 +106,65 @@ def test_glisur_dielectric_path_is_geant4s(name):
     with np.load(os.path.join(GOLDEN, case["file"])) as z:
         mine = np.histogram2d(wo[:, 2], np.abs(np.arctan2(wo[:, 1], wo[:, 0])), bins=[z["edges_z"], z["edges_phi"]])[0]
         assert _chi2_p(z["hist"], mine) > P_MIN, name
+
+
+RANDOM_CASES = sorted(k for k, v in MANIFEST.items() if v["pol"] == "random")
+
+
+def _ring_counts(tal, h, n_photons):
+    """The 36 rings' counts from a trace's arrivals (each photon of weight 1/n ends in one ring)."""
+    return O.light_of(tal, h, 36, 1, n_photons)["arrivals"] * n_photons
+
+
+@pytest.mark.parametrize("name", RANDOM_CASES)
+def test_glisur_toy_in_the_oracle_counts_geant4s_rings(name):
+    """The oracle's trace through the ring counter (the path within a detector: the side it ends on decides the
+    medium it goes on in) against Geant4's ring histogram."""
+    from toys import glisur_beam, glisur_toy
+
+    case = MANIFEST[name]
+    det = glisur_toy("dielectric", case["n1"], case["n2"], case["value"])
+    n = 1 << 16
+    tal, h = O.OracleRun(det).trace(glisur_beam(case["theta"]), n, seed=case["seed"])
+    got = _ring_counts(tal, h, n)
+    assert abs(got.sum() - n) < 1e-6 * n  # every photon counted once
+    with np.load(os.path.join(GOLDEN, case["file"])) as z:
+        assert _chi2_p(z["rings"], got) > P_MIN, name
+
+
+@pytest.mark.parametrize("keyed", [False, True])
+def test_glisur_metal_toy_in_the_oracle(keyed):
+    """GLISUR_METAL in the ring counter: the rings of the oracle's trace against its facet path's own directions;
+    keyed, drawn from the cover mixture, the weights average out to the same rings."""
+    from toys import glisur_beam, glisur_toy
+
+    det = glisur_toy("metal", polish=0.4, keyed=keyed)
+    n = 1 << 16
+    tal, h = O.OracleRun(det).trace(glisur_beam(40.0), n, seed=9)
+    got = _ring_counts(tal, h, n)
+    t = np.deg2rad(40.0)
+    wo, _ = O.glisur_paths(0.6, [np.sin(t), 0.0, -np.cos(t)], 1 << 18, seed=10)
+    ref = np.histogram(np.degrees(np.arccos(np.clip(wo[:, 2], -1, 1))), np.linspace(0, 180, 37))[0]
+    assert abs(got.sum() - n) < 1e-3 * n
+    assert got[18:].sum() == 0.0
+    if not keyed:
+        assert _chi2_p(ref, got) > P_MIN
+    else:  # weighted counts: compare shares within their own errors
+        share = ref / ref.sum()
+        e = np.sqrt(share * (1 - share) / ref.sum() + share / n * 1.2)
+        assert np.all(np.abs(got / n - share) < 5 * e + 1e-4)
+
+
+def test_float32_model_keeps_the_crossed_face_behind_a_crossing_near_the_origin():
+    """A crossing at the origin reached from 0.5 m away rounds off the face by up to its ray's band (~1e-7 m), far
+    above the band of the crossing point's own coordinates (the 1e-9 m floor): the float32 model (the engine's
+    arithmetic and band rule) takes the face behind by the crossing ray's scale, so it walks the same paths as FP64,
+    with no zero-length steps and no pushes."""
+    from toys import glisur_beam, glisur_toy
+
+    det = glisur_toy("dielectric", 1.67, 1.8, 0.01)
+    n = 1 << 14
+    a, b = (O.OracleRun(det).trace(glisur_beam(45.0), n, seed=3, fp32=f) for f in (False, True))
+    ra, rb = (O.light_of(t, h, 36, 1, n)["arrivals"] for t, h in (a, b))
+    assert np.abs(ra - rb).sum() < 2e-3
+    assert float(b[1][9]) == 0.0 and float(b[1][12]) == float(a[1][12])  # pushed; vertices

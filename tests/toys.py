@@ -74,6 +74,64 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
                     ["shell"], [SensorGroup("pmt", np.arange(1), ce=ce_key)], Params(values))
 
 
+def glisur_toy(kind="dielectric", n1=1.0, n2=1.5, polish=0.3, R=1.0, T=0.0, eff=0.0, keyed=False):
+    """Return the GLISUR ring counter: a flat interface z = 0 between an upper medium (index n1, a world orb of 1.2
+    m) and a lower box (|x|, |y| < 1.1 m, -1.1 < z < 0; index n2, or opaque for ``kind`` "metal") with a GLISUR row
+    (GLISUR_DIELECTRIC or GLISUR_METAL) on the crossing down (and up, for the dielectric), inside a counting shell r
+    in [1, 1.01] m cut into 36 polar rings of 5 degrees about +z by z slabs (rings 0-17 in the upper medium, 18-35
+    in the box), each its own sensor absorbing and counting every photon that enters it.  A beam aimed at the origin
+    counts the polar angle of its photons' last direction off the surface, the rings of Geant4's boundary runs.
+    ``keyed``: polish (and R) as keys, which draws the facets from the cover mixture."""  # noqa: D205, D209
+    kinds = [solids.orb(1.2, name="world"), solids.box(1.1, 1.1, 0.55, name="lower")]
+    for i in range(36):  # the shell cut by the slab cos(theta_1) <= z <= cos(theta_0): planes, no cone apex at the
+        # origin where every photon starts (a photon from the origin enters the shell at r = 1, z = cos(theta))
+        z1, z0 = np.cos(np.deg2rad(5.0 * (i + 1))), np.cos(np.deg2rad(5.0 * i))
+        shell = solids.boolean("sub", solids.orb(1.01), solids.orb(1.0), name=f"ring{i}")
+        slab = solids.box(1.1, 1.1, 0.5 * (z0 - z1))
+        kinds.append(solids.boolean("inter", shell, slab, t=np.array([0.0, 0.0, 0.5 * (z0 + z1)]), name=f"ring{i}"))
+    vols = [navarrays.Volume("world", 0, np.eye(3), np.zeros(3), -1, 0),
+            navarrays.Volume("world/lower", 1, np.eye(3), np.array([0.0, 0.0, -0.55]), 0, 2)]
+    vols += [navarrays.Volume(f"world/ring{i}" if i < 18 else f"world/lower/ring{i}", 2 + i, np.eye(3), np.zeros(3),
+                              0 if i < 18 else 1, 1) for i in range(36)]
+    nav = navarrays.build(vols, kinds, outside_cls=3)
+    values = {"materials/Upper/RINDEX": Table(E_KNOTS, np.array([n1, n1])),
+              "materials/Lower/RINDEX": Table(E_KNOTS, np.array([n2, n2])),
+              "materials/SensLike/RINDEX": Table(E_KNOTS, np.array([1.0, 1.0]))}
+    media = [Medium("Upper", "materials/Upper/RINDEX"), Medium("Lower", "materials/Lower/RINDEX"),
+             Medium("SensLike", "materials/SensLike/RINDEX")]
+    cols = {"R": R, "polish": polish, "eff": eff}
+    if keyed:
+        values["surfaces/Glisur/REFLECTIVITY"] = np.float64(R)
+        values["surfaces/Glisur/polish"] = np.float64(polish)
+        cols.update(R="surfaces/Glisur/REFLECTIVITY", polish="surfaces/Glisur/polish")
+    if kind == "dielectric":
+        cols["T"] = T
+        row = SurfaceRow("Glisur", "GLISUR_DIELECTRIC", cols)
+    else:
+        row = SurfaceRow("Glisur", "GLISUR_METAL", cols)
+    rows = [SurfaceRow("ring", "SPECULAR_METAL", {"R": 0.0, "eff": 1.0}), SurfaceRow("absorber", "ABSORBER"), row]
+    row_of = np.full((4, 4), 1, np.int32)
+    row_of[0, 1] = row_of[2, 1] = 0
+    row_of[0, 2] = 2
+    if kind == "dielectric":
+        row_of[2, 0] = 2
+    class_medium = np.array([0, 2, 1 if kind == "dielectric" else -1, -1], np.int32)
+    sensor_of_vol = np.array([-1, -1] + list(range(36)) + [-1], np.int32)
+    return Detector(f"glisur_{kind}", nav, media, class_medium, ["Upper", "SensLike", "Lower", "OUT"], rows, row_of,
+                    sensor_of_vol, [f"ring{i}" for i in range(36)], [SensorGroup("rings", np.arange(36))],
+                    Params(values))
+
+
+def glisur_beam(theta_deg, lam=420.0):
+    """A beam from 0.5 m up the incidence line onto the origin at ``theta_deg`` off the normal +z (unpolarised: a
+    random linear state per photon, Geant4's random polarisation)."""  # noqa: D205, D209
+    from lumen_amd import sources
+
+    t = np.deg2rad(theta_deg)
+    return sources.Point((-0.5 * np.sin(t), 0.0, 0.5 * np.cos(t)), lam=lam, n_photons=1.0,
+                         direction=(np.sin(t), 0.0, -np.cos(t)))
+
+
 MIE_CASES = {"A": (0.6, 0.3, 0.7), "B": (0.0, 0.8, 0.5), "water": (0.95, 0.3, 1.0)}  # (g_f, g_b, r)
 MIE_R, MIE_L, MIE_N, MIE_E = 1.0, 20.0, 1.33, 3.0  # sphere radius (m), MIEHG (m), RINDEX, photon energy (eV)
 
