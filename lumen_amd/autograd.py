@@ -36,6 +36,8 @@ class _Run:
     n_grad: int
     value_index: object  # torch long: the packed values that carry a slot
     slot_of_value: object  # torch long: their slots
+    first_index: object = None  # torch long: the first packed value of each slot (a key read by several tables
+    first_slot: object = None  # is packed once per table: its gradient goes to one of its copies)
     health: object = None
     adj_health: object = None
     kernel_ms: float = float("nan")
@@ -109,7 +111,7 @@ def _function():
             run = ctx.run
             gs = adjoint(run, values.detach().contiguous(), g)
             gv = torch.zeros(values.shape, dtype=torch.float64, device=values.device)
-            gv[run.value_index] = gs[run.slot_of_value]
+            gv[run.first_index] = gs[run.first_slot]  # the copies of a slot share one value of the key
             return gv.to(values.dtype), None
 
     return PhotonLoop
@@ -147,7 +149,7 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
                              "a re-emission's spectrum, yield and delay are calibrated by value (finite differences)")
     pk = packing.pack(det, p, grad_keys=keys, edge_rows=edge_rows)
     values = eng.upload(pk.values)
-    slots, value_index, slot_of_value = _slot_tensors(eng, pk.slots)
+    slots, value_index, slot_of_value, first_index, first_slot = _slot_tensors(eng, pk.slots)
     # scatter the tensors into the values, differentiably: each key's slots in order are its table's values
     starts = {k: (s0, n) for k, s0, n in pk.grad_keys}
     parts = []
@@ -170,7 +172,7 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
         ref_values = torch.as_tensor(packing.pack(det, reference, edge_rows=edge_rows).values, device=eng.device)
     rows, n_ev = source_rows(det, source, samples, first_id)
     run = _Run(eng, rows, n_ev, int(seed) & 0xFFFFFFFFFFFFFFFF, controls, ref_values, slots, pk.n_grad,
-               value_index, slot_of_value)
+               value_index, slot_of_value, first_index, first_slot)
     tally = _PL.apply(vals, run)
     health = run.health
     light = light_of(eng, tally, lambda: health.detach().cpu().numpy(),
@@ -182,16 +184,21 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
 
 
 def _slot_tensors(eng, slots: np.ndarray):
-    """Return (slots int32, the packed values' indices that carry a slot, their slots) on the device, the previous
-    ones again when the slots are the same (uploads from pageable memory wait for the stream)."""  # noqa: D205, D209
+    """Return (slots int32, the packed values' indices that carry a slot, their slots, the first such value of each
+    slot, its slot) on the device, the previous ones again when the slots are the same (uploads from pageable memory
+    wait for the stream)."""  # noqa: D205, D209
     import torch
 
     last = getattr(eng, "_slot_upload", None)
     if last is not None and np.array_equal(last[0], slots):
         return last[1]
     idx = np.flatnonzero(slots >= 0)
+    _, first = np.unique(slots[idx], return_index=True)
+    first = idx[first]
     out = (torch.as_tensor(slots, device=eng.device), torch.as_tensor(idx, device=eng.device, dtype=torch.long),
-           torch.as_tensor(slots[idx], device=eng.device, dtype=torch.long))
+           torch.as_tensor(slots[idx], device=eng.device, dtype=torch.long),
+           torch.as_tensor(first, device=eng.device, dtype=torch.long),
+           torch.as_tensor(slots[first], device=eng.device, dtype=torch.long))
     eng._slot_upload = (slots.copy(), out)
     return out
 

@@ -99,6 +99,10 @@ struct Args {
   // the diffusion jump (null: off): per medium (lo, h) and (n, offset into jclear, -1 no grid) as float4 pairs
   const float4 *__restrict__ jrec;
   const float *__restrict__ jclear;
+  // the sensor-crediting rules (offsets into L, 0: none): a (C x C) table whose non-zero pairs credit a crossing's
+  // detected share to the sensor of the volume left (an exit sensor), and per class the row whose detected share a
+  // photon emitted inside a sensor volume of that class deposits at its emission (-1: none)
+  int left_at, born_at;
 };
 
 struct Photon {
@@ -663,6 +667,34 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   rng.draw4(p.vertex, 1, ul);  // lobe (3), polarisation
   float3 er0 = p.er, ei0 = p.ei;
   pol_fill(p.d, er0, ei0, ul[3]);
+  if (a.born_at && p.vertex == 0) {  // emitted inside a sensor volume that counts by volume (step.py:574-612 `_born`)
+    const int br = a.L[a.born_at + p.cls], sensor = g.vinfo[p.vol].w;
+    if (br >= 0 && sensor >= 0) {  // the entry's detected share, deposited at the emission; the photon ends there
+      const SurfEval se = surface_eval(a.L, a.P, br, p.lam, p.d, mul3(-1.0f, p.d), 1.0f, 1.0f, er0, ei0);
+      const float dep_q = p.w * se.detect;
+      const float theta = a.Kt > 1 ? cathode_theta(g, p.vol, p.x) : 0.0f;
+      if (!ADJ) {
+        deposit<F>(a, p, acc, sensor, dep_q, p.t, theta);
+        acc.surf += p.w * p.w0;
+      } else {
+        const float D = dep_q * p.w0 * deposit_cotangent(a, b, p, sensor, theta);
+        if (D != 0.0f) {
+          p.hasdep = true;
+          if (p.pass == 1) {
+            float dsh[N_COL], ddet[N_COL];
+            const int *Rw = a.L + a.L[L_ROWS_AT] + ROW_WORDS * br;
+            surface_dlogs(Rw, a.P, HC / p.lam, -1, dsh, ddet);
+            add_columns(sg, Rw, a.P, b->slots, HC / p.lam, ddet, D);
+          }
+          p.Dsum += D;
+        }
+      }
+      p.w = p.wr = 0.0f;
+      p.vertex += 1;
+      acc.vertices += 1;
+      return false;
+    }
+  }
   medium_cache<ADJ, F>(a, p, b);
   const MediumAt ml = p.ml, mr = REF ? p.mr : p.ml;
   // the total rates, summed in Geant4's channel order (Rayleigh, Mie, WLS)
@@ -727,9 +759,11 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
     o = choose6(pprop, u[2]);
     // the deposit: the expected detection at the crossing (step.py:522-524)
     const float dep_q = p.w * reach_live * att * sl.detect;
-    const int sensor = g.vinfo[cr.vol_after].w;
+    // the sensor entered, or across an exit sensor's border the region left (step.py:544-547 `_sensor_dep`)
+    const int vs = (a.left_at && a.L[a.left_at + cb * a.L[L_CLASSES] + ca]) ? p.vol : cr.vol_after;
+    const int sensor = g.vinfo[vs].w;
     if (sensor >= 0 && !cont) {
-      const float theta = a.Kt > 1 ? cathode_theta(g, cr.vol_after, fma3(dist, p.d, p.x)) : 0.0f;
+      const float theta = a.Kt > 1 ? cathode_theta(g, vs, fma3(dist, p.d, p.x)) : 0.0f;
       if (!ADJ) {
         deposit<F>(a, p, acc, sensor, dep_q, p.t + dist / v_g, theta);
       } else {
@@ -1211,6 +1245,7 @@ struct lh_model_s {
   int occ_per[OCC_CACHE] = {};
   int occ_next = 0;
   unsigned char *scratch = nullptr;  // this device's copy of the static scratch array (lh_trace_static)
+  int left_at = 0, born_at = 0;      // the layout's sensor-crediting tables (words 16, 17; 0: none)
 };
 
 static thread_local std::string g_err;
@@ -1241,7 +1276,7 @@ LH_API const char *lh_last_error(void) { return g_err.c_str(); }
 
 LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t layout_words, lh_model *out) {
   if (!G || !layout || !out) return fail(LH_ERR_ARG, "lh_model_create: null argument");
-  if (layout_words < 16 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
+  if (layout_words < 18 || layout[0] != 0x4C484F31) return fail(LH_ERR_LAYOUT, "lh_model_create: not a packed layout");
   if (layout[14] < 1 || layout[14] > 4096 || layout[15] < 0 || (long long)layout[5] * layout[14] > (1 << 20))
     return fail(LH_ERR_LAYOUT, "lh_model_create: the charge cells (wavelength x theta knots) are out of range");
   {  // the time histogram's grid
@@ -1318,6 +1353,18 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     for (int c = 0; c < nc * nc; ++c)
       if (layout[layout[8] + c] < -1 || layout[layout[8] + c] >= nr)
         return fail(LH_ERR_LAYOUT, "lh_model_create: a surface row index is out of range");
+    // the sensor-crediting tables: exit-sensor flags per class pair (0 or 1), a born row per class (-1 or a row)
+    if (layout[16] < 0 || layout[17] < 0 || (layout[16] > 0 && (long long)layout[16] + (long long)nc * nc > layout_words) ||
+        (layout[17] > 0 && (long long)layout[17] + nc > layout_words))
+      return fail(LH_ERR_LAYOUT, "lh_model_create: the sensor-crediting tables are out of range");
+    if (layout[16] > 0)
+      for (int c = 0; c < nc * nc; ++c)
+        if (layout[layout[16] + c] != 0 && layout[layout[16] + c] != 1)
+          return fail(LH_ERR_LAYOUT, "lh_model_create: an exit-sensor flag is not 0 or 1");
+    if (layout[17] > 0)
+      for (int c = 0; c < nc; ++c)
+        if (layout[layout[17] + c] < -1 || layout[layout[17] + c] >= nr)
+          return fail(LH_ERR_LAYOUT, "lh_model_create: a born row is out of range");
     for (int r = 0; r < nr; ++r) {
       const int32_t *R = layout + layout[9] + ROW_WORDS * r;
       if (R[0] < 0 || R[0] > 2) return fail(LH_ERR_LAYOUT, "lh_model_create: an unknown surface model");
@@ -1540,6 +1587,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   m->Kt = layout[14];
   m->K = m->Kl * m->Kt;
   m->NT = layout[11];
+  m->left_at = layout[16];
+  m->born_at = layout[17];
   memcpy(&m->t0, &layout[12], 4);
   memcpy(&m->dt, &layout[13], 4);
   m->tally_stride = 3LL * m->S + (long long)m->S * m->K + (m->NT > 0 ? (long long)m->S * (m->NT + 4) : 0);
@@ -1773,6 +1822,8 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.src_end = m->src_end;
   a.jrec = c.jump && m->jrec ? m->jrec : nullptr;
   a.jclear = m->jclear;
+  a.left_at = m->left_at;
+  a.born_at = m->born_at;
   return LH_OK;
 }
 

@@ -33,7 +33,7 @@ enum { FLAG_NORMAL = 1, FLAG_FAR_OPAQUE = 2, FLAG_EDGE = 4, FLAG_PER_NORMAL = 8,
 enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_POLISH, N_COL };
 // the packed layout's header words (lumen_amd/pack.py)
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
-       L_ROWS_AT, L_QE_TABLE, L_NT, L_T0, L_DT, L_CE_KNOTS, L_CE_TABLE };
+       L_ROWS_AT, L_QE_TABLE, L_NT, L_T0, L_DT, L_CE_KNOTS, L_CE_TABLE, L_LEFT_AT, L_BORN_AT };
 constexpr int MEDIUM_WORDS = 25;
 enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4, MED_MIE = 8 };  // a medium's flag word (its word 8)
 constexpr float MIE_G_MAX = 0.999f;  // the largest |g| of a Mie lobe

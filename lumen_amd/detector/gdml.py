@@ -63,12 +63,28 @@ class Recipe:
 
 @dataclass
 class Group:
-    """A sensor group of a recipe: rule ``border_into`` a logical volume ``lv``; ``values``: a sensor value set."""
+    """A sensor group of a recipe: the placements of a logical volume ``lv``; ``values``: a sensor value set.
+
+    ``rule`` (the reference's ``rules/sensors.py``): "border_into", counted at the optical surfaces bordering into
+    the volume (their EFFICIENCY); "any_entry", a sensitive volume: every photon entering it from any side is absorbed
+    and counted with ``entry_efficiency`` (1 with a value set, whose efficiency acts after tracking; else the setting,
+    default 1), and with ``count`` "volume" (Geant4's sensitive-detector convention, the default) a photon emitted
+    inside it is counted at its emission with the entry's detection ("entry": it leaves uncounted).  Leaving a sensor
+    volume through a border surface that detects (an exit sensor) credits the volume left, whatever the rule.
+    """
 
     name: str
     lv: str
     values: str | None = None
     rule: str = "border_into"
+    count: str = "volume"
+    entry_efficiency: float | None = None
+
+    def __post_init__(self):
+        if self.rule not in ("border_into", "any_entry"):
+            raise ValueError(f"sensor group {self.name!r}: rule {self.rule!r} is not border_into or any_entry")
+        if self.count not in ("volume", "entry"):
+            raise ValueError(f"sensor group {self.name!r}: count {self.count!r} is not volume or entry")
 
 
 @dataclass
@@ -336,18 +352,23 @@ def load(path: str, recipe: Recipe) -> Detector:
                 values[f"surfaces/{sname}/{prop}"] = _table(d, ref)
         if s["type"] == "dielectric_metal" and s["finish"] == "ground" and s["model"] == "unified":
             values[f"surfaces/{sname}/sigma_alpha"] = np.float64(s["value"])
-    border_in = {}
+    border_in, border_out = {}, {}
     for sname, a, b in d.borders:
         if a in pv_index and b in pv_index:
             border_in.setdefault(pv_index[b], []).append((sname, pv_index[a]))
+            border_out.setdefault(pv_index[a], []).append(sname)
     skin_lv = {lv: sname for sname, lv in d.skins}
+    group_of_lv = {g.lv: g for g in recipe.groups}
     cls_key, cls_names, vol_cls = {}, [], []
-    for i, (pvname, lv, _, _, _) in enumerate(placed):
-        key = (d.volumes[lv]["material"], tuple(sorted(s for s, _ in border_in.get(i, []))), skin_lv.get(lv))
+    for i, (pvname, lv, _, _, _) in enumerate(placed):  # a class: medium, borders in and out, skin, sensor group
+        g = group_of_lv.get(lv)
+        key = (d.volumes[lv]["material"], tuple(sorted(s for s, _ in border_in.get(i, []))), skin_lv.get(lv),
+               None if g is None else g.name, tuple(sorted(border_out.get(i, []))))
         if key not in cls_key:
             cls_key[key] = len(cls_names)
             cls_names.append(f"{key[0]}" + (f"|{','.join(key[1])}" if key[1] else "")
-                             + (f"|skin:{key[2]}" if key[2] else ""))
+                             + (f"|skin:{key[2]}" if key[2] else "") + (f"|sensor:{key[3]}" if key[3] else "")
+                             + (f"|out:{','.join(key[4])}" if key[4] else ""))
         vol_cls.append(cls_key[key])
     out_cls = len(cls_names)
     cls_names.append("OUT")
@@ -387,6 +408,19 @@ def load(path: str, recipe: Recipe) -> Detector:
             if p >= 0:
                 put(vol_cls[p], vol_cls[i], r)
                 put(vol_cls[i], vol_cls[p], r)
+    born_row = np.full(C, -1, np.int32)
+    for g in recipe.groups:  # a sensitive volume: every entry counted (a counting row), a photon born inside too
+        if g.rule != "any_entry":
+            continue
+        eff = 1.0 if g.values is not None else (1.0 if g.entry_efficiency is None else float(g.entry_efficiency))
+        r = len(srows)
+        srows.append(SurfaceRow(f"{g.name}/entry", "COUNTING", {"eff": eff}))
+        for sc in {vol_cls[i] for i, p in enumerate(placed) if p[1] == g.lv}:
+            for a in range(C):
+                if a != sc and a != out_cls and class_medium[a] >= 0:
+                    row_of[a, sc] = r
+            if g.count == "volume":
+                born_row[sc] = r
     vols = [navarrays.Volume(p[0], kind_of[d.volumes[p[1]]["solid"]], p[2], p[3], p[4], vol_cls[i])
             for i, p in enumerate(placed)]
     nav, _ = navarrays.cached(vols, kinds, out_cls)
@@ -414,13 +448,20 @@ def load(path: str, recipe: Recipe) -> Detector:
             values[f"sensors/{g.name}/efficiency"] = np.float64(vs.get("efficiency", 1.0))
             eff = f"sensors/{g.name}/efficiency"
         groups.append(SensorGroup(g.name, np.asarray(ids, np.int64), qe, eff, ce))
+    credit_left = np.zeros((C, C), bool)
+    for sname, a, b in d.borders:  # an exit sensor: a detecting border out of a sensor volume credits the volume left
+        if a in pv_index and b in pv_index and sensor_of_vol[pv_index[a]] >= 0:
+            row = srows[srow_of[sname]]
+            if "eff" in row.params() and row._fixed("eff") != 0.0:
+                credit_left[vol_cls[pv_index[a]], vol_cls[pv_index[b]]] = True
     origin = {k: "detector file" for k in values}
     for g in groups:
         for k in (g.qe, g.efficiency, g.ce):
             if k is not None:
                 origin[k] = "sensor model defaults"
     return Detector(recipe.name, nav, media, class_medium, cls_names, srows, row_of, sensor_of_vol, names, groups,
-                    Params(values, origin))
+                    Params(values, origin), credit_left=credit_left if credit_left.any() else None,
+                    born_row=born_row if (born_row >= 0).any() else None)
 
 
 def _path(placed, i):

@@ -289,6 +289,12 @@ class Detector:
     values: Params
     source_box: tuple = None
     time_grid: "TimeGrid | None" = None  # the time histogram's static nodes (None: no time histogram)
+    # the sensor-crediting rules (the reference's ``build/lookup.py:413-417`` ``deposit_pre``, ``born_pair``):
+    # credit_left (C, C) bool, a detected share crossing c_from -> c_to goes to the sensor of the volume left (an
+    # exit sensor: a border out of a sensor volume whose surface detects); born_row (C,) int32, the row whose detected
+    # share a photon emitted inside a sensor volume of the class deposits at its emission, where it ends (-1: none)
+    credit_left: np.ndarray | None = None
+    born_row: np.ndarray | None = None
 
     def timed(self, t0: float, dt: float, nt: int) -> "Detector":
         """Return this detector with a time histogram on the nodes t0 + j dt, j < nt (ns)."""

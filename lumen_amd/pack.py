@@ -9,7 +9,10 @@ differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` decl
     [0] magic  [1] media  [2] classes  [3] rows  [4] sensors  [5] qe knots  [6] media at  [7] class_medium at
     [8] row_of at  [9] rows at  [10] qe table (offset of its knots in values)  [11] time nodes nt (0: no time
     histogram)  [12] t0 (ns, float32 bits)  [13] dt (ns, float32 bits)  [14] theta knots (1: no theta axis)
-    [15] ce table (offset of its theta knots, radians, in values)
+    [15] ce table (offset of its theta knots, radians, in values)  [16] exit-sensor flags at (C x C, 1: a detected
+    share crossing that class pair is credited to the sensor of the volume left; 0: no table)  [17] born rows at (per
+    class the row whose detected share a photon emitted inside a sensor volume of the class deposits at its emission,
+    -1 none; 0: no table)
     The charge cells per sensor are (qe knots) x (theta knots), row-major (the wavelength's knot major).
     media:  per medium 25 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
             (1 opaque, 2 a wavelength shifter, 4 its exponential time profile, 8 Mie scattering); (offset, knots)
@@ -33,7 +36,7 @@ from .detector.model import COLUMNS, FLAG_EDGE, Detector
 from .params import Params, Table
 
 MAGIC = 0x4C484F31
-HEADER = 16
+HEADER = 18
 MEDIUM_WORDS = 25
 MED_OPAQUE, MED_WLS, MED_WLS_EXP, MED_MIE = 1, 2, 4, 8
 MIE_G_MAX = 0.999  # the largest |g| of a Mie lobe (the reference's rules/choices.py MIE_G_MAX)
@@ -213,7 +216,7 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
     f32bits = lambda x: int(np.array([x], np.float32).view(np.int32)[0])  # noqa: E731
     L = [MAGIC, nm, nc, nr, det.n_sensors, n_qe, 0, 0, 0, 0, qe_off,
          0 if tg is None else tg.nt, 0 if tg is None else f32bits(tg.t0), 0 if tg is None else f32bits(tg.dt),
-         n_ce, ce_off]
+         n_ce, ce_off, 0, 0]
     L[6] = len(L)
     L += list(med.ravel())
     L[7] = len(L)
@@ -222,5 +225,13 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
     L += list(np.asarray(det.row_of, np.int32).ravel())
     L[9] = len(L)
     L += list(rows.ravel())
+    left = getattr(det, "credit_left", None)
+    if left is not None and np.any(left):
+        L[16] = len(L)
+        L += list(np.asarray(left, np.int32).ravel())
+    born = getattr(det, "born_row", None)
+    if born is not None and np.any(np.asarray(born) >= 0):
+        L[17] = len(L)
+        L += list(np.asarray(born, np.int32))
     return Packed(np.asarray(b.vals, np.float32), np.asarray(L, np.int32), np.asarray(b.slots, np.int32),
                   b.n_grad, b.grad_map)

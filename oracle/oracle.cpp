@@ -1293,6 +1293,70 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
     er0 = R(std::cos(TWO_PI * ul[3])) * t1 + R(std::sin(TWO_PI * ul[3])) * t2;
     ei0 = v3<R>(R(0.0), R(0.0), R(0.0));
   }
+  double *tal = &T.t[(size_t)p.event * r.stride];
+  // a record at a sensor: the tallies, the charge cells (theta about volume `vol`'s axis at the point xc), the time
+  // histogram and the photon's moment
+  auto deposit = [&](int sensor, int vol, V3<R> xc, double v, double time) {
+    T.h[LH_H_RECORDS] += 1;
+    if (v != 0.0) {
+      tal[sensor] += v;
+      tal[r.S + sensor] += v * time;
+      // the charge cells: hat weights on the QE wavelength knots and on the CE knots of the cathode position
+      // angle (readout/grid.py:161-185 `hats`, `cells`, `charge_cells`; sensors/features.py `theta`: the polar
+      // angle of the crossing point about the sensor volume's +z axis), the wavelength's knot major
+      int il = 0, it = 0;
+      double fl = 0.0, ft = 0.0, accept = 1.0;
+      if (r.Kl > 1) {
+        r.live.tab(r.live.L[10], r.Kl, p.lam, &il, &fl);
+        accept *= r.live.tab(r.live.L[10], r.Kl, p.lam);
+      }
+      if (r.Kt > 1) {
+        V3<R> xl, dl;
+        to_local(s, vol, xc, p.d, xl, dl);
+        const double th = std::atan2(std::sqrt(val(xl.x) * val(xl.x) + val(xl.y) * val(xl.y)), val(xl.z));
+        r.live.tab(r.live.L[15], r.Kt, th, &it, &ft);
+        accept *= r.live.tab(r.live.L[15], r.Kt, th);
+      }
+      double *cc = &tal[3 * r.S + (long long)sensor * r.K + il * r.Kt + it];
+      cc[0] += v * (1 - fl) * (1 - ft);
+      if (fl > 0) cc[r.Kt] += v * fl * (1 - ft);
+      if (ft > 0) cc[1] += v * (1 - fl) * ft;
+      if (fl > 0 && ft > 0) cc[r.Kt + 1] += v * fl * ft;
+      if (r.nt > 0) {  // the time histogram (readout/grid.py:190-225 `_time_weights`, `time_cells`)
+        const double dq = v * accept;  // weighted by the acceptance: the detected light's time shape
+        const double x = std::min(std::max((time - r.t0) / r.dt, -4.0), r.nt + 4.0);
+        const double jf = std::floor(x), g = x - jf;
+        const int j = (int)jf;
+        const double wk[4] = {-g * (g - 1) * (g - 2) / 6, (g + 1) * (g - 1) * (g - 2) / 2,
+                              -(g + 1) * g * (g - 2) / 2, (g + 1) * g * (g - 1) / 6};
+        double late = 0, early = 0;
+        for (int k = 0; k < 4; ++k) {
+          const int nn = j - 1 + k;
+          if (nn >= r.nt) late += wk[k];
+          else if (nn < 0) early += wk[k];
+          else tal[r.time_at + (long long)sensor * r.nt + nn] += dq * wk[k];
+        }
+        double *ov = tal + r.time_at + (long long)r.S * r.nt + 4LL * sensor;
+        ov[0] += dq * late;
+        ov[1] += dq * late * time;
+        ov[2] += dq * early;
+        ov[3] += dq * early * time;
+      }
+      mom.add(sensor, v);
+    }
+  };
+  if (r.live.L[17] && p.vertex == 0) {  // emitted inside a sensor volume that counts by volume (step.py:574-612)
+    const int br = r.live.L[r.live.L[17] + p.cls], sensor = s.sensor[p.vol];
+    if (br >= 0 && sensor >= 0) {  // the entry's detected share at the emission (normal incidence, indices 1)
+      const Surf<R> se = surface_eval<R>(r.live, br, p.lam, p.d, R(-1.0) * p.d, 1.0, 1.0, er0, ei0);
+      deposit(sensor, p.vol, p.x, val(p.w * se.detect) * p.w0, val(p.t));
+      T.h[LH_H_ABSORBED_SURFACE] += val(p.w) * p.w0;
+      p.w = p.wr = R(0.0);
+      p.vertex += 1;
+      T.h[LH_H_VERTICES] += 1;
+      return false;
+    }
+  }
   const int m = r.live.cls_medium(p.cls);
   const Med<R> ml = medium_at<R>(r.live, m, p.lam);
   const Med<R> mr = r.has_ref ? medium_at<R>(r.ref, m, p.lam) : ml;
@@ -1314,7 +1378,6 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   R pdraw[NOUT], pprop[NOUT];
   int o = -1;
   const bool at_if = hit && row >= 0;
-  double *tal = &T.t[(size_t)p.event * r.stride];
   if (at_if) {
     const int m2 = r.live.cls_medium(ca);
     sl = surface_eval<R>(r.live, row, p.lam, p.d, cr.n, val(ml.n), index_at(r.live, m2, p.lam), er0, ei0);
@@ -1339,58 +1402,11 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
     for (int k = 0; k < NOUT; ++k) tot += pdraw[k];
     for (int k = 0; k < NOUT; ++k) pprop[k] = tot > R(0.0) ? pdraw[k] / tot : R(0.0);
     o = choose<R>(pprop, NOUT, u[2]);
-    const int sensor = s.sensor[cr.vol_after];
-    if (sensor >= 0 && !cont) {
-      const double q = val(p.w * reach_live * att * sl.detect), v = q * p.w0;
-      const double time = val(p.t + dist / v_g);
-      T.h[LH_H_RECORDS] += 1;
-      if (v != 0.0) {
-        tal[sensor] += v;
-        tal[r.S + sensor] += v * time;
-        // the charge cells: hat weights on the QE wavelength knots and on the CE knots of the cathode position
-        // angle (readout/grid.py:161-185 `hats`, `cells`, `charge_cells`; sensors/features.py `theta`: the polar
-        // angle of the crossing point about the sensor volume's +z axis), the wavelength's knot major
-        int il = 0, it = 0;
-        double fl = 0.0, ft = 0.0, accept = 1.0;
-        if (r.Kl > 1) {
-          r.live.tab(r.live.L[10], r.Kl, p.lam, &il, &fl);
-          accept *= r.live.tab(r.live.L[10], r.Kl, p.lam);
-        }
-        if (r.Kt > 1) {
-          V3<R> xl, dl;
-          to_local(s, cr.vol_after, p.x + dist * p.d, p.d, xl, dl);
-          const double th = std::atan2(std::sqrt(val(xl.x) * val(xl.x) + val(xl.y) * val(xl.y)), val(xl.z));
-          r.live.tab(r.live.L[15], r.Kt, th, &it, &ft);
-          accept *= r.live.tab(r.live.L[15], r.Kt, th);
-        }
-        double *cc = &tal[3 * r.S + (long long)sensor * r.K + il * r.Kt + it];
-        cc[0] += v * (1 - fl) * (1 - ft);
-        if (fl > 0) cc[r.Kt] += v * fl * (1 - ft);
-        if (ft > 0) cc[1] += v * (1 - fl) * ft;
-        if (fl > 0 && ft > 0) cc[r.Kt + 1] += v * fl * ft;
-        if (r.nt > 0) {  // the time histogram (readout/grid.py:190-225 `_time_weights`, `time_cells`)
-          const double dq = v * accept;  // weighted by the acceptance: the detected light's time shape
-          const double x = std::min(std::max((time - r.t0) / r.dt, -4.0), r.nt + 4.0);
-          const double jf = std::floor(x), g = x - jf;
-          const int j = (int)jf;
-          const double wk[4] = {-g * (g - 1) * (g - 2) / 6, (g + 1) * (g - 1) * (g - 2) / 2,
-                                -(g + 1) * g * (g - 2) / 2, (g + 1) * g * (g - 1) / 6};
-          double late = 0, early = 0;
-          for (int k = 0; k < 4; ++k) {
-            const int nn = j - 1 + k;
-            if (nn >= r.nt) late += wk[k];
-            else if (nn < 0) early += wk[k];
-            else tal[r.time_at + (long long)sensor * r.nt + nn] += dq * wk[k];
-          }
-          double *ov = tal + r.time_at + (long long)r.S * r.nt + 4LL * sensor;
-          ov[0] += dq * late;
-          ov[1] += dq * late * time;
-          ov[2] += dq * early;
-          ov[3] += dq * early * time;
-        }
-        mom.add(sensor, v);
-      }
-    }
+    // the sensor entered, or across an exit sensor's border the region left (step.py:544-547 `_sensor_dep`)
+    const int vs = (r.live.L[16] && r.live.L[r.live.L[16] + cb * r.live.L[2] + ca]) ? p.vol : cr.vol_after;
+    const int sensor = s.sensor[vs];
+    if (sensor >= 0 && !cont)
+      deposit(sensor, vs, p.x + dist * p.d, val(p.w * reach_live * att * sl.detect) * p.w0, val(p.t + dist / v_g));
   }
   bool ok;
   R w_new = R(0.0), wr_new = R(0.0);

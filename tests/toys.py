@@ -74,6 +74,46 @@ def sphere_toy(a=1.0, surface="SPECULAR_METAL", R=0.5, L_abs=10.0, L_ray=None, n
                     ["shell"], [SensorGroup("pmt", np.arange(1), ce=ce_key)], Params(values))
 
 
+def sensor_volume_toy(rule="born", eff=0.6, R=0.3, T=0.4, n=1.33, L_abs=None, keyed=False):
+    """Return a sensor sphere (radius 0.3 m, index n) in a water world (an orb of 1.2 m, index n), sensor 0.
+
+    ``rule`` "born": the sphere is a sensitive volume counting by volume (every entry a COUNTING row with ``eff``, and
+    a photon emitted inside counted at its emission with that detection: ``born_row``); "entry": the same volume
+    counting only entries (a photon born inside leaves uncounted); "exit": the sphere's border out into the water is a
+    DIELECTRIC_RT row (R, T, eff) that detects, credited to the sphere (the volume left: ``credit_left``); "exit_off":
+    that border without the crediting rule (the volume entered, water, holds no sensor).  ``keyed``: eff as the key
+    surfaces/sensor/EFFICIENCY (shared by both rows)."""  # noqa: D205, D209
+    kinds = [solids.orb(1.2, name="world"), solids.orb(0.3, name="sensor")]
+    vols = [navarrays.Volume("world", 0, np.eye(3), np.zeros(3), -1, 0),
+            navarrays.Volume("world/sensor", 1, np.eye(3), np.zeros(3), 0, 1)]
+    nav = navarrays.build(vols, kinds, outside_cls=2)
+    values = {"materials/Water/RINDEX": Table(E_KNOTS, np.array([n, n])),
+              "materials/SensLike/RINDEX": Table(E_KNOTS, np.array([n, n]))}
+    absorption = None
+    if L_abs is not None:
+        values["materials/Water/ABSLENGTH"] = Table(E_KNOTS, np.array([L_abs, L_abs]))
+        absorption = "materials/Water/ABSLENGTH"
+    media = [Medium("Water", "materials/Water/RINDEX", absorption), Medium("SensLike", "materials/SensLike/RINDEX")]
+    if keyed:
+        values["surfaces/sensor/EFFICIENCY"] = np.float64(eff)
+        eff = "surfaces/sensor/EFFICIENCY"
+    rows = [SurfaceRow("absorber", "ABSORBER"), SurfaceRow("dielectric", "DIELECTRIC"),
+            SurfaceRow("entry", "COUNTING", {"eff": eff}),
+            SurfaceRow("exit", "DIELECTRIC_RT", {"R": R, "T": T, "eff": eff})]
+    row_of = np.full((3, 3), 0, np.int32)
+    row_of[0, 1] = 2 if rule in ("born", "entry") else 1
+    row_of[1, 0] = 3 if rule.startswith("exit") else 1
+    credit_left = born_row = None
+    if rule == "born":
+        born_row = np.array([-1, 2, -1], np.int32)
+    if rule == "exit":
+        credit_left = np.zeros((3, 3), bool)
+        credit_left[1, 0] = True
+    return Detector(f"sensor_volume_{rule}", nav, media, np.array([0, 1, -1], np.int32), ["Water", "SensLike", "OUT"],
+                    rows, row_of, np.array([-1, 0, -1], np.int32), ["sphere"], [SensorGroup("s", np.arange(1))],
+                    Params(values), credit_left=credit_left, born_row=born_row)
+
+
 def glisur_toy(kind="dielectric", n1=1.0, n2=1.5, polish=0.3, R=1.0, T=0.0, eff=0.0, keyed=False):
     """Return the GLISUR ring counter: a flat interface z = 0 between an upper medium (index n1, a world orb of 1.2
     m) and a lower box (|x|, |y| < 1.1 m, -1.1 < z < 0; index n2, or opaque for ``kind`` "metal") with a GLISUR row
