@@ -91,6 +91,7 @@ def cpu_baseline(det, src, seconds=10.0):
                 break
             n *= 4
         n = int(min(1 << 24, max(n, n * seconds / max(dt, 1e-3))))
+        log(f"cpu baseline {'fp32' if fp32 else 'fp64'}: {n} photons on {threads} threads")
         t0 = time.perf_counter()
         run.trace(src, n, seed=100, threads=threads, bvh=True, fp32=fp32)
         dt = time.perf_counter() - t0

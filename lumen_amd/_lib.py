@@ -39,7 +39,7 @@ class LhGeometry(C.Structure):
         ("gR", _f), ("g0", _i), ("g1", _i), ("lo", _f), ("hi", _f), ("skip", _i), ("start", _i), ("count", _i),
         ("items", _i), ("leaf_typ", _i), ("leaf_p0", _i), ("leaf_np", _i), ("leaf_o", _f), ("leaf_q", _f),
         ("leaf_A", _f), ("leaf_b", _f), ("leaf_c", _f), ("plane_n", _f), ("plane_h", _f), ("kind_l0", _i),
-        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float), ("skin", _i)]
+        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float), ("skin", _i), ("touch", _i)]
 
 
 class LhSource(C.Structure):
@@ -138,6 +138,8 @@ def geometry_struct(nav, sensor_of_vol, reach: float):
         prog_op=i32(G["op"]), prog_arg=i32(G["arg"]))
     if "skin" in h:  # the daughter-free cores (absent from the reference's arrays: no acceleration)
         keep["skin"] = i32(h["skin"])
+    if "touch" in h:  # the volumes a sibling's box touches (absent: every exit looks for a sibling past the face)
+        keep["touch"] = i32(h["touch"])
     n_nodes = len(keep["skip"]) if nav.stats.get("groups", 1) else 0
     g = LhGeometry(n_vol=nav.n, n_nodes=n_nodes, n_items=len(keep["items"]),
                    n_groups=int(nav.stats.get("groups", len(keep["g0"]))), n_leaves=n_leaves,

@@ -709,7 +709,9 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const int cb = p.cls;
   const int ca = hit ? g.vinfo[cr.vol_after].z : -1;
   int row = hit ? a.L[a.L[L_ROW_OF_AT] + cb * a.L[L_CLASSES] + ca] : -1;
-  const bool cont = hit && zero && row < 0;  // a zero-length crossing no surface joins: a straight pass
+  // a crossing no surface joins, of zero length or between two volumes of one class (the same medium and surfaces,
+  // which the reference's navigator does not stop at: lookup.py:26-31): a straight pass
+  const bool cont = hit && (zero || ca == cb) && row < 0;
   if (cont) row = 0;
   const bool scatter = mu_r > 0.0f && s < dist;
   const float reach_live = __expf(-mu * dist), reach_ref = REF ? __expf(-mu_r * dist) : reach_live;
@@ -1408,7 +1410,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     }
   }
   for (int v = 0; v <= n; ++v) {
-    vinfo[v] = make_int4(v < n ? G->kind[v] : 0, v < n ? G->parent[v] : n, G->cls[v], G->sensor[v]);
+    vinfo[v] = make_int4(v < n ? (G->touch ? G->touch[v] & 1 : 1) : 0, v < n ? G->parent[v] : n, G->cls[v],
+                         G->sensor[v]);
     vgrp[v] = make_int2(G->vg0[v], G->vg1[v]);
   }
   for (int gi = 0; gi < G->n_groups; ++gi) {

@@ -35,7 +35,7 @@ extern __shared__ double lds_cells[];
 struct Geo {
   int nvol;
   const float4 *__restrict__ xf;    // 3 per volume: rows of R^T with the offset -R^T t (world -> local)
-  const int4 *__restrict__ vinfo;   // per navigator id: kind, parent, cls, sensor
+  const int4 *__restrict__ vinfo;   // per navigator id: touch flag (a sibling's box comes near), parent, cls, sensor
   const int2 *__restrict__ vgrp;    // per navigator id: frame groups [x, y)
   const float4 *__restrict__ gxf;   // 3 per group: rows of gR^T (w: 1 if identity)
   const int2 *__restrict__ grange;  // per group: BVH nodes [x, y)
@@ -446,11 +446,8 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
         if (lo > 0.0f && lo < hi && lo < ch.t) { ch.t = lo; ch.leaf = 0; ch.face = iv.flo[p]; }
       }
     }
-    if (exit_query && ch.leaf < 0) {  // the start is outside its own solid (rounding): leave it where it is
-      ch.t = 0.0f;
-      ch.leaf = 0;
-      ch.face = 0;
-    }
+    if (exit_query && ch.leaf < 0)  // the start is outside its own solid (rounding): leave it where it is, the
+      return Change{0.0f, 0, 0, mul3(-1.0f, d)};  // normal facing the photon (no face is met)
     if (ch.leaf >= 0) ch.n = leaf_normal<TOR>(g, K.x, ch.face, o, d, ch.t);
     return ch;
   }
@@ -474,7 +471,7 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
     return run_program(g, K.z, K.w, bits);
   };
   const bool start = member(0.0f);
-  if (exit_query && !start) return Change{0.0f, 0, 0, leaf_normal<TOR>(g, K.x, 0, o, d, 0.0f)};
+  if (exit_query && !start) return Change{0.0f, 0, 0, mul3(-1.0f, d)};  // outside its own solid: leave it now
   if (!exit_query && start) return ch;  // inside a daughter already (rounding): no entry ahead
   float cur = 0.0f;
   for (int it = 0; it < 4 * nl + 5; ++it) {
@@ -494,7 +491,7 @@ __device__ inline Change first_change(const Geo &g, int4 K, int vol, float3 o, f
       return Change{nxt, nl_, nf, leaf_normal<TOR>(g, K.x + nl_, nf, o, d, nxt)};
     cur = nxt;
   }
-  if (exit_query) return Change{0.0f, 0, 0, leaf_normal<TOR>(g, K.x, 0, o, d, 0.0f)};
+  if (exit_query) return Change{0.0f, 0, 0, mul3(-1.0f, d)};
   return ch;
 }
 
@@ -551,6 +548,9 @@ __device__ inline void core_interval(const Geo &g, int vol, float3 xl, float3 dl
     hi = t1;
   }
 }
+
+template <bool TOR>
+__device__ inline int locate(const Geo &g, float3 x, int start = -1);
 
 // `scale`: the coordinate scale of the ray that put the start on face `last` (its crossing's rounding is within
 // that ray's band, not the start's own: a crossing near the origin reached from far away)
@@ -642,6 +642,33 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
     best.n = unit3(best.n);
     if (dot3(best.n, d) > 0.0f) best.n = mul3(-1.0f, best.n);
   }
+  // an exit into the mother at a point another volume holds (touching siblings, a daughter flush with its
+  // mother's face, whose exit is the mother's too): the crossing goes on into the volume holding the point K_ON
+  // bands past the face (the reference's relocation past the face met, navigate.py:526-555), the crossing point
+  // moved there and the face behind the photon
+  // An entry whose point past the face a daughter of the entered volume holds (a daughter flush with its mother's
+  // face) goes on into that daughter alike: faces within K_ON bands are crossed as one
+  // Looked for only where it can happen: past an exit, outside the mother (a daughter flush with its face), or in
+  // a sibling whose box comes near (vinfo.x, the touch flag); past an entry, in the entered volume's daughters
+  if (best.vol_after >= 0 && best.vol_after < g.nvol) {
+    const float step = K_ON * band_at(maxabs3(x) + best.t);
+    const float3 xp = fma3(best.t + step, d, x);
+    int v2 = best.vol_after;
+    if (best.own) {
+      float3 xl, dl;
+      to_local(g, best.vol_after, xp, d, xl, dl);
+      if (!solid_inside<TOR>(g, g.vkind[best.vol_after], xl)) v2 = locate<TOR>(g, xp);
+      else if (g.vinfo[vol].x & 1) v2 = locate<TOR>(g, xp, best.vol_after);
+    } else {
+      const int2 gr = g.vgrp[best.vol_after];
+      if (gr.x < gr.y) v2 = locate<TOR>(g, xp, best.vol_after);
+    }
+    if (v2 != vol && v2 != best.vol_after) {
+      best.vol_after = v2;
+      best.t += step;
+      best.srf = -1;
+    }
+  }
   return best;
 }
 
@@ -663,10 +690,10 @@ __device__ inline bool core_holds(const Geo &g, int vol, float3 xl) {
   return !(C0.w < HUGEF && y.x * y.x + y.y * y.y > C0.w);
 }
 
-// the deepest navigator id holding x (the outside when no volume does)
+// the deepest navigator id holding x (the outside when no volume does), searched from `start` (-1: the outside)
 template <bool TOR>
-__device__ inline int locate(const Geo &g, float3 x) {
-  int cur = g.nvol;
+__device__ inline int locate(const Geo &g, float3 x, int start) {
+  int cur = start < 0 ? g.nvol : start;
   for (int depth = 0; depth < 64; ++depth) {
     if (cur < g.nvol) {  // inside the current volume's daughter-free core: no daughter to look for
       float3 xl, dl;

@@ -11,11 +11,13 @@ lie inside their mother (no overlaps, so no ownership links), and daughter group
 that share a rotation their world boxes fit badly, in that rotation's frame (``frame_groups``, ``gR``).
 Navigator ids: volumes 0..n-1, the outside n.  ``R``, ``t`` map a volume's frame to the world: x_w = R x + t.
 
-One array is this build's own, an acceleration the reference's format does not hold (absent, nothing changes):
-``host["skin"]`` (n+1,), per volume the CSG leaf of its daughter-free core, or -1.  A volume whose solid is one
-convex leaf (a box, a full tube, an orb) and whose daughters all sit near its boundary (sensors on a tank's walls)
-gets the leaf shrunk by the depth D of its deepest daughter plus a margin: no daughter's box meets the core, so a ray
-skips every BVH box over the part of its path inside the core.  ``skin_core`` says how D is found.
+Two arrays are this build's own, accelerations the reference's format does not hold (absent, nothing changes but
+the speed): ``host["skin"]`` (n+1,), per volume the CSG leaf of its daughter-free core, or -1.  A volume whose solid
+is one convex leaf (a box, a full tube, an orb) and whose daughters all sit near its boundary (sensors on a tank's
+walls) gets the leaf shrunk by the depth D of its deepest daughter plus a margin: no daughter's box meets the core,
+so a ray skips every BVH box over the part of its path inside the core.  ``skin_core`` says how D is found.
+``host["touch"]`` (n+1,), 1 where a sibling's box lies within the navigator's relocation step of the volume's box
+(``touch_flags``): only such a volume's exits look for a sibling holding the point past the face.
 """
 
 from __future__ import annotations
@@ -83,7 +85,7 @@ class NavArrays:
     def reference_record(self) -> "_Record":
         """Return the arrays as the reference's ``NavArrays`` record (``geometry/navigator_build.py:51-90``)."""
         n, T = int(self.n), self.csg["T"]
-        host = dict(self.host)
+        host = {k: v for k, v in self.host.items() if k != "touch"}  # derived from the boxes: load recomputes it
         fdt = np.float64 if self.precision == "float64" else np.float32
         for k in ("V0", "V1", "V2", "N"):  # no meshes: one placeholder row, as the reference writes for none
             host.setdefault(k, np.zeros((0, 3), fdt))
@@ -126,13 +128,18 @@ class NavArrays:
             csg = {"T": dict(c["T"]), "K": dict(c["K"]), "G": dict(c["G"]), "empty_leaf": int(c["empty_leaf"])}
             host = {k: v for k, v in f["host"].items() if k not in ("V0", "V1", "V2", "N", "cstart", "cn", "cmesh",
                                                                         "tchunk")}
-            return NavArrays(host, csg, int(f["n"]), tuple(f["names"]), np.asarray(f["lo_w"]), np.asarray(f["hi_w"]),
+            lo_w, hi_w = np.asarray(f["lo_w"]), np.asarray(f["hi_w"])
+            if "touch" not in host:  # derived from the boxes: an entry written elsewhere lacks it
+                host["touch"] = touch_flags(host["parent"], lo_w, hi_w)
+            return NavArrays(host, csg, int(f["n"]), tuple(f["names"]), lo_w, hi_w,
                              0, str(f.get("precision", "float32")), dict(f.get("stats", {})))
         if isinstance(meta, dict) and meta.get("format") == "lumen-navarrays/1":
             host = {k[5:]: v for k, v in store.items() if k.startswith("host/")}
             csg = {tab: {k[len(tab) + 5:]: v for k, v in store.items() if k.startswith(f"csg/{tab}/")}
                    for tab in ("T", "K", "G")}
             csg["empty_leaf"] = meta.get("empty_leaf", len(csg["T"]["typ"]) - 1)
+            if "touch" not in host:
+                host["touch"] = touch_flags(host["parent"], store["lo_w"], store["hi_w"])
             return NavArrays(host, csg, int(meta["n"]), tuple(meta["names"]), store["lo_w"], store["hi_w"],
                              int(meta.get("n_mesh", 0)), meta.get("precision", "float32"), meta.get("stats", {}))
         raise ValueError(f"navigator cache entry {path}: an unknown layout")
@@ -447,10 +454,36 @@ def build(volumes: list[Volume], kinds: list[Solid], outside_cls: int, precision
         ub=np.concatenate([[0], np.cumsum(nl)]).astype(np.int32),
         skin=np.where(core_of >= 0, csg["K"]["l0"][np.maximum(core_of, 0)], -1).astype(np.int32),
     )
+    host["touch"] = touch_flags(parent, lo_w, hi_w)
     framed_groups = int(sum(not np.array_equal(r, np.eye(3)) for r in gR))
     stats = dict(volumes=n, kinds=len(kinds), groups=G, framed=framed_groups, bvh_depth=depth,
                  max_daughters=max(len(k) for k in kids), skin_cores=len(cores))
     return NavArrays(host, csg, n, tuple(v.name for v in volumes), lo_w, hi_w, 0, precision, stats)
+
+
+def touch_flags(parent, lo_w, hi_w) -> np.ndarray:
+    """Return (n+1,) int32: 1 where a sibling's world box lies within the relocation step of the volume's own, K_ON
+    float32 bands at the geometry's coordinate scale (the largest coordinate plus the diagonal: the scale of any
+    crossing): an exit from such a volume may land in that sibling.  Boxes hold their solids, so a volume whose box
+    no sibling's box comes near can meet no sibling past its faces."""  # noqa: D205, D209
+    n = len(lo_w)
+    out = np.zeros(n + 1, np.int32)
+    if n == 0:
+        return out
+    scale = float(np.max(np.abs(np.concatenate([lo_w, hi_w])))) + float(np.linalg.norm(hi_w.max(0) - lo_w.min(0)))
+    tol = 4.0 * (8.0 * 2.0**-23 * scale + 1e-9)
+    par = np.asarray(parent)[:n]
+    for p in np.unique(par):
+        ks = np.flatnonzero(par == p)
+        if len(ks) < 2:
+            continue
+        lo, hi = lo_w[ks] - tol, hi_w[ks] + tol
+        for a in range(0, len(ks), 512):  # pairwise box overlaps, in slabs
+            b = slice(a, a + 512)
+            ov = np.all((lo[b, None, :] <= hi[None, :, :]) & (lo[None, :, :] <= hi[b, None, :]), axis=2)
+            ov[np.arange(ov.shape[0]), np.arange(a, a + ov.shape[0])] = False  # not itself
+            out[ks[b]] |= ov.any(axis=1).astype(np.int32)
+    return out
 
 
 # ---------------------------------------------------------------------------------------------- the build cache

@@ -503,6 +503,8 @@ template <class R> V3<R> normal_at(const Scene &s, int v, int leaf, int face, V3
 }
 
 // the next visible crossing: the own exit, then every daughter (a daughter wins a tie)
+inline int locate_from(const Scene &s, int cur, V3<double> x);
+
 // `scale`: the coordinate scale of the ray that put the start on face `last` (float32: its band bounds the start's
 // rounding off that face)
 template <class R>
@@ -562,7 +564,23 @@ Cross<R> query(const Scene &s, V3<R> x, V3<R> d, int vol, long long last, R tmax
   }
   if (bv >= 0) {
     best.srf = ((long long)bv << 10) | (bl << 5) | bf;
-    best.n = normal_at(s, bv, bl, bf, x, d, best.t);
+    // an own exit at t = 0 is a start outside its own solid (rounding): no face is met, the normal faces the photon
+    best.n = (own && best.t == R(0.0)) ? R(-1.0) * d : normal_at(s, bv, bl, bf, x, d, best.t);
+  }
+  // an exit into the mother at a point another volume holds (touching siblings, a daughter flush with its mother's
+  // face, whose exit is the mother's too): the crossing goes on into the volume holding the point a little past the
+  // face (src/lumen/geometry/navigate.py:526-555: K_ON float32 bands at the crossing's scale, so faces closer than
+  // that are crossed as one, in FP64 as in float32), the crossing point moved there
+  // (an entry whose point past the face a daughter of the entered volume holds goes on into that daughter alike)
+  if (best.vol_after >= 0 && best.vol_after < s.n) {
+    const double step = 4.0 * (8.0 * 1.1920928955078125e-7 * (xa + (double)val(best.t)) + 1e-9);
+    const V3<double> xp = v3(val(x.x), val(x.y), val(x.z)) + ((double)val(best.t) + step) * v3(val(d.x), val(d.y), val(d.z));
+    const int v2 = locate_from(s, own ? s.n : best.vol_after, xp);
+    if (v2 != vol && v2 != best.vol_after) {
+      best.vol_after = v2;
+      best.t = best.t + R(step);
+      best.srf = -1;
+    }
   }
   return best;
 }
@@ -594,8 +612,7 @@ inline bool inside(const Scene &s, int v, V3<double> x) {
   return run_prog(K, in);
 }
 
-inline int locate(const Scene &s, V3<double> x) {
-  int cur = s.n;
+inline int locate_from(const Scene &s, int cur, V3<double> x) {
   for (;;) {
     int found = -1;
     for (int u : s.kids[cur])
@@ -604,6 +621,8 @@ inline int locate(const Scene &s, V3<double> x) {
     cur = found;
   }
 }
+
+inline int locate(const Scene &s, V3<double> x) { return locate_from(s, s.n, x); }
 
 // ------------------------------------------------------------------------------------------ optics (FP64)
 struct Optics {
@@ -1368,7 +1387,7 @@ template <class R> bool step(const Run &r, Photon<R> &p, Tally &T, Moments<R> &m
   const bool zero = dist <= R(ZERO_STEP);
   const int cb = p.cls, ca = hit ? s.cls[cr.vol_after] : -1;
   int row = hit ? r.live.row_of(cb, ca) : -1;
-  const bool cont = hit && zero && row < 0;
+  const bool cont = hit && (zero || ca == cb) && row < 0;  // no surface: zero length, or one class on both sides
   if (cont) row = 0;
   const bool scatter = mu_r > R(0.0) && s_free < dist;
   const R reach_live = EXP(-mu * dist), reach_ref = r.has_ref ? EXP(-mu_r * dist) : reach_live;
