@@ -48,8 +48,15 @@ struct TabPos {
 __device__ __forceinline__ TabPos tab_pos(const float *__restrict__ P, int off, int n, float x) {
   if (n <= 1 || x <= P[off]) return TabPos{0, 0.0f};
   if (x >= P[off + n - 1]) return TabPos{n - 1, 0.0f};
-  int i = 0;
-  while (i < n - 2 && P[off + i + 1] <= x) ++i;
+  // the last knot i <= n - 2 with E_i <= x, by bisection: ceil(log2(n - 1)) loads whatever x, so the lanes of a
+  // wavefront stay in step (a scan from the first knot runs as long as its farthest lane)
+  int i = 0, cnt = n - 1;
+  while (cnt > 1) {
+    const int half = cnt >> 1;
+    const bool up = P[off + i + half] <= x;
+    i = up ? i + half : i;
+    cnt = up ? cnt - half : half;
+  }
   const float e0 = P[off + i], e1 = P[off + i + 1];
   return TabPos{i, (x - e0) / (e1 - e0)};
 }
