@@ -1377,8 +1377,10 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
       const int32_t *M = layout + layout[6] + MEDIUM_WORDS * mi;
       if (!(M[8] & MED_OPAQUE) && (M[1] < 1 || M[3] < 1))
         return fail(LH_ERR_LAYOUT, "lh_model_create: a medium without RINDEX");
-      if (M[8] & ~(MED_OPAQUE | MED_WLS | MED_WLS_EXP | MED_MIE))
+      if (M[8] & ~(MED_OPAQUE | MED_WLS | MED_WLS_EXP | MED_MIE | MED_SAME_ABS | MED_SAME_RAY))
         return fail(LH_ERR_LAYOUT, "lh_model_create: unknown medium flags");
+      if (((M[8] & MED_SAME_ABS) && M[5] != M[1]) || ((M[8] & MED_SAME_RAY) && M[7] != M[1]))
+        return fail(LH_ERR_LAYOUT, "lh_model_create: a table on RINDEX's knots with another knot count");
       for (int j = 0; j < 8; j += 2)
         if (M[j] < 0 || M[j + 1] < 0) return fail(LH_ERR_LAYOUT, "lh_model_create: a negative table offset");
       if (M[8] & MED_WLS) {  // WLSABSLENGTH, WLSCOMPONENT (two knots or more), the yield and the time constant

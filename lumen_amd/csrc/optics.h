@@ -35,7 +35,8 @@ enum { COL_R = 0, COL_T, COL_EFF, COL_SPIKE, COL_LOBE, COL_BACK, COL_SIGMA, COL_
 enum { L_MAGIC = 0, L_MEDIA, L_CLASSES, L_ROWS, L_SENSORS, L_QE, L_MEDIA_AT, L_CLASS_MEDIUM_AT, L_ROW_OF_AT,
        L_ROWS_AT, L_QE_TABLE, L_NT, L_T0, L_DT, L_CE_KNOTS, L_CE_TABLE, L_LEFT_AT, L_BORN_AT };
 constexpr int MEDIUM_WORDS = 25;
-enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4, MED_MIE = 8 };  // a medium's flag word (its word 8)
+// a medium's flag word (its word 8); MED_SAME_ABS / _RAY: ABSLENGTH / RAYLEIGH tabulated on RINDEX's knots
+enum { MED_OPAQUE = 1, MED_WLS = 2, MED_WLS_EXP = 4, MED_MIE = 8, MED_SAME_ABS = 16, MED_SAME_RAY = 32 };
 constexpr float MIE_G_MAX = 0.999f;  // the largest |g| of a Mie lobe
 constexpr int ROW_WORDS = 2 + 2 * N_COL;
 
@@ -83,13 +84,14 @@ __device__ inline MediumAt medium_at(const int *__restrict__ L, const float *__r
   const int *M = L + L[L_MEDIA_AT] + MEDIUM_WORDS * m;
   if (M[8] & MED_OPAQUE) return r;
   const float E = HC / lam;
-  r.n = tab(P, M[0], M[1], E);
+  const TabPos pn = tab_pos(P, M[0], M[1], E);
+  r.n = tab_at(P, M[0], M[1], pn);
   const TabPos pg = tab_pos(P, M[2], M[3], E);  // Geant4 interpolates the group velocity (1 / n_g) in energy
   const float g0 = 1.0f / P[M[2] + M[3] + pg.i];
   const float inv = pg.f == 0.0f ? g0 : fmaf(pg.f, 1.0f / P[M[2] + M[3] + pg.i + 1] - g0, g0);
   r.ng = 1.0f / inv;
-  if (M[5] > 0) r.mu_abs = rate_of(tab(P, M[4], M[5], E));
-  if (M[7] > 0) r.mu_ray = rate_of(tab(P, M[6], M[7], E));
+  if (M[5] > 0) r.mu_abs = rate_of((M[8] & MED_SAME_ABS) ? tab_at(P, M[4], M[5], pn) : tab(P, M[4], M[5], E));
+  if (M[7] > 0) r.mu_ray = rate_of((M[8] & MED_SAME_RAY) ? tab_at(P, M[6], M[7], pn) : tab(P, M[6], M[7], E));
   if (M[8] & MED_MIE) r.mu_mie = rate_of(tab(P, M[17], M[18], E));
   if (M[8] & MED_WLS) r.mu_wls = rate_of(tab(P, M[9], M[10], E));
   return r;

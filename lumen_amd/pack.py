@@ -15,7 +15,8 @@ differentiates (-1: none).  Layout, in int32 words (``include/lumen_hip.h`` decl
     -1 none; 0: no table)
     The charge cells per sensor are (qe knots) x (theta knots), row-major (the wavelength's knot major).
     media:  per medium 25 words: (offset, knots) of RINDEX, of the group index, of ABSLENGTH, of RAYLEIGH; flags
-            (1 opaque, 2 a wavelength shifter, 4 its exponential time profile, 8 Mie scattering); (offset, knots)
+            (1 opaque, 2 a wavelength shifter, 4 its exponential time profile, 8 Mie scattering, 16 ABSLENGTH and 32
+            RAYLEIGH on RINDEX's knots); (offset, knots)
             of WLSABSLENGTH, of WLSCOMPONENT (emission intensity over energy), of WLSMEANNUMBERPHOTONS, of
             WLSTIMECONSTANT; of MIEHG, of MIEHG_FORWARD, of MIEHG_BACKWARD, of MIEHG_FORWARD_RATIO
     rows:   per row 18 words: model, flags, then (offset, knots) of R, T, eff, p_spike, p_lobe, p_back, sigma_alpha,
@@ -39,6 +40,7 @@ MAGIC = 0x4C484F31
 HEADER = 18
 MEDIUM_WORDS = 25
 MED_OPAQUE, MED_WLS, MED_WLS_EXP, MED_MIE = 1, 2, 4, 8
+MED_SAME_ABS, MED_SAME_RAY = 16, 32  # ABSLENGTH (RAYLEIGH) tabulated on RINDEX's knots
 MIE_G_MAX = 0.999  # the largest |g| of a Mie lobe (the reference's rules/choices.py MIE_G_MAX)
 ROW_WORDS = 2 + 2 * len(COLUMNS)
 
@@ -161,6 +163,11 @@ def pack(det: Detector, params: Params | None = None, grad_keys=(), edge_rows="s
             med[i, 6:8] = b.table(rt.energy, water_rayleigh(rt.energy, rt.value))
         elif m.rayleigh is not None:
             med[i, 6:8] = b.value(p, m.rayleigh)
+        knots = np.float32(b.vals[med[i, 0]:med[i, 0] + med[i, 1]])
+        for w, flag in ((4, MED_SAME_ABS), (6, MED_SAME_RAY)):  # on RINDEX's knots: one knot search serves both
+            o, k = med[i, w], med[i, w + 1]
+            if k == med[i, 1] and k > 1 and np.array_equal(np.float32(b.vals[o:o + k]), knots):
+                med[i, 8] |= flag
         if m.wls_length is not None:
             med[i, 8] |= MED_WLS | (MED_WLS_EXP if m.wls_profile == "exponential" else 0)
             med[i, 9:11] = b.value(p, m.wls_length)
