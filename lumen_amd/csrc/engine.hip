@@ -96,6 +96,8 @@ struct Args {
   unsigned long long *__restrict__ queue;
   const float *__restrict__ ut;  // the unified facet tables (unified_tables.cpp)
   int lds_tally;                 // the block's tallies live in LDS (flushed once at exit), else global atomics
+  int lds_values, lds_values_off;  // the forward kernel stages the packed values (lds_values floats) in LDS at
+                                   // float offset lds_values_off of lds_cells: its table searches read LDS
   // the diffusion jump (null: off): per medium (lo, h) and (n, offset into jclear, -1 no grid) as float4 pairs
   const float4 *__restrict__ jrec;
   const float *__restrict__ jclear;
@@ -1069,11 +1071,18 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 
 // dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
 template <unsigned F>
-__global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a, int n_lds) {
-  if (a.lds_tally)
+__global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a0, int n_lds) {
+  if (a0.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
-  stage_nodes(a.g);
+  stage_nodes(a0.g);
+  Args a = a0;
+  if (a0.lds_values) {  // the packed values in LDS: every table search of the loop reads them there
+    float *v = reinterpret_cast<float *>(lds_cells) + a0.lds_values_off;
+    for (int i = threadIdx.x; i < a0.lds_values; i += blockDim.x) v[i] = a0.P[i];
+    if (a0.Pr == a0.P) a.Pr = v;
+    a.P = v;
+  }
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -1823,6 +1832,7 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.queue = m->queue;
   a.ut = m->ut;
   a.lds_tally = 0;
+  a.lds_values = a.lds_values_off = 0;
   a.src = m->src;
   a.src_end = m->src_end;
   a.jrec = c.jump && m->jrec ? m->jrec : nullptr;
@@ -1918,11 +1928,21 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
   a.g.lds_nodes = nodes ? 1 : 0;
   const size_t off = cells ? cells_al : 0;
   a.g.lds_node_off = (int)(off / 16);
+  size_t end = off + (nodes ? node_bytes : 0);
+  // the forward kernel's packed values after them, when they fit (their table searches then read LDS)
+  const size_t value_bytes = ((size_t)m->values_needed * sizeof(float) + 15) / 16 * 16;
+  a.lds_values = 0;
+  a.lds_values_off = 0;
+  if (cells_optional && value_bytes > 0 && end + value_bytes <= budget && !(ctl && ctl->nodes_global == 1)) {
+    a.lds_values = (int)m->values_needed;
+    a.lds_values_off = (int)(end / sizeof(float));
+    end += value_bytes;
+  }
   m->last_launch[3] = a.lds_tally;
   m->last_launch[4] = a.g.lds_nodes;
   m->last_launch[5] = (int32_t)budget;
   m->last_launch[6] = m->lds_per_cu;
-  return off + (nodes ? node_bytes : 0);
+  return end;
 }
 
 static void launch_adjoint(lh_model m, const Args &a, const AdjArgs &b, const lh_controls *ctl, size_t bytes,
