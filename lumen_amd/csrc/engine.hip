@@ -98,6 +98,8 @@ struct Args {
   int lds_tally;                 // the block's tallies live in LDS (flushed once at exit), else global atomics
   int lds_values, lds_values_off;  // the forward kernel stages the packed values (lds_values floats) in LDS at
                                    // float offset lds_values_off of lds_cells: its table searches read LDS
+  int lds_layout, lds_layout_off;  // ... the layout (words), at that word offset
+  int lds_items, lds_ixf_off, lds_ikind_off;  // ... the BVH items' transforms and kinds (float4 / int4 offsets)
   // the diffusion jump (null: off): per medium (lo, h) and (n, offset into jclear, -1 no grid) as float4 pairs
   const float4 *__restrict__ jrec;
   const float *__restrict__ jclear;
@@ -1083,6 +1085,19 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
     if (a0.Pr == a0.P) a.Pr = v;
     a.P = v;
   }
+  if (a0.lds_layout) {  // the layout: rows, media, the class-pair table
+    int *l = reinterpret_cast<int *>(lds_cells) + a0.lds_layout_off;
+    for (int i = threadIdx.x; i < a0.lds_layout; i += blockDim.x) l[i] = a0.L[i];
+    a.L = l;
+  }
+  if (a0.lds_items) {  // the BVH items' transforms and kinds, read by every item test of the walk
+    float4 *x = reinterpret_cast<float4 *>(lds_cells) + a0.lds_ixf_off;
+    int4 *k = reinterpret_cast<int4 *>(lds_cells) + a0.lds_ikind_off;
+    for (int i = threadIdx.x; i < 3 * a0.lds_items; i += blockDim.x) x[i] = a0.g.ixf[i];
+    for (int i = threadIdx.x; i < a0.lds_items; i += blockDim.x) k[i] = a0.g.ikind[i];
+    a.g.ixf = x;
+    a.g.ikind = k;
+  }
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -1257,6 +1272,7 @@ struct lh_model_s {
   int occ_next = 0;
   unsigned char *scratch = nullptr;  // this device's copy of the static scratch array (lh_trace_static)
   int left_at = 0, born_at = 0;      // the layout's sensor-crediting tables (words 16, 17; 0: none)
+  int n_items = 0;                   // the BVH items (the forward kernel may stage their transforms in LDS)
 };
 
 static thread_local std::string g_err;
@@ -1564,6 +1580,7 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.vkind = (const int4 *)(buf + o_vkind);
   g.ixf = (const float4 *)(buf + o_ixf);
   g.ikind = (const int4 *)(buf + o_ikind);
+  m->n_items = G->n_items;
   g.vgrp = (const int2 *)(buf + o_vgrp);
   g.grange = (const int2 *)(buf + o_grange);
   g.items = (const int *)(buf + o_items);
@@ -1832,7 +1849,8 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.queue = m->queue;
   a.ut = m->ut;
   a.lds_tally = 0;
-  a.lds_values = a.lds_values_off = 0;
+  a.lds_values = a.lds_values_off = a.lds_layout = a.lds_layout_off = 0;
+  a.lds_items = a.lds_ixf_off = a.lds_ikind_off = 0;
   a.src = m->src;
   a.src_end = m->src_end;
   a.jrec = c.jump && m->jrec ? m->jrec : nullptr;
@@ -1930,13 +1948,28 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
   a.g.lds_node_off = (int)(off / 16);
   size_t end = off + (nodes ? node_bytes : 0);
   // the forward kernel's packed values after them, when they fit (their table searches then read LDS)
+  // then the layout, then the BVH items' transforms and kinds: each where it still fits, in that order
   const size_t value_bytes = ((size_t)m->values_needed * sizeof(float) + 15) / 16 * 16;
-  a.lds_values = 0;
-  a.lds_values_off = 0;
-  if (cells_optional && value_bytes > 0 && end + value_bytes <= budget && !(ctl && ctl->nodes_global == 1)) {
+  const size_t layout_bytes = ((size_t)m->layout_words * sizeof(int) + 15) / 16 * 16;
+  const size_t item_bytes = (size_t)m->n_items * 64;  // 3 float4 rows + an int4 kind
+  a.lds_values = a.lds_values_off = a.lds_layout = a.lds_layout_off = 0;
+  a.lds_items = a.lds_ixf_off = a.lds_ikind_off = 0;
+  const bool stage = cells_optional && !(ctl && ctl->nodes_global == 1);
+  if (stage && value_bytes > 0 && end + value_bytes <= budget) {
     a.lds_values = (int)m->values_needed;
     a.lds_values_off = (int)(end / sizeof(float));
     end += value_bytes;
+  }
+  if (stage && layout_bytes > 0 && end + layout_bytes <= budget) {
+    a.lds_layout = m->layout_words;
+    a.lds_layout_off = (int)(end / sizeof(int));
+    end += layout_bytes;
+  }
+  if (stage && m->n_items > 0 && end + item_bytes <= budget) {
+    a.lds_items = m->n_items;
+    a.lds_ixf_off = (int)(end / 16);
+    a.lds_ikind_off = (int)(end / 16) + 3 * m->n_items;
+    end += item_bytes;
   }
   m->last_launch[3] = a.lds_tally;
   m->last_launch[4] = a.g.lds_nodes;
