@@ -1930,8 +1930,11 @@ static int grid_of(lh_model m, const lh_controls *ctl, const void *fn, size_t ld
 static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t cells_bytes, bool cells_optional,
                         const lh_controls *ctl) {
   const int per_regs = std::max(1, grid_of(m, nullptr, fn, 0, block) / std::max(m->cus, 1));
-  const size_t budget =
-      std::min<size_t>((size_t)m->lds_per_block, (size_t)m->lds_per_cu / (size_t)per_regs) / 16 * 16;
+  // the kernel's own static LDS (its __shared__ arrays) comes out of the same per-block limit
+  hipFuncAttributes attr{};
+  const size_t fixed = hipFuncGetAttributes(&attr, fn) == hipSuccess ? (size_t)attr.sharedSizeBytes : 1024;
+  const size_t limit = std::min<size_t>((size_t)m->lds_per_block, (size_t)m->lds_per_cu / (size_t)per_regs);
+  const size_t budget = (limit > fixed + 16 ? limit - fixed - 16 : 0) / 16 * 16;
   const size_t node_bytes = (size_t)m->g.n_nodes * 32;
   const bool want_nodes = m->g.n_nodes > 0 && !(ctl && ctl->nodes_global == 1);
   const bool want_cells = cells_bytes > 0 && !(cells_optional && ctl && ctl->tally_global == 1);
