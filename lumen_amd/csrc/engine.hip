@@ -1072,25 +1072,23 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 }
 
 // dynamic LDS of a launch: [the block's tallies (lds_tally) or gradient accumulators (the adjoint)][BVH nodes]
-template <unsigned F>
-__global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a0, int n_lds) {
-  if (a0.lds_tally)
-    for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
-  if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
-  stage_nodes(a0.g);
+// the launch's arguments with the tables the launch placed in LDS copied there and pointed at (every thread of the
+// block calls it; a barrier follows): the packed values (every table search reads them), the layout (rows, media,
+// the class-pair table), the BVH items' transforms and kinds (every item test of the walk)
+__device__ inline Args stage_tables(const Args &a0) {
   Args a = a0;
-  if (a0.lds_values) {  // the packed values in LDS: every table search of the loop reads them there
+  if (a0.lds_values) {
     float *v = reinterpret_cast<float *>(lds_cells) + a0.lds_values_off;
     for (int i = threadIdx.x; i < a0.lds_values; i += blockDim.x) v[i] = a0.P[i];
     if (a0.Pr == a0.P) a.Pr = v;
     a.P = v;
   }
-  if (a0.lds_layout) {  // the layout: rows, media, the class-pair table
+  if (a0.lds_layout) {
     int *l = reinterpret_cast<int *>(lds_cells) + a0.lds_layout_off;
     for (int i = threadIdx.x; i < a0.lds_layout; i += blockDim.x) l[i] = a0.L[i];
     a.L = l;
   }
-  if (a0.lds_items) {  // the BVH items' transforms and kinds, read by every item test of the walk
+  if (a0.lds_items) {
     float4 *x = reinterpret_cast<float4 *>(lds_cells) + a0.lds_ixf_off;
     int4 *k = reinterpret_cast<int4 *>(lds_cells) + a0.lds_ikind_off;
     for (int i = threadIdx.x; i < 3 * a0.lds_items; i += blockDim.x) x[i] = a0.g.ixf[i];
@@ -1098,6 +1096,16 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
     a.g.ixf = x;
     a.g.ikind = k;
   }
+  return a;
+}
+
+template <unsigned F>
+__global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a0, int n_lds) {
+  if (a0.lds_tally)
+    for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
+  if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
+  stage_nodes(a0.g);
+  const Args a = stage_tables(a0);
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -1154,11 +1162,12 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
 template <unsigned F>
 __global__ void __launch_bounds__(BLOCK)
     __attribute__((amdgpu_waves_per_eu(((F) & (F_REF | F_CHAN)) ? 1 : LH_ADJ_WAVES_PER_EU)))
-    adjoint_kernel(Args a, AdjArgs b) {
+    adjoint_kernel(Args a0, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
-  stage_nodes(a.g);
+  stage_nodes(a0.g);
+  const Args a = stage_tables(a0);
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -1957,7 +1966,7 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
   const size_t item_bytes = (size_t)m->n_items * 64;  // 3 float4 rows + an int4 kind
   a.lds_values = a.lds_values_off = a.lds_layout = a.lds_layout_off = 0;
   a.lds_items = a.lds_ixf_off = a.lds_ikind_off = 0;
-  const bool stage = cells_optional && !(ctl && ctl->nodes_global == 1);
+  const bool stage = !(ctl && ctl->nodes_global == 1);
   if (stage && value_bytes > 0 && end + value_bytes <= budget) {
     a.lds_values = (int)m->values_needed;
     a.lds_values_off = (int)(end / sizeof(float));
