@@ -141,6 +141,10 @@ struct Acc {  // per-lane health accumulators of the frequent fates, flushed onc
 // held all launch long for them would crowd the step's working set
 enum { RF_LOST = 0, RF_ESC, RF_DCUT, RF_ABAND, RF_TRUNC, RF_PUSHED, RF_SPILLS, RF_N };
 __shared__ float s_rare[RF_N];
+// per wavefront of the block, the volume its last located emission started in: a new photon whose start lies in that
+// volume's daughter-free core starts in that volume (no daughter holds a point of the core), with no descent from the
+// top (-1: none yet; lanes overwrite it as they like, any value is only a hint that the core test checks)
+__shared__ int s_hint[32];
 
 __device__ __forceinline__ void rare_add(int cell, float v) {
   if (v != 0.0f) atomicAdd(&s_rare[cell], v);
@@ -351,7 +355,18 @@ __device__ inline bool emit(const Args &a, long long gi, Photon &p, Acc &acc) {
   }
   p.w = 1.0f;
   p.wr = 1.0f;
-  p.vol = locate<TOR>(a.g, p.x);
+  int vol = -1;
+  const int hint = s_hint[threadIdx.x >> 6];
+  if (hint >= 0 && hint < a.g.nvol) {
+    float3 xl, dl;
+    to_local(a.g, hint, p.x, mk3(0.f, 0.f, 1.f), xl, dl);
+    if (core_holds(a.g, hint, xl)) vol = hint;
+  }
+  if (vol < 0) {
+    vol = locate<TOR>(a.g, p.x);
+    s_hint[threadIdx.x >> 6] = vol;
+  }
+  p.vol = vol;
   p.cls = a.g.vinfo[p.vol].z;
   p.srf = -1;
   p.xs = 0.0f;
@@ -1104,6 +1119,7 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
   if (a0.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
+  if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
   const Args a = stage_tables(a0);
   __syncthreads();
@@ -1166,6 +1182,7 @@ __global__ void __launch_bounds__(BLOCK)
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
+  if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
   const Args a = stage_tables(a0);
   __syncthreads();
