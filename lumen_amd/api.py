@@ -108,6 +108,49 @@ class Light:
     late: Any = None  # (S, 2) (q, q t) beyond the last node
     early: Any = None  # (S, 2) (q, q t) before the first node
     charge_axes: tuple = (1, 1)  # (wavelength knots, theta knots) of the charge cells
+    sensor_group: Any = None  # a group's view (``groups``): its SensorGroup, whose keys ``readout.pe`` reads
+    group_specs: list = field(default_factory=list)  # the detector's SensorGroups (``groups`` is built from them)
+
+    @property
+    def groups(self) -> dict | None:
+        """{group name: this Light restricted to the group's sensors} (the reference's ``Light.groups``,
+        ``api.py:515-537``; None for a detector without groups).  Each view's sensor axis holds the group's sensors
+        in the group's order and carries the group (``sensor_group``): the readout takes ``(light.group(name),
+        params)``.  Built on first read and kept."""  # noqa: D205, D209
+        if not self.group_specs:
+            return None
+        got = self.__dict__.get("_groups")
+        if got is None:
+            got = {g.name: self._view(g) for g in self.group_specs}
+            self.__dict__["_groups"] = got
+        return got
+
+    def group(self, name: str) -> "Light":
+        """Return the view of sensor group ``name`` (KeyError naming the groups there are)."""
+        gs = self.groups
+        if not gs or name not in gs:
+            raise KeyError(f"Light: no sensor group {name!r}; the groups are {list(gs or ())}")
+        return gs[name]
+
+    def _view(self, g) -> "Light":
+        """Return this Light on group g's sensors (gathered copies along the sensor axis)."""
+        import torch
+
+        s = int(self.arrivals.shape[-1])
+        idx = np.asarray(g.sensors, np.int64)
+        if idx.size and (idx.min() < 0 or idx.max() >= s):
+            raise ValueError(f"Light: group {g.name!r} names sensors outside 0..{s - 1}")
+        ix = torch.as_tensor(idx, device=self.arrivals.device)
+        ev = self.arrivals.dim() - 1  # 1 with an event axis
+
+        def cut(t):
+            return None if t is None else t.index_select(ev, ix)
+
+        return Light(cut(self.arrivals), cut(self.arrivals_error), cut(self.mean_arrival_time), cut(self.charge),
+                     cut(self.arrival2), self.__dict__.get("_health"), self.samples,
+                     provenance=dict(self.provenance, group=g.name), kernel_ms=self.__dict__.get("_kernel_ms"),
+                     time=cut(self.time), late=cut(self.late), early=cut(self.early), charge_axes=self.charge_axes,
+                     sensor_group=g)
 
     def to_dlpack(self, name="arrivals"):
         """Return a DLPack capsule of one output (device memory: ``jax.dlpack.from_dlpack`` takes it)."""
@@ -232,6 +275,13 @@ class Engine:
             _lib.check(_lib.lib().lh_kernel_ms_of(self.handle, int(launch), _lib.C.byref(ms)), "lh_kernel_ms_of")
         return ms.value
 
+    def kernel_ms_sum(self, launches: range) -> float:
+        """Return the summed kernel time (ms) of a range of launches; NaN once the oldest is past the model's last
+        256 (its timing overwritten)."""  # noqa: D205, D209
+        if self.launches() - launches.start > 256:
+            return float("nan")
+        return sum(self.kernel_ms(i) for i in launches)
+
     def launches(self) -> int:
         """Return the number of lh_trace / lh_adjoint launches made on this model."""
         n = _lib.C.c_int64()
@@ -298,6 +348,32 @@ def source_rows(det, source, samples, first_id=0):
     return rows, 1
 
 
+def launch_rows(rows, chunk: int | None) -> list:
+    """Return the launches of a run as LhSource arrays of at most ``chunk`` photons each (None: one launch of every
+    row; the reference's ``readout/grid.py:358-362`` ``photon_chunks`` per source).  A row longer than a launch's room
+    goes on in the next launch with its ``first_id`` advanced by the photons already taken, so every photon keeps its
+    id and its random numbers; rows of no photons are dropped (a run of only such rows is one launch of them)."""  # noqa: D205, D209
+    if chunk is None:
+        return [rows]
+    out, cur, room = [], [], chunk
+    for r in rows:
+        done, n = 0, int(r.n)
+        while done < n:
+            take = min(n - done, room)
+            part = type(r).from_buffer_copy(r)
+            part.n, part.first_id = take, int(r.first_id) + done
+            cur.append(part)
+            done, room = done + take, room - take
+            if room == 0:
+                out.append(cur)
+                cur, room = [], chunk
+    if cur:
+        out.append(cur)
+    if not out:
+        return [rows]
+    return [(type(rows[0]) * len(c))(*c) for c in out]
+
+
 _GATED: dict = {}  # (id(detector), t0, dt, nt) -> (the detector, its timed copy): one model per gate
 
 
@@ -323,8 +399,9 @@ def gated(det, gate, step: float | None = None):
 
 def simulate(det, source, params: Params | None = None, *, samples: int | None = None, seed: int = 0,
              gate=None, events: int | None = None, reference: Params | None = None, controls: Controls | None = None,
-             first_id: int = 0, edge_rows="structure", device="cuda", stream=None, gate_step: float | None = None
-             ) -> Light:
+             backend: str = "auto", chunk: int | None = None, records: int | None = None, diagnostics=(),
+             truth=None, first_id: int = 0, edge_rows="structure", device="cuda", stream=None,
+             gate_step: float | None = None) -> Light:
     """Trace ``samples`` Monte Carlo photons of ``source`` through ``det`` and return the light at each sensor.
 
     ``params``: the values (default ``det.params()``); ``reference``: the values every random choice is made at
@@ -335,9 +412,30 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     histogram's extent (``gated``; ``gate_step`` its node spacing).  ``edge_rows``: the surface rows drawn with the
     defensive mixture ("structure": the reference's rule; ``()``: none -- the same light in expectation, without the
     zero-weight photons an edge row sends on).
+
+    ``chunk`` (the reference's memory lever): photons per launch; a run of more is several ``lh_trace`` launches
+    added into the same tallies, each photon the same whatever the chunk (its id, hence its random numbers, is
+    fixed), the results differing only by the order of the FP64 sums.  The engine's memory does not grow with the
+    photons (no deposit records: every deposit is added into the tallies in place), so the default is one launch.
+    ``records`` (the reference's deposit-record slots per launch): accepted (an int >= 1) and unused, for that
+    reason; nothing can overflow.  ``backend``: "auto" (or "hip"), the HIP engine; the reference's "cpu" and "xla"
+    photon loops have no counterpart here (the CPU oracle is a test checker, not a backend): refused.
+    ``diagnostics`` and ``truth`` (the per-photon probes and the analog truth rows) are not built: refused unless
+    empty.
     """
     import torch
 
+    if backend not in ("auto", "hip"):
+        raise ValueError(f"simulate: backend {backend!r}: this engine runs the photon loop on the GPU only "
+                         f"('auto' or 'hip')")
+    if diagnostics:
+        raise ValueError("simulate: diagnostics (the per-photon probes) are not part of this engine")
+    if truth is not None:
+        raise ValueError("simulate: truth rows are not part of this engine")
+    if chunk is not None and int(chunk) < 1:
+        raise ValueError(f"simulate: chunk is a number of photons >= 1, got {chunk}")
+    if records is not None and int(records) < 1:
+        raise ValueError(f"simulate: records is a number of deposit records >= 1, got {records}")
     if gate is not None:
         det = gated(det, gate, gate_step)
     eng = engine(det, device)
@@ -369,13 +467,15 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
     ctl = controls.struct()
     if controls.jump:
         eng.ensure_jump()
-    _lib.check(_lib.lib().lh_trace(eng.handle, _lib.C.c_void_p(values.data_ptr()),
-                                   None if ref_values is None else _lib.C.c_void_p(ref_values.data_ptr()),
-                                   int(values.numel()), rows, len(rows),
-                                   _lib.C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), _lib.C.byref(ctl),
-                                   _lib.C.c_void_p(buf.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
-                                   _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
-    launch = eng.launches() - 1
+    first_launch = eng.launches()
+    for part in launch_rows(rows, None if chunk is None else int(chunk)):
+        _lib.check(_lib.lib().lh_trace(eng.handle, _lib.C.c_void_p(values.data_ptr()),
+                                       None if ref_values is None else _lib.C.c_void_p(ref_values.data_ptr()),
+                                       int(values.numel()), part, len(part),
+                                       _lib.C.c_uint64(int(seed) & 0xFFFFFFFFFFFFFFFF), _lib.C.byref(ctl),
+                                       _lib.C.c_void_p(buf.data_ptr()), n_ev, _lib.C.c_void_p(health.data_ptr()),
+                                       _lib.C.c_void_p(st.cuda_stream)), "lh_trace")
+    launches = range(first_launch, eng.launches())
     if st != current:
         for t in (values, ref_values, buf, health):  # in use on `st` after this returns: kept from reuse meanwhile
             if t is not None:
@@ -383,11 +483,13 @@ def simulate(det, source, params: Params | None = None, *, samples: int | None =
         current.wait_stream(st)  # the results, for what the caller does next on the current stream
     light = light_of(eng, tally, lambda: health.cpu().numpy(), samples if not isinstance(source, Batch) else None,
                      source, keep_events=events is not None)
-    light.kernel_ms = lambda: eng.kernel_ms(launch)  # the call returns before the kernel ends: read on first use
+    # the call returns before the kernels end: read on first use
+    light.kernel_ms = lambda: eng.kernel_ms_sum(launches)
+    light.group_specs = list(det.groups or [])
     tg = getattr(det, "time_grid", None)
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, controls=controls._asdict(),
-                            engine="liblumen_hip (gfx950)", reference=reference is not None,
-                            time_grid=None if tg is None else (tg.t0, tg.dt, tg.nt))
+                            engine="liblumen_hip (gfx950)", reference=reference is not None, chunk=chunk,
+                            launches=len(launches), time_grid=None if tg is None else (tg.t0, tg.dt, tg.nt))
     return light
 
 

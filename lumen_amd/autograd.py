@@ -178,6 +178,7 @@ def simulate_diff(det, source, params_t: dict, *, samples=None, seed=0, params: 
     light = light_of(eng, tally, lambda: health.detach().cpu().numpy(),
                      samples if not hasattr(source, "sources") else None, source)
     light.kernel_ms = lambda: run.kernel_ms
+    light.group_specs = list(det.groups or [])
     light.provenance = dict(seed=int(seed), samples=samples, detector=det.name, derivative="adjoint (replay)",
                             keys=keys, run=run)
     return light

@@ -18,13 +18,17 @@ from .params import Params, Table
 
 
 def pe(light, params: Params, det=None, group: int = 0):
-    """Return expected photoelectrons (S,) (or (E, S) for a batched Light) at ``params``."""
+    """Return expected photoelectrons (S,) (or (E, S) for a batched Light) at ``params``: the sensor group's keys
+    are those of ``light``'s own group when it is a group's view (``light.group(name)``, the reference's
+    ``readout.pe(light.group(name), params)``), else of ``det.groups[group]``."""  # noqa: D205, D209
     import torch
 
     charge = light.charge
     dev, dt = charge.device, charge.dtype
     kl, kt = getattr(light, "charge_axes", None) or (charge.shape[-1], 1)
-    g = det.groups[group] if det is not None and det.groups else None
+    g = getattr(light, "sensor_group", None)  # a group's view (``Light.group(name)``) carries its group
+    if g is None:
+        g = det.groups[group] if det is not None and det.groups else None
     qe_key = g.qe if g is not None else None
     eff_key = g.efficiency if g is not None else None
     ce_key = getattr(g, "ce", None) if g is not None else None

@@ -499,3 +499,29 @@ def test_gate_and_events_arguments(gpu, tank1):
     assert tuple(d.arrivals.shape) == (4, 8) and float(d.arrivals[1].abs().sum()) == 0.0
     with pytest.raises(ValueError, match="events"):
         L.simulate(tank1, batch, seed=3, events=2)
+
+
+def test_chunk_records_and_groups_arguments(gpu, tank1):
+    """``simulate(chunk=, records=)`` and ``Light.group`` (the reference's api.py:876-878, 557-562): a run split
+    into launches of ``chunk`` photons traces the same photons (ids fixed) into the same tallies, to the order of
+    the FP64 sums; ``records`` changes nothing; the group view is the light on the group's sensors."""
+    n = (1 << 17) + 123
+    a = L.simulate(tank1, SRC1, samples=n, seed=5)
+    b = L.simulate(tank1, SRC1, samples=n, seed=5, chunk=(1 << 15) + 7, records=16)
+    assert b.provenance["launches"] == 5 and a.provenance["launches"] == 1
+    for k in ("arrivals", "arrival2", "mean_arrival_time", "charge"):
+        np.testing.assert_allclose(getattr(b, k).cpu().numpy(), getattr(a, k).cpu().numpy(), rtol=1e-12, atol=0)
+    ha, hb = np.asarray(a.health), np.asarray(b.health)
+    np.testing.assert_allclose(hb, ha, rtol=1e-12, atol=1e-12)
+    assert hb[13] == n and b.kernel_ms > 0
+    batch = L.sources.Batch((SRC1, SRC1, SRC1), (5000, 0, 70000), events=(1, 0, 1))
+    c = L.simulate(tank1, batch, seed=2, events=3)
+    d = L.simulate(tank1, batch, seed=2, events=3, chunk=4096)
+    np.testing.assert_allclose(d.arrivals.cpu().numpy(), c.arrivals.cpu().numpy(), rtol=1e-12, atol=0)
+    np.testing.assert_allclose(d.arrivals_error.cpu().numpy(), c.arrivals_error.cpu().numpy(), rtol=1e-9, atol=0)
+    g = a.group("pmt")
+    np.testing.assert_array_equal(g.arrivals.cpu().numpy(), a.arrivals.cpu().numpy()[tank1.groups[0].sensors])
+    np.testing.assert_allclose(L.readout.pe(g, tank1.params()).cpu().numpy(),
+                               L.readout.pe(a, tank1.params(), tank1).cpu().numpy(), rtol=1e-15)
+    with pytest.raises(ValueError, match="backend"):
+        L.simulate(tank1, SRC1, samples=16, seed=5, backend="xla")

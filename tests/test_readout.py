@@ -175,3 +175,56 @@ def test_delay_is_differentiable_in_its_table():
         e[i] = 1e-6
         fd = (f(pdf.detach() + e) - f(pdf.detach() - e)) / 2e-6
         assert float(pdf.grad[i]) == pytest.approx(float(fd), rel=1e-6, abs=1e-9)
+
+
+def test_launch_rows_keep_every_photon_id_once():
+    """``simulate(chunk=)``'s launches (the reference's readout/grid.py:358-362 photon_chunks): every launch holds at
+    most the chunk's photons and the launches cover each row's ids first_id .. first_id + n - 1 once, in order."""
+    from lumen_amd import _lib
+    from lumen_amd.api import launch_rows
+
+    ns, firsts = [5, 0, 17, 1, 8], [0, 5, 5, 22, 100]
+    rows = (_lib.LhSource * len(ns))()
+    for i, (n, f) in enumerate(zip(ns, firsts)):
+        rows[i].n, rows[i].first_id, rows[i].event, rows[i].weight = n, f, i, 0.5 + i
+    want = sorted((i, f + k) for i, (n, f) in enumerate(zip(ns, firsts)) for k in range(n))
+    for chunk in (1, 3, 4, 7, 31, 1000):
+        got = []
+        parts = launch_rows(rows, chunk)
+        for part in parts:
+            assert 0 < sum(r.n for r in part) <= chunk
+            for r in part:
+                got += [(r.event, r.first_id + k) for k in range(r.n)]
+                assert r.weight == 0.5 + r.event  # the per-photon weight is the row's, not the launch's
+        assert sorted(got) == want and len(got) == len(want)
+        assert len(parts) == -(-sum(ns) // chunk)
+    assert launch_rows(rows, None) is not None and len(launch_rows(rows, None)) == 1
+
+
+def test_group_views_and_pe_of_a_group():
+    """``Light.groups`` / ``Light.group(name)`` (the reference's api.py:515-562): each view holds its group's sensors
+    in the group's order, and ``pe`` of a view reads that group's keys (the reference's ``pe(light.group(name),
+    params)``)."""
+    from lumen_amd.detector.model import SensorGroup
+    from lumen_amd.readout import pe
+
+    a = torch.arange(1.0, 6.0, dtype=torch.float64)
+    light = Light(a, a * 0.1, a * 2.0, a[:, None].clone(), a * a, Health(*([0.0] * 15)), 100)
+    light.group_specs = [SensorGroup("odd", np.array([4, 2, 0]), "sensors/odd/qe", "sensors/odd/efficiency"),
+                         SensorGroup("even", np.array([1, 3]), "sensors/even/qe")]
+    p = Params({"sensors/odd/qe": np.array([0.1, 0.2, 0.3]), "sensors/odd/efficiency": 0.5,
+                "sensors/even/qe": np.array([1.0, 2.0])})
+    odd = light.group("odd")
+    np.testing.assert_array_equal(odd.arrivals.numpy(), [5.0, 3.0, 1.0])
+    np.testing.assert_array_equal(odd.mean_arrival_time.numpy(), [10.0, 6.0, 2.0])
+    np.testing.assert_array_equal(odd.arrival2.numpy(), [25.0, 9.0, 1.0])
+    np.testing.assert_allclose(pe(odd, p).numpy(), 0.5 * np.array([0.1, 0.2, 0.3]) * [5.0, 3.0, 1.0])
+    np.testing.assert_allclose(pe(light.groups["even"], p).numpy(), [2.0, 8.0])
+    assert odd.provenance["group"] == "odd" and light.group("odd") is odd  # built once
+    with pytest.raises(KeyError, match="odd"):
+        light.group("missing")
+    e2 = torch.stack([a, 2 * a])
+    ev = Light(e2, e2, e2, e2[..., None], e2, Health(*([0.0] * 15)), 100)
+    ev.group_specs = light.group_specs
+    np.testing.assert_array_equal(ev.group("even").arrivals.numpy(), [[2.0, 4.0], [4.0, 8.0]])
+    assert Light(a, a, a, a[:, None], a, Health(*([0.0] * 15)), 1).groups is None
