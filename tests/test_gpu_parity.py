@@ -512,8 +512,9 @@ def test_chunk_records_and_groups_arguments(gpu, tank1):
     for k in ("arrivals", "arrival2", "mean_arrival_time", "charge"):
         np.testing.assert_allclose(getattr(b, k).cpu().numpy(), getattr(a, k).cpu().numpy(), rtol=1e-12, atol=0)
     ha, hb = np.asarray(a.health), np.asarray(b.health)
-    np.testing.assert_allclose(hb, ha, rtol=1e-12, atol=1e-12)
-    assert hb[13] == n and b.kernel_ms > 0
+    # the fates' weights are summed per wavefront in float32 before their FP64 cells: another split, other roundings
+    np.testing.assert_allclose(hb, ha, rtol=1e-6, atol=1e-9)
+    assert hb[13] == n and (hb[[10, 12]] == ha[[10, 12]]).all() and b.kernel_ms > 0  # the counts: exact
     batch = L.sources.Batch((SRC1, SRC1, SRC1), (5000, 0, 70000), events=(1, 0, 1))
     c = L.simulate(tank1, batch, seed=2, events=3)
     d = L.simulate(tank1, batch, seed=2, events=3, chunk=4096)
