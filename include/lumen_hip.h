@@ -47,6 +47,7 @@ extern "C" {
 #define LH_H_VERTICES 12      /* steps taken */
 #define LH_H_PHOTONS 13       /* photons started */
 #define LH_H_START_WEIGHT 14  /* their row weights summed */
+#define LH_H_BOUNDED 15       /* steps whose query the reach bound stopped a few bands past the free path */
 #define LH_HEALTH_N 16
 
 typedef struct lh_model_s *lh_model;
@@ -90,6 +91,12 @@ typedef struct {
    * volume may land in that sibling (touching siblings: the crossing goes on into it); no counterpart in the
    * reference's arrays (lumen_amd/detector/navarrays.py touch_flags) */
   const int32_t *touch;
+  /* n_vol*3 each, or NULL (then no reach bound): each volume's world box (lo, hi), padded as the navigator's BVH
+   * boxes (lumen_amd/detector/navarrays.py `lo_w`, `hi_w`).  lh_model_create builds the reach bound from them (the
+   * reference's src/lumen/geometry/reach.py:174-222 `ReachGeometry`): the volumes' boxes padded by the relocation
+   * step at the geometry's scale, and a BVH over the sensor volumes' boxes, so that a step whose free path ends
+   * inside its volume's box before any sensor box along the ray queries no further than a few bands past it */
+  const float *box_lo, *box_hi;
 } lh_geometry;
 
 /* One emission source.  Photon i of it (0 <= i < n) has the id first_id + i: its random numbers are a function of
@@ -125,6 +132,10 @@ typedef struct {
                              * drew an outcome the live values give no probability), as the reference does; 0: end
                              * it there.  It deposits nothing more either way: only the records and vertices
                              * diagnostics differ */
+  int32_t reach_bound;      /* 1: the reach bound where the model has one (lh_geometry box_lo / box_hi); 0: every step
+                             * queries to the next crossing.  The same photons and results either way; the bound
+                             * pays where the query past a step's free path is costly (many volumes that are not
+                             * sensors), and costs registers elsewhere */
 } lh_controls;
 
 /* Build a model: uploads the navigator arrays and the packed layout (lumen_amd/pack.py) to the current device.

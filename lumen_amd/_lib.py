@@ -39,7 +39,8 @@ class LhGeometry(C.Structure):
         ("gR", _f), ("g0", _i), ("g1", _i), ("lo", _f), ("hi", _f), ("skip", _i), ("start", _i), ("count", _i),
         ("items", _i), ("leaf_typ", _i), ("leaf_p0", _i), ("leaf_np", _i), ("leaf_o", _f), ("leaf_q", _f),
         ("leaf_A", _f), ("leaf_b", _f), ("leaf_c", _f), ("plane_n", _f), ("plane_h", _f), ("kind_l0", _i),
-        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float), ("skin", _i), ("touch", _i)]
+        ("kind_nl", _i), ("kind_p0", _i), ("kind_pl", _i), ("prog_op", _i), ("prog_arg", _i), ("reach", C.c_float), ("skin", _i), ("touch", _i),
+        ("box_lo", _f), ("box_hi", _f)]
 
 
 class LhSource(C.Structure):
@@ -52,7 +53,8 @@ class LhSource(C.Structure):
 class LhControls(C.Structure):
     _fields_ = [("roulette", C.c_float), ("max_distance", C.c_float), ("max_vertices", C.c_int32),
                 ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32), ("tally_global", C.c_int32),
-                ("nodes_global", C.c_int32), ("jump", C.c_int32), ("keep_zero_weight", C.c_int32)]
+                ("nodes_global", C.c_int32), ("jump", C.c_int32), ("keep_zero_weight", C.c_int32),
+                ("reach_bound", C.c_int32)]
 
 
 _LIB = None
@@ -140,6 +142,9 @@ def geometry_struct(nav, sensor_of_vol, reach: float):
         keep["skin"] = i32(h["skin"])
     if "touch" in h:  # the volumes a sibling's box touches (absent: every exit looks for a sibling past the face)
         keep["touch"] = i32(h["touch"])
+    lo_w, hi_w = getattr(nav, "lo_w", None), getattr(nav, "hi_w", None)
+    if lo_w is not None and hi_w is not None and np.shape(lo_w) == (nav.n, 3) == np.shape(hi_w):
+        keep["box_lo"], keep["box_hi"] = f32(lo_w).reshape(-1), f32(hi_w).reshape(-1)  # the reach bound's boxes
     n_nodes = len(keep["skip"]) if nav.stats.get("groups", 1) else 0
     g = LhGeometry(n_vol=nav.n, n_nodes=n_nodes, n_items=len(keep["items"]),
                    n_groups=int(nav.stats.get("groups", len(keep["g0"]))), n_leaves=n_leaves,

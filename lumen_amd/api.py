@@ -36,7 +36,11 @@ class Controls(NamedTuple):
     clearance grids are built on first use, ``lumen_amd.detector.clearance``), ``keep_zero_weight`` (trace a photon
     on after its live weight became exactly 0, as the reference does -- an edge row's defensive floor drew an outcome
     the live values give no probability; such a photon deposits nothing more, so only the ``records`` and
-    ``vertices`` diagnostics change; off: it ends there)."""  # noqa: D205, D209
+    ``vertices`` diagnostics change; off: it ends there), ``reach_bound`` (the reference's reach bound,
+    ``geometry/reach.py``: a step whose free path ends inside its volume's box with no sensor volume's box ahead of
+    it on the ray queries only a few bands past the free path; the same photons and results either way.  Off by
+    default: on the water tank and the LAr box the query past the free path is cheap, and the bound's registers
+    slow the loop by 14-20 %)."""  # noqa: D205, D209
 
     max_distance: Any = None
     roulette: float = ROULETTE
@@ -47,6 +51,7 @@ class Controls(NamedTuple):
     nodes_global: bool = False
     jump: bool = False
     keep_zero_weight: bool = False
+    reach_bound: bool = False
 
     def struct(self) -> _lib.LhControls:
         if not (float(self.roulette) > 0.0):
@@ -54,7 +59,7 @@ class Controls(NamedTuple):
         return _lib.LhControls(float(self.roulette), 0.0 if self.max_distance is None else float(self.max_distance),
                                int(self.max_vertices), int(self.chunk), int(self.blocks_per_cu),
                                int(bool(self.tally_global)), int(bool(self.nodes_global)), int(bool(self.jump)),
-                               int(bool(self.keep_zero_weight)))
+                               int(bool(self.keep_zero_weight)), int(bool(self.reach_bound)))
 
 
 class Health(NamedTuple):
@@ -75,6 +80,7 @@ class Health(NamedTuple):
     vertices: float
     photons: float
     start_weight: float
+    bounded: float = 0.0  # steps whose query the reach bound stopped a few bands past the free path
 
     FATES = ("absorbed_bulk", "absorbed_surface", "lost", "escaped", "distance_cut", "abandoned", "truncated",
              "rouletted", "reweighted")
@@ -85,7 +91,7 @@ class Health(NamedTuple):
 
     @staticmethod
     def of(h: np.ndarray) -> "Health":
-        return Health(*[float(x) for x in np.asarray(h, np.float64)[:15]])
+        return Health(*[float(x) for x in np.asarray(h, np.float64)[:16]])
 
 
 @dataclass

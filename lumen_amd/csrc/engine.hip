@@ -59,8 +59,9 @@ constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many ce
 constexpr int OCC_CACHE = 8;
 // a kernel's features (template parameter): separate reference values, bulk channels beyond Rayleigh (Mie,
 // WLS), the time histogram, BVH nodes read from global memory
-constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8, F_TORUS = 16;
-constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES | F_TORUS;  // the features an adjoint kernel is specialised on
+constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8, F_TORUS = 16, F_REACH = 32;
+// the features an adjoint kernel is specialised on
+constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES | F_TORUS | F_REACH;
 
 // The forward kernel's register budget: 3 wavefronts per SIMD (168 VGPRs) hold the step without spills; at 4
 // (128 VGPRs) about 77 VGPRs spill to scratch, several GB of scratch traffic per launch, and the 1,000-PMT tank
@@ -85,6 +86,7 @@ struct Args {
   float roulette, max_dist, reach;
   int max_vert, chunk;
   int keep_zero;  // 1: a photon whose live weight became 0 walks on (the reference's records); 0: it ends
+  int reach_on;   // 1: the reach bound where the model has one (the controls' reach_bound)
   double *__restrict__ tally;
   long long tally_stride;  // doubles per event
   int S, K;  // sensors; charge cells per sensor, Kl wavelength (QE) knots x Kt theta (CE) knots
@@ -145,6 +147,8 @@ __shared__ float s_rare[RF_N];
 // volume's daughter-free core starts in that volume (no daughter holds a point of the core), with no descent from the
 // top (-1: none yet; lanes overwrite it as they like, any value is only a hint that the core test checks)
 __shared__ int s_hint[32];
+// the block's steps whose query the reach bound limited (a wavefront adds its count once per step)
+__shared__ unsigned s_bounded;
 
 __device__ __forceinline__ void rare_add(int cell, float v) {
   if (v != 0.0f) atomicAdd(&s_rare[cell], v);
@@ -721,7 +725,13 @@ __device__ inline bool step(const Args &a, Photon &p, Acc &acc, const AdjArgs *b
   const float mu_r = CHAN ? (mr.mu_ray + mr.mu_mie) + mr.mu_wls : mr.mu_ray;
   const float s = -__logf(u[0]) / fmaxf(mu_r, FLOOR32);  // the free path at the reference rate
   const float max_dist = a.max_dist;
-  const Cross cr = query_t<!(F & F_GNODES), (F & F_TORUS) != 0>(g, p.x, p.d, p.vol, p.srf, max_dist, p.xs);
+  float q_lim = max_dist;
+  if constexpr ((F & F_REACH) != 0) {  // the reach bound: not on a face, in a volume
+    if (p.srf < 0 && p.vol < g.nvol) q_lim = reach_limit(g, p.x, p.d, p.vol, s, max_dist);
+    const unsigned long long bm = __ballot(q_lim < max_dist);
+    if (bm != 0ull && (int)(threadIdx.x & 63) == __ffsll((long long)bm) - 1) atomicAdd(&s_bounded, (unsigned)__popcll(bm));
+  }
+  const Cross cr = query_t<!(F & F_GNODES), (F & F_TORUS) != 0>(g, p.x, p.d, p.vol, p.srf, q_lim, p.xs);
   const bool hit = cr.vol_after >= 0;
   const float dist = cr.t;
   const bool zero = dist <= ZERO_STEP_BANDS * b0;
@@ -1056,6 +1066,7 @@ __device__ inline void flush_health(const Args &a, const Acc &acc) {
     const float v = s_rare[threadIdx.x];
     if (v != 0.0f) unsafeAtomicAdd(a.health + cell[threadIdx.x], (double)v);
   }
+  if (threadIdx.x == RF_N && s_bounded != 0u) unsafeAtomicAdd(a.health + LH_H_BOUNDED, (double)s_bounded);
 }
 
 __global__ void reset_queue(unsigned long long *q) { *q = 0ull; }
@@ -1118,6 +1129,7 @@ template <unsigned F>
 __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args a0, int n_lds) {
   if (a0.lds_tally)
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x) lds_cells[i] = 0.0;
+  if (threadIdx.x == 0) s_bounded = 0u;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
@@ -1181,6 +1193,7 @@ __global__ void __launch_bounds__(BLOCK)
     adjoint_kernel(Args a0, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
+  if (threadIdx.x == 0) s_bounded = 0u;
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
@@ -1556,6 +1569,60 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
     }
   }
 
+  // the reach bound's boxes: each volume's box and a BVH (one box a leaf) over the sensor volumes' boxes, all padded
+  // by twice the relocation step (K_ON bands) at the geometry's scale, the largest coordinate plus the diagonal
+  const bool has_reach = G->box_lo && G->box_hi;
+  std::vector<float4> rbox(has_reach ? 2 * (size_t)n : 1), rnode(2);
+  int n_rnode = -1;
+  if (has_reach) {
+    double mx = 0.0, ulo[3] = {HUGEF, HUGEF, HUGEF}, uhi[3] = {-HUGEF, -HUGEF, -HUGEF};
+    for (int v = 0; v < n; ++v)
+      for (int k = 0; k < 3; ++k) {
+        const double l = G->box_lo[3 * v + k], h = G->box_hi[3 * v + k];
+        if (!(l <= h) || !std::isfinite(l) || !std::isfinite(h))
+          return fail(LH_ERR_ARG, "lh_model_create: a volume's box is empty or not finite");
+        mx = std::max(mx, std::max(std::fabs(l), std::fabs(h)));
+        ulo[k] = std::min(ulo[k], l);
+        uhi[k] = std::max(uhi[k], h);
+      }
+    const double diag = std::sqrt((uhi[0] - ulo[0]) * (uhi[0] - ulo[0]) + (uhi[1] - ulo[1]) * (uhi[1] - ulo[1]) +
+                                  (uhi[2] - ulo[2]) * (uhi[2] - ulo[2]));
+    const double pad = 2.0 * K_ON * (BAND_ULPS * (double)EPS32 * (mx + diag) + BAND_FLOOR);
+    auto down = [](double x) { return std::nextafter((float)x, -HUGEF); };  // float32 bounds that hold the box
+    auto up = [](double x) { return std::nextafter((float)x, HUGEF); };
+    for (int v = 0; v < n; ++v) {
+      rbox[2 * v] = make_float4(down(G->box_lo[3 * v] - pad), down(G->box_lo[3 * v + 1] - pad),
+                                down(G->box_lo[3 * v + 2] - pad), 0.0f);
+      rbox[2 * v + 1] = make_float4(up(G->box_hi[3 * v] + pad), up(G->box_hi[3 * v + 1] + pad),
+                                    up(G->box_hi[3 * v + 2] + pad), 0.0f);
+    }
+    std::vector<double> slo, shi;
+    for (int v = 0; v < n; ++v)
+      if (G->sensor[v] >= 0)
+        for (int k = 0; k < 3; ++k) {
+          slo.push_back(G->box_lo[3 * v + k] - pad);
+          shi.push_back(G->box_hi[3 * v + k] + pad);
+        }
+    const int ns = (int)(slo.size() / 3);
+    n_rnode = 0;
+    if (ns > 0) {
+      std::vector<double> nlo(6 * (size_t)ns), nhi(6 * (size_t)ns);
+      std::vector<int32_t> nskip(2 * (size_t)ns), nstart(2 * (size_t)ns), ncount(2 * (size_t)ns), nitems(ns);
+      int32_t nn = 0, depth = 0;
+      const int rc = lh_bvh_build(slo.data(), shi.data(), ns, 1, 16, nlo.data(), nhi.data(), nskip.data(),
+                                  nstart.data(), ncount.data(), nitems.data(), &nn, &depth);
+      if (rc != LH_OK) return rc;
+      rnode.assign(2 * (size_t)nn, make_float4(0.f, 0.f, 0.f, 0.f));
+      for (int i = 0; i < nn; ++i) {
+        if (nskip[i] <= i || nskip[i] > nn) return fail(LH_ERR_ARG, "lh_model_create: the sensor boxes' BVH is malformed");
+        const int leaf = ncount[i] > 0 ? 1 : 0;
+        rnode[2 * i] = make_float4(down(nlo[3 * i]), down(nlo[3 * i + 1]), down(nlo[3 * i + 2]), int_bits(nskip[i]));
+        rnode[2 * i + 1] = make_float4(up(nhi[3 * i]), up(nhi[3 * i + 1]), up(nhi[3 * i + 2]), int_bits(leaf));
+      }
+      n_rnode = nn;
+    }
+  }
+
   size_t off = 0;
   auto place = [&](size_t bytes) {
     const size_t o = off;
@@ -1567,7 +1634,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
                o_kinds = place(kinds.size() * 16), o_vkind = place(vkind.size() * 16),
                o_ixf = place(ixf.size() * 16), o_ikind = place(ikind.size() * 16), o_vgrp = place(vgrp.size() * 8), o_grange = place(grange.size() * 8),
                o_items = place(items.size() * 4), o_prog = place(prog.size() * 4),
-               o_vcore = place(vcore.size() * 16);
+               o_vcore = place(vcore.size() * 16), o_rbox = place(rbox.size() * 16),
+               o_rnode = place(rnode.size() * 16);
   lh_model m = new lh_model_s();
   m->has_torus = has_torus;
   char *buf = nullptr;
@@ -1593,6 +1661,8 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   memcpy(host.data() + o_items, items.data(), items.size() * 4);
   memcpy(host.data() + o_prog, prog.data(), prog.size() * 4);
   memcpy(host.data() + o_vcore, vcore.data(), vcore.size() * 16);
+  memcpy(host.data() + o_rbox, rbox.data(), rbox.size() * 16);
+  memcpy(host.data() + o_rnode, rnode.data(), rnode.size() * 16);
   HIP_TRY(hipMemcpy(buf, host.data(), off, hipMemcpyHostToDevice));
   Geo &g = m->g;
   g.nvol = n;
@@ -1612,6 +1682,9 @@ LH_API int lh_model_create(const lh_geometry *G, const int32_t *layout, int32_t 
   g.items = (const int *)(buf + o_items);
   g.prog = (const int *)(buf + o_prog);
   g.vcore = (const float4 *)(buf + o_vcore);
+  g.rbox = (const float4 *)(buf + o_rbox);
+  g.rnode = (const float4 *)(buf + o_rnode);
+  g.n_rnode = n_rnode;
   g.reach = G->reach;
   g.n_nodes = G->n_nodes;
   g.lds_nodes = 0;
@@ -1861,6 +1934,7 @@ static int fill_args(lh_model m, Args &a, const float *values, const float *ref,
   a.max_vert = c.max_vertices > 0 ? c.max_vertices : (1 << 22);
   a.chunk = c.chunk > 0 ? c.chunk : DEFAULT_CHUNK;
   a.keep_zero = c.keep_zero_weight ? 1 : 0;
+  a.reach_on = c.reach_bound ? 1 : 0;
   a.tally = tally;
   a.tally_stride = m->tally_stride;
   a.nt = m->NT;
@@ -1893,14 +1967,17 @@ static const void *kernel_f(bool adj) {
   return adj ? (const void *)adjoint_kernel<F & F_ADJ_MASK> : (const void *)trace_kernel<F>;
 }
 
+// the reach bound where the model has its boxes and the controls allow it; not compiled with the torus roots
 static unsigned features_of(lh_model m, const Args &a, bool adj) {
   return (a.has_ref ? F_REF : 0u) | (m->has_chan ? F_CHAN : 0u) | (!adj && m->NT > 0 ? F_TIME : 0u) |
-         (m->has_torus ? F_TORUS : 0u);
+         (m->has_torus ? F_TORUS : 0u) | (m->g.n_rnode >= 0 && a.reach_on && !m->has_torus ? F_REACH : 0u);
 }
 
+// every feature set a launch can take: any of the first five bits, and the reach bound without the torus roots
 #define LH_FEATURES(X)                                                                                         \
   X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) \
-      X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+      X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38)  \
+      X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47)
 
 // the workgroup size of a launch's kernel
 static int block_for(bool adj, unsigned f) { return adj || (f & F_REF) ? BLOCK : LH_FWD_BLOCK; }

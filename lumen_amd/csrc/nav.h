@@ -49,6 +49,11 @@ struct Geo {
   const int4 *__restrict__ ikind;   // per BVH item: its volume's kind record
   const int *__restrict__ prog;     // op << 24 | arg
   const float4 *__restrict__ vcore;  // 3 per navigator id: its daughter-free core (core_interval)
+  // the reach bound (reach_limit): 2 per volume its world box padded by twice the relocation step at the geometry's
+  // scale; 2 per node of a BVH over the sensor volumes' boxes (padded alike), (lo, skip), (hi, 1 at a leaf: one box)
+  const float4 *__restrict__ rbox;
+  const float4 *__restrict__ rnode;
+  int n_rnode;  // the sensor-box BVH's nodes (0: no sensor); -1: no reach bound
   float reach;
   int n_nodes;
   int lds_nodes;     // 1: the kernel staged all BVH nodes in LDS at float4 offset lds_node_off of lds_cells
@@ -670,6 +675,42 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
     }
   }
   return best;
+}
+
+// The reach bound (the reference's geometry/reach.py:1-40, 174-222): the limit for the query of a step whose free
+// path is s, from x along d in volume vol, not sitting on a face, within tmax.  Every crossing the query can return
+// lies in the volume's padded box B (its own exit, a daughter's entry, or a point the relocation step past one), and
+// the ray leaves the convex B once, at T; the crossing deposits only where it enters a sensor volume or leaves the
+// one the photon is in, at a point of that volume's box.  So where s < T and the segment [0, T] meets no sensor
+// volume's padded box, the query may stop a few bands past s: every crossing at t <= s is found as before (the
+// limit only prunes candidates beyond it), and one beyond s, a bulk event either way, deposits nothing.  The boxes'
+// padding (twice the relocation step at the geometry's scale) holds the float32 slab tests' rounding.  Otherwise,
+// or for a start outside B, the limit is tmax.  The sensor boxes are walked any-hit: the first leaf box met ends it.
+__device__ inline float reach_limit(const Geo &g, float3 x, float3 d, int vol, float s, float tmax) {
+  const float4 lo = g.rbox[2 * vol], hi = g.rbox[2 * vol + 1];
+  if (!(x.x >= lo.x && x.x <= hi.x && x.y >= lo.y && x.y <= hi.y && x.z >= lo.z && x.z <= hi.z)) return tmax;
+  const float3 inv = mk3(1.0f / (fabsf(d.x) > 1e-30f ? d.x : copysignf(1e-30f, d.x)),
+                         1.0f / (fabsf(d.y) > 1e-30f ? d.y : copysignf(1e-30f, d.y)),
+                         1.0f / (fabsf(d.z) > 1e-30f ? d.z : copysignf(1e-30f, d.z)));
+  const float T = fminf(fminf(fmaxf((lo.x - x.x) * inv.x, (hi.x - x.x) * inv.x),
+                              fmaxf((lo.y - x.y) * inv.y, (hi.y - x.y) * inv.y)),
+                        fmaxf((lo.z - x.z) * inv.z, (hi.z - x.z) * inv.z));
+  const float lim = s + (K_ON + 2.0f) * band_at(maxabs3(x) + s);
+  if (!(s < T) || !(lim < tmax)) return tmax;
+  const float3 xi = mk3(x.x * inv.x, x.y * inv.y, x.z * inv.z);
+  int ni = 0;
+  while (ni < g.n_rnode) {
+    const float4 a = g.rnode[2 * ni], b = g.rnode[2 * ni + 1];
+    const float tx0 = fmaf(a.x, inv.x, -xi.x), tx1 = fmaf(b.x, inv.x, -xi.x);
+    const float ty0 = fmaf(a.y, inv.y, -xi.y), ty1 = fmaf(b.y, inv.y, -xi.y);
+    const float tz0 = fmaf(a.z, inv.z, -xi.z), tz1 = fmaf(b.z, inv.z, -xi.z);
+    const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), T));
+    const bool in = !(te > tl);
+    if (in && __float_as_int(b.w) != 0) return tmax;  // a sensor volume's box on the segment
+    ni = in ? ni + 1 : __float_as_int(a.w);
+  }
+  return lim;
 }
 
 template <bool TOR>

@@ -28,10 +28,18 @@ def test_abi_exports_every_declared_symbol():
 
 
 def test_abi_structs_match_the_header():
-    """ctypes' structs have the header's sizes (lh_source: 4 ints, 14 floats, 2 int64)."""
+    """ctypes' structs have the header's sizes and order (lh_source: 4 ints, 14 floats, 2 int64; lh_controls: 10
+    words; lh_geometry's pointers after `reach`: skin, touch, box_lo, box_hi)."""  # noqa: D205, D209
     assert ctypes.sizeof(_lib.LhSource) == 4 * 4 + 14 * 4 + 2 * 8
-    assert ctypes.sizeof(_lib.LhControls) == 9 * 4
+    assert ctypes.sizeof(_lib.LhControls) == 10 * 4
+    assert _lib.LhControls.reach_bound.offset == 9 * 4
     assert _lib.LhGeometry.skin.offset == _lib.LhGeometry.reach.offset + 8  # the float, padding, the pointer
+    g = _lib.LhGeometry
+    assert (g.touch.offset, g.box_lo.offset, g.box_hi.offset) == tuple(g.skin.offset + 8 * k for k in (1, 2, 3))
+    assert ctypes.sizeof(g) == g.box_hi.offset + 8
+    header = open(os.path.join(ROOT, "include", "lumen_hip.h")).read()
+    assert "const float *box_lo, *box_hi;\n} lh_geometry;" in header
+    assert "int32_t reach_bound;" in header.split("} lh_controls;")[0].split("typedef struct {")[-1]
 
 
 def _leaf(T, li, c=None, h_shift=0.0):
