@@ -53,6 +53,10 @@ constexpr int BLOCK = 256;  // the adjoint's and the reference-value kernels' wo
 #endif
 template <unsigned F>
 constexpr int block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
+// the adjoint's workgroup: the plain adjoint (3 wavefronts per SIMD) one block of 12 wavefronts per CU, whose LDS then
+// holds the BVH nodes beside the gradient slots; the variants at their natural (larger) allocation 256
+template <unsigned F>
+constexpr int adj_block_of() { return (F & 3u) ? BLOCK : LH_FWD_BLOCK; }
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
@@ -1188,7 +1192,7 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
 #define LH_ADJ_WAVES_PER_EU 3
 #endif
 template <unsigned F>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(adj_block_of<F>())
     __attribute__((amdgpu_waves_per_eu(((F) & (F_REF | F_CHAN)) ? 1 : LH_ADJ_WAVES_PER_EU)))
     adjoint_kernel(Args a0, AdjArgs b) {
   double *sg = lds_cells;
@@ -1980,7 +1984,9 @@ static unsigned features_of(lh_model m, const Args &a, bool adj) {
       X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47)
 
 // the workgroup size of a launch's kernel
-static int block_for(bool adj, unsigned f) { return adj || (f & F_REF) ? BLOCK : LH_FWD_BLOCK; }
+static int block_for(bool adj, unsigned f) {
+  return (adj ? (f & (F_REF | F_CHAN)) != 0 : (f & F_REF) != 0) ? BLOCK : LH_FWD_BLOCK;
+}
 
 static const void *kernel_of(bool adj, unsigned f) {
   switch (f) {
@@ -1995,7 +2001,7 @@ static const void *kernel_of(bool adj, unsigned f) {
 template <unsigned F>
 static void launch_f(bool adj, dim3 grid, size_t bytes, hipStream_t st, const Args &a, const AdjArgs &b, int n_lds) {
   if (adj)
-    hipLaunchKernelGGL(adjoint_kernel<F & F_ADJ_MASK>, grid, dim3(BLOCK), bytes, st, a, b);
+    hipLaunchKernelGGL(adjoint_kernel<F & F_ADJ_MASK>, grid, dim3(adj_block_of<F & F_ADJ_MASK>()), bytes, st, a, b);
   else
     hipLaunchKernelGGL(trace_kernel<F>, grid, dim3(block_of<F>()), bytes, st, a, n_lds);
 }
@@ -2235,7 +2241,7 @@ LH_API int lh_adjoint(lh_model m, const float *values, const float *ref, int64_t
   HIP_TRY(hipGetLastError());
   const int slot = (int)(m->launches % TIME_RING);
   HIP_TRY(hipEventRecord(m->ev0[slot], st));
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), BLOCK,
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), block_for(true, features_of(m, a, true)),
                                  (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, b, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
@@ -2344,7 +2350,7 @@ LH_API int lh_batch_adjoint(lh_batch b, const float *values, const float *ref, i
   a.queue = b->queue;
   AdjArgs g{cot, b->slots, grad, n_grad};
   hipLaunchKernelGGL(reset_queue, dim3(1), dim3(1), 0, st, a.queue);
-  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), BLOCK,
+  const size_t bytes = place_lds(m, a, kernel_of(true, features_of(m, a, true)), block_for(true, features_of(m, a, true)),
                                  (size_t)n_grad * sizeof(double), false, ctl);
   launch_adjoint(m, a, g, ctl, bytes, st);
   HIP_TRY(hipGetLastError());
