@@ -116,6 +116,30 @@ class Light:
     charge_axes: tuple = (1, 1)  # (wavelength knots, theta knots) of the charge cells
     sensor_group: Any = None  # a group's view (``groups``): its SensorGroup, whose keys ``readout.pe`` reads
     group_specs: list = field(default_factory=list)  # the detector's SensorGroups (``groups`` is built from them)
+    n_events: int = 0  # E of a leading event axis (``simulate(events=E)`` or a Batch); 0: none
+
+    @property
+    def events(self) -> int:
+        """The number of events of the outputs' leading event axis (0: none; the reference's ``Light.events``)."""
+        return int(self.n_events)
+
+    def event(self, e: int) -> "Light":
+        """Return event ``e``'s Light (the reference's ``Light.event``, ``api.py:541-555``): every per-sensor array at
+        that event; the health stays the run's."""  # noqa: D205, D209
+        if not self.n_events:
+            raise ValueError("Light.event: this light has no event axis (simulate(..., events=E) or a Batch)")
+        if not 0 <= int(e) < self.n_events:
+            raise IndexError(f"Light.event: event {e} of {self.n_events}")
+        e = int(e)
+
+        def cut(t):
+            return None if t is None else t[e]
+
+        return Light(cut(self.arrivals), cut(self.arrivals_error), cut(self.mean_arrival_time), cut(self.charge),
+                     cut(self.arrival2), self.__dict__.get("_health"), self.samples,
+                     provenance=dict(self.provenance, event=e), kernel_ms=self.__dict__.get("_kernel_ms"),
+                     time=cut(self.time), late=cut(self.late), early=cut(self.early), charge_axes=self.charge_axes,
+                     sensor_group=self.sensor_group, group_specs=self.group_specs)
 
     @property
     def groups(self) -> dict | None:
@@ -156,7 +180,7 @@ class Light:
                      cut(self.arrival2), self.__dict__.get("_health"), self.samples,
                      provenance=dict(self.provenance, group=g.name), kernel_ms=self.__dict__.get("_kernel_ms"),
                      time=cut(self.time), late=cut(self.late), early=cut(self.early), charge_axes=self.charge_axes,
-                     sensor_group=g)
+                     sensor_group=g, n_events=self.n_events)
 
     def to_dlpack(self, name="arrivals"):
         """Return a DLPack capsule of one output (device memory: ``jax.dlpack.from_dlpack`` takes it)."""
@@ -529,5 +553,6 @@ def light_of(eng, tally, health, samples, source=None, keep_events=False) -> Lig
         if time is not None:
             time, late, early = time[0], late[0], early[0]
     health = health if callable(health) else Health.of(health)
+    n_events = 0 if arr.dim() == 1 else int(arr.shape[0])
     return Light(arr, err, mean_t, charge, arr2, health, samples or 0, time=time, late=late, early=early,
-                 charge_axes=(eng.Kl, eng.Kt))
+                 charge_axes=(eng.Kl, eng.Kt), n_events=n_events)

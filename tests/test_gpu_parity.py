@@ -494,6 +494,8 @@ def test_gate_and_events_arguments(gpu, tank1):
     assert tuple(c.arrivals.shape) == (3, 8)
     np.testing.assert_allclose(c.arrivals[0].cpu().numpy(), b.arrivals.cpu().numpy(), rtol=1e-9, atol=1e-12)
     assert float(c.arrivals[1:].abs().sum()) == 0.0
+    assert c.events == 3 and b.events == 0  # the reference's Light.event(e): event e's light
+    np.testing.assert_array_equal(c.event(0).arrivals.cpu().numpy(), c.arrivals[0].cpu().numpy())
     batch = L.sources.Batch((SRC1, SRC1), (n, n), events=(2, 0))
     d = L.simulate(tank1, batch, seed=3, events=4)
     assert tuple(d.arrivals.shape) == (4, 8) and float(d.arrivals[1].abs().sum()) == 0.0

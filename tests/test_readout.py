@@ -228,3 +228,24 @@ def test_group_views_and_pe_of_a_group():
     ev.group_specs = light.group_specs
     np.testing.assert_array_equal(ev.group("even").arrivals.numpy(), [[2.0, 4.0], [4.0, 8.0]])
     assert Light(a, a, a, a[:, None], a, Health(*([0.0] * 15)), 1).groups is None
+
+
+def test_event_views():
+    """``Light.events`` / ``Light.event(e)`` (the reference's api.py:536-555): event e's arrays, its groups' views
+    on them, and the reference's errors for a light without an event axis or an event out of range."""
+    from lumen_amd.detector.model import SensorGroup
+
+    a = torch.arange(1.0, 7.0, dtype=torch.float64).reshape(2, 3)
+    light = Light(a, a * 0.5, a + 1.0, a[..., None].clone(), a * a, Health(*([0.0] * 15)), 10, n_events=2)
+    light.group_specs = [SensorGroup("last_two", np.array([2, 1]))]
+    assert light.events == 2
+    e1 = light.event(1)
+    np.testing.assert_array_equal(e1.arrivals.numpy(), [4.0, 5.0, 6.0])
+    np.testing.assert_array_equal(e1.mean_arrival_time.numpy(), [5.0, 6.0, 7.0])
+    np.testing.assert_array_equal(e1.group("last_two").arrivals.numpy(), [6.0, 5.0])
+    np.testing.assert_array_equal(light.group("last_two").event(0).arrivals.numpy(), [3.0, 2.0])
+    assert e1.events == 0 and e1.provenance["event"] == 1
+    with pytest.raises(IndexError):
+        light.event(2)
+    with pytest.raises(ValueError, match="no event axis"):
+        e1.event(0)
