@@ -53,10 +53,10 @@ constexpr int BLOCK = 256;  // the adjoint's and the reference-value kernels' wo
 #endif
 template <unsigned F>
 constexpr int block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
-// the adjoint's workgroup: the plain adjoint (3 wavefronts per SIMD) one block of 12 wavefronts per CU, whose LDS then
-// holds the BVH nodes beside the gradient slots; the variants at their natural (larger) allocation 256
+// the adjoint's workgroup: at 3 wavefronts per SIMD one block of 12 wavefronts per CU, whose LDS then holds the BVH
+// nodes beside the gradient slots; the reference-value variants at their natural (larger) allocation 256
 template <unsigned F>
-constexpr int adj_block_of() { return (F & 3u) ? BLOCK : LH_FWD_BLOCK; }
+constexpr int adj_block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
@@ -1185,15 +1185,16 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
-// The plain adjoint (no separate reference values, no Mie or WLS channel) runs at 3 wavefronts per SIMD (168
-// VGPRs, 27 spilled): 26.4 ms against 31.8 at its natural 2 (217 VGPRs) and 32.3 at 4 (123 spilled) on the
-// 1,000-PMT tank's 2^24 photons; the others keep their natural allocation (at 3 they would spill 110 or more).
+// The adjoint without separate reference values runs at 3 wavefronts per SIMD (168 VGPRs, some spilled), one
+// 768-thread block per CU: on the 1,000-PMT tank's 2^24 photons 12.1 ms against 14.0 at four 256-thread blocks with
+// the nodes in memory (and 31.8 at its natural 2 wavefronts); with a Mie or WLS channel (240 VGPRs at its natural
+// 2) the LAr voxels' adjoint takes 8 % less time at 3.  The reference-value variants keep their natural allocation.
 #ifndef LH_ADJ_WAVES_PER_EU
 #define LH_ADJ_WAVES_PER_EU 3
 #endif
 template <unsigned F>
 __global__ void __launch_bounds__(adj_block_of<F>())
-    __attribute__((amdgpu_waves_per_eu(((F) & (F_REF | F_CHAN)) ? 1 : LH_ADJ_WAVES_PER_EU)))
+    __attribute__((amdgpu_waves_per_eu(((F) & F_REF) ? 1 : LH_ADJ_WAVES_PER_EU)))
     adjoint_kernel(Args a0, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
@@ -1984,8 +1985,8 @@ static unsigned features_of(lh_model m, const Args &a, bool adj) {
       X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47)
 
 // the workgroup size of a launch's kernel
-static int block_for(bool adj, unsigned f) {
-  return (adj ? (f & (F_REF | F_CHAN)) != 0 : (f & F_REF) != 0) ? BLOCK : LH_FWD_BLOCK;
+static int block_for(bool /* adjoint or forward: the same rule */, unsigned f) {
+  return (f & F_REF) != 0 ? BLOCK : LH_FWD_BLOCK;
 }
 
 static const void *kernel_of(bool adj, unsigned f) {
