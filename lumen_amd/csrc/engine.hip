@@ -45,18 +45,17 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int MAX_FACETS = 256;
 constexpr int MAX_FACET_ATTEMPTS = 4096;
-constexpr int BLOCK = 256;  // the adjoint's and the reference-value kernels' workgroup
-// The plain forward kernels run one workgroup of 12 wavefronts per CU (3 per SIMD at their 168-VGPR budget): the
-// BVH nodes and the tally copy are staged once per CU instead of once per 4 wavefronts, leaving room for both.
+// Every photon-loop kernel runs one workgroup of 12 wavefronts per CU (3 per SIMD at a 168-VGPR budget): the BVH
+// nodes, the packed tables and the tally or gradient copy are staged once per CU instead of once per 4 wavefronts,
+// leaving room for all of them.
 #ifndef LH_FWD_BLOCK
 #define LH_FWD_BLOCK 768
 #endif
+constexpr int BLOCK = LH_FWD_BLOCK;
 template <unsigned F>
-constexpr int block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
-// the adjoint's workgroup: at 3 wavefronts per SIMD one block of 12 wavefronts per CU, whose LDS then holds the BVH
-// nodes beside the gradient slots; the reference-value variants at their natural (larger) allocation 256
+constexpr int block_of() { return BLOCK; }
 template <unsigned F>
-constexpr int adj_block_of() { return (F & 1u) ? BLOCK : LH_FWD_BLOCK; }
+constexpr int adj_block_of() { return BLOCK; }
 constexpr int DEFAULT_CHUNK = 256;
 constexpr int MAX_GRAD = 8192;  // gradient slots a block accumulates in LDS (64 KiB)
 constexpr long long LDS_TALLY_CELLS = 20480;  // tallies of at most this many cells (160 KiB) may go to LDS
@@ -67,14 +66,14 @@ constexpr unsigned F_REF = 1, F_CHAN = 2, F_TIME = 4, F_GNODES = 8, F_TORUS = 16
 // the features an adjoint kernel is specialised on
 constexpr unsigned F_ADJ_MASK = F_REF | F_CHAN | F_GNODES | F_TORUS | F_REACH;
 
-// The forward kernel's register budget: 3 wavefronts per SIMD (168 VGPRs) hold the step without spills; at 4
+// The forward kernel's register budget: 3 wavefronts per SIMD (168 VGPRs) hold the step with few spills; at 4
 // (128 VGPRs) about 77 VGPRs spill to scratch, several GB of scratch traffic per launch, and the 1,000-PMT tank
-// runs 4 % slower (10.7 against 10.3 ms for 2^24 photons); at 2 the walk's load latency shows.  The adjoint and
-// the kernels with separate reference values keep their natural allocation.
+// runs 4 % slower; at 2 the walk's load latency shows: the kernel with separate reference values (224 VGPRs at its
+// natural 2) takes 9.7 ms for the tank's 2^24 photons against 6.3 at 3.
 #ifndef LH_WAVES_PER_EU
 #define LH_WAVES_PER_EU 3
 #endif
-#define LH_OCC_FWD(F) __attribute__((amdgpu_waves_per_eu(((F) & 1u) ? 1 : LH_WAVES_PER_EU)))
+#define LH_OCC_FWD(F) __attribute__((amdgpu_waves_per_eu(LH_WAVES_PER_EU)))
 
 struct Args {
   Geo g;
@@ -1185,16 +1184,16 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
 }
 
 // the adjoint: every photon walked twice from its Philox counter (adjoint.h); per-block FP64 accumulators in LDS
-// The adjoint without separate reference values runs at 3 wavefronts per SIMD (168 VGPRs, some spilled), one
-// 768-thread block per CU: on the 1,000-PMT tank's 2^24 photons 12.1 ms against 14.0 at four 256-thread blocks with
-// the nodes in memory (and 31.8 at its natural 2 wavefronts); with a Mie or WLS channel (240 VGPRs at its natural
-// 2) the LAr voxels' adjoint takes 8 % less time at 3.  The reference-value variants keep their natural allocation.
+// The adjoint runs at 3 wavefronts per SIMD (168 VGPRs, some spilled), one 768-thread block per CU: on the
+// 1,000-PMT tank's 2^24 photons 12.1 ms against 14.0 at four 256-thread blocks with the nodes in memory (and 31.8
+// at its natural 2 wavefronts); with a Mie or WLS channel (240 VGPRs at its natural 2) the LAr voxels' adjoint
+// takes 8 % less time at 3, with separate reference values (256 VGPRs, 1 wavefront) about a third less.
 #ifndef LH_ADJ_WAVES_PER_EU
 #define LH_ADJ_WAVES_PER_EU 3
 #endif
 template <unsigned F>
 __global__ void __launch_bounds__(adj_block_of<F>())
-    __attribute__((amdgpu_waves_per_eu(((F) & F_REF) ? 1 : LH_ADJ_WAVES_PER_EU)))
+    __attribute__((amdgpu_waves_per_eu(LH_ADJ_WAVES_PER_EU)))
     adjoint_kernel(Args a0, AdjArgs b) {
   double *sg = lds_cells;
   for (int i = threadIdx.x; i < b.n_grad; i += blockDim.x) sg[i] = 0.0;
@@ -1985,9 +1984,7 @@ static unsigned features_of(lh_model m, const Args &a, bool adj) {
       X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47)
 
 // the workgroup size of a launch's kernel
-static int block_for(bool /* adjoint or forward: the same rule */, unsigned f) {
-  return (f & F_REF) != 0 ? BLOCK : LH_FWD_BLOCK;
-}
+static int block_for(bool /* adjoint or forward */, unsigned /* features */) { return BLOCK; }
 
 static const void *kernel_of(bool adj, unsigned f) {
   switch (f) {
