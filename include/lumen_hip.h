@@ -124,7 +124,7 @@ typedef struct {
   float max_distance;    /* <= 0: the geometry's reach */
   int32_t max_vertices;  /* the guard (steps per photon) */
   int32_t chunk;         /* photons a wavefront takes from the work queue at once (0: the engine's default) */
-  int32_t blocks_per_cu; /* resident blocks of 256 per CU of the persistent grid (0: as many as fit) */
+  int32_t blocks_per_cu; /* resident workgroups (768 threads) per CU of the persistent grid (0: as many as fit) */
   int32_t tally_global;  /* 1: accumulate the tallies with global atomics even where they fit in LDS; else 0 */
   int32_t nodes_global;  /* 1: read the BVH nodes from global memory even where they fit in LDS; else 0 */
   int32_t jump;          /* 1: the diffusion jump where the model holds a clearance grid (lh_model_jump); else 0 */
