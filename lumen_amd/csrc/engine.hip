@@ -1104,20 +1104,30 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 // the launch's arguments with the tables the launch placed in LDS copied there and pointed at (every thread of the
 // block calls it; a barrier follows): the packed values (every table search reads them), the layout (rows, media,
 // the class-pair table), the BVH items' transforms and kinds (every item test of the walk)
+// A kernel that reads its BVH nodes from LDS (no F_GNODES) has its packed values, layout and BVH items' transforms
+// and kinds staged there too (the host launches the F_GNODES variant otherwise), so their pointers are LDS
+// addresses the compiler knows as such: LDS reads instead of generic ones that wait on both memory counters (2 %
+// of the tank's forward).  The F_GNODES variant stages what fits, behind generic pointers.
+template <unsigned F>
 __device__ inline Args stage_tables(const Args &a0) {
   Args a = a0;
-  if (a0.lds_values) {
+  constexpr bool ALL = (F & F_GNODES) == 0;
+  if (ALL || a0.lds_values) {
     float *v = reinterpret_cast<float *>(lds_cells) + a0.lds_values_off;
     for (int i = threadIdx.x; i < a0.lds_values; i += blockDim.x) v[i] = a0.P[i];
-    if (a0.Pr == a0.P) a.Pr = v;
+    if constexpr (ALL) {
+      if constexpr ((F & F_REF) == 0) a.Pr = v;
+    } else if (a0.Pr == a0.P) {
+      a.Pr = v;
+    }
     a.P = v;
   }
-  if (a0.lds_layout) {
+  if (ALL || a0.lds_layout) {
     int *l = reinterpret_cast<int *>(lds_cells) + a0.lds_layout_off;
     for (int i = threadIdx.x; i < a0.lds_layout; i += blockDim.x) l[i] = a0.L[i];
     a.L = l;
   }
-  if (a0.lds_items) {
+  if (ALL || a0.lds_items) {
     float4 *x = reinterpret_cast<float4 *>(lds_cells) + a0.lds_ixf_off;
     int4 *k = reinterpret_cast<int4 *>(lds_cells) + a0.lds_ikind_off;
     for (int i = threadIdx.x; i < 3 * a0.lds_items; i += blockDim.x) x[i] = a0.g.ixf[i];
@@ -1136,7 +1146,7 @@ __global__ void __launch_bounds__(block_of<F>()) LH_OCC_FWD(F) trace_kernel(Args
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
-  const Args a = stage_tables(a0);
+  const Args a = stage_tables<F>(a0);
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -1201,7 +1211,7 @@ __global__ void __launch_bounds__(adj_block_of<F>())
   if (threadIdx.x < RF_N) s_rare[threadIdx.x] = 0.0f;
   if (threadIdx.x < 32) s_hint[threadIdx.x] = -1;
   stage_nodes(a0.g);
-  const Args a = stage_tables(a0);
+  const Args a = stage_tables<F>(a0);
   __syncthreads();
   Photon p;
   Acc acc{};
@@ -2046,9 +2056,15 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
   const bool want_nodes = m->g.n_nodes > 0 && !(ctl && ctl->nodes_global == 1);
   const bool want_cells = cells_bytes > 0 && !(cells_optional && ctl && ctl->tally_global == 1);
   const size_t cells_al = (cells_bytes + 15) / 16 * 16;
+  // the packed values, the layout and the BVH items' transforms and kinds (3 float4 rows + an int4 kind each): a
+  // kernel that reads its nodes from LDS reads all of them there too, so the nodes come with all three or not at all
+  const size_t value_bytes = ((size_t)m->values_needed * sizeof(float) + 15) / 16 * 16;
+  const size_t layout_bytes = ((size_t)m->layout_words * sizeof(int) + 15) / 16 * 16;
+  const size_t item_bytes = (size_t)m->n_items * 64;
+  const size_t nodes_need = node_bytes + value_bytes + layout_bytes + item_bytes;
   bool cells = want_cells && (!cells_optional || cells_al <= budget);
-  bool nodes = want_nodes && (cells ? cells_al : 0) + node_bytes <= budget;
-  if (cells_optional && cells && want_nodes && !nodes && node_bytes <= budget) {  // nodes first
+  bool nodes = want_nodes && (cells ? cells_al : 0) + nodes_need <= budget;
+  if (cells_optional && cells && want_nodes && !nodes && nodes_need <= budget) {  // nodes first
     cells = false;
     nodes = true;
   }
@@ -2057,11 +2073,8 @@ static size_t place_lds(lh_model m, Args &a, const void *fn, int block, size_t c
   const size_t off = cells ? cells_al : 0;
   a.g.lds_node_off = (int)(off / 16);
   size_t end = off + (nodes ? node_bytes : 0);
-  // the forward kernel's packed values after them, when they fit (their table searches then read LDS)
-  // then the layout, then the BVH items' transforms and kinds: each where it still fits, in that order
-  const size_t value_bytes = ((size_t)m->values_needed * sizeof(float) + 15) / 16 * 16;
-  const size_t layout_bytes = ((size_t)m->layout_words * sizeof(int) + 15) / 16 * 16;
-  const size_t item_bytes = (size_t)m->n_items * 64;  // 3 float4 rows + an int4 kind
+  // then the values (their table searches then read LDS), the layout and the items, in that order: all of them
+  // beside LDS nodes, else each where it still fits
   a.lds_values = a.lds_values_off = a.lds_layout = a.lds_layout_off = 0;
   a.lds_items = a.lds_ixf_off = a.lds_ikind_off = 0;
   const bool stage = !(ctl && ctl->nodes_global == 1);
