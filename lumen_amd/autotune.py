@@ -3,8 +3,8 @@ the persistent loop, whose 64 lanes each refill from that claim) and the persist
 a staged batch, for its forward and its adjoint launch separately, by timing launches of it, and remembers the
 choices per (detector, launch kind, photons, sources, GPU) in ``$LUMEN_AMD_CACHE`` (default ``~/.cache/lumen_amd/autotune.json``).
 
-Small chunks balance uneven sources at the end of a launch; large ones cut queue traffic.  Fewer blocks per CU
-trade latency hiding for cache footprint.  The search is exhaustive over a small grid; each point is timed on the
+Small chunks balance uneven sources at the end of a launch; large ones cut queue traffic.  Blocks per CU stay a
+search dimension (``tune(blocks=...)``), but the kernels' one 768-thread workgroup per CU leaves a single value.  The search is exhaustive over a small grid; each point is timed on the
 real launch (best of two), which costs a few launches once per configuration.
 """
 
@@ -15,7 +15,9 @@ import os
 import time
 
 CHUNKS = (64, 128, 256, 512, 1024, 4096)
-BLOCKS = (0, 1, 2)  # 0: as many as fit
+# the persistent grid's resident workgroups per CU: each photon-loop kernel's 768-thread workgroup of 12 wavefronts
+# fills a CU at its register budget, so "as many as fit" (0) is one and the search has nothing else to try
+BLOCKS = (0,)
 
 
 def _cache_path():
