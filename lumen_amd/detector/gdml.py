@@ -1,6 +1,6 @@
 """A GDML-subset reader and the detector builder behind ``lumen_amd.load`` (written for this engine).
 
-It reads what the BASELINE detectors use: ``define`` (``matrix``, ``constant``, ``position``, ``rotation``),
+It reads what the tutorial tank, the 1,000-PMT water tank and the LAr box use: ``define`` (``matrix``, ``constant``, ``position``, ``rotation``),
 ``materials`` (material properties by matrix reference: RINDEX, ABSLENGTH, RAYLEIGH, the Mie MIEHG constants, the
 wavelength shifter's WLS properties), ``solids`` (``box``, ``tube`` (a phi cut too), ``cone``, ``polycone`` without a
 phi cut, ``orb``, ``sphere`` with polar cuts, ``ellipsoid``, ``trd``, ``torus``, the booleans, ``opticalsurface``),
