@@ -611,8 +611,9 @@ __device__ inline Cross query_t(const Geo &g, float3 x, float3 d, int vol, int l
         const float te = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
         const float tl = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
         const int sc = __float_as_int(hi.w);
-        const bool in = !(te > tl) && !(te >= core_lo && tl <= core_hi);
-        const bool leaf = in && (sc & 7) != 0;
+        // non-short-circuit: the compares as a mask, no branch per node (3 % of the tank's forward)
+        const bool in = !(te > tl) & !((te >= core_lo) & (tl <= core_hi));
+        const bool leaf = in & ((sc & 7) != 0);
         pl2 = leaf ? pl1 : pl2;
         pl1 = leaf ? pl0 : pl1;
         pl0 = leaf ? sc : pl0;
